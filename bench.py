@@ -1,0 +1,220 @@
+"""Benchmark: end-to-end LFMM2 SNPs/s on one MI355X.
+
+Workload (n=10k, K=20): n=10,000 samples, p=10,000,000 SNPs (packed .bed
+payload ~25 GB generated directly in HBM, Balding-Nichols with 20
+populations, Fst ~ U(0.02, 0.15), 2% missing), d=4 covariates with
+pairwise correlation 0.3, RSVD on a 1% est-rate subset (100,000 SNPs),
+K=20, oversampling 10, 3 power iterations, eigenstrat normalisation.
+
+One step = fit_lfmm2 end to end with the genotypes resident in HBM:
+thin precompute, randomized SVD, the association pass over every SNP,
+t^2 histogram, GIF and calibrated p-values (outputs stay in HBM).
+The headline value is the mixed-precision path; the FP64 GPU path is timed
+beside it and the max |delta log10 p| between the two is reported.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=10_000)
+    ap.add_argument("--p", type=int, default=10_000_000)
+    ap.add_argument("--k", type=int, default=20)
+    ap.add_argument("--d", type=int, default=4)
+    ap.add_argument("--est-rate", type=float, default=0.01)
+    ap.add_argument("--power-iter", type=int, default=3)
+    ap.add_argument("--precision", choices=["auto", "i8", "fp64"], default="auto")
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--cpu-snps", type=int, default=1500)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
+                                   "note (SNP-range sharding + histogram all-reduce), not run here",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    import schnellfmm_amd as S
+    from schnellfmm_amd._lib import AssocOut, lib
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    n, p, K, d = args.n, args.p, args.k, args.d
+    ctx = S.Context(0)
+    t0 = time.time()
+    log(f"[bench] generating n={n} p={p} panel in HBM ...")
+    geno = S.BedFile.synthetic(n, p, n_pops=20, fst=(0.02, 0.15), maf=(0.05, 0.5), miss_rate=0.02,
+                               seed=3, ctx=ctx)
+    torch.cuda.synchronize()
+    log(f"[bench] panel ready in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(3)
+    cov = np.full((d, d), 0.3) + 0.7 * np.eye(d)
+    x = np.ascontiguousarray(rng.multivariate_normal(np.zeros(d), cov, size=n))
+    est_idx = S.subset_indices(args.est_rate, p)
+
+    def alloc_out():
+        t = {k: torch.empty(shape, dtype=torch.float64, device=dev) for k, shape in
+             (("beta", (p, d)), ("t", (p, d)), ("r2", (p, 3)), ("p", (p, d)), ("gpt", (d,)), ("gif", (1,)))}
+        t["u"] = torch.empty((n, K), dtype=torch.float64, device=dev)
+        o = AssocOut()
+        o.beta, o.tstat, o.r2, o.pvalue = (t[k].data_ptr() for k in ("beta", "t", "r2", "p"))
+        o.hist = None
+        o.gif_per_trait, o.gif = t["gpt"].data_ptr(), t["gif"].data_ptr()
+        return t, o
+
+    def make_cfg(prec):
+        return S.Lfmm2Config(k=K, oversampling=10, n_power_iter=args.power_iter, norm=S.EIGENSTRAT,
+                             precision=prec, seed=42).to_c()
+
+    def run(prec, tensors, out):
+        cfg = make_cfg(prec)
+        xs_ptr = x.ctypes.data_as(C.c_void_p)
+        idx_ptr = est_idx.ctypes.data_as(C.c_void_p)
+        ctx.check(lib().slfmm_fit_lfmm2(ctx.h, geno.h, idx_ptr, len(est_idx), geno.h, xs_ptr, d,
+                                        C.byref(cfg), None, C.c_void_p(tensors["u"].data_ptr()),
+                                        C.byref(out), 1))
+
+    def timed(prec, steps, warmup, label):
+        tensors, out = alloc_out()
+        for i in range(warmup):
+            run(prec, tensors, out)
+            log(f"[bench] {label} warmup {i + 1}/{warmup} done")
+        torch.cuda.synchronize()
+        lib().slfmm_profile_enable(ctx.h, 1)
+        for name in (b"assoc_stream", b"rsvd_project", b"rsvd_forward"):
+            lib().slfmm_profile_read(ctx.h, name, None, None, 1)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for i in range(steps):
+            run(prec, tensors, out)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t1) / steps
+        kern = {}
+        for name in ("assoc_stream", "rsvd_project", "rsvd_forward"):
+            ms, cnt = C.c_double(), C.c_uint64()
+            lib().slfmm_profile_read(ctx.h, name.encode(), C.byref(ms), C.byref(cnt), 1)
+            kern[name] = (ms.value, cnt.value)
+        lib().slfmm_profile_enable(ctx.h, 0)
+        log(f"[bench] {label}: {dt * 1e3:.2f} ms/step, kernels {kern}")
+        return dt, kern, tensors
+
+    prec_main = {"i8": S.PREC_I8, "fp64": S.PREC_FP64}.get(args.precision)
+    if prec_main is None:
+        prec_main = S.PREC_I8
+        try:
+            t_, o_ = alloc_out()
+            run(S.PREC_I8, t_, o_)
+            del t_, o_
+        except Exception as e:
+            log(f"[bench] mixed path unavailable ({e}); reporting the FP64 path")
+            prec_main = S.PREC_FP64
+    dt, kern, res_main = timed(prec_main, args.steps, args.warmup, "main")
+    result_main = {k: v for k, v in res_main.items()}
+
+    # FP64 GPU path beside it, and the error of the mixed path against it
+    fp64 = None
+    max_dlp = None
+    if prec_main != S.PREC_FP64 and args.fp64_steps > 0:
+        dt64, kern64, res64 = timed(S.PREC_FP64, args.fp64_steps, 0, "fp64")
+        fp64 = {"ms_per_step": dt64 * 1e3, "snps_per_s": p / dt64}
+        import oracle as O
+        t_m = result_main["t"].cpu().numpy(); g_m = result_main["gpt"].cpu().numpy()
+        t_f = res64["t"].cpu().numpy(); g_f = res64["gpt"].cpu().numpy()
+        lp_m = O.log10_pvalue(t_m, g_m); lp_f = O.log10_pvalue(t_f, g_f)
+        max_dlp = float(np.max(np.abs(lp_m - lp_f)))
+        fp64["max_abs_dlog10p_vs_fp64"] = max_dlp
+        fp64["gif_mixed"] = g_m.tolist(); fp64["gif_fp64"] = g_f.tolist()
+        del res64
+
+    # roofline of the dominant kernel (association genotype stream)
+    ams, acnt = kern["assoc_stream"]
+    avg_ms = ams / max(acnt, 1)
+    bytes_per_launch = p * math.ceil(n / 4)
+    flops_per_launch = p * 2.0 * n * (2 + 2 * d + K)
+    if prec_main == S.PREC_FP64:
+        achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "kernel": "k_func_f64", "avg_ms": avg_ms, "bytes_per_launch": bytes_per_launch}
+    else:
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_assoc_i8", "avg_ms": avg_ms, "bytes_per_launch": bytes_per_launch}
+
+    cpu = None
+    if args.cpu_baseline:
+        cpu = cpu_baseline(geno, x, result_main["u"].cpu().numpy(), args, n, K, d)
+
+    line = {
+        "metric": "SNPs/sec end-to-end (n=10k, K=20): mixed-prec vs FP64 GPU @ max|Δlog10p|≤1e-3",
+        "value": p / dt, "unit": "SNPs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "i8-mfma/i32-acc+f64" if prec_main == S.PREC_I8 else "f64",
+        "data": "synthetic (Balding-Nichols, generated in HBM)",
+        "config": {"workload": "lfmm2_fit n=10000 p=10000000 K=20 d=4 est_rate=0.01 q=3 eigenstrat",
+                   "n": n, "p": p, "K": K, "d": d, "est_snps": int(len(est_idx)),
+                   "power_iter": args.power_iter, "parallelism": "single-gpu"},
+        "roofline": roof,
+        "kernels_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kern.items()},
+        "fp64_gpu": fp64,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(geno, x, u_hat, args, n, K, d):
+    """CPU restatement of the reference algorithm on a bounded SNP prefix."""
+    import oracle as O
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+    except Exception:
+        threads = os.cpu_count() or 1
+    m = int(args.cpu_snps)
+    idx = np.arange(m)
+    raw = geno.decode_raw(idx)                 # n x m genotypes (exact), re-packed for the oracle
+    g = np.where(np.isnan(raw), 255, raw).astype(np.uint8)
+    packed = O.encode_bed(g)
+    bps = O.bytes_per_snp(n)
+    xs = O.center_covariates(x)
+    t0 = time.perf_counter()
+    pre = O.precompute(xs, 1e-5)
+    t1 = time.perf_counter()
+    O.test_associations_fused(packed, bps, n, m, xs, u_hat, pre, K, 500, O.EIGENSTRAT, O.Impute())
+    t2 = time.perf_counter()
+    rate = m / (t2 - t1)
+    return {"value": rate, "unit": "SNPs/s", "cores": int(threads), "kind": "port",
+            "sample": f"oracle dense reference-form association on the first {m} SNPs at n={n}, "
+                      f"K={K}, d={d} (chunks of 500; precompute {t1 - t0:.1f}s not counted; "
+                      f"RSVD excluded, U_hat from the GPU run)",
+            "seconds": t2 - t1}
+
+
+if __name__ == "__main__":
+    main()

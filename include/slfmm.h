@@ -1,0 +1,198 @@
+/*
+ * slfmm.h -- C ABI of the MI355X (gfx950) streaming LFMM2 library.
+ *
+ * Drop-in boundary for schnelLFMM's Rust library entry points (reference
+ * crate `schnellfmm` v0.3.2).  Every entry point returns an int status
+ * (SLFMM_OK = 0), never throws across the ABI, and records a message that
+ * slfmm_last_error() returns.  Matrices are row-major float64 unless noted.
+ * Caller memory is never freed by the library.  Output pointers are host
+ * memory unless SLFMM_OUT_DEVICE is passed in `flags`, in which case they
+ * are device pointers on the context's device (inputs stay host memory).
+ *
+ * Threading: one context owns one HIP stream at default priority; calls on
+ * one context must not overlap.  Several contexts may coexist.
+ */
+#ifndef SLFMM_H
+#define SLFMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct slfmm_ctx slfmm_ctx;
+typedef struct slfmm_geno slfmm_geno;
+
+/* Status codes.  Validation codes mirror the reference's `bail!` sites. */
+enum {
+  SLFMM_OK = 0,
+  SLFMM_E_ARG = 1,        /* bad argument / shape                              */
+  SLFMM_E_HIP = 2,        /* HIP runtime error                                 */
+  SLFMM_E_IO = 3,         /* file open / read (src/bed.rs:86-97)               */
+  SLFMM_E_BED_MAGIC = 4,  /* bad magic bytes (src/bed.rs:105-110)              */
+  SLFMM_E_BED_SIZE = 5,   /* size mismatch (src/bed.rs:113-123)                */
+  SLFMM_E_PARSE = 6,      /* .bim/.fam field count or number (src/bed.rs:456-503) */
+  SLFMM_E_LAMBDA = 7,     /* lambda <= 0 (src/precompute.rs:31-38)             */
+  SLFMM_E_DF = 8,         /* n <= 1+d+K (src/testing.rs:272-278)               */
+  SLFMM_E_SINGULAR = 9,   /* safe_inv failed even after epsilon (src/testing.rs:1014-1021) */
+  SLFMM_E_NOMEM = 10,     /* device allocation failed                          */
+  SLFMM_E_NUMERIC = 11    /* non-finite intermediate                           */
+};
+
+enum { SLFMM_NORM_CENTER_ONLY = 0, SLFMM_NORM_EIGENSTRAT = 1 };   /* src/bed.rs:19-26 */
+enum { SLFMM_IMPUTE_MEAN = 0, SLFMM_IMPUTE_NMF = 1 };              /* src/lib.rs:39-44 */
+
+/* Arithmetic of the genotype-stream GEMMs.  Small dense algebra (QR of the
+ * sketch, small SVD, ridge / K x K solves, GIF calibration) is FP64 on the
+ * GPU in every mode. */
+enum {
+  SLFMM_PREC_FP64 = 0,    /* FP64 throughout                                   */
+  SLFMM_PREC_I8 = 1       /* int8 MFMA on exact genotypes, multi-term int8
+                             operands, exact int32 accumulation, FP64 epilogue */
+};
+
+enum { SLFMM_OUT_DEVICE = 1u };
+
+/* Lfmm2Config (src/lib.rs:47-98) plus the GPU precision switch. */
+typedef struct {
+  uint64_t k;             /* latent factors K            (default 5)       */
+  double lambda;          /* ridge penalty               (default 1e-5)    */
+  uint64_t chunk_size;    /* SNPs per chunk; only seeds CV masks (10000)   */
+  uint64_t oversampling;  /* RSVD oversampling           (default 10)      */
+  uint64_t n_power_iter;  /* RSVD power iterations       (default 2)       */
+  uint64_t seed;          /* RNG seed                    (default 42)      */
+  int32_t norm;           /* SLFMM_NORM_*                (CENTER_ONLY)     */
+  int32_t scale_cov;      /* divide covariates by sd     (0)               */
+  int32_t impute;         /* SLFMM_IMPUTE_*              (MEAN)            */
+  int32_t precision;      /* SLFMM_PREC_*                (FP64)            */
+} slfmm_config;
+
+/* NmfConfig (src/nmf.rs:15-50). */
+typedef struct {
+  uint64_t k;             /* components                  (default 5)       */
+  uint64_t n_iter;        /* multiplicative iterations   (default 10)      */
+  double eps;             /* clamp / update threshold    (default 1e-16)   */
+  double cv_rate;         /* CV hold-out rate            (default 5e-4)    */
+  uint64_t chunk_size;    /* CV mask seeding granularity (default 10000)   */
+  uint64_t seed;          /* base seed                   (default 42)      */
+} slfmm_nmf_config;
+
+/* TestResults (src/testing.rs:76-93) outputs of the association pass. */
+typedef struct {
+  double* beta;           /* p x d ridge effect sizes (src/testing.rs:359-362)     */
+  double* tstat;          /* p x d t statistics       (src/testing.rs:387-390)     */
+  double* r2;             /* p x 3 r2_cov, r2_latent, r2_resid (:392-404)          */
+  double* pvalue;         /* p x d calibrated p values (src/testing.rs:685-713)    */
+  uint64_t* hist;         /* d x 100000 t^2 histogram, may be NULL (:114-149)      */
+  double* gif_per_trait;  /* d, clamped >= 1 (src/testing.rs:159-177)              */
+  double* gif;            /* 1, average of unclamped estimates                     */
+} slfmm_assoc_out;
+
+void slfmm_config_default(slfmm_config* cfg);
+void slfmm_nmf_config_default(slfmm_nmf_config* cfg);
+const char* slfmm_version(void);
+
+int slfmm_create(int device, slfmm_ctx** out);
+void slfmm_destroy(slfmm_ctx* ctx);
+const char* slfmm_last_error(const slfmm_ctx* ctx);
+int slfmm_synchronize(slfmm_ctx* ctx);
+
+/* ---- L1: PLINK .bed (src/bed.rs) --------------------------------------- */
+
+/* BedFile::open validation (src/bed.rs:78-135): parses <stem>.fam/.bim,
+ * checks magic and exact size; reports n_samples and n_snps. */
+int slfmm_bed_open(slfmm_ctx* ctx, const char* bed_path, uint64_t* n_samples, uint64_t* n_snps);
+
+/* BedFile::open + mmap upload; sample_keep (nullable) mirrors
+ * BedFile::subset_samples (src/bed.rs:155-159): physical indices, new order. */
+int slfmm_geno_load_bed(slfmm_ctx* ctx, const char* bed_path, const int64_t* sample_keep,
+                        uint64_t n_keep, slfmm_geno** out);
+
+/* Upload packed SNP-major bytes (the .bed payload after the 3 magic bytes,
+ * ceil(n_phys/4) bytes per SNP). */
+int slfmm_geno_upload(slfmm_ctx* ctx, const uint8_t* packed, uint64_t n_phys, uint64_t n_snps,
+                      const int64_t* sample_keep, uint64_t n_keep, slfmm_geno** out);
+
+/* Wrap packed bytes already resident on the device (row stride `ld` bytes,
+ * ld % 16 == 0, >= ceil(n/4); bytes past ceil(n/4) and padding codes must be
+ * zero, 64 readable bytes must follow the last row).  Not owned. */
+int slfmm_geno_wrap_device(slfmm_ctx* ctx, void* d_packed, uint64_t n, uint64_t n_snps,
+                           uint64_t ld, slfmm_geno** out);
+
+void slfmm_geno_free(slfmm_geno* g);
+int slfmm_geno_info(const slfmm_geno* g, uint64_t* n_samples, uint64_t* n_snps, uint64_t* ld);
+
+/* Per-SNP integer genotype counts: missing calls, sum of dosages, count of
+ * hom-alt calls (exact; mean = sum_g / (n - n_miss)). */
+int slfmm_decode_stats(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* snp_idx, uint64_t n_idx,
+                       int32_t* n_miss, int32_t* sum_g, int32_t* n_hom_alt);
+
+/* decode_bed_chunk_into (src/bed.rs:211-235, :300-379): n x c row-major,
+ * optional n x c fill for missing entries (NULL = mean imputation). */
+int slfmm_decode(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* snp_idx, uint64_t c,
+                 int32_t norm, const double* fill, double* out);
+
+/* decode_raw_bed_chunk_into (src/bed.rs:246-281): {0,1,2,NaN}, n x c. */
+int slfmm_decode_raw(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* snp_idx, uint64_t c,
+                     double* out);
+
+/* ---- L3 algorithms ------------------------------------------------------- */
+
+/* center_covariates (src/lib.rs:102-117).  Host arithmetic (n x d). */
+int slfmm_center_covariates(const double* x, uint64_t n, uint64_t d, int32_t scale, double* out);
+
+/* safe_inv (src/testing.rs:996-1024) on the GPU in FP64; *regularized = 1
+ * when the epsilon*I fallback was taken. */
+int slfmm_safe_inv(slfmm_ctx* ctx, const double* a, uint64_t m, double* out, int32_t* regularized);
+
+/* precompute (src/precompute.rs:30-88) in thin form: sigma (d), d_lambda
+ * (first d entries; the rest are 1), Q_d (n x d, left singular vectors) and
+ * ridge_inv (d x d).  No n x n matrix is formed. */
+int slfmm_precompute(slfmm_ctx* ctx, const double* xs, uint64_t n, uint64_t d, double lambda,
+                     double* sigma, double* d_lambda, double* q_thin, double* ridge_inv);
+
+/* estimate_factors_streaming (src/rsvd.rs:26-177).  xs is the centred
+ * covariate matrix.  b0 = M^T Omega (n x l, l = k + oversampling) may be
+ * injected; NULL draws Omega on the device from cfg->seed.  u_hat: n x k. */
+int slfmm_estimate_factors(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* subset_idx,
+                           uint64_t n_idx, const double* xs, uint64_t d, const slfmm_config* cfg,
+                           const double* b0, double* u_hat, uint32_t flags);
+
+/* test_associations_fused (src/testing.rs:252-528) with mean imputation,
+ * GIF and calibration; outputs per slfmm_assoc_out. */
+int slfmm_test_associations(slfmm_ctx* ctx, const slfmm_geno* g, const double* xs, uint64_t d,
+                            const double* u_hat, const slfmm_config* cfg, slfmm_assoc_out* out,
+                            uint32_t flags);
+
+/* fit_lfmm2 (src/lib.rs:162-263), mean imputation: centres x, estimates
+ * U_hat on est (subset_idx of est, NULL = all), tests every SNP of g. */
+int slfmm_fit_lfmm2(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset_idx,
+                    uint64_t n_idx, const slfmm_geno* g, const double* x, uint64_t d,
+                    const slfmm_config* cfg, const double* b0, double* u_hat,
+                    slfmm_assoc_out* out, uint32_t flags);
+
+/* ---- Kernel timing (no reference counterpart; used by the benchmark) ---- */
+
+/* Enable HIP-event timing of the genotype-stream kernels on the context's
+ * stream (names: "assoc_stream", "rsvd_project", "rsvd_forward"). */
+int slfmm_profile_enable(slfmm_ctx* ctx, int32_t on);
+/* Accumulated device time (ms) and launch count of one timed kernel; reset=1
+ * clears it after reading. */
+int slfmm_profile_read(slfmm_ctx* ctx, const char* name, double* total_ms, uint64_t* launches,
+                       int32_t reset);
+
+/* ---- Synthetic data (test / benchmark inputs, no reference counterpart) --- */
+
+/* Balding-Nichols panel generated on the device into a packed SNP-major
+ * buffer owned by a new geno handle: n samples split evenly into n_pops
+ * populations, F_k ~ U(fst_lo, fst_hi), p_anc ~ U(maf_lo, maf_hi),
+ * missing calls with probability miss_rate. */
+int slfmm_synth_bn(slfmm_ctx* ctx, uint64_t n, uint64_t n_snps, uint32_t n_pops, double fst_lo,
+                   double fst_hi, double maf_lo, double maf_hi, double miss_rate, uint64_t seed,
+                   slfmm_geno** out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SLFMM_H */

@@ -1,0 +1,91 @@
+"""ctypes binding of libslfmm.so (the C ABI declared in include/slfmm.h).
+
+The product path has no CPU fallback: if the HIP library is missing this
+module raises at import of any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libslfmm.so")
+
+u64, i64, i32, u32, dbl = C.c_uint64, C.c_int64, C.c_int32, C.c_uint32, C.c_double
+vp = C.c_void_p
+
+
+class Config(C.Structure):
+    _fields_ = [("k", u64), ("lam", dbl), ("chunk_size", u64), ("oversampling", u64),
+                ("n_power_iter", u64), ("seed", u64), ("norm", i32), ("scale_cov", i32),
+                ("impute", i32), ("precision", i32)]
+
+
+class NmfConfig(C.Structure):
+    _fields_ = [("k", u64), ("n_iter", u64), ("eps", dbl), ("cv_rate", dbl),
+                ("chunk_size", u64), ("seed", u64)]
+
+
+class AssocOut(C.Structure):
+    _fields_ = [("beta", vp), ("tstat", vp), ("r2", vp), ("pvalue", vp), ("hist", vp),
+                ("gif_per_trait", vp), ("gif", vp)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "slfmm_config_default": (None, [vp]),
+    "slfmm_nmf_config_default": (None, [vp]),
+    "slfmm_version": (C.c_char_p, []),
+    "slfmm_create": (C.c_int, [C.c_int, vp]),
+    "slfmm_destroy": (None, [vp]),
+    "slfmm_last_error": (C.c_char_p, [vp]),
+    "slfmm_synchronize": (C.c_int, [vp]),
+    "slfmm_bed_open": (C.c_int, [vp, C.c_char_p, vp, vp]),
+    "slfmm_geno_load_bed": (C.c_int, [vp, C.c_char_p, vp, u64, vp]),
+    "slfmm_geno_upload": (C.c_int, [vp, vp, u64, u64, vp, u64, vp]),
+    "slfmm_geno_wrap_device": (C.c_int, [vp, vp, u64, u64, u64, vp]),
+    "slfmm_geno_free": (None, [vp]),
+    "slfmm_geno_info": (C.c_int, [vp, vp, vp, vp]),
+    "slfmm_decode_stats": (C.c_int, [vp, vp, vp, u64, vp, vp, vp]),
+    "slfmm_decode": (C.c_int, [vp, vp, vp, u64, i32, vp, vp]),
+    "slfmm_decode_raw": (C.c_int, [vp, vp, vp, u64, vp]),
+    "slfmm_center_covariates": (C.c_int, [vp, u64, u64, i32, vp]),
+    "slfmm_safe_inv": (C.c_int, [vp, vp, u64, vp, vp]),
+    "slfmm_precompute": (C.c_int, [vp, vp, u64, u64, dbl, vp, vp, vp, vp]),
+    "slfmm_estimate_factors": (C.c_int, [vp, vp, vp, u64, vp, u64, vp, vp, vp, u32]),
+    "slfmm_test_associations": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, u32]),
+    "slfmm_fit_lfmm2": (C.c_int, [vp, vp, vp, u64, vp, vp, u64, vp, vp, vp, vp, u32]),
+    "slfmm_profile_enable": (C.c_int, [vp, i32]),
+    "slfmm_profile_read": (C.c_int, [vp, C.c_char_p, vp, vp, i32]),
+    "slfmm_synth_bn": (C.c_int, [vp, u64, u64, u32, dbl, dbl, dbl, dbl, dbl, u64, vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libslfmm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built: run python -m schnellfmm_amd.build "
+                               "(there is no CPU fallback for the LFMM2 hot path)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class SlfmmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def check(ctx, rc: int) -> None:
+    if rc != 0:
+        msg = lib().slfmm_last_error(ctx).decode(errors="replace") if ctx else ""
+        raise SlfmmError(rc, msg)

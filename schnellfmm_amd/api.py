@@ -1,0 +1,310 @@
+"""Host-side mirror of schnelLFMM's library interface over the C ABI.
+
+Names, argument meaning and error behaviour follow the reference crate
+(src/lib.rs, src/bed.rs); every numeric step runs in libslfmm.so on the GPU.
+Results come back as numpy arrays instead of the reference's on-disk
+fragments (the output writer, src/testing.rs:530-983, is not part of this
+path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Optional, Sequence, Union
+
+import numpy as np
+
+from ._lib import AssocOut, Config, NmfConfig, SlfmmError, check, lib
+
+CENTER_ONLY, EIGENSTRAT = 0, 1          # SnpNorm, src/bed.rs:19-26
+PREC_FP64, PREC_I8 = 0, 1
+N_BINS = 100_000
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+class Context:
+    """One device, one HIP stream (slfmm_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(None, lib().slfmm_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().slfmm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        check(self.h, rc)
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+@dataclass
+class Lfmm2Config:
+    """src/lib.rs:47-98 (+ precision)."""
+    k: int = 5
+    lam: float = 1e-5
+    chunk_size: int = 10_000
+    oversampling: int = 10
+    n_power_iter: int = 2
+    seed: int = 42
+    n_workers: int = 0          # accepted for interface parity; the GPU needs no host workers
+    progress: bool = False
+    norm: int = CENTER_ONLY
+    scale_cov: bool = False
+    impute: str = "mean"
+    precision: int = PREC_FP64
+
+    def to_c(self) -> Config:
+        c = Config()
+        lib().slfmm_config_default(C.byref(c))
+        c.k, c.lam, c.chunk_size = self.k, self.lam, self.chunk_size
+        c.oversampling, c.n_power_iter, c.seed = self.oversampling, self.n_power_iter, self.seed
+        c.norm, c.scale_cov = self.norm, int(self.scale_cov)
+        c.impute = 0 if self.impute == "mean" else 1
+        c.precision = self.precision
+        return c
+
+
+def subset_indices(spec, n_snps: int) -> np.ndarray:
+    """SubsetSpec expansion (src/parallel.rs:113-122): None/'all', float rate, or indices."""
+    if spec is None or (isinstance(spec, str) and spec == "all"):
+        return np.arange(n_snps, dtype=np.int64)
+    if isinstance(spec, float):
+        if not (0.0 < spec <= 1.0):
+            raise ValueError("est-rate must be in (0, 1]")
+        step = int(math.ceil(1.0 / spec))
+        return np.arange(0, n_snps, step, dtype=np.int64)
+    return np.ascontiguousarray(np.asarray(spec, dtype=np.int64))
+
+
+class BedFile:
+    """A device-resident PLINK genotype panel (BedFile, src/bed.rs:60-172)."""
+
+    def __init__(self, handle, n_samples: int, n_snps: int, ctx: Context, path: Optional[str] = None):
+        self.h = handle
+        self.n_samples = n_samples
+        self.n_snps = n_snps
+        self.ctx = ctx
+        self.path = path
+
+    @classmethod
+    def open(cls, bed_path: str, sample_keep: Optional[Sequence[int]] = None,
+             ctx: Optional[Context] = None) -> "BedFile":
+        """BedFile::open (+ subset_samples): validate, mmap, upload to HBM."""
+        ctx = ctx or default_context()
+        keep = None if sample_keep is None else np.ascontiguousarray(np.asarray(sample_keep, np.int64))
+        h = C.c_void_p()
+        ctx.check(lib().slfmm_geno_load_bed(ctx.h, bed_path.encode(), _ptr(keep),
+                                            0 if keep is None else len(keep), C.byref(h)))
+        n, p = C.c_uint64(), C.c_uint64()
+        lib().slfmm_geno_info(h, C.byref(n), C.byref(p), None)
+        return cls(h, n.value, p.value, ctx, bed_path)
+
+    @staticmethod
+    def validate(bed_path: str, ctx: Optional[Context] = None):
+        ctx = ctx or default_context()
+        n, p = C.c_uint64(), C.c_uint64()
+        ctx.check(lib().slfmm_bed_open(ctx.h, bed_path.encode(), C.byref(n), C.byref(p)))
+        return n.value, p.value
+
+    @classmethod
+    def from_packed(cls, packed: bytes, n_phys: int, n_snps: int,
+                    sample_keep: Optional[Sequence[int]] = None, ctx: Optional[Context] = None):
+        ctx = ctx or default_context()
+        buf = np.frombuffer(packed, dtype=np.uint8)
+        keep = None if sample_keep is None else np.ascontiguousarray(np.asarray(sample_keep, np.int64))
+        h = C.c_void_p()
+        ctx.check(lib().slfmm_geno_upload(ctx.h, _ptr(buf), n_phys, n_snps, _ptr(keep),
+                                          0 if keep is None else len(keep), C.byref(h)))
+        n = len(keep) if keep is not None else n_phys
+        return cls(h, n, n_snps, ctx)
+
+    @classmethod
+    def synthetic(cls, n: int, p: int, n_pops: int = 20, fst=(0.02, 0.15), maf=(0.05, 0.5),
+                  miss_rate: float = 0.02, seed: int = 3, ctx: Optional[Context] = None):
+        """Balding-Nichols panel generated directly in HBM (slfmm_synth_bn)."""
+        ctx = ctx or default_context()
+        h = C.c_void_p()
+        ctx.check(lib().slfmm_synth_bn(ctx.h, n, p, n_pops, fst[0], fst[1], maf[0], maf[1],
+                                       miss_rate, seed, C.byref(h)))
+        return cls(h, n, p, ctx)
+
+    def subset_snp_count(self, spec) -> int:
+        return len(subset_indices(spec, self.n_snps))
+
+    def free(self):
+        if self.h:
+            lib().slfmm_geno_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # ---- decode (src/bed.rs:211-281) ----
+    def decode(self, snp_idx, norm: int = CENTER_ONLY, fill: Optional[np.ndarray] = None) -> np.ndarray:
+        idx = np.ascontiguousarray(np.asarray(snp_idx, np.int64))
+        out = np.empty((self.n_samples, len(idx)))
+        f = None if fill is None else _f64(fill)
+        self.ctx.check(lib().slfmm_decode(self.ctx.h, self.h, _ptr(idx), len(idx), norm, _ptr(f), _ptr(out)))
+        return out
+
+    def decode_raw(self, snp_idx) -> np.ndarray:
+        idx = np.ascontiguousarray(np.asarray(snp_idx, np.int64))
+        out = np.empty((self.n_samples, len(idx)))
+        self.ctx.check(lib().slfmm_decode_raw(self.ctx.h, self.h, _ptr(idx), len(idx), _ptr(out)))
+        return out
+
+    def stats(self, snp_idx):
+        idx = np.ascontiguousarray(np.asarray(snp_idx, np.int64))
+        a, b, c = (np.empty(len(idx), np.int32) for _ in range(3))
+        self.ctx.check(lib().slfmm_decode_stats(self.ctx.h, self.h, _ptr(idx), len(idx), _ptr(a), _ptr(b), _ptr(c)))
+        return a, b, c
+
+
+def center_covariates(x, scale: bool = False) -> np.ndarray:
+    """src/lib.rs:102-117."""
+    x = _f64(x)
+    out = np.empty_like(x)
+    rc = lib().slfmm_center_covariates(_ptr(x), x.shape[0], x.shape[1], int(scale), _ptr(out))
+    if rc:
+        raise SlfmmError(rc, "center_covariates")
+    return out
+
+
+def safe_inv(a, ctx: Optional[Context] = None):
+    """safe_inv (src/testing.rs:996-1024) on the GPU: returns (inverse, regularized)."""
+    ctx = ctx or default_context()
+    a = _f64(a)
+    out = np.empty_like(a)
+    reg = C.c_int32()
+    ctx.check(lib().slfmm_safe_inv(ctx.h, _ptr(a), a.shape[0], _ptr(out), C.byref(reg)))
+    return out, bool(reg.value)
+
+
+@dataclass
+class Precomputed:
+    sigma: np.ndarray
+    d_lambda: np.ndarray
+    q_thin: np.ndarray
+    ridge_inv: np.ndarray
+
+
+def precompute(xs, lam: float, ctx: Optional[Context] = None) -> Precomputed:
+    """Thin-form precompute (src/precompute.rs:30-88)."""
+    ctx = ctx or default_context()
+    xs = _f64(xs)
+    n, d = xs.shape
+    s, dl, q, r = np.empty(d), np.empty(d), np.empty((n, d)), np.empty((d, d))
+    ctx.check(lib().slfmm_precompute(ctx.h, _ptr(xs), n, d, float(lam), _ptr(s), _ptr(dl), _ptr(q), _ptr(r)))
+    return Precomputed(s, dl, q, r)
+
+
+@dataclass
+class TestResults:
+    """src/testing.rs:76-93 plus the per-SNP arrays."""
+    u_hat: np.ndarray
+    gif: float
+    gif_per_trait: np.ndarray
+    beta: np.ndarray
+    tstat: np.ndarray
+    r2: np.ndarray
+    pvalue: np.ndarray
+    hist: Optional[np.ndarray] = None
+    nmf_cv: Optional[list] = None
+    nmf_cv_mean: Optional[float] = None
+
+    __test__ = False   # not a pytest class
+
+
+def estimate_factors_streaming(y_est: BedFile, subset, xs, config: Lfmm2Config, b0=None) -> np.ndarray:
+    """src/rsvd.rs:26-177 on centred covariates xs; b0 = M^T Omega may be injected."""
+    ctx = y_est.ctx
+    xs = _f64(xs)
+    idx = None if subset is None or (isinstance(subset, str) and subset == "all") else subset_indices(subset, y_est.n_snps)
+    cfg = config.to_c()
+    u = np.empty((y_est.n_samples, config.k))
+    b = None if b0 is None else _f64(b0)
+    ctx.check(lib().slfmm_estimate_factors(ctx.h, y_est.h, _ptr(idx), 0 if idx is None else len(idx),
+                                           _ptr(xs), xs.shape[1], C.byref(cfg), _ptr(b), _ptr(u), 0))
+    return u
+
+
+def estimate_factors(y_est: BedFile, subset, x, config: Lfmm2Config, b0=None) -> np.ndarray:
+    """src/lib.rs:127-136."""
+    return estimate_factors_streaming(y_est, subset, center_covariates(x, config.scale_cov), config, b0)
+
+
+def _assoc_arrays(p: int, d: int, with_hist: bool):
+    return (np.empty((p, d)), np.empty((p, d)), np.empty((p, 3)), np.empty((p, d)),
+            np.empty((d, N_BINS), np.uint64) if with_hist else None, np.empty(d), np.empty(1))
+
+
+def _out_struct(arrs) -> AssocOut:
+    o = AssocOut()
+    o.beta, o.tstat, o.r2, o.pvalue, o.hist, o.gif_per_trait, o.gif = (_ptr(a) for a in arrs)
+    return o
+
+
+def test_associations_fused(y_full: BedFile, xs, u_hat, config: Lfmm2Config, with_hist: bool = False) -> TestResults:
+    """src/testing.rs:252-528 on centred covariates."""
+    ctx = y_full.ctx
+    xs, u_hat = _f64(xs), _f64(u_hat)
+    arrs = _assoc_arrays(y_full.n_snps, xs.shape[1], with_hist)
+    out = _out_struct(arrs)
+    cfg = config.to_c()
+    ctx.check(lib().slfmm_test_associations(ctx.h, y_full.h, _ptr(xs), xs.shape[1], _ptr(u_hat),
+                                            C.byref(cfg), C.byref(out), 0))
+    beta, t, r2, p, hist, gpt, gif = arrs
+    return TestResults(u_hat, float(gif[0]), gpt, beta, t, r2, p, hist)
+
+
+def test_associations(y_full: BedFile, x, u_hat, config: Lfmm2Config, with_hist: bool = False) -> TestResults:
+    """src/lib.rs:144-154."""
+    return test_associations_fused(y_full, center_covariates(x, config.scale_cov), u_hat, config, with_hist)
+
+
+def fit_lfmm2(y_est: BedFile, est_subset, y_full: BedFile, x, config: Lfmm2Config, b0=None,
+              with_hist: bool = False) -> TestResults:
+    """Full pipeline, src/lib.rs:162-263 (mean imputation)."""
+    ctx = y_full.ctx
+    x = _f64(x)
+    idx = None if est_subset is None or (isinstance(est_subset, str) and est_subset == "all") \
+        else subset_indices(est_subset, y_est.n_snps)
+    d = x.shape[1]
+    arrs = _assoc_arrays(y_full.n_snps, d, with_hist)
+    out = _out_struct(arrs)
+    u = np.empty((y_full.n_samples, config.k))
+    cfg = config.to_c()
+    b = None if b0 is None else _f64(b0)
+    ctx.check(lib().slfmm_fit_lfmm2(ctx.h, y_est.h, _ptr(idx), 0 if idx is None else len(idx), y_full.h,
+                                    _ptr(x), d, C.byref(cfg), _ptr(b), _ptr(u), C.byref(out), 0))
+    beta, t, r2, p, hist, gpt, gif = arrs
+    return TestResults(u, float(gif[0]), gpt, beta, t, r2, p, hist)

@@ -1,0 +1,513 @@
+// C ABI of the slfmm library and the host-side orchestration that mirrors
+// the reference's library entry points (src/lib.rs, src/bed.rs).
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "internal.h"
+
+using namespace slfmm;
+
+void* slfmm_ctx::scratch(const std::string& name, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  auto it = pool.find(name);
+  if (it != pool.end() && it->second.second >= bytes) return it->second.first;
+  if (it != pool.end()) {
+    SLFMM_HIP(hipStreamSynchronize(stream));
+    SLFMM_HIP(hipFree(it->second.first));
+    pool.erase(it);
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(SLFMM_E_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes (" + name + ") failed");
+  }
+  pool[name] = {p, bytes};
+  return p;
+}
+
+std::pair<hipEvent_t, hipEvent_t> slfmm_ctx::tic(const std::string&) {
+  hipEvent_t a = nullptr, b = nullptr;
+  SLFMM_HIP(hipEventCreate(&a));
+  SLFMM_HIP(hipEventCreate(&b));
+  SLFMM_HIP(hipEventRecord(a, stream));
+  return {a, b};
+}
+
+void slfmm_ctx::toc(const std::string& name, std::pair<hipEvent_t, hipEvent_t> ev) {
+  (void)hipEventRecord(ev.second, stream);
+  timers[name].pending.push_back(ev);
+}
+
+void slfmm_ctx::collect() {
+  for (auto& kv : timers) {
+    for (auto& ev : kv.second.pending) {
+      float ms = 0.f;
+      SLFMM_HIP(hipEventSynchronize(ev.second));
+      SLFMM_HIP(hipEventElapsedTime(&ms, ev.first, ev.second));
+      kv.second.ms += ms;
+      kv.second.launches += 1;
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+    kv.second.pending.clear();
+  }
+}
+
+slfmm_ctx::~slfmm_ctx() {
+  for (auto& kv : timers)
+    for (auto& ev : kv.second.pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& kv : pool) (void)hipFree(kv.second.first);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+template <class F>
+int guarded(slfmm_ctx* ctx, F&& f) {
+  try {
+    if (ctx) SLFMM_HIP(hipSetDevice(ctx->device));
+    f();
+    if (ctx) ctx->err.clear();
+    return SLFMM_OK;
+  } catch (const Error& e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (ctx) ctx->err = "host allocation failed";
+    return SLFMM_E_NOMEM;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return SLFMM_E_ARG;
+  }
+}
+
+// Device copy of a host array through a named scratch buffer.
+template <class T>
+T* to_dev(slfmm_ctx* ctx, const char* name, const T* h, size_t count) {
+  T* d = ctx->buf<T>(name, count);
+  if (count) SLFMM_HIP(hipMemcpyAsync(d, h, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return d;
+}
+
+template <class T>
+void to_host(slfmm_ctx* ctx, T* h, const T* d, size_t count) {
+  if (count && h) SLFMM_HIP(hipMemcpyAsync(h, d, count * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+}
+
+// ---- .bim / .fam parsing (src/bed.rs:456-503) -------------------------------
+size_t count_records(const std::string& path, const char* kind, bool bim) {
+  std::ifstream f(path);
+  if (!f) throw Error(SLFMM_E_IO, "Failed to parse " + path + ": cannot open");
+  std::string line;
+  size_t i = 0;
+  while (std::getline(f, line)) {
+    ++i;
+    std::istringstream ss(line);
+    std::vector<std::string> fld;
+    std::string w;
+    while (ss >> w) fld.push_back(w);
+    if (fld.size() < 6)
+      throw Error(SLFMM_E_PARSE, "Failed to parse " + path + ": " + kind + " line " + std::to_string(i) + " has " +
+                                     std::to_string(fld.size()) + " fields, expected 6");
+    char* end = nullptr;
+    if (bim) {
+      std::strtod(fld[2].c_str(), &end);
+      if (*end) throw Error(SLFMM_E_PARSE, "Failed to parse " + path + ": BIM line " + std::to_string(i) + ": invalid cM '" + fld[2] + "'");
+      for (char c : fld[3])
+        if (c < '0' || c > '9') throw Error(SLFMM_E_PARSE, "Failed to parse " + path + ": BIM line " + std::to_string(i) + ": invalid position '" + fld[3] + "'");
+    } else {
+      long v = std::strtol(fld[4].c_str(), &end, 10);
+      if (*end || v < 0 || v > 255) throw Error(SLFMM_E_PARSE, "Failed to parse " + path + ": FAM line " + std::to_string(i) + ": invalid sex '" + fld[4] + "'");
+    }
+  }
+  return i;
+}
+
+std::string with_ext(const std::string& bed, const char* ext) {
+  size_t slash = bed.find_last_of('/');
+  size_t dot = bed.find_last_of('.');
+  std::string stem = (dot != std::string::npos && (slash == std::string::npos || dot > slash)) ? bed.substr(0, dot) : bed;
+  return stem + "." + ext;
+}
+
+struct Mapped {
+  const uint8_t* p = nullptr;
+  size_t len = 0;
+  int fd = -1;
+  ~Mapped() {
+    if (p && len) munmap((void*)p, len);
+    if (fd >= 0) close(fd);
+  }
+};
+
+void bed_open(const std::string& path, uint64_t& n, uint64_t& p, Mapped* map) {
+  n = count_records(with_ext(path, "fam"), "FAM", false);
+  p = count_records(with_ext(path, "bim"), "BIM", true);
+  Mapped local;
+  Mapped& m = map ? *map : local;
+  m.fd = open(path.c_str(), O_RDONLY);
+  if (m.fd < 0) throw Error(SLFMM_E_IO, "Failed to open " + path);
+  struct stat st;
+  fstat(m.fd, &st);
+  m.len = (size_t)st.st_size;
+  if (m.len) {
+    void* a = mmap(nullptr, m.len, PROT_READ, MAP_PRIVATE, m.fd, 0);
+    if (a == MAP_FAILED) throw Error(SLFMM_E_IO, "mmap failed for " + path);
+    m.p = (const uint8_t*)a;
+    madvise(a, m.len, MADV_SEQUENTIAL);
+  }
+  if (m.len < 3 || m.p[0] != 0x6C || m.p[1] != 0x1B || m.p[2] != 0x01)
+    throw Error(SLFMM_E_BED_MAGIC, "Invalid .bed file (bad magic bytes): " + path);
+  const uint64_t expected = 3 + cdiv(n, 4) * p;
+  if (m.len != expected)
+    throw Error(SLFMM_E_BED_SIZE, "BED file size mismatch: expected " + std::to_string(expected) + " bytes (" +
+                                      std::to_string(n) + " samples × " + std::to_string(p) + " SNPs), got " +
+                                      std::to_string(m.len));
+}
+
+void check_cfg(const slfmm_config* cfg) {
+  SLFMM_CHECK(cfg != nullptr, SLFMM_E_ARG, "config is NULL");
+  SLFMM_CHECK(cfg->precision == SLFMM_PREC_FP64 || cfg->precision == SLFMM_PREC_I8, SLFMM_E_ARG, "unknown precision");
+  SLFMM_CHECK(cfg->norm == SLFMM_NORM_CENTER_ONLY || cfg->norm == SLFMM_NORM_EIGENSTRAT, SLFMM_E_ARG, "unknown norm");
+  SLFMM_CHECK(cfg->impute == SLFMM_IMPUTE_MEAN, SLFMM_E_ARG, "only mean imputation is available in this entry point");
+}
+
+void center_host(const double* x, uint64_t n, uint64_t d, int scale, double* out) {
+  // src/lib.rs:102-117: subtract column means; optionally divide by sd (n-1)
+  for (uint64_t j = 0; j < d; ++j) {
+    double s = 0.0;
+    for (uint64_t i = 0; i < n; ++i) s += x[i * d + j];
+    double mean = s / (double)n;
+    for (uint64_t i = 0; i < n; ++i) out[i * d + j] = x[i * d + j] - mean;
+    if (scale) {
+      double v = 0.0;
+      for (uint64_t i = 0; i < n; ++i) v += out[i * d + j] * out[i * d + j];
+      double sd = std::sqrt(v / ((double)n - 1.0));
+      if (sd > 1e-14)
+        for (uint64_t i = 0; i < n; ++i) out[i * d + j] /= sd;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void slfmm_config_default(slfmm_config* c) {
+  if (!c) return;
+  c->k = 5; c->lambda = 1e-5; c->chunk_size = 10000; c->oversampling = 10; c->n_power_iter = 2;
+  c->seed = 42; c->norm = SLFMM_NORM_CENTER_ONLY; c->scale_cov = 0; c->impute = SLFMM_IMPUTE_MEAN;
+  c->precision = SLFMM_PREC_FP64;
+}
+
+void slfmm_nmf_config_default(slfmm_nmf_config* c) {
+  if (!c) return;
+  c->k = 5; c->n_iter = 10; c->eps = 1e-16; c->cv_rate = 0.0005; c->chunk_size = 10000; c->seed = 42;
+}
+
+const char* slfmm_version(void) { return "slfmm 0.1.0 (gfx950)"; }
+
+int slfmm_create(int device, slfmm_ctx** out) {
+  if (!out) return SLFMM_E_ARG;
+  *out = nullptr;
+  slfmm_ctx* c = new slfmm_ctx();
+  c->device = device;
+  int rc = guarded(c, [&] { SLFMM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)); });
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return SLFMM_OK;
+}
+
+void slfmm_destroy(slfmm_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  delete ctx;
+}
+
+const char* slfmm_last_error(const slfmm_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int slfmm_synchronize(slfmm_ctx* ctx) {
+  if (!ctx) return SLFMM_E_ARG;
+  return guarded(ctx, [&] { SLFMM_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int slfmm_bed_open(slfmm_ctx* ctx, const char* bed_path, uint64_t* n_samples, uint64_t* n_snps) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(bed_path && n_samples && n_snps, SLFMM_E_ARG, "NULL argument");
+    bed_open(bed_path, *n_samples, *n_snps, nullptr);
+  });
+}
+
+int slfmm_geno_load_bed(slfmm_ctx* ctx, const char* bed_path, const int64_t* keep, uint64_t n_keep,
+                        slfmm_geno** out) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && bed_path && out, SLFMM_E_ARG, "NULL argument");
+    Mapped m;
+    uint64_t n = 0, p = 0;
+    bed_open(bed_path, n, p, &m);
+    auto* g = new slfmm_geno();
+    try {
+      geno_upload(ctx, m.p + 3, n, p, keep, n_keep, g);
+      SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+      slfmm_geno_free(g);
+      throw;
+    }
+    *out = g;
+  });
+}
+
+int slfmm_geno_upload(slfmm_ctx* ctx, const uint8_t* packed, uint64_t n_phys, uint64_t n_snps,
+                      const int64_t* keep, uint64_t n_keep, slfmm_geno** out) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && out && (packed || n_snps == 0), SLFMM_E_ARG, "NULL argument");
+    auto* g = new slfmm_geno();
+    try {
+      geno_upload(ctx, packed, n_phys, n_snps, keep, n_keep, g);
+      SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+      slfmm_geno_free(g);
+      throw;
+    }
+    *out = g;
+  });
+}
+
+int slfmm_geno_wrap_device(slfmm_ctx* ctx, void* d_packed, uint64_t n, uint64_t n_snps, uint64_t ld,
+                           slfmm_geno** out) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && out && d_packed, SLFMM_E_ARG, "NULL argument");
+    SLFMM_CHECK(ld % 16 == 0 && ld >= cdiv(n, 4), SLFMM_E_ARG, "row stride must be a multiple of 16 >= ceil(n/4)");
+    auto* g = new slfmm_geno();
+    g->ctx = ctx; g->d = (uint8_t*)d_packed; g->n = n; g->p = n_snps; g->ld = ld; g->owned = false;
+    *out = g;
+  });
+}
+
+void slfmm_geno_free(slfmm_geno* g) {
+  if (!g) return;
+  if (g->owned && g->d) {
+    if (g->ctx) {
+      (void)hipSetDevice(g->ctx->device);
+      (void)hipStreamSynchronize(g->ctx->stream);
+    }
+    (void)hipFree(g->d);
+  }
+  delete g;
+}
+
+int slfmm_geno_info(const slfmm_geno* g, uint64_t* n, uint64_t* p, uint64_t* ld) {
+  if (!g) return SLFMM_E_ARG;
+  if (n) *n = g->n;
+  if (p) *p = g->p;
+  if (ld) *ld = g->ld;
+  return SLFMM_OK;
+}
+
+int slfmm_decode_stats(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* snp_idx, uint64_t n_idx,
+                       int32_t* n_miss, int32_t* sum_g, int32_t* n_hom) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && g && snp_idx, SLFMM_E_ARG, "NULL argument");
+    for (uint64_t i = 0; i < n_idx; ++i)
+      SLFMM_CHECK(snp_idx[i] >= 0 && (uint64_t)snp_idx[i] < g->p, SLFMM_E_ARG, "SNP index out of range");
+    hipStream_t s = ctx->stream;
+    int64_t* didx = to_dev(ctx, "api_idx", snp_idx, n_idx);
+    uint8_t* Y = ctx->buf<uint8_t>("api_Y", n_idx * g->ld + 64);
+    SLFMM_HIP(hipMemsetAsync(Y + n_idx * g->ld, 0, 64, s));
+    geno_gather(ctx, g, didx, n_idx, Y);
+    int32_t* a = ctx->buf<int32_t>("api_s0", n_idx);
+    int32_t* b = ctx->buf<int32_t>("api_s1", n_idx);
+    int32_t* c = ctx->buf<int32_t>("api_s2", n_idx);
+    launch_stats(s, Y, g->ld, g->n, n_idx, a, b, c);
+    to_host(ctx, n_miss, a, n_idx);
+    to_host(ctx, sum_g, b, n_idx);
+    to_host(ctx, n_hom, c, n_idx);
+    SLFMM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+static int decode_common(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* snp_idx, uint64_t c, int norm,
+                         const double* fill, double* out, int raw) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && g && snp_idx && out, SLFMM_E_ARG, "NULL argument");
+    for (uint64_t i = 0; i < c; ++i)
+      SLFMM_CHECK(snp_idx[i] >= 0 && (uint64_t)snp_idx[i] < g->p, SLFMM_E_ARG, "SNP index out of range");
+    hipStream_t s = ctx->stream;
+    int64_t* didx = to_dev(ctx, "api_idx", snp_idx, c);
+    const double* dfill = fill ? to_dev(ctx, "api_fill", fill, g->n * c) : nullptr;
+    double* dout = ctx->buf<double>("api_out", g->n * c);
+    launch_decode_f64(s, g->d, g->ld, g->n, didx, c, norm, dfill, dout, raw);
+    to_host(ctx, out, dout, g->n * c);
+    SLFMM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int slfmm_decode(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* snp_idx, uint64_t c, int32_t norm,
+                 const double* fill, double* out) {
+  return decode_common(ctx, g, snp_idx, c, norm, fill, out, 0);
+}
+
+int slfmm_decode_raw(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* snp_idx, uint64_t c, double* out) {
+  return decode_common(ctx, g, snp_idx, c, 0, nullptr, out, 1);
+}
+
+int slfmm_center_covariates(const double* x, uint64_t n, uint64_t d, int32_t scale, double* out) {
+  if (!x || !out || n == 0) return SLFMM_E_ARG;
+  center_host(x, n, d, scale, out);
+  return SLFMM_OK;
+}
+
+int slfmm_safe_inv(slfmm_ctx* ctx, const double* a, uint64_t m, double* out, int32_t* regularized) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && a && out, SLFMM_E_ARG, "NULL argument");
+    double* da = to_dev(ctx, "api_inv_a", a, m * m);
+    double* dout = ctx->buf<double>("api_inv_o", m * m);
+    int inf = safe_inv_dev(ctx, da, m, dout);
+    if (regularized) *regularized = inf == 1;
+    if (inf == 1) fprintf(stderr, "Warning: matrix is singular, adding epsilon diagonal regularization\n");
+    SLFMM_CHECK(inf != 2, SLFMM_E_SINGULAR, "inversion failed even with epsilon regularization");
+    to_host(ctx, out, dout, m * m);
+    SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int slfmm_precompute(slfmm_ctx* ctx, const double* xs, uint64_t n, uint64_t d, double lambda, double* sigma,
+                     double* d_lambda, double* q_thin, double* ridge_inv) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && xs, SLFMM_E_ARG, "NULL argument");
+    double* dx = to_dev(ctx, "api_xs", xs, n * d);
+    double* ds = ctx->buf<double>("api_sig", d);
+    double* dl = ctx->buf<double>("api_dl", d);
+    double* dq = ctx->buf<double>("api_qd", n * d);
+    double* dr = ctx->buf<double>("api_rg", d * d);
+    precompute_dev(ctx, dx, n, d, lambda, ds, dl, dq, dr);
+    to_host(ctx, sigma, ds, d);
+    to_host(ctx, d_lambda, dl, d);
+    to_host(ctx, q_thin, dq, n * d);
+    to_host(ctx, ridge_inv, dr, d * d);
+    SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int slfmm_estimate_factors(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* subset_idx, uint64_t n_idx,
+                           const double* xs, uint64_t d, const slfmm_config* cfg, const double* b0,
+                           double* u_hat, uint32_t flags) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && g && xs && u_hat, SLFMM_E_ARG, "NULL argument");
+    check_cfg(cfg);
+    const uint64_t n = g->n, l = cfg->k + cfg->oversampling;
+    double* dx = to_dev(ctx, "api_xs", xs, n * d);
+    const double* db0 = b0 ? to_dev(ctx, "api_b0", b0, n * l) : nullptr;
+    double* du = (flags & SLFMM_OUT_DEVICE) ? u_hat : ctx->buf<double>("api_uhat", n * cfg->k);
+    rsvd_run(ctx, g, subset_idx, subset_idx ? n_idx : g->p, dx, d, *cfg, db0, du);
+    if (!(flags & SLFMM_OUT_DEVICE)) to_host(ctx, u_hat, du, n * cfg->k);
+    SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+static void assoc_host(slfmm_ctx* ctx, const slfmm_geno* g, const double* dx, uint64_t d, const double* du,
+                       const slfmm_config* cfg, slfmm_assoc_out* out, uint32_t flags) {
+  const uint64_t p = g->p;
+  const bool dev = flags & SLFMM_OUT_DEVICE;
+  double* beta = dev ? out->beta : ctx->buf<double>("api_beta", p * d);
+  double* t = dev ? out->tstat : ctx->buf<double>("api_t", p * d);
+  double* r2 = dev ? out->r2 : ctx->buf<double>("api_r2", p * 3);
+  double* pv = dev ? out->pvalue : ctx->buf<double>("api_p", p * d);
+  uint64_t* hist = (dev && out->hist) ? out->hist : (out->hist ? ctx->buf<uint64_t>("api_hist", d * kBins) : nullptr);
+  double* gpt = dev ? out->gif_per_trait : ctx->buf<double>("api_gpt", d);
+  double* gif = dev ? out->gif : ctx->buf<double>("api_gif", 1);
+  SLFMM_CHECK(beta && t && r2 && pv && gpt && gif, SLFMM_E_ARG, "NULL output pointer");
+  assoc_run(ctx, g, dx, d, du, cfg->k, cfg->lambda, *cfg, beta, t, r2, pv, hist, gpt, gif);
+  if (!dev) {
+    to_host(ctx, out->beta, beta, p * d);
+    to_host(ctx, out->tstat, t, p * d);
+    to_host(ctx, out->r2, r2, p * 3);
+    to_host(ctx, out->pvalue, pv, p * d);
+    if (out->hist) to_host(ctx, out->hist, hist, d * kBins);
+    to_host(ctx, out->gif_per_trait, gpt, d);
+    to_host(ctx, out->gif, gif, 1);
+  }
+}
+
+int slfmm_test_associations(slfmm_ctx* ctx, const slfmm_geno* g, const double* xs, uint64_t d,
+                            const double* u_hat, const slfmm_config* cfg, slfmm_assoc_out* out, uint32_t flags) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && g && xs && u_hat && out, SLFMM_E_ARG, "NULL argument");
+    check_cfg(cfg);
+    double* dx = to_dev(ctx, "api_xs", xs, g->n * d);
+    double* du = to_dev(ctx, "api_u", u_hat, g->n * cfg->k);
+    assoc_host(ctx, g, dx, d, du, cfg, out, flags);
+    SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int slfmm_fit_lfmm2(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset_idx, uint64_t n_idx,
+                    const slfmm_geno* g, const double* x, uint64_t d, const slfmm_config* cfg, const double* b0,
+                    double* u_hat, slfmm_assoc_out* out, uint32_t flags) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && est && g && x && out, SLFMM_E_ARG, "NULL argument");
+    SLFMM_CHECK(est->n == g->n, SLFMM_E_ARG, "estimation and test panels must have the same samples");
+    check_cfg(cfg);
+    const uint64_t n = g->n, K = cfg->k, l = K + cfg->oversampling;
+    std::vector<double> xs(n * d);
+    center_host(x, n, d, cfg->scale_cov, xs.data());                     // src/lib.rs:170
+    double* dx = to_dev(ctx, "api_xs", xs.data(), n * d);
+    const double* db0 = b0 ? to_dev(ctx, "api_b0", b0, n * l) : nullptr;
+    const bool dev = flags & SLFMM_OUT_DEVICE;
+    double* du = (dev && u_hat) ? u_hat : ctx->buf<double>("api_uhat", n * K);
+    rsvd_run(ctx, est, subset_idx, subset_idx ? n_idx : est->p, dx, d, *cfg, db0, du);   // src/lib.rs:231
+    assoc_host(ctx, g, dx, d, du, cfg, out, flags);                    // src/lib.rs:254
+    if (!dev && u_hat) to_host(ctx, u_hat, du, n * K);
+    SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int slfmm_profile_enable(slfmm_ctx* ctx, int32_t on) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx, SLFMM_E_ARG, "NULL context");
+    ctx->profiling = on != 0;
+  });
+}
+
+int slfmm_profile_read(slfmm_ctx* ctx, const char* name, double* total_ms, uint64_t* launches, int32_t reset) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && name, SLFMM_E_ARG, "NULL argument");
+    ctx->collect();
+    auto& t = ctx->timers[name];
+    if (total_ms) *total_ms = t.ms;
+    if (launches) *launches = t.launches;
+    if (reset) { t.ms = 0.0; t.launches = 0; }
+  });
+}
+
+int slfmm_synth_bn(slfmm_ctx* ctx, uint64_t n, uint64_t n_snps, uint32_t n_pops, double fst_lo, double fst_hi,
+                   double maf_lo, double maf_hi, double miss_rate, uint64_t seed, slfmm_geno** out) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && out && n > 0 && n_pops > 0, SLFMM_E_ARG, "bad argument");
+    SLFMM_CHECK(fst_lo > 0.0 && fst_hi < 1.0 && fst_lo <= fst_hi, SLFMM_E_ARG, "Fst range must lie in (0,1)");
+    auto* g = new slfmm_geno();
+    g->n = n; g->p = n_snps;
+    try {
+      geno_alloc(ctx, g);
+      launch_synth_bn(ctx->stream, g->d, g->ld, n, n_snps, n_pops, fst_lo, fst_hi, maf_lo, maf_hi, miss_rate, seed);
+      SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+      slfmm_geno_free(g);
+      throw;
+    }
+    *out = g;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,401 @@
+// Dense FP64 helpers that stay on the GPU: tall-skinny Gram / GEMM with
+// deterministic reductions, and one-workgroup solvers for the small
+// matrices of the pipeline (m <= 64): Cholesky (CholeskyQR2 of the sketch),
+// LU inverse with the reference's safe_inv fallback, and one-sided Jacobi
+// (small SVD / symmetric eigendecomposition).
+#include "internal.h"
+
+namespace slfmm {
+
+constexpr int kSm = 64;   // max order of the one-workgroup solvers
+
+__global__ void k_fill(double* p, double v, uint64_t count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+void launch_fill(hipStream_t s, double* p, double v, uint64_t count) {
+  if (!count) return;
+  k_fill<<<(unsigned)std::min<uint64_t>(cdiv(count, 256), 4096), 256, 0, s>>>(p, v, count);
+  SLFMM_HIP(hipGetLastError());
+}
+
+// ---- tall-skinny Gram: out = A^T B ------------------------------------------
+__global__ void __launch_bounds__(256) k_gram_partial(const double* __restrict__ A, uint64_t lda, int a,
+                                                      const double* __restrict__ B, uint64_t ldb, int b,
+                                                      uint64_t rows, uint64_t rpb, double* __restrict__ part) {
+  __shared__ double sa[32][kSm + 1];
+  __shared__ double sb[32][kSm + 1];
+  const int t = threadIdx.x;
+  const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
+  const uint64_t r1 = std::min(rows, r0 + rpb);
+  const int nout = a * b;
+  double acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+  for (uint64_t rb = r0; rb < r1; rb += 32) {
+    const int nr = (int)std::min<uint64_t>(32, r1 - rb);
+    __syncthreads();
+    for (int e = t; e < 32 * kSm; e += 256) {
+      int rr = e / kSm, c = e % kSm;
+      sa[rr][c] = (rr < nr && c < a) ? A[(rb + rr) * lda + c] : 0.0;
+      sb[rr][c] = (rr < nr && c < b) ? B[(rb + rr) * ldb + c] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      int e = t + 256 * q;
+      if (e < nout) {
+        int i = e / b, j = e % b;
+        double s = acc[q];
+        for (int rr = 0; rr < nr; ++rr) s += sa[rr][i] * sb[rr][j];
+        acc[q] = s;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    int e = t + 256 * q;
+    if (e < nout) part[(uint64_t)blockIdx.x * nout + e] = acc[q];
+  }
+}
+
+__global__ void k_gram_reduce(const double* __restrict__ part, int nblk, int nout, double* out) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nout) return;
+  double s = 0.0;
+  for (int g = 0; g < nblk; ++g) s += part[(uint64_t)g * nout + e];
+  out[e] = s;
+}
+
+void ts_gram(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t a, const double* B,
+             uint64_t ldb, uint64_t b, uint64_t rows, double* out) {
+  SLFMM_CHECK(a <= (uint64_t)kSm && b <= (uint64_t)kSm, SLFMM_E_ARG, "ts_gram: too many columns");
+  hipStream_t s = ctx->stream;
+  if (rows == 0) { launch_fill(s, out, 0.0, a * b); return; }
+  uint64_t nblk = std::min<uint64_t>(cdiv(rows, 128), 2048);
+  uint64_t rpb = cdiv(rows, nblk);
+  nblk = cdiv(rows, rpb);
+  double* part = ctx->buf<double>("gram_part", nblk * a * b);
+  k_gram_partial<<<(unsigned)nblk, 256, 0, s>>>(A, lda, (int)a, B, ldb, (int)b, rows, rpb, part);
+  k_gram_reduce<<<(unsigned)cdiv(a * b, 256), 256, 0, s>>>(part, (int)nblk, (int)(a * b), out);
+  SLFMM_HIP(hipGetLastError());
+}
+
+// ---- tall-skinny GEMM: out = alpha A M + beta out -------------------------
+__global__ void __launch_bounds__(256) k_ts_gemm(const double* __restrict__ A, uint64_t lda, int m,
+                                                 const double* __restrict__ M, uint64_t ldm, int q,
+                                                 uint64_t rows, double alpha, double beta, double* out,
+                                                 uint64_t ldo) {
+  __shared__ double sm[kSm * kSm];
+  for (int e = threadIdx.x; e < m * q; e += 256) sm[e] = M[(uint64_t)(e / q) * ldm + (e % q)];
+  __syncthreads();
+  const uint64_t total = rows * (uint64_t)q;
+  for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256) {
+    uint64_t r = e / q;
+    int c = (int)(e % q);
+    const double* ar = A + r * lda;
+    double s = 0.0;
+    for (int t = 0; t < m; ++t) s += ar[t] * sm[t * q + c];
+    double v = alpha * s;
+    if (beta != 0.0) v += beta * out[r * ldo + c];
+    out[r * ldo + c] = v;
+  }
+}
+
+void ts_gemm(hipStream_t s, const double* A, uint64_t lda, uint64_t m, const double* M,
+             uint64_t ldm, uint64_t q, uint64_t rows, double alpha, double beta, double* out,
+             uint64_t ldo) {
+  if (rows == 0 || q == 0) return;
+  SLFMM_CHECK(m <= (uint64_t)kSm && q <= (uint64_t)kSm, SLFMM_E_ARG, "ts_gemm: too many columns");
+  uint64_t blocks = std::min<uint64_t>(cdiv(rows * q, 256), 8192);
+  k_ts_gemm<<<(unsigned)blocks, 256, 0, s>>>(A, lda, (int)m, M, ldm, (int)q, rows, alpha, beta, out, ldo);
+  SLFMM_HIP(hipGetLastError());
+}
+
+// Column means, one thread per column, sequential (deterministic) sums.
+__global__ void k_colmeans(const double* A, uint64_t lda, uint64_t rows, int m, double* means) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  double s = 0.0;
+  for (uint64_t r = 0; r < rows; ++r) s += A[r * lda + c];
+  means[c] = rows ? s / (double)rows : 0.0;
+}
+
+void launch_colmeans(hipStream_t s, const double* A, uint64_t lda, uint64_t rows, uint64_t m, double* means) {
+  k_colmeans<<<1, 64, 0, s>>>(A, lda, rows, (int)m, means);
+  SLFMM_HIP(hipGetLastError());
+}
+
+// ---- one-workgroup Cholesky:  G + shift*I = R^T R,  outputs R and R^{-1} ---
+__global__ void __launch_bounds__(256) k_chol(const double* __restrict__ G, int m, double shift, double tol,
+                                              double* R, double* Rinv, int* info) {
+  extern __shared__ double A[];   // m*m
+  __shared__ double diag0[kSm];
+  __shared__ int fail;
+  const int t = threadIdx.x;
+  for (int e = t; e < m * m; e += 256) A[e] = G[e] + ((e / m == e % m) ? shift : 0.0);
+  if (t == 0) fail = 0;
+  __syncthreads();
+  if (t < m) diag0[t] = A[t * m + t];
+  __syncthreads();
+  for (int j = 0; j < m; ++j) {
+    if (t == 0) {
+      double dj = A[j * m + j];
+      if (!(dj > tol * diag0[j]) || !(dj < 1e308)) fail = j + 1;
+      else A[j * m + j] = sqrt(dj);
+    }
+    __syncthreads();
+    if (fail) break;
+    const double rjj = A[j * m + j];
+    for (int i = j + 1 + t; i < m; i += 256) A[j * m + i] /= rjj;
+    __syncthreads();
+    const int w = m - j - 1;
+    for (int e = t; e < w * w; e += 256) {
+      int i = j + 1 + e / w, l = j + 1 + e % w;
+      if (l >= i) A[i * m + l] -= A[j * m + i] * A[j * m + l];
+    }
+    __syncthreads();
+  }
+  if (fail) {
+    if (t == 0) *info = fail;
+    return;
+  }
+  if (t < m) {   // back substitution, column t of R^{-1} (written straight to Rinv)
+    const int c = t;
+    for (int i = m - 1; i >= 0; --i) {
+      double s = (i == c) ? 1.0 : 0.0;
+      for (int k = i + 1; k <= c; ++k) s -= A[i * m + k] * Rinv[k * m + c];
+      Rinv[i * m + c] = (i <= c) ? s / A[i * m + i] : 0.0;
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < m * m; e += 256) {
+    int i = e / m, l = e % m;
+    R[e] = (l >= i) ? A[e] : 0.0;
+  }
+  if (t == 0) *info = 0;
+}
+
+// one chol call; returns host-visible info
+static int chol_dev(slfmm_ctx* ctx, const double* G, uint64_t m, double shift, double tol, double* R, double* Rinv) {
+  int* dinfo = ctx->buf<int>("chol_info", 1);
+  k_chol<<<1, 256, m * m * sizeof(double), ctx->stream>>>(G, (int)m, shift, tol, R, Rinv, dinfo);
+  SLFMM_HIP(hipGetLastError());
+  int info = 0;
+  SLFMM_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+  return info;
+}
+
+bool cholqr2(slfmm_ctx* ctx, double* A, uint64_t lda, uint64_t m, uint64_t rows, double* d_R) {
+  SLFMM_CHECK(m <= (uint64_t)kSm, SLFMM_E_ARG, "cholqr2: too many columns");
+  hipStream_t s = ctx->stream;
+  double* G = ctx->buf<double>("cqr_G", m * m);
+  double* R1 = ctx->buf<double>("cqr_R1", m * m);
+  double* R1i = ctx->buf<double>("cqr_R1i", m * m);
+  double* R2 = ctx->buf<double>("cqr_R2", m * m);
+  double* R2i = ctx->buf<double>("cqr_R2i", m * m);
+  double* Racc = ctx->buf<double>("cqr_Racc", m * m);
+  double* Rtmp = ctx->buf<double>("cqr_Rtmp", m * m);
+  double* T = ctx->buf<double>("cqr_T", rows * m);
+  // pass 1 (shifted if plain Cholesky breaks down)
+  ts_gram(ctx, A, lda, m, A, lda, m, rows, G);
+  int info = chol_dev(ctx, G, m, 0.0, 1e-13, R1, R1i);
+  int passes = 2;
+  if (info) {
+    // shifted CholeskyQR: G + s I with s ~ u * ||A||_F^2 * (rows*m), then two clean passes
+    std::vector<double> hG(m * m);
+    SLFMM_HIP(hipMemcpyAsync(hG.data(), G, m * m * 8, hipMemcpyDeviceToHost, s));
+    SLFMM_HIP(hipStreamSynchronize(s));
+    double tr = 0.0;
+    for (uint64_t i = 0; i < m; ++i) tr += hG[i * m + i];
+    double shift = 11.0 * (double)(rows * m + m * (m + 1)) * 1.1102230246251565e-16 * tr;
+    info = chol_dev(ctx, G, m, shift, 0.0, R1, R1i);
+    if (info) return false;
+    passes = 3;
+  }
+  ts_gemm(s, A, lda, m, R1i, m, m, rows, 1.0, 0.0, T, m);
+  SLFMM_HIP(hipMemcpyAsync(Racc, R1, m * m * 8, hipMemcpyDeviceToDevice, s));
+  for (int ps = 1; ps < passes; ++ps) {
+    ts_gram(ctx, T, m, m, T, m, m, rows, G);
+    info = chol_dev(ctx, G, m, 0.0, 1e-13, R2, R2i);
+    if (info) return false;
+    double* dst = (ps == passes - 1) ? nullptr : T;
+    if (dst) {
+      // T <- T R2^{-1}  (through A as a temporary)
+      ts_gemm(s, T, m, m, R2i, m, m, rows, 1.0, 0.0, A, lda);
+      SLFMM_HIP(hipMemcpy2DAsync(T, m * 8, A, lda * 8, m * 8, rows, hipMemcpyDeviceToDevice, s));
+    } else {
+      ts_gemm(s, T, m, m, R2i, m, m, rows, 1.0, 0.0, A, lda);
+    }
+    ts_gemm(s, R2, m, m, Racc, m, m, m, 1.0, 0.0, Rtmp, m);
+    SLFMM_HIP(hipMemcpyAsync(Racc, Rtmp, m * m * 8, hipMemcpyDeviceToDevice, s));
+  }
+  if (d_R) SLFMM_HIP(hipMemcpyAsync(d_R, Racc, m * m * 8, hipMemcpyDeviceToDevice, s));
+  return true;
+}
+
+// ---- one-workgroup LU inverse with the safe_inv fallback -------------------
+// src/testing.rs:996-1024: invert; if LU meets an exactly zero pivot, add
+// eps = 1e-8 * max(max|diag|, 1e-300) to the diagonal and invert again.
+__global__ void __launch_bounds__(256) k_safe_inv(const double* __restrict__ In, int m, double* Out, int* info) {
+  extern __shared__ double A[];   // m*m
+  __shared__ int piv[kSm];
+  __shared__ int sing;
+  __shared__ double eps;
+  const int t = threadIdx.x;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    __syncthreads();
+    for (int e = t; e < m * m; e += 256) A[e] = In[e] + ((attempt == 1 && e / m == e % m) ? eps : 0.0);
+    if (t == 0) sing = 0;
+    __syncthreads();
+    for (int j = 0; j < m; ++j) {
+      if (t == 0) {
+        int p = j;
+        double best = fabs(A[j * m + j]);
+        for (int i = j + 1; i < m; ++i) {
+          double v = fabs(A[i * m + j]);
+          if (v > best) { best = v; p = i; }
+        }
+        piv[j] = p;
+        if (A[p * m + j] == 0.0) sing = j + 1;
+      }
+      __syncthreads();
+      if (sing) break;
+      const int p = piv[j];
+      if (p != j)
+        for (int c = t; c < m; c += 256) {
+          double tmp = A[j * m + c];
+          A[j * m + c] = A[p * m + c];
+          A[p * m + c] = tmp;
+        }
+      __syncthreads();
+      const double inv = 1.0 / A[j * m + j];
+      for (int i = j + 1 + t; i < m; i += 256) A[i * m + j] *= inv;
+      __syncthreads();
+      const int w = m - j - 1;
+      for (int e = t; e < w * w; e += 256) {
+        int i = j + 1 + e / w, l = j + 1 + e % w;
+        A[i * m + l] -= A[i * m + j] * A[j * m + l];
+      }
+      __syncthreads();
+    }
+    if (!sing) {
+      if (t < m) {   // column t of A^{-1}: P b, forward (unit L), backward (U)
+        double x[kSm];
+        for (int i = 0; i < m; ++i) x[i] = (i == t) ? 1.0 : 0.0;
+        for (int j = 0; j < m; ++j) { int p = piv[j]; double tmp = x[j]; x[j] = x[p]; x[p] = tmp; }
+        for (int i = 0; i < m; ++i) { double s = x[i]; for (int k = 0; k < i; ++k) s -= A[i * m + k] * x[k]; x[i] = s; }
+        for (int i = m - 1; i >= 0; --i) { double s = x[i]; for (int k = i + 1; k < m; ++k) s -= A[i * m + k] * x[k]; x[i] = s / A[i * m + i]; }
+        for (int i = 0; i < m; ++i) Out[i * m + t] = x[i];
+      }
+      if (t == 0) *info = attempt;   // 0 = plain, 1 = regularised
+      return;
+    }
+    if (t == 0) {
+      double dmax = 0.0;
+      for (int i = 0; i < m; ++i) dmax = fmax(dmax, fabs(In[i * m + i]));
+      eps = 1e-8 * fmax(dmax, 1e-300);
+    }
+  }
+  if (t == 0) *info = 2;
+}
+
+int safe_inv_dev(slfmm_ctx* ctx, const double* d_a, uint64_t m, double* d_out) {
+  SLFMM_CHECK(m >= 1 && m <= (uint64_t)kSm, SLFMM_E_ARG, "safe_inv: order must be 1..64");
+  int* dinfo = ctx->buf<int>("inv_info", 1);
+  k_safe_inv<<<1, 256, m * m * sizeof(double), ctx->stream>>>(d_a, (int)m, d_out, dinfo);
+  SLFMM_HIP(hipGetLastError());
+  int info = 0;
+  SLFMM_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+  return info;
+}
+
+// ---- one-workgroup one-sided Jacobi -----------------------------------------
+// Rotates column pairs of A (m x m) until they are mutually orthogonal;
+// V accumulates the rotations, so A_in V = U diag(sv).  Pairs of a sweep are
+// scheduled by the circle method: m/2 disjoint pairs per step, 8 lanes each.
+__global__ void __launch_bounds__(256) k_jacobi(const double* __restrict__ In, int m, double* Vout, double* svout) {
+  extern __shared__ double A[];   // m*m, then V (m*m)
+  double* V = A + m * m;
+  __shared__ double sv[kSm];
+  __shared__ int order[kSm];
+  __shared__ int rotated;
+  const int t = threadIdx.x;
+  const int mp = (m + 1) & ~1;
+  for (int e = t; e < m * m; e += 256) {
+    A[e] = In[e];
+    V[e] = (e / m == e % m) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const int grp = t >> 3, sub = t & 7;          // 32 groups of 8 lanes
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    if (t == 0) rotated = 0;
+    __syncthreads();
+    for (int r = 0; r < mp - 1; ++r) {
+      for (int pi = grp; pi < mp / 2; pi += 32) {
+        int pos_a = pi, pos_b = mp - 1 - pi;
+        int a = pos_a == 0 ? 0 : ((pos_a - 1 + r) % (mp - 1)) + 1;
+        int b = pos_b == 0 ? 0 : ((pos_b - 1 + r) % (mp - 1)) + 1;
+        bool live = a < m && b < m;
+        double al = 0.0, be = 0.0, ga = 0.0;
+        if (live)
+          for (int i = sub; i < m; i += 8) {
+            double x = A[i * m + a], y = A[i * m + b];
+            al += x * x; be += y * y; ga += x * y;
+          }
+        for (int off = 4; off > 0; off >>= 1) {
+          al += __shfl_xor(al, off, 8);
+          be += __shfl_xor(be, off, 8);
+          ga += __shfl_xor(ga, off, 8);
+        }
+        if (live && ga != 0.0 && fabs(ga) > 4e-16 * (double)m * sqrt(al * be)) {
+          double zeta = (be - al) / (2.0 * ga);
+          double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          double c = 1.0 / sqrt(1.0 + tt * tt), s = c * tt;
+          for (int i = sub; i < m; i += 8) {
+            double x = A[i * m + a], y = A[i * m + b];
+            A[i * m + a] = c * x - s * y;
+            A[i * m + b] = s * x + c * y;
+            x = V[i * m + a]; y = V[i * m + b];
+            V[i * m + a] = c * x - s * y;
+            V[i * m + b] = s * x + c * y;
+          }
+          if (sub == 0) rotated = 1;
+        }
+      }
+      __syncthreads();
+    }
+    int any = rotated;
+    __syncthreads();
+    if (!any) break;
+  }
+  if (t < m) {
+    double s = 0.0;
+    for (int i = 0; i < m; ++i) s += A[i * m + t] * A[i * m + t];
+    sv[t] = sqrt(s);
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int i = 0; i < m; ++i) order[i] = i;
+    for (int i = 0; i < m; ++i)
+      for (int j = i + 1; j < m; ++j)
+        if (sv[order[j]] > sv[order[i]]) { int tmp = order[i]; order[i] = order[j]; order[j] = tmp; }
+  }
+  __syncthreads();
+  for (int e = t; e < m * m; e += 256) {
+    int i = e / m, c = e % m;
+    Vout[e] = V[i * m + order[c]];
+  }
+  if (t < m) svout[t] = sv[order[t]];
+}
+
+void jacobi_svd_small(slfmm_ctx* ctx, const double* d_a, uint64_t m, double* d_v, double* d_sv) {
+  SLFMM_CHECK(m >= 1 && m <= (uint64_t)kSm, SLFMM_E_ARG, "jacobi: order must be 1..64");
+  k_jacobi<<<1, 256, 2 * m * m * sizeof(double), ctx->stream>>>(d_a, (int)m, d_v, d_sv);
+  SLFMM_HIP(hipGetLastError());
+}
+
+}  // namespace slfmm

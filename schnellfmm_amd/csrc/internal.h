@@ -1,0 +1,143 @@
+// Internal declarations shared by the slfmm HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/slfmm.h"
+
+namespace slfmm {
+
+// Error carried from deep inside a call back to the C ABI boundary.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define SLFMM_HIP(expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      throw ::slfmm::Error(SLFMM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define SLFMM_CHECK(cond, code, msg)                 \
+  do {                                               \
+    if (!(cond)) throw ::slfmm::Error((code), (msg)); \
+  } while (0)
+
+constexpr int kBins = 100000;        // TsqHistogram bins (src/testing.rs:125)
+constexpr double kBinWidth = 0.001;  // (src/testing.rs:126)
+constexpr int kMaxRows = 64;         // max functionals per SNP handled by one launch row-block
+constexpr int kRowBlk = 32;          // rows per FP64 functional row-block
+
+inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+inline uint64_t rup(uint64_t a, uint64_t b) { return cdiv(a, b) * b; }
+
+}  // namespace slfmm
+
+// Opaque handle definitions.
+struct slfmm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // Named, grow-only device scratch buffers reused across calls.
+  std::map<std::string, std::pair<void*, size_t>> pool;
+  void* scratch(const std::string& name, size_t bytes);
+  template <class T>
+  T* buf(const std::string& name, size_t count) {
+    return static_cast<T*>(scratch(name, count * sizeof(T)));
+  }
+  // HIP-event timing of named kernels (enabled by slfmm_profile_enable)
+  bool profiling = false;
+  struct Timer { double ms = 0.0; uint64_t launches = 0; std::vector<std::pair<hipEvent_t, hipEvent_t>> pending; };
+  std::map<std::string, Timer> timers;
+  std::pair<hipEvent_t, hipEvent_t> tic(const std::string& name);
+  void toc(const std::string& name, std::pair<hipEvent_t, hipEvent_t> ev);
+  void collect();
+  ~slfmm_ctx();
+};
+
+// RAII timing scope for one kernel launch on ctx->stream.
+struct slfmm_timed {
+  slfmm_ctx* ctx; std::string name; std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+  slfmm_timed(slfmm_ctx* c, const char* n) : ctx(c), name(n) { if (ctx->profiling) ev = ctx->tic(name); }
+  ~slfmm_timed() { if (ctx->profiling && ev.first) ctx->toc(name, ev); }
+};
+
+struct slfmm_geno {
+  slfmm_ctx* ctx = nullptr;
+  uint8_t* d = nullptr;   // packed codes, SNP-major, row stride ld bytes
+  uint64_t n = 0;         // samples represented (after sample_keep)
+  uint64_t p = 0;         // SNPs
+  uint64_t ld = 0;        // row stride in bytes (multiple of 16)
+  bool owned = false;
+};
+
+namespace slfmm {
+
+// ---- geno.hip -------------------------------------------------------------
+uint64_t geno_ld(uint64_t n);
+void geno_alloc(slfmm_ctx* ctx, slfmm_geno* g);
+void geno_upload(slfmm_ctx* ctx, const uint8_t* packed, uint64_t n_phys, uint64_t n_snps,
+                 const int64_t* keep, uint64_t n_keep, slfmm_geno* g);
+// Gather rows idx[0..m) of g into dst (m x g->ld), contiguous.
+void geno_gather(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* d_idx, uint64_t m, uint8_t* dst);
+// Integer stats per SNP (rows of a packed matrix).
+void launch_stats(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
+                  int32_t* n_miss, int32_t* sum_g, int32_t* n_hom);
+// mean and scale per SNP from the integer stats (compute_scale semantics).
+void launch_mean_scale(hipStream_t s, const int32_t* n_miss, const int32_t* sum_g, uint64_t n,
+                       uint64_t p, int norm, double* mu, double* sc);
+void launch_decode_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t n,
+                       const int64_t* idx, uint64_t c, int norm, const double* fill, double* out,
+                       int raw);
+void launch_synth_bn(hipStream_t s, uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
+                     uint32_t n_pops, double fst_lo, double fst_hi, double maf_lo, double maf_hi,
+                     double miss_rate, uint64_t seed);
+// Transposed copy (sample-major packed: row = sample, 4 SNPs per byte).
+void launch_transpose(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
+                      uint8_t* out, uint64_t ldt);
+
+// ---- dense.hip: tall-skinny FP64 helpers and one-workgroup small solvers ---
+// out (a x b) = A^T B, A rows x a (lda), B rows x b (ldb); deterministic.
+void ts_gram(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t a, const double* B,
+             uint64_t ldb, uint64_t b, uint64_t rows, double* out);
+// out (rows x q, ldo) = alpha * A (rows x m, lda) * M (m x q, ldm) + beta * out
+void ts_gemm(hipStream_t s, const double* A, uint64_t lda, uint64_t m, const double* M,
+             uint64_t ldm, uint64_t q, uint64_t rows, double alpha, double beta, double* out,
+             uint64_t ldo);
+void launch_fill(hipStream_t s, double* p, double v, uint64_t count);
+void launch_colmeans(hipStream_t s, const double* A, uint64_t lda, uint64_t rows, uint64_t m,
+                     double* means);
+// In-place thin QR of A (rows x m, lda) by CholeskyQR2; R (m x m upper) to
+// d_R if non-null.  Returns false (and leaves A unchanged) if not full rank.
+bool cholqr2(slfmm_ctx* ctx, double* A, uint64_t lda, uint64_t m, uint64_t rows, double* d_R);
+// safe_inv on device (m <= 64): returns 0 ok, 1 regularised, 2 failed.
+int safe_inv_dev(slfmm_ctx* ctx, const double* d_a, uint64_t m, double* d_out);
+// Symmetric/one-sided Jacobi on an m x m matrix (m <= 64): columns of
+// A V orthogonal; outputs V (m x m) and column norms, sorted descending.
+void jacobi_svd_small(slfmm_ctx* ctx, const double* d_a, uint64_t m, double* d_v, double* d_sv);
+
+// ---- assoc.hip --------------------------------------------------------------
+struct AssocPlan;   // defined in assoc.hip
+void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t d,
+               const double* d_u, uint64_t k, double lambda, const slfmm_config& cfg,
+               double* d_beta, double* d_t, double* d_r2, double* d_p, uint64_t* d_hist,
+               double* d_gif_pt, double* d_gif);
+
+// ---- rsvd.hip ---------------------------------------------------------------
+void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_t n_idx,
+              const double* d_xs, uint64_t d, const slfmm_config& cfg, const double* d_b0,
+              double* d_uhat);
+// Thin precompute on the device: sigma(d), dlam(d), Qd (n x d), ridge_inv (d x d).
+void precompute_dev(slfmm_ctx* ctx, const double* d_xs, uint64_t n, uint64_t d, double lambda,
+                    double* d_sigma, double* d_dlam, double* d_qd, double* d_ridge);
+
+}  // namespace slfmm

@@ -1,0 +1,294 @@
+// Steps 0-2: thin precompute and the streaming randomized SVD on the GPU.
+//
+// The reference forms M = D_lambda Q^T (n x n, src/precompute.rs:66-69) and
+// multiplies it into every forward-pass chunk (src/rsvd.rs:93-94).  With
+// N = Q M = I - Q_d diag(1 - d_lambda) Q_d^T (identity plus rank d, Q_d the
+// thin left singular vectors of X) every step has an exact low-rank form:
+//   sketch     Z = Y^T B0                 (B0 = M^T Omega, src/rsvd.rs:51)
+//   forward    W = Y Q_z;  Q_W = qr(N W)  (reference: qr(M Y Q_z) = Q^T qr(N W))
+//   backward   Z = Y^T (N Q_W)            (reference: Y^T M^T Q_a)
+//   recovery   U_hat = N^{-1} U_{NW}[:, :K] (reference: Q D^{-1} U_small[:, :K])
+// so U_hat equals the reference's up to column signs, with no n x n array.
+#include "internal.h"
+
+namespace slfmm {
+
+void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t p, uint64_t npad,
+                     const double* A, uint64_t lda, int nrb, const double* mu, double* F, uint64_t ldF,
+                     const double* outscale);
+void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
+                     const double* d_B, uint64_t l, const int32_t* nmiss, const int32_t* sumg,
+                     const double* mu, const double* sc, double* Z, uint64_t ldZ);
+void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* packedT, uint64_t ldt, uint64_t n, uint64_t p,
+                        const double* Qz, uint64_t ldq, uint64_t l, const double* mu, const double* sc,
+                        double* W, uint64_t ldw);
+
+// ---- forward pass: W[k][c] = sum_j v_jk * sc_j * Qz[j][c] --------------------
+// One workgroup per (256-sample tile, SNP chunk, 32-column block); partial
+// sums per chunk are reduced in a fixed order afterwards.
+__global__ void __launch_bounds__(256) k_acc_f64(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t n,
+                                                 uint64_t p, uint64_t snps_per_chunk, const double* __restrict__ Qz,
+                                                 uint64_t ldq, int l, const double* __restrict__ mu,
+                                                 const double* __restrict__ sc, double* __restrict__ part,
+                                                 uint64_t ldp) {
+  __shared__ double sq[32][33];
+  __shared__ double smu[32];
+  __shared__ uint32_t sw[32][16];
+  const int t = threadIdx.x;
+  const uint64_t k = (uint64_t)blockIdx.x * 256 + t;
+  const uint64_t j0 = (uint64_t)blockIdx.y * snps_per_chunk;
+  const uint64_t j1 = std::min(p, j0 + snps_per_chunk);
+  const int cb = blockIdx.z;
+  const int ncol = min(32, l - cb * 32);
+  double acc[32];
+#pragma unroll
+  for (int c = 0; c < 32; ++c) acc[c] = 0.0;
+  for (uint64_t jb = j0; jb < j1; jb += 32) {
+    const int nj = (int)std::min<uint64_t>(32, j1 - jb);
+    __syncthreads();
+    for (int e = t; e < 32 * 32; e += 256) {
+      int jj = e >> 5, c = e & 31;
+      sq[jj][c] = (jj < nj && c < ncol) ? Qz[(jb + jj) * ldq + cb * 32 + c] * sc[jb + jj] : 0.0;
+    }
+    if (t < 32) smu[t] = t < nj ? mu[jb + t] : 0.0;
+    for (int e = t; e < 32 * 16; e += 256) {
+      int jj = e >> 4, wd = e & 15;
+      sw[jj][wd] = jj < nj ? reinterpret_cast<const uint32_t*>(packed + (jb + jj) * ld + (uint64_t)blockIdx.x * 64)[wd] : 0u;
+    }
+    __syncthreads();
+    for (int jj = 0; jj < nj; ++jj) {
+      uint32_t code = (sw[jj][t >> 4] >> (2 * (t & 15))) & 3u;
+      double m = smu[jj];
+      double v = (code & 2u) ? ((code & 1u) ? 2.0 - m : 1.0 - m) : ((code & 1u) ? 0.0 : -m);
+#pragma unroll
+      for (int c = 0; c < 32; ++c) acc[c] += v * sq[jj][c];
+    }
+  }
+  if (k < n)
+    for (int c = 0; c < ncol; ++c) part[((uint64_t)blockIdx.y * n + k) * ldp + cb * 32 + c] = acc[c];
+}
+
+__global__ void k_acc_reduce(const double* __restrict__ part, uint64_t n, int l, uint64_t ldp, int nchunk,
+                             double* W, uint64_t ldw) {
+  uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * (uint64_t)l) return;
+  uint64_t k = e / l;
+  int c = (int)(e % l);
+  double s = 0.0;
+  for (int g = 0; g < nchunk; ++g) s += part[((uint64_t)g * n + k) * ldp + c];
+  W[k * ldw + c] = s;
+}
+
+static void accumulate_f64(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
+                           const double* Qz, uint64_t ldq, uint64_t l, const double* mu, const double* sc,
+                           double* W, uint64_t ldw) {
+  hipStream_t s = ctx->stream;
+  const uint64_t ntile = cdiv(n, 256), ncb = cdiv(l, 32);
+  uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(cdiv(p, 256), cdiv(2048, ntile * ncb)));
+  const uint64_t spc = cdiv(p, nchunk);
+  nchunk = cdiv(p, spc);
+  const uint64_t ldp = ncb * 32;
+  double* part = ctx->buf<double>("rs_accpart", nchunk * n * ldp);
+  k_acc_f64<<<dim3((unsigned)ntile, (unsigned)nchunk, (unsigned)ncb), 256, 0, s>>>(packed, ld, n, p, spc, Qz, ldq,
+                                                                                  (int)l, mu, sc, part, ldp);
+  k_acc_reduce<<<(unsigned)cdiv(n * l, 256), 256, 0, s>>>(part, n, (int)l, ldp, (int)nchunk, W, ldw);
+  SLFMM_HIP(hipGetLastError());
+}
+
+// ---- low-rank operator N (and N^{-1}) ----------------------------------------
+__global__ void k_scale_rows(double* C, int d, int c, const double* dlam, int inverse) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d * c) return;
+  double dl = dlam[e / c];
+  C[e] *= inverse ? (1.0 / dl - 1.0) : -(1.0 - dl);
+}
+
+// V (rows x c, ldv) <- N V  or  N^{-1} V
+static void apply_N(slfmm_ctx* ctx, const double* Qd, uint64_t d, const double* dlam, double* V, uint64_t ldv,
+                    uint64_t c, uint64_t rows, bool inverse) {
+  double* C = ctx->buf<double>("rs_NC", d * c);
+  ts_gram(ctx, Qd, d, d, V, ldv, c, rows, C);
+  k_scale_rows<<<(unsigned)cdiv(d * c, 256), 256, 0, ctx->stream>>>(C, (int)d, (int)c, dlam, inverse ? 1 : 0);
+  ts_gemm(ctx->stream, Qd, d, d, C, c, c, rows, 1.0, 1.0, V, ldv);
+}
+
+__global__ void k_normal(double* out, uint64_t n, int l, uint64_t ld, uint64_t seed) {
+  uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * (uint64_t)l) return;
+  uint64_t z0, z1;
+  {
+    auto mix = [](uint64_t sd, uint64_t i) {
+      uint64_t z = sd * 0xD1B54A32D192ED03ull + (i + 1) * 0x9E3779B97F4A7C15ull;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      return z ^ (z >> 31);
+    };
+    z0 = mix(seed, 2 * e);
+    z1 = mix(seed, 2 * e + 1);
+  }
+  double u1 = 1.0 - (double)(z0 >> 11) * (1.0 / 9007199254740992.0);
+  double u2 = (double)(z1 >> 11) * (1.0 / 9007199254740992.0);
+  out[(e / l) * ld + (e % l)] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+__global__ void k_pad_rows(const double* __restrict__ src, uint64_t lds, uint64_t rows, int cols,
+                           double* dst, uint64_t ldd, uint64_t rows_pad) {
+  uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= rows_pad * ldd) return;
+  uint64_t r = e / ldd;
+  int c = (int)(e % ldd);
+  dst[e] = (r < rows && c < cols) ? src[r * lds + c] : 0.0;
+}
+
+__global__ void k_transpose_small(const double* in, int m, double* out) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < m * m) out[(e % m) * m + e / m] = in[e];
+}
+
+// ---- thin precompute -----------------------------------------------------------
+// X = Q_d diag(sigma) R^T from the eigendecomposition of X^T X (one-sided
+// Jacobi on the d x d Gram), d_lambda = sqrt(lambda / (lambda + sigma^2))
+// (src/precompute.rs:52-59), ridge_inv = safe_inv(X^T X + lambda I).
+__global__ void k_prec_finish(const double* ev, const double* V, int d, double lambda, double* sigma,
+                              double* dlam, double* M, double* Gr) {
+  if (threadIdx.x != 0) return;
+  double smax = ev[0] > 0.0 ? sqrt(ev[0]) : 0.0;
+  for (int j = 0; j < d; ++j) {
+    double sg = ev[j] > 0.0 ? sqrt(ev[j]) : 0.0;
+    sigma[j] = sg;
+    double s2 = sg * sg;
+    dlam[j] = sqrt(lambda / (lambda + s2));
+    double inv = (sg > 1e-12 * smax && sg > 0.0) ? 1.0 / sg : 0.0;
+    for (int i = 0; i < d; ++i) M[i * d + j] = V[i * d + j] * inv;
+  }
+  for (int i = 0; i < d; ++i) Gr[i * d + i] += lambda;
+}
+
+void precompute_dev(slfmm_ctx* ctx, const double* d_xs, uint64_t n, uint64_t d, double lambda,
+                    double* d_sigma, double* d_dlam, double* d_qd, double* d_ridge) {
+  SLFMM_CHECK(lambda > 0.0, SLFMM_E_LAMBDA,
+              "Ridge penalty λ must be positive, got " + std::to_string(lambda) +
+                  ". λ=0 makes the D_λ diagonal singular (d_λ[j] = 0/(0+σ²) = 0, d_λ_inv = ∞). "
+                  "Use a small positive value like 1e-10.");
+  SLFMM_CHECK(d >= 1 && d <= 64, SLFMM_E_ARG, "need 1 <= d <= 64 covariates");
+  hipStream_t s = ctx->stream;
+  double* G = ctx->buf<double>("pc_G", d * d);
+  ts_gram(ctx, d_xs, d, d, d_xs, d, d, n, G);
+  double* V = ctx->buf<double>("pc_V", d * d);
+  double* ev = ctx->buf<double>("pc_ev", d);
+  jacobi_svd_small(ctx, G, d, V, ev);
+  double* M = ctx->buf<double>("pc_M", d * d);
+  k_prec_finish<<<1, 64, 0, s>>>(ev, V, (int)d, lambda, d_sigma, d_dlam, M, G);
+  ts_gemm(s, d_xs, d, d, M, d, d, n, 1.0, 0.0, d_qd, d);
+  int inf = safe_inv_dev(ctx, G, d, d_ridge);
+  if (inf == 1) fprintf(stderr, "Warning: X^T X + λI (ridge) is singular, adding epsilon diagonal regularization\n");
+  SLFMM_CHECK(inf != 2, SLFMM_E_SINGULAR, "X^T X + λI (ridge) inversion failed even with epsilon regularization");
+}
+
+// ---- randomized SVD ---------------------------------------------------------------
+void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_t n_idx, const double* d_xs,
+              uint64_t d, const slfmm_config& cfg, const double* d_b0, double* d_uhat) {
+  hipStream_t s = ctx->stream;
+  const uint64_t n = g->n, K = cfg.k, l = cfg.k + cfg.oversampling;
+  SLFMM_CHECK(K >= 1 && l <= 64, SLFMM_E_ARG, "need 1 <= K and K + oversampling <= 64");
+  SLFMM_CHECK(l <= n, SLFMM_E_ARG, "K + oversampling must not exceed the number of samples");
+  double* sigma = ctx->buf<double>("rs_sigma", d);
+  double* dlam = ctx->buf<double>("rs_dlam", d);
+  double* Qd = ctx->buf<double>("rs_Qd", n * d);
+  double* ridge = ctx->buf<double>("rs_ridge", d * d);
+  precompute_dev(ctx, d_xs, n, d, cfg.lambda, sigma, dlam, Qd, ridge);
+
+  // estimation SNPs as a contiguous packed block
+  const uint8_t* Y = g->d;
+  uint64_t pe = g->p;
+  bool identity = (h_idx == nullptr);
+  if (h_idx) {
+    identity = (n_idx == g->p);
+    for (uint64_t i = 0; identity && i < n_idx; ++i) identity = (h_idx[i] == (int64_t)i);
+  }
+  if (!identity) {
+    for (uint64_t i = 0; i < n_idx; ++i)
+      SLFMM_CHECK(h_idx[i] >= 0 && (uint64_t)h_idx[i] < g->p, SLFMM_E_ARG, "estimation SNP index out of range");
+    int64_t* didx = ctx->buf<int64_t>("rs_idx", n_idx);
+    SLFMM_HIP(hipMemcpyAsync(didx, h_idx, n_idx * 8, hipMemcpyHostToDevice, s));
+    uint8_t* Yg = ctx->buf<uint8_t>("rs_Y", n_idx * g->ld + 64);
+    SLFMM_HIP(hipMemsetAsync(Yg + n_idx * g->ld, 0, 64, s));
+    geno_gather(ctx, g, didx, n_idx, Yg);
+    Y = Yg;
+    pe = n_idx;
+  }
+  SLFMM_CHECK(pe >= l, SLFMM_E_ARG, "need at least K + oversampling estimation SNPs");
+  int32_t* nmiss = ctx->buf<int32_t>("rs_nmiss", pe);
+  int32_t* sumg = ctx->buf<int32_t>("rs_sumg", pe);
+  int32_t* nhom = ctx->buf<int32_t>("rs_nhom", pe);
+  double* mu = ctx->buf<double>("rs_mu", pe);
+  double* sc = ctx->buf<double>("rs_sc", pe);
+  launch_stats(s, Y, g->ld, n, pe, nmiss, sumg, nhom);
+  launch_mean_scale(s, nmiss, sumg, n, pe, cfg.norm, mu, sc);
+
+  const bool mixed = cfg.precision == SLFMM_PREC_I8;
+  uint8_t* YT = nullptr;
+  uint64_t ldt = 0;
+  if (mixed) {   // sample-major copy for the forward passes
+    ldt = rup(cdiv(pe, 4), 16);
+    YT = ctx->buf<uint8_t>("rs_YT", n * ldt + 64);
+    SLFMM_HIP(hipMemsetAsync(YT, 0, n * ldt + 64, s));
+    launch_transpose(s, Y, g->ld, n, pe, YT, ldt);
+  }
+
+  const uint64_t npad = rup(n, 64);
+  const int nrb = (int)cdiv(l, kRowBlk);
+  const uint64_t lda = (uint64_t)nrb * kRowBlk;
+  double* B = ctx->buf<double>("rs_B", n * l);
+  double* A = ctx->buf<double>("rs_A", npad * lda);
+  double* Z = ctx->buf<double>("rs_Z", pe * lda);
+  double* W = ctx->buf<double>("rs_W", n * l);
+  if (d_b0) {
+    SLFMM_HIP(hipMemcpyAsync(B, d_b0, n * l * 8, hipMemcpyDeviceToDevice, s));
+  } else {
+    k_normal<<<(unsigned)cdiv(n * l, 256), 256, 0, s>>>(B, n, (int)l, l, cfg.seed);
+    apply_N(ctx, Qd, d, dlam, B, l, l, n, false);
+  }
+  auto project = [&]() {   // Z = Y^T B, scaled per SNP
+    slfmm_timed tm(ctx, "rsvd_project");
+    if (mixed) {
+      rsvd_project_i8(ctx, Y, g->ld, n, pe, B, l, nmiss, sumg, mu, sc, Z, lda);
+    } else {
+      k_pad_rows<<<(unsigned)cdiv(npad * lda, 256), 256, 0, s>>>(B, l, n, (int)l, A, lda, npad);
+      launch_func_f64(s, Y, g->ld, pe, npad, A, lda, nrb, mu, Z, lda, sc);
+    }
+  };
+  auto orth_z = [&]() {
+    SLFMM_CHECK(cholqr2(ctx, Z, lda, l, pe, nullptr), SLFMM_E_NUMERIC, "QR of the RSVD sketch failed (rank deficient)");
+  };
+  auto forward = [&]() {   // W = N Y Q_z
+    {
+      slfmm_timed tm(ctx, "rsvd_forward");
+      if (mixed) rsvd_accumulate_i8(ctx, YT, ldt, n, pe, Z, lda, l, mu, sc, W, l);
+      else accumulate_f64(ctx, Y, g->ld, n, pe, Z, lda, l, mu, sc, W, l);
+    }
+    apply_N(ctx, Qd, d, dlam, W, l, l, n, false);
+  };
+  project();                                           // src/rsvd.rs:56-75
+  orth_z();
+  for (uint64_t it = 0; it < cfg.n_power_iter; ++it) {  // src/rsvd.rs:79-133
+    forward();
+    SLFMM_CHECK(cholqr2(ctx, W, l, l, n, nullptr), SLFMM_E_NUMERIC, "QR of the power-iteration block failed");
+    SLFMM_HIP(hipMemcpyAsync(B, W, n * l * 8, hipMemcpyDeviceToDevice, s));
+    apply_N(ctx, Qd, d, dlam, B, l, l, n, false);
+    project();
+    orth_z();
+  }
+  forward();                                           // src/rsvd.rs:138-157
+  double* R = ctx->buf<double>("rs_R", l * l);
+  double* Rt = ctx->buf<double>("rs_Rt", l * l);
+  double* Vs = ctx->buf<double>("rs_V", l * l);
+  double* sv = ctx->buf<double>("rs_sv", l);
+  SLFMM_CHECK(cholqr2(ctx, W, l, l, n, R), SLFMM_E_NUMERIC, "QR of the final projection failed");
+  k_transpose_small<<<(unsigned)cdiv(l * l, 256), 256, 0, s>>>(R, (int)l, Rt);
+  jacobi_svd_small(ctx, Rt, l, Vs, sv);                // left singular vectors of R
+  ts_gemm(s, W, l, l, Vs, l, K, n, 1.0, 0.0, d_uhat, K);
+  apply_N(ctx, Qd, d, dlam, d_uhat, K, K, n, true);    // src/rsvd.rs:168-172
+}
+
+}  // namespace slfmm
