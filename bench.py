@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--precision", choices=["auto", "i8", "fp64"], default="auto")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--cpu-snps", type=int, default=1500)
+    ap.add_argument("--cpu-snps", type=int, default=5000)
     args = ap.parse_args()
 
     if args.gpus != 1:

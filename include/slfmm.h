@@ -116,7 +116,7 @@ int slfmm_geno_upload(slfmm_ctx* ctx, const uint8_t* packed, uint64_t n_phys, ui
 
 /* Wrap packed bytes already resident on the device (row stride `ld` bytes,
  * ld % 16 == 0, >= ceil(n/4); bytes past ceil(n/4) and padding codes must be
- * zero, 64 readable bytes must follow the last row).  Not owned. */
+ * zero, 128 readable bytes must follow the last row).  Not owned. */
 int slfmm_geno_wrap_device(slfmm_ctx* ctx, void* d_packed, uint64_t n, uint64_t n_snps,
                            uint64_t ld, slfmm_geno** out);
 
@@ -181,6 +181,11 @@ int slfmm_profile_enable(slfmm_ctx* ctx, int32_t on);
  * clears it after reading. */
 int slfmm_profile_read(slfmm_ctx* ctx, const char* name, double* total_ms, uint64_t* launches,
                        int32_t reset);
+
+/* Diagnostic (no reference counterpart): one int8 MFMA 16x16x64 with the
+ * fragment mapping of the mixed-precision kernels; D = A (16x64) B (64x16),
+ * all row-major.  A GPU test pins the lane layout with it. */
+int slfmm_debug_mfma_i8(slfmm_ctx* ctx, const int8_t* a, const int8_t* b, int32_t* d);
 
 /* ---- Synthetic data (test / benchmark inputs, no reference counterpart) --- */
 

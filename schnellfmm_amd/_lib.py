@@ -55,6 +55,7 @@ SIGNATURES = {
     "slfmm_estimate_factors": (C.c_int, [vp, vp, vp, u64, vp, u64, vp, vp, vp, u32]),
     "slfmm_test_associations": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, u32]),
     "slfmm_fit_lfmm2": (C.c_int, [vp, vp, vp, u64, vp, vp, u64, vp, vp, vp, vp, u32]),
+    "slfmm_debug_mfma_i8": (C.c_int, [vp, vp, vp, vp]),
     "slfmm_profile_enable": (C.c_int, [vp, i32]),
     "slfmm_profile_read": (C.c_int, [vp, C.c_char_p, vp, vp, i32]),
     "slfmm_synth_bn": (C.c_int, [vp, u64, u64, u32, dbl, dbl, dbl, dbl, dbl, u64, vp]),

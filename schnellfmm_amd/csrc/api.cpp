@@ -323,8 +323,8 @@ int slfmm_decode_stats(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* snp_i
       SLFMM_CHECK(snp_idx[i] >= 0 && (uint64_t)snp_idx[i] < g->p, SLFMM_E_ARG, "SNP index out of range");
     hipStream_t s = ctx->stream;
     int64_t* didx = to_dev(ctx, "api_idx", snp_idx, n_idx);
-    uint8_t* Y = ctx->buf<uint8_t>("api_Y", n_idx * g->ld + 64);
-    SLFMM_HIP(hipMemsetAsync(Y + n_idx * g->ld, 0, 64, s));
+    uint8_t* Y = ctx->buf<uint8_t>("api_Y", n_idx * g->ld + kSlack);
+    SLFMM_HIP(hipMemsetAsync(Y + n_idx * g->ld, 0, kSlack, s));
     geno_gather(ctx, g, didx, n_idx, Y);
     int32_t* a = ctx->buf<int32_t>("api_s0", n_idx);
     int32_t* b = ctx->buf<int32_t>("api_s1", n_idx);
@@ -470,6 +470,13 @@ int slfmm_fit_lfmm2(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset
     assoc_host(ctx, g, dx, d, du, cfg, out, flags);                    // src/lib.rs:254
     if (!dev && u_hat) to_host(ctx, u_hat, du, n * K);
     SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int slfmm_debug_mfma_i8(slfmm_ctx* ctx, const int8_t* a, const int8_t* b, int32_t* d) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && a && b && d, SLFMM_E_ARG, "NULL argument");
+    probe_mfma_i8(ctx, a, b, d);
   });
 }
 

@@ -16,11 +16,10 @@
 // uses the quadratic form of the (safe_inv-regularised) Ginv, as the
 // reference does.
 #include "internal.h"
+#include "epilogue.h"
 
 namespace slfmm {
 
-constexpr int kMaxM = 64;   // K + d
-constexpr int kMaxD = 16;   // traits
 
 // ---------------------------------------------------------------------------
 // FP64 functional kernel:  F[j][32*rb + r] = sum_k A[k][32*rb + r] * v_jk
@@ -199,94 +198,6 @@ __global__ void k_plan(const double* __restrict__ Gfull, const double* __restric
 // ---------------------------------------------------------------------------
 // Per-SNP epilogue (shared by the FP64 path and the fused mixed path).
 // ---------------------------------------------------------------------------
-struct EpiArgs {
-  uint64_t n, p;
-  int d, K, m, fallback, norm;
-  double df;
-  const double* plan;
-  const int32_t* n_miss;
-  const int32_t* sum_g;
-  const int32_t* n_hom;
-  double* beta;   // p x d
-  double* tstat;  // p x d
-  double* r2;     // p x 3
-};
-
-// compute_scale (src/bed.rs:382-395); contraction-free in this file too.
-__device__ __forceinline__ double epi_scale(double mean, int norm) {
-  double pp = __dmul_rn(mean, 0.5);
-  double twopq = __dmul_rn(__dmul_rn(2.0, pp), __dsub_rn(1.0, pp));
-  if (norm == SLFMM_NORM_CENTER_ONLY) return twopq < 1e-20 ? 0.0 : 1.0;
-  double den = sqrt(twopq);
-  return den > 1e-10 ? 1.0 / den : 0.0;
-}
-
-template <int MAXM>
-__device__ void snp_epilogue(const EpiArgs& a, const double* sPlan, uint64_t j, const double* f_unscaled) {
-  const int d = a.d, m = a.m;
-  const int64_t nmiss = a.n_miss[j], sg = a.sum_g[j], nh = a.n_hom[j];
-  const int64_t nobs = (int64_t)a.n - nmiss;
-  const double mean = nobs > 0 ? (double)sg / (double)nobs : 0.0;
-  const double sc = epi_scale(mean, a.norm);
-  const int64_t sg2 = sg + 2 * nh;
-  const double tss = nobs > 0 ? sc * sc * ((double)(nobs * sg2 - sg * sg) / (double)nobs) : 0.0;
-  double fy[MAXM];
-#pragma unroll
-  for (int r = 0; r < MAXM; ++r) fy[r] = r < m ? sc * f_unscaled[r] : 0.0;
-  const double* P = sPlan;
-  const double* Vd = P + d * m;
-  const double* Rm = Vd + d;
-  const double* Fm = Rm + d * m;
-  const double* L = Fm + d * m;
-  double proj = 0.0;
-  if (!a.fallback) {
-#pragma unroll
-    for (int r = 0; r < MAXM; ++r) proj += fy[r] * fy[r];
-  } else {
-    for (int c = 0; c < m; ++c) {
-      double s = 0.0;
-#pragma unroll
-      for (int r = 0; r < MAXM; ++r) if (r < m) s += L[r * m + c] * fy[r];
-      proj += s * s;
-    }
-  }
-  double rss = tss - proj;
-  if (rss < 0.0) rss = 0.0;
-  const double sigma2 = rss / a.df;
-  double sscov = 0.0;
-  for (int jj = 0; jj < d; ++jj) {
-    double coef = 0.0, b = 0.0, fc = 0.0;
-#pragma unroll
-    for (int r = 0; r < MAXM; ++r)
-      if (r < m) {
-        coef += P[jj * m + r] * fy[r];
-        b += Rm[jj * m + r] * fy[r];
-        fc += Fm[jj * m + r] * fy[r];
-      }
-    sscov += fc * fc;
-    // t_test (src/testing.rs:1036-1048)
-    double se = sqrt(sigma2 * Vd[jj]);
-    double tv = 0.0;
-    if (se >= 1e-300 && isfinite(se)) {
-      tv = coef / se;
-      if (!isfinite(tv)) tv = 0.0;
-    }
-    a.tstat[j * d + jj] = tv;
-    a.beta[j * d + jj] = b;
-  }
-  double r0 = 0.0, r1 = 0.0, r2v = 0.0;
-  if (tss > 1e-300) {                    // src/testing.rs:394-404
-    double sslat = tss - rss - sscov;
-    if (sslat < 0.0) sslat = 0.0;
-    r0 = sscov / tss;
-    r1 = sslat / tss;
-    r2v = rss / tss;
-  }
-  a.r2[j * 3 + 0] = r0;
-  a.r2[j * 3 + 1] = r1;
-  a.r2[j * 3 + 2] = r2v;
-}
-
 template <int MAXM>
 __global__ void __launch_bounds__(128) k_epilogue(EpiArgs a, const double* __restrict__ F, uint64_t ldF) {
   extern __shared__ double sPlan[];
@@ -295,7 +206,10 @@ __global__ void __launch_bounds__(128) k_epilogue(EpiArgs a, const double* __res
   __syncthreads();
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.p) return;
-  snp_epilogue<MAXM>(a, sPlan, j, F + j * ldF);
+  double fu[MAXM];
+#pragma unroll
+  for (int r = 0; r < MAXM; ++r) fu[r] = r < a.m ? F[j * ldF + r] : 0.0;
+  snp_epilogue<MAXM>(a, sPlan, j, fu, a.n_miss[j], a.sum_g[j], a.n_hom[j]);
 }
 
 // ---------------------------------------------------------------------------

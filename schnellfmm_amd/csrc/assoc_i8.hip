@@ -1,18 +1,952 @@
-// Mixed-precision (int8 MFMA) genotype-stream kernels -- placeholder until
-// the kernels land; the FP64 path is complete without them.
+// Mixed-precision genotype-stream kernels on gfx950 int8 MFMA.
+//
+// Genotype codes are exact small integers, so they enter the matrix cores
+// as int8 with no rounding: per 16 samples of one SNP the lane decodes one
+// 32-bit word of packed codes into two int8 operand fragments,
+//   g0   = dosage with missing -> 0   (00->0, 01->0, 10->1, 11->2)
+//   miss = missing indicator          (01->1, else 0)
+// and v_mfma_i32_16x16x64_i8 accumulates exactly in int32.  The dense
+// operand (an orthonormal basis Q of [U_c|X] for the association, the
+// sketch B for the RSVD) is carried as 2 or 3 int8 terms per entry with a
+// per-column scale, Q ~ sc (t0 + t1/254 + t2/254^2) (about 15 / 22
+// significant bits).  Per SNP and functional i, with G_t, M_t, O_t the int32
+// sums of term t against g0, miss and 1:
+//   f_i = sum_t w_t (G_t - mu O_t) + mu sum_t w_t M_t,   w_t = sc_i 254^-t
+// is the functional of the centred column (g - mu on observed calls, 0 on
+// missing calls; src/bed.rs:339-343), and the FP64 per-SNP epilogue of the
+// association pass runs on it unchanged.
+//
+// Operand fragments: for a block of 64*KS samples, lane group g = lane/16
+// of a wave loads KS consecutive words of its SNP row, word s serving k-step
+// s; byte c of decoded register v holds sample
+//   64*KS*blk + 16*KS*g + 16*s + 4*c + v,
+// and the dense operand is pre-permuted in memory into exactly this order
+// (one 16-byte fragment per lane), so the same K index meets on both sides.
 #include "internal.h"
+#include "epilogue.h"
 
 namespace slfmm {
-struct EpiArgs;
-void launch_func_i8(slfmm_ctx*, const slfmm_geno*, const double*, uint64_t, const EpiArgs&, bool) {
-  throw Error(SLFMM_E_ARG, "SLFMM_PREC_I8 association kernel is not available in this build");
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Quantisation of the dense operand
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int clamp127(double v) {
+  int x = (int)v;
+  return x > 127 ? 127 : (x < -127 ? -127 : x);
 }
-void rsvd_project_i8(slfmm_ctx*, const uint8_t*, uint64_t, uint64_t, uint64_t, const double*, uint64_t,
-                     const int32_t*, const int32_t*, const double*, const double*, double*, uint64_t) {
-  throw Error(SLFMM_E_ARG, "SLFMM_PREC_I8 RSVD kernel is not available in this build");
+
+__device__ __forceinline__ int quant_term(double q, int t) {
+  double h = rint(q);
+  if (t == 0) return clamp127(h);
+  double r1 = (q - h) * 254.0;
+  double md = rint(r1);
+  if (t == 1) return clamp127(md);
+  return clamp127(rint((r1 - md) * 254.0));
 }
-void rsvd_accumulate_i8(slfmm_ctx*, const uint8_t*, uint64_t, uint64_t, uint64_t, const double*, uint64_t,
-                        uint64_t, const double*, const double*, double*, uint64_t) {
-  throw Error(SLFMM_E_ARG, "SLFMM_PREC_I8 RSVD kernel is not available in this build");
+
+// Column max |A[k][c]| over rows, one workgroup per column.
+__global__ void __launch_bounds__(256) k_colabsmax(const double* __restrict__ A, uint64_t lda, uint64_t rows,
+                                                   const double* __restrict__ rowscale, const double* __restrict__ rowmul,
+                                                   double* out) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  double m = 0.0;
+  for (uint64_t r = threadIdx.x; r < rows; r += 256) {
+    double v = A[r * lda + c];
+    if (rowscale) v *= rowscale[r];
+    if (rowmul) v *= rowmul[r];
+    m = fmax(m, fabs(v));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0] > 0.0 ? red[0] / 127.0 : 1.0;
 }
+
+// Apack for the SNP-side kernels: rows = functional terms (rowinfo), K = samples.
+__global__ void k_pack_rows(const double* __restrict__ Q, uint64_t ldq, uint64_t n, const int2* __restrict__ rowinfo,
+                            const double* __restrict__ sc, int NT, int KS, uint64_t total, int8_t* __restrict__ out) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e & 15), lane = (int)((e >> 4) & 63);
+    const uint64_t frag = e >> 10;                 // (blk*KS + s)*NT + rt
+    const int rt = (int)(frag % NT);
+    const uint64_t ks = frag / NT;
+    const int s = (int)(ks % KS);
+    const uint64_t blk = ks / KS;
+    const int v = j >> 2, c = j & 3, g = lane >> 4;
+    const uint64_t sample = blk * 64 * KS + 16 * KS * g + 16 * s + 4 * c + v;
+    const int2 ri = rowinfo[16 * rt + (lane & 15)];
+    int val = 0;
+    if (ri.x >= 0 && sample < n) val = quant_term(Q[sample * ldq + ri.x] / sc[ri.x], ri.y);
+    out[e] = (int8_t)val;
+  }
+}
+
+// O_t = sum_k term_t(Q[k][i]) over the n samples (exact integers).
+__global__ void __launch_bounds__(256) k_term_sums(const double* __restrict__ Q, uint64_t ldq, uint64_t n,
+                                                   const double* __restrict__ sc, int nterm, double* out) {
+  __shared__ long long red[256];
+  const int i = blockIdx.x, t = blockIdx.y;
+  long long s = 0;
+  for (uint64_t k = threadIdx.x; k < n; k += 256) s += quant_term(Q[k * ldq + i] / sc[i], t);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i * nterm + t] = (double)red[0];
+}
+
+// Row recombination table of one packed operand (device copy).
+struct Recomb {
+  int m;
+  int ng[kMaxM];           // terms on the g0 side (2 or 3)
+  int rg[kMaxM][3];        // their rows
+  int rm[kMaxM][2];        // rows of the two terms on the miss side
+  double w[kMaxM][3];      // sc_i / 254^t
+  double O[kMaxM][3];      // sum over samples of each term
+};
+
+struct Operand {
+  int8_t* pack = nullptr;
+  int NT0 = 0, NT1 = 0, KS = 4;
+  uint64_t nblk = 0;
+  Recomb* rc = nullptr;
+};
+
+static int ks_for(int NT) { return NT <= 8 ? 4 : 2; }   // k-steps per staged block of the SNP-side kernels
+
+// Quantise Q (n x m, ldq) into the packed fragment layout; terms_g per column.
+static Operand prepare_operand(slfmm_ctx* ctx, const char* tag, const double* Q, uint64_t ldq, uint64_t n, int m,
+                               const std::vector<int>& terms_g) {
+  hipStream_t s = ctx->stream;
+  Operand op;
+  std::string T(tag);
+  double* sc = ctx->buf<double>(T + "_sc", m);
+  k_colabsmax<<<m, 256, 0, s>>>(Q, ldq, n, nullptr, nullptr, sc);
+  // rows: g0 block (terms_g[i] rows per functional), then miss block (2 rows each)
+  std::vector<int2> rows;
+  Recomb h{};
+  h.m = m;
+  for (int i = 0; i < m; ++i) {
+    h.ng[i] = terms_g[i];
+    for (int t = 0; t < 3; ++t) h.rg[i][t] = -1;
+    for (int t = 0; t < terms_g[i]; ++t) { h.rg[i][t] = (int)rows.size(); rows.push_back(make_int2(i, t)); }
+  }
+  const int R0 = (int)rows.size();
+  op.NT0 = (int)cdiv(R0, 16);
+  rows.resize(16 * op.NT0, make_int2(-1, 0));
+  for (int i = 0; i < m; ++i)
+    for (int t = 0; t < 2; ++t) { h.rm[i][t] = (int)rows.size(); rows.push_back(make_int2(i, t)); }
+  op.NT1 = (int)cdiv(rows.size() - 16 * op.NT0, 16);
+  rows.resize(16 * (op.NT0 + op.NT1), make_int2(-1, 0));
+  const int NT = op.NT0 + op.NT1;
+  op.KS = ks_for(NT);
+  op.nblk = cdiv(n, 64 * op.KS);
+  int2* drows = ctx->buf<int2>(T + "_rows", rows.size());
+  SLFMM_HIP(hipMemcpyAsync(drows, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+  const uint64_t total = op.nblk * op.KS * NT * 1024;
+  op.pack = ctx->buf<int8_t>(T + "_pack", total);
+  k_pack_rows<<<(unsigned)std::min<uint64_t>(cdiv(total, 256), 16384), 256, 0, s>>>(Q, ldq, n, drows, sc, NT, op.KS,
+                                                                                  total, op.pack);
+  double* O = ctx->buf<double>(T + "_O", m * 3);
+  k_term_sums<<<dim3(m, 3), 256, 0, s>>>(Q, ldq, n, sc, 3, O);
+  std::vector<double> hsc(m), hO(m * 3);
+  SLFMM_HIP(hipMemcpyAsync(hsc.data(), sc, m * 8, hipMemcpyDeviceToHost, s));
+  SLFMM_HIP(hipMemcpyAsync(hO.data(), O, m * 3 * 8, hipMemcpyDeviceToHost, s));
+  SLFMM_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < m; ++i)
+    for (int t = 0; t < 3; ++t) {
+      h.w[i][t] = hsc[i] * (t == 0 ? 1.0 : (t == 1 ? 1.0 / 254.0 : 1.0 / 64516.0));
+      h.O[i][t] = hO[i * 3 + t];
+    }
+  op.rc = ctx->buf<Recomb>(T + "_rc", 1);
+  SLFMM_HIP(hipMemcpyAsync(op.rc, &h, sizeof(Recomb), hipMemcpyHostToDevice, s));
+  SLFMM_HIP(hipGetLastError());
+  return op;
+}
+
+// ---------------------------------------------------------------------------
+// Genotype fragment decode
+// ---------------------------------------------------------------------------
+struct Frag {
+  v4i g0, miss;
+};
+
+__device__ __forceinline__ Frag decode_word(uint32_t w, uint32_t& c_hom, uint32_t& c_miss, uint32_t& c_hi) {
+  const uint32_t lo = w & 0x55555555u, hi = (w >> 1) & 0x55555555u, hb = lo & hi;
+  const uint32_t g0w = hi + hb, mw = lo ^ hb;
+  c_hom += __popc(hb);
+  c_miss += __popc(mw);
+  c_hi += __popc(hi);
+  Frag f;
+  f.g0[0] = (int)(g0w & 0x03030303u);
+  f.g0[1] = (int)((g0w >> 2) & 0x03030303u);
+  f.g0[2] = (int)((g0w >> 4) & 0x03030303u);
+  f.g0[3] = (int)((g0w >> 6) & 0x03030303u);
+  f.miss[0] = (int)(mw & 0x01010101u);
+  f.miss[1] = (int)((mw >> 2) & 0x01010101u);
+  f.miss[2] = (int)((mw >> 4) & 0x01010101u);
+  f.miss[3] = (int)((mw >> 6) & 0x01010101u);
+  return f;
+}
+
+template <int KS>
+__device__ __forceinline__ void load_words(const uint8_t* p, uint32_t (&w)[KS]) {
+  if constexpr (KS == 4) {
+    uint4 q = *reinterpret_cast<const uint4*>(p);
+    w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
+  } else if constexpr (KS == 2) {
+    uint2 q = *reinterpret_cast<const uint2*>(p);
+    w[0] = q.x; w[1] = q.y;
+  } else {
+    w[0] = *reinterpret_cast<const uint32_t*>(p);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SNP-side kernel: functionals of every SNP row against a packed operand.
+// MODE 0: fused association epilogue;  MODE 1: store s_c * f (RSVD sketch).
+// Workgroup: 4 waves x 4 SNP tiles of 16 SNPs (256 SNPs).
+// ---------------------------------------------------------------------------
+struct GtArgs {
+  const uint8_t* packed;
+  uint64_t ld, p, n;
+  int nblk, m, norm;
+  const v4i* A;
+  const Recomb* rc;
+  EpiArgs ea;
+  double* Z;
+  uint64_t ldZ;
+};
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+// Asynchronous copy of one 1 KB fragment chunk (64 lanes x 16 B) from global
+// memory straight into LDS (global_load_lds_dwordx4): lane l's 16 bytes land
+// at dst + 16 l.  The destination is uniform across the wave.
+__device__ __forceinline__ void chunk_to_lds(const unsigned char* src_chunk, unsigned char* dst_chunk, int lane) {
+  __builtin_amdgcn_global_load_lds((const void*)(src_chunk + 16 * lane), (lds_ptr_t)(dst_chunk), 16, 0, 0);
+}
+
+template <int NT0, int NT1, int MAXM, int MODE>
+__global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
+  constexpr int NT = NT0 + NT1;
+  constexpr int KS = 2;                                    // k-steps (64 samples) per staged block
+  constexpr int ST = 4;                                    // SNP tiles of 16 per wave
+  constexpr int CH = KS * NT;                              // 1 KB operand chunks per block
+  constexpr int LDS_A = 2 * CH * 1024;
+  constexpr int LDS_RED = 4 * NT * 1024;                   // one SNP tile of int32 sums per wave
+  constexpr int PLAN_MAX = MODE == 0 ? 3 * kMaxD * MAXM + kMaxD + MAXM * MAXM : 0;
+  constexpr int LDS_E = LDS_RED + PLAN_MAX * 8;
+  constexpr int LDS = LDS_A > LDS_E ? LDS_A : LDS_E;
+  __shared__ __align__(16) unsigned char lds[LDS];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, l16 = lane & 15;
+  const uint64_t base = ((uint64_t)blockIdx.x * 4 + wave) * (ST * 16);
+  const unsigned char* Ag = reinterpret_cast<const unsigned char*>(a.A);
+  const uint8_t* rowp[ST];
+#pragma unroll
+  for (int st = 0; st < ST; ++st) {
+    uint64_t j = base + 16 * st + l16;
+    if (j >= a.p) j = a.p - 1;
+    rowp[st] = a.packed + j * a.ld + 4 * KS * g;
+  }
+  for (int c = wave; c < CH; c += 4) chunk_to_lds(Ag + (uint64_t)c * 1024, lds + c * 1024, lane);
+  uint32_t gw[ST][KS], gn[ST][KS];
+#pragma unroll
+  for (int st = 0; st < ST; ++st) load_words<KS>(rowp[st], gw[st]);
+  v4i acc[ST][NT];
+#pragma unroll
+  for (int st = 0; st < ST; ++st)
+#pragma unroll
+    for (int rt = 0; rt < NT; ++rt) acc[st][rt] = v4i{0, 0, 0, 0};
+  uint32_t c_hom[ST], c_miss[ST], c_hi[ST];
+#pragma unroll
+  for (int st = 0; st < ST; ++st) c_hom[st] = c_miss[st] = c_hi[st] = 0;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+
+  for (int kb = 0; kb < a.nblk; ++kb) {
+    const bool more = kb + 1 < a.nblk;
+    if (more) {
+      unsigned char* dst = lds + ((kb + 1) & 1) * CH * 1024;
+      const unsigned char* src = Ag + (uint64_t)(kb + 1) * CH * 1024;
+      for (int c = wave; c < CH; c += 4) chunk_to_lds(src + (uint64_t)c * 1024, dst + c * 1024, lane);
+#pragma unroll
+      for (int st = 0; st < ST; ++st) load_words<KS>(rowp[st] + (uint64_t)(kb + 1) * 16 * KS, gn[st]);
+    }
+    if (kb == a.nblk - 1) {   // bytes at or past the row stride belong to the next row
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int64_t lim = (int64_t)a.ld - ((int64_t)kb * 16 * KS + 4 * KS * g + 4 * s);
+        const uint32_t mask = lim >= 4 ? 0xFFFFFFFFu : (lim <= 0 ? 0u : ((1u << (8 * lim)) - 1u));
+#pragma unroll
+        for (int st = 0; st < ST; ++st) gw[st][s] &= mask;
+      }
+    }
+    const v4i* Ab = reinterpret_cast<const v4i*>(lds + (kb & 1) * CH * 1024);
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      Frag f[ST];
+#pragma unroll
+      for (int st = 0; st < ST; ++st) f[st] = decode_word(gw[st][s], c_hom[st], c_miss[st], c_hi[st]);
+#pragma unroll
+      for (int rt = 0; rt < NT; ++rt) {
+        const v4i af = Ab[(s * NT + rt) * 64 + lane];
+#pragma unroll
+        for (int st = 0; st < ST; ++st)
+          acc[st][rt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, rt < NT0 ? f[st].g0 : f[st].miss, acc[st][rt], 0, 0, 0);
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int st = 0; st < ST; ++st)
+#pragma unroll
+        for (int s = 0; s < KS; ++s) gw[st][s] = gn[st][s];
+    }
+    __builtin_amdgcn_s_waitcnt(0);   // the LDS copy of block kb+1 has landed
+    __syncthreads();
+  }
+
+  // ---- epilogue: lane (g, l16) owns SNP base + 16 g + l16 -------------------
+#pragma unroll
+  for (int st = 0; st < ST; ++st) {
+    c_hom[st] += __shfl_xor(c_hom[st], 16); c_hom[st] += __shfl_xor(c_hom[st], 32);
+    c_miss[st] += __shfl_xor(c_miss[st], 16); c_miss[st] += __shfl_xor(c_miss[st], 32);
+    c_hi[st] += __shfl_xor(c_hi[st], 16); c_hi[st] += __shfl_xor(c_hi[st], 32);
+  }
+  uint32_t o_miss = c_miss[0], o_hom = c_hom[0], o_hi = c_hi[0];
+#pragma unroll
+  for (int st = 1; st < ST; ++st) {
+    o_miss = (g == st) ? c_miss[st] : o_miss;
+    o_hom = (g == st) ? c_hom[st] : o_hom;
+    o_hi = (g == st) ? c_hi[st] : o_hi;
+  }
+  const int64_t nmiss = o_miss, nhom = o_hom;
+  const int64_t sg = (int64_t)o_hi + nhom;
+  const int64_t nobs = (int64_t)a.n - nmiss;
+  const double mu = nobs > 0 ? (double)sg / (double)nobs : 0.0;
+  const uint64_t j = base + 16 * g + l16;
+  const double scale = MODE == 1 ? epi_scale(mu, a.norm) : 1.0;
+  int* red = reinterpret_cast<int*>(lds) + wave * (NT * 256);   // [16*NT rows][16 SNPs]
+  const Recomb* rc = a.rc;
+  double fy[MODE == 0 ? MAXM : 1];
+#pragma unroll
+  for (int st = 0; st < ST; ++st) {
+#pragma unroll
+    for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[(16 * rt + 4 * g + r) * 16 + l16] = acc[st][rt][r];
+    __syncthreads();
+    if (g == st) {
+#pragma unroll
+      for (int i = 0; i < (MODE == 0 ? MAXM : kMaxM); ++i) {
+        if (i < a.m) {
+          double fg = 0.0, fm = 0.0;
+          const int ng = rc->ng[i];
+          for (int tt = 0; tt < ng; ++tt)
+            fg += rc->w[i][tt] * ((double)red[rc->rg[i][tt] * 16 + l16] - mu * rc->O[i][tt]);
+          fm = rc->w[i][0] * (double)red[rc->rm[i][0] * 16 + l16] + rc->w[i][1] * (double)red[rc->rm[i][1] * 16 + l16];
+          const double f = fg + mu * fm;
+          if constexpr (MODE == 1) {
+            if (j < a.p) a.Z[j * a.ldZ + i] = scale * f;
+          } else {
+            fy[i] = f;
+          }
+        } else if constexpr (MODE == 0) {
+          fy[i] = 0.0;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if constexpr (MODE == 0) {
+    double* plan = reinterpret_cast<double*>(lds + LDS_RED);
+    const int plan_len = a.ea.d * a.ea.m * 3 + a.ea.d + (a.ea.fallback ? a.ea.m * a.ea.m : 0);
+    for (int e = t; e < plan_len; e += 256) plan[e] = a.ea.plan[e];
+    __syncthreads();
+    if (j < a.p) snp_epilogue<MAXM>(a.ea, plan, j, fy, nmiss, sg, nhom);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined variant for NT <= 8: 8 waves x 64 SNPs per workgroup, blocks of
+// 4 k-steps (256 samples, 64 bytes per SNP row).  Genotype segments are
+// copied by LDS-DMA into a 3-slot ring (blocks kb+1 and kb+2 in flight
+// while kb is computed), the operand block into a 2-slot ring.  At the top
+// of each block the wave waits for all but its 4 newest loads (the genotype
+// block kb+1), so block kb's genotypes and operand have landed.
+// ---------------------------------------------------------------------------
+constexpr int kPipeWaves = 8;
+constexpr int kPipeTiles = 3;   // SNP tiles of 16 per wave
+
+// One 4-k-step block of the pipelined kernel with statically chosen ring
+// slots (separate LDS objects, so the compiler can tell the slot being read
+// from the slots being written by LDS-DMA and inserts no drain in between).
+// Reads ST 16-byte words, 1024 bytes apart, from LDS and waits for them in
+// the same asm block.  The genotype ring is filled by LDS-DMA; the ordering
+// against those copies is the explicit vmcnt wait plus barrier that precedes
+// every read, so the compiler must not add its own full drain of the DMA
+// counter (its alias tracking of DMA'd LDS objects is incomplete across the
+// unrolled ring and falls back to vmcnt(0)).
+template <int ST>
+__device__ __forceinline__ void lds_read_tiles(const unsigned char* p, v4i (&q)[ST]) {
+  const uint32_t addr = (uint32_t)(uintptr_t)(lds_ptr_t)p;
+  static_assert(ST >= 1 && ST <= 4, "tile count");
+  if constexpr (ST == 1) {
+    asm volatile("ds_read_b128 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=&v"(q[0]) : "v"(addr) : "memory");
+  } else if constexpr (ST == 2) {
+    asm volatile("ds_read_b128 %0, %2\n\tds_read_b128 %1, %2 offset:1024\n\ts_waitcnt lgkmcnt(0)"
+                 : "=&v"(q[0]), "=&v"(q[1]) : "v"(addr) : "memory");
+  } else if constexpr (ST == 3) {
+    asm volatile("ds_read_b128 %0, %3\n\tds_read_b128 %1, %3 offset:1024\n\t"
+                 "ds_read_b128 %2, %3 offset:2048\n\ts_waitcnt lgkmcnt(0)"
+                 : "=&v"(q[0]), "=&v"(q[1]), "=&v"(q[2]) : "v"(addr) : "memory");
+  } else {
+    asm volatile("ds_read_b128 %0, %4\n\tds_read_b128 %1, %4 offset:1024\n\t"
+                 "ds_read_b128 %2, %4 offset:2048\n\tds_read_b128 %3, %4 offset:3072\n\ts_waitcnt lgkmcnt(0)"
+                 : "=&v"(q[0]), "=&v"(q[1]), "=&v"(q[2]), "=&v"(q[3]) : "v"(addr) : "memory");
+  }
+}
+
+template <int NT0, int NT, int ST>
+__device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, int wave, int lane, int g,
+                                           const uint8_t* row0, uint64_t tile_stride, uint64_t jl,
+                                           const uint8_t* lastrow, const unsigned char* Ag,
+                                           const unsigned char* a_rd, unsigned char* a_wr,
+                                           const unsigned char* g_rd, unsigned char* g_wr,
+                                           v4i (&acc)[ST][NT], uint32_t (&c_hom)[ST], uint32_t (&c_miss)[ST],
+                                           uint32_t (&c_hi)[ST]) {
+  constexpr int KS = 4;
+  constexpr int W = kPipeWaves;
+  constexpr int CH = KS * NT;
+  // block kb's operand and genotypes have landed once at most the ST newest
+  // loads of this wave (genotype block kb+1, one load per SNP tile) are
+  // still in flight.  s_waitcnt immediate: vmcnt[3:0] = ST, expcnt and
+  // lgkmcnt at their maxima (no wait on them).
+  static_assert(ST >= 1 && ST <= 15, "vmcnt immediate");
+  __builtin_amdgcn_s_waitcnt(0x0F70 | ST);
+  __builtin_amdgcn_s_barrier();
+  const v4i* Ab = reinterpret_cast<const v4i*>(a_rd);
+  uint32_t gw[ST][KS];
+  {
+    v4i q[ST];
+    lds_read_tiles<ST>(g_rd + wave * (ST * 1024) + 16 * lane, q);
+#pragma unroll
+    for (int st = 0; st < ST; ++st)
+#pragma unroll
+      for (int s = 0; s < KS; ++s) gw[st][s] = (uint32_t)q[st][s];
+  }
+  if (kb == a.nblk - 1) {   // bytes at or past the row stride belong to the next row
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int64_t lim = (int64_t)a.ld - ((int64_t)kb * 64 + 16 * g + 4 * s);
+      const uint32_t mask = lim >= 4 ? 0xFFFFFFFFu : (lim <= 0 ? 0u : ((1u << (8 * lim)) - 1u));
+#pragma unroll
+      for (int st = 0; st < ST; ++st) gw[st][s] &= mask;
+    }
+  }
+  // the ring slots written below are not the ones read above
+  if (kb + 1 < a.nblk) {
+    const unsigned char* src = Ag + (uint64_t)(kb + 1) * CH * 1024;
+    for (int c = wave; c < CH; c += W) chunk_to_lds(src + (uint64_t)c * 1024, a_wr + c * 1024, lane);
+  }
+  {
+    // past the last block a dummy copy of block 0 keeps the count of
+    // in-flight genotype loads per wave at exactly ST
+    const int src = kb + 2 < a.nblk ? kb + 2 : 0;
+    unsigned char* dst = g_wr + wave * (ST * 1024);
+#pragma unroll
+    for (int st = 0; st < ST; ++st) {
+      // rows past the last SNP are clamped onto it: read, never stored
+      const uint8_t* rp = (jl + 16 * st < a.p) ? row0 + st * tile_stride : lastrow;
+      __builtin_amdgcn_global_load_lds((const void*)(rp + (uint64_t)src * 64), (lds_ptr_t)(dst + st * 1024), 16, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    Frag f[ST];
+#pragma unroll
+    for (int st = 0; st < ST; ++st) f[st] = decode_word(gw[st][s], c_hom[st], c_miss[st], c_hi[st]);
+#pragma unroll
+    for (int rt = 0; rt < NT; rt += 2) {
+      const v4i af0 = Ab[(s * NT + rt) * 64 + lane];
+      const v4i af1 = rt + 1 < NT ? Ab[(s * NT + rt + 1) * 64 + lane] : af0;
+#pragma unroll
+      for (int st = 0; st < ST; ++st) {
+        acc[st][rt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af0, rt < NT0 ? f[st].g0 : f[st].miss, acc[st][rt], 0, 0, 0);
+        if (rt + 1 < NT)
+          acc[st][rt + 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af1, rt + 1 < NT0 ? f[st].g0 : f[st].miss,
+                                                                  acc[st][rt + 1], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined variant for NT <= 8: 8 waves x 64 SNPs per workgroup, blocks of
+// 4 k-steps (256 samples, 64 bytes per SNP row).  Genotype segments are
+// copied by LDS-DMA into a 3-slot ring (blocks kb+1 and kb+2 in flight
+// while kb is computed), the operand block into a 2-slot ring; the block
+// loop is unrolled by 6 so every slot is a compile-time LDS object.
+// ---------------------------------------------------------------------------
+template <int NT0, int NT1, int MAXM, int MODE>
+__global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
+  constexpr int NT = NT0 + NT1;
+  constexpr int KS = 4;
+  constexpr int ST = kPipeTiles;
+  constexpr int W = kPipeWaves;
+  constexpr int CH = KS * NT;
+  constexpr int A_SLOT = CH * 1024;
+  constexpr int G_SLOT = W * ST * 1024;
+  static_assert(NT * 1024 * W <= 3 * G_SLOT, "epilogue sums of all waves must fit the genotype ring");
+  __shared__ __align__(16) unsigned char A0[A_SLOT];
+  __shared__ __align__(16) unsigned char A1[A_SLOT];
+  __shared__ __align__(16) unsigned char G0[G_SLOT];
+  __shared__ __align__(16) unsigned char G1[G_SLOT];
+  __shared__ __align__(16) unsigned char G2[G_SLOT];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, l16 = lane & 15;
+  const uint64_t base = ((uint64_t)blockIdx.x * W + wave) * (ST * 16);
+  const unsigned char* Ag = reinterpret_cast<const unsigned char*>(a.A);
+  // rows past the last SNP are clamped onto it: read, never stored
+  const uint64_t jl = base + l16;
+  const uint8_t* lastrow = a.packed + (a.p - 1) * a.ld + 16 * g;
+  const uint8_t* row0 = jl < a.p ? a.packed + jl * a.ld + 16 * g : lastrow;
+  const uint64_t tile_stride = (uint64_t)16 * a.ld;
+  v4i acc[ST][NT];
+#pragma unroll
+  for (int st = 0; st < ST; ++st)
+#pragma unroll
+    for (int rt = 0; rt < NT; ++rt) acc[st][rt] = v4i{0, 0, 0, 0};
+  uint32_t c_hom[ST], c_miss[ST], c_hi[ST];
+#pragma unroll
+  for (int st = 0; st < ST; ++st) c_hom[st] = c_miss[st] = c_hi[st] = 0;
+  // prologue: operand block 0 -> A0, genotype blocks 0 -> G0 and 1 -> G1
+  for (int c = wave; c < CH; c += W) chunk_to_lds(Ag + (uint64_t)c * 1024, A0 + c * 1024, lane);
+#pragma unroll
+  for (int st = 0; st < ST; ++st) {
+    const uint8_t* rp = (jl + 16 * st < a.p) ? row0 + st * tile_stride : lastrow;
+    __builtin_amdgcn_global_load_lds((const void*)(rp), (lds_ptr_t)(G0 + wave * (ST * 1024) + st * 1024), 16, 0, 0);
+  }
+  {
+    const int src = a.nblk > 1 ? 1 : 0;
+#pragma unroll
+    for (int st = 0; st < ST; ++st) {
+      const uint8_t* rp = (jl + 16 * st < a.p) ? row0 + st * tile_stride : lastrow;
+      __builtin_amdgcn_global_load_lds((const void*)(rp + (uint64_t)src * 64),
+                                       (lds_ptr_t)(G1 + wave * (ST * 1024) + st * 1024), 16, 0, 0);
+    }
+  }
+#define SLFMM_PIPE(PH, AR, AW, GR, GW)                                                              \
+  if (kb + PH < a.nblk)                                                                             \
+    pipe_block<NT0, NT, ST>(kb + PH, a, wave, lane, g, row0, tile_stride, jl, lastrow, Ag, AR, AW, GR, GW, acc, c_hom, \
+                            c_miss, c_hi);
+  for (int kb = 0; kb < a.nblk; kb += 6) {
+    SLFMM_PIPE(0, A0, A1, G0, G2)
+    SLFMM_PIPE(1, A1, A0, G1, G0)
+    SLFMM_PIPE(2, A0, A1, G2, G1)
+    SLFMM_PIPE(3, A1, A0, G0, G2)
+    SLFMM_PIPE(4, A0, A1, G1, G0)
+    SLFMM_PIPE(5, A1, A0, G2, G1)
+  }
+#undef SLFMM_PIPE
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+
+  // ---- epilogue: lane (g, l16) owns SNP base + 16 g + l16 -------------------
+#pragma unroll
+  for (int st = 0; st < ST; ++st) {
+    c_hom[st] += __shfl_xor(c_hom[st], 16); c_hom[st] += __shfl_xor(c_hom[st], 32);
+    c_miss[st] += __shfl_xor(c_miss[st], 16); c_miss[st] += __shfl_xor(c_miss[st], 32);
+    c_hi[st] += __shfl_xor(c_hi[st], 16); c_hi[st] += __shfl_xor(c_hi[st], 32);
+  }
+  uint32_t o_miss = c_miss[0], o_hom = c_hom[0], o_hi = c_hi[0];
+#pragma unroll
+  for (int st = 1; st < ST; ++st) {
+    o_miss = (g == st) ? c_miss[st] : o_miss;
+    o_hom = (g == st) ? c_hom[st] : o_hom;
+    o_hi = (g == st) ? c_hi[st] : o_hi;
+  }
+  const int64_t nmiss = o_miss, nhom = o_hom;
+  const int64_t sg = (int64_t)o_hi + nhom;
+  const int64_t nobs = (int64_t)a.n - nmiss;
+  const double mu = nobs > 0 ? (double)sg / (double)nobs : 0.0;
+  const bool owner = g < ST;
+  const uint64_t j = owner ? base + 16 * g + l16 : a.p;
+  const double scale = MODE == 1 ? epi_scale(mu, a.norm) : 1.0;
+  // per-wave int32 sums of one SNP tile: waves spread over the three genotype slots
+  constexpr int WPS = G_SLOT / (NT * 1024);   // waves per slot
+  unsigned char* gslot = wave < WPS ? G0 : (wave < 2 * WPS ? G1 : G2);
+  int* red = reinterpret_cast<int*>(gslot) + (wave % WPS) * (NT * 256);
+  const Recomb* rc = a.rc;
+  double fy[MODE == 0 ? MAXM : 1];
+#pragma unroll
+  for (int st = 0; st < ST; ++st) {
+#pragma unroll
+    for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[(16 * rt + 4 * g + r) * 16 + l16] = acc[st][rt][r];
+    __syncthreads();
+    if (g == st) {   // lane group st owns this tile's SNPs
+#pragma unroll
+      for (int i = 0; i < (MODE == 0 ? MAXM : kMaxM); ++i) {
+        if (i < a.m) {
+          double fg = 0.0;
+          const int ng = rc->ng[i];
+          for (int tt = 0; tt < ng; ++tt)
+            fg += rc->w[i][tt] * ((double)red[rc->rg[i][tt] * 16 + l16] - mu * rc->O[i][tt]);
+          const double fm = rc->w[i][0] * (double)red[rc->rm[i][0] * 16 + l16] +
+                            rc->w[i][1] * (double)red[rc->rm[i][1] * 16 + l16];
+          const double f = fg + mu * fm;
+          if constexpr (MODE == 1) {
+            if (j < a.p) a.Z[j * a.ldZ + i] = scale * f;
+          } else {
+            fy[i] = f;
+          }
+        } else if constexpr (MODE == 0) {
+          fy[i] = 0.0;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if constexpr (MODE == 0) {
+    double* plan = reinterpret_cast<double*>(A0);
+    const int plan_len = a.ea.d * a.ea.m * 3 + a.ea.d + (a.ea.fallback ? a.ea.m * a.ea.m : 0);
+    for (int e = t; e < plan_len; e += 512) plan[e] = a.ea.plan[e];
+    __syncthreads();
+    if (j < a.p) snp_epilogue<MAXM>(a.ea, plan, j, fy, nmiss, sg, nhom);
+  }
+}
+
+#define SLFMM_GT_CASES(X) \
+  X(1, 1) X(2, 1) X(3, 1) X(2, 2) X(3, 2) X(4, 2) X(3, 3) X(4, 3) X(5, 3) X(4, 4) X(5, 4) X(6, 4)
+
+template <int A0, int A1>
+static bool launch_pipe(hipStream_t s, unsigned grid, GtArgs& a, int mode, int maxm) {
+  if constexpr (A0 + A1 <= 8) {
+    if (mode == 1) k_gt_i8p<A0, A1, 1, 1><<<grid, 512, 0, s>>>(a);
+    else if (maxm <= 32) k_gt_i8p<A0, A1, 32, 0><<<grid, 512, 0, s>>>(a);
+    else return false;
+    return true;
+  }
+  return false;
+}
+
+static void launch_gt(hipStream_t s, const Operand& op, GtArgs& a, int mode, int maxm) {
+  if (a.p == 0) return;
+  const int key = op.NT0 * 100 + op.NT1;
+  const int NT = op.NT0 + op.NT1;
+  bool done = false;
+  if (NT <= 8) {
+    const unsigned grid = (unsigned)cdiv(a.p, 16 * kPipeTiles * kPipeWaves);
+#define X(A0, A1)                                                                      \
+    if (!done && key == A0 * 100 + A1) done = launch_pipe<A0, A1>(s, grid, a, mode, maxm);
+    SLFMM_GT_CASES(X)
+#undef X
+  } else {
+    const unsigned grid = (unsigned)cdiv(a.p, 256);
+#define X(A0, A1)                                                                      \
+    if (!done && key == A0 * 100 + A1 && A0 + A1 > 8) {                                \
+      if (mode == 1) k_gt_i8<A0, A1, 1, 1><<<grid, 256, 0, s>>>(a);                    \
+      else if (maxm <= 32) k_gt_i8<A0, A1, 32, 0><<<grid, 256, 0, s>>>(a);             \
+      done = true;                                                                     \
+    }
+    SLFMM_GT_CASES(X)
+#undef X
+    if (!done && mode == 1 && op.NT0 == op.NT1) {
+      switch (op.NT0) {
+        case 5: k_gt_i8<5, 5, 1, 1><<<grid, 256, 0, s>>>(a); done = true; break;
+        case 6: k_gt_i8<6, 6, 1, 1><<<grid, 256, 0, s>>>(a); done = true; break;
+        case 7: k_gt_i8<7, 7, 1, 1><<<grid, 256, 0, s>>>(a); done = true; break;
+        case 8: k_gt_i8<8, 8, 1, 1><<<grid, 256, 0, s>>>(a); done = true; break;
+        default: break;
+      }
+    }
+  }
+  SLFMM_CHECK(done, SLFMM_E_ARG, "no int8 kernel for this tile shape (K + d too large)");
+  SLFMM_HIP(hipGetLastError());
+}
+
+// Association pass over every SNP of g (fused epilogue).
+void launch_func_i8(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_Q, uint64_t m, const EpiArgs& ea,
+                    bool /*fused_epilogue*/) {
+  // Latent directions: 2 int8 terms; covariate directions (the t-statistic
+  // numerators, last d columns of the basis): 3 terms while K + d <= 32.
+  SLFMM_CHECK(m <= 32, SLFMM_E_ARG, "the int8 association path supports K + d <= 32 (use SLFMM_PREC_FP64)");
+  std::vector<int> terms(m, 2);
+  for (int i = ea.K; i < (int)m; ++i) terms[i] = 3;
+  Operand op = prepare_operand(ctx, "as_i8", d_Q, m, g->n, (int)m, terms);
+  const uint64_t plan_bytes = 8 * (ea.d * m * 3 + ea.d + (ea.fallback ? m * m : 0));
+  SLFMM_CHECK(op.NT0 + op.NT1 > 8 || plan_bytes <= (uint64_t)4 * (op.NT0 + op.NT1) * 1024, SLFMM_E_ARG,
+              "association plan does not fit the int8 kernel's LDS (use SLFMM_PREC_FP64)");
+  GtArgs a{};
+  a.packed = g->d; a.ld = g->ld; a.p = g->p; a.n = g->n; a.nblk = (int)op.nblk; a.m = (int)m; a.norm = ea.norm;
+  a.A = reinterpret_cast<const v4i*>(op.pack); a.rc = op.rc; a.ea = ea;
+  launch_gt(ctx->stream, op, a, 0, (int)m);
+}
+
+// RSVD sketch / backward pass: Z[j][c] = s_j * (B^T y_j)_c for every SNP row.
+void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p, const double* d_B,
+                     uint64_t l, int norm, double* Z, uint64_t ldZ) {
+  std::vector<int> terms(l, 2);
+  Operand op = prepare_operand(ctx, "rs_i8", d_B, l, n, (int)l, terms);
+  GtArgs a{};
+  a.packed = packed; a.ld = ld; a.p = p; a.n = n; a.nblk = (int)op.nblk; a.m = (int)l; a.norm = norm;
+  a.A = reinterpret_cast<const v4i*>(op.pack); a.rc = op.rc; a.Z = Z; a.ldZ = ldZ;
+  launch_gt(ctx->stream, op, a, 1, 1);
+}
+
+// ---------------------------------------------------------------------------
+// RSVD forward pass on the sample-major copy: W = Y Q_z (n x l).
+//   W[k][c] = sum_j g0_jk Qt_jc + sum_j miss_jk (mu Qt)_jc - sum_j mu_j Qt_jc
+// with Qt = diag(s) Q_z.  A side: genotype fragments of 16 samples x 16*KS
+// SNPs; B side: Qt and mu*Qt as 2 int8 terms per column, pre-permuted.
+// Workgroup: 4 waves x 4 sample tiles (256 samples) x one SNP chunk.
+// ---------------------------------------------------------------------------
+__global__ void k_pack_cols(const double* __restrict__ Qz, uint64_t ldq, uint64_t pe, int l,
+                            const double* __restrict__ ssnp, const double* __restrict__ mu,
+                            const double* __restrict__ csc, int NCB, int KS, uint64_t total, int8_t* __restrict__ out) {
+  const int NTB = 4 * NCB;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e & 15), lane = (int)((e >> 4) & 63);
+    const uint64_t frag = e >> 10;
+    const int ct = (int)(frag % NTB);
+    const uint64_t ks = frag / NTB;
+    const int s = (int)(ks % KS);
+    const uint64_t blk = ks / KS;
+    const int v = j >> 2, c = j & 3, g = lane >> 4;
+    const uint64_t snp = blk * 64 * KS + 16 * KS * g + 16 * s + 4 * c + v;
+    const int col = 16 * (ct % NCB) + (lane & 15);
+    const int term = (ct / NCB) & 1, part = ct / (2 * NCB);
+    int val = 0;
+    if (col < l && snp < pe) {
+      double x = Qz[snp * ldq + col] * ssnp[snp];
+      if (part) x *= mu[snp];
+      val = quant_term(x / csc[part * l + col], term);
+    }
+    out[e] = (int8_t)val;
+  }
+}
+
+// vhat_c = sum_j mu_j * Qt^_jc (2-term value of Qt), one workgroup per column.
+__global__ void __launch_bounds__(256) k_vhat(const double* __restrict__ Qz, uint64_t ldq, uint64_t pe,
+                                              const double* __restrict__ ssnp, const double* __restrict__ mu,
+                                              const double* __restrict__ csc, double* vhat) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  double acc = 0.0;
+  for (uint64_t jj = threadIdx.x; jj < pe; jj += 256) {
+    double q = Qz[jj * ldq + c] * ssnp[jj] / csc[c];
+    double val = csc[c] * ((double)quant_term(q, 0) + (double)quant_term(q, 1) / 254.0);
+    acc += mu[jj] * val;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) vhat[c] = red[0];
+}
+
+template <int NCB>
+__global__ void __launch_bounds__(256) k_fwd_i8(const uint8_t* __restrict__ YT, uint64_t ldt, uint64_t n, int blk0,
+                                                int nblk_chunk, int nblk_total, const v4i* __restrict__ Bp,
+                                                const double* __restrict__ csc, int l, double* __restrict__ part,
+                                                uint64_t ldp) {
+  constexpr int NTB = 4 * NCB;
+  constexpr int KS = 4;
+  constexpr int ST = 4;
+  constexpr int BLK = KS * NTB * 64;
+  __shared__ v4i Bs[2 * BLK];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, l16 = lane & 15;
+  const uint64_t sbase = (uint64_t)blockIdx.x * 256 + wave * (ST * 16);
+  const int b0 = blk0 + blockIdx.y * nblk_chunk;
+  const int b1 = min(nblk_total, b0 + nblk_chunk);
+  const uint8_t* rowp[ST];
+#pragma unroll
+  for (int st = 0; st < ST; ++st) {
+    uint64_t k = sbase + 16 * st + l16;
+    if (k >= n) k = n - 1;
+    rowp[st] = YT + k * ldt + 4 * KS * g;
+  }
+  v4i acc[ST][NTB];
+#pragma unroll
+  for (int st = 0; st < ST; ++st)
+#pragma unroll
+    for (int ct = 0; ct < NTB; ++ct) acc[st][ct] = v4i{0, 0, 0, 0};
+  uint32_t dummy0 = 0, dummy1 = 0, dummy2 = 0;
+  constexpr int PER = (BLK + 255) / 256;
+  if (b0 < b1) {
+    for (int q = 0; q < PER; ++q) {
+      int idx = t + 256 * q;
+      if (idx < BLK) Bs[idx] = Bp[(uint64_t)b0 * BLK + idx];
+    }
+  }
+  uint32_t gw[ST][KS], gn[ST][KS];
+  if (b0 < b1)
+#pragma unroll
+    for (int st = 0; st < ST; ++st) load_words<KS>(rowp[st] + (uint64_t)b0 * 16 * KS, gw[st]);
+  __syncthreads();
+  for (int kb = b0; kb < b1; ++kb) {
+    const bool more = kb + 1 < b1;
+    v4i pre[PER];
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        int idx = t + 256 * q;
+        if (idx < BLK) pre[q] = Bp[(uint64_t)(kb + 1) * BLK + idx];
+      }
+#pragma unroll
+      for (int st = 0; st < ST; ++st) load_words<KS>(rowp[st] + (uint64_t)(kb + 1) * 16 * KS, gn[st]);
+    }
+    if (kb == nblk_total - 1) {
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int64_t lim = (int64_t)ldt - ((int64_t)kb * 16 * KS + 4 * KS * g + 4 * s);
+        const uint32_t mask = lim >= 4 ? 0xFFFFFFFFu : (lim <= 0 ? 0u : ((1u << (8 * lim)) - 1u));
+#pragma unroll
+        for (int st = 0; st < ST; ++st) gw[st][s] &= mask;
+      }
+    }
+    const v4i* Bb = Bs + ((kb - b0) & 1) * BLK;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      v4i bf[NTB];
+#pragma unroll
+      for (int ct = 0; ct < NTB; ++ct) bf[ct] = Bb[(s * NTB + ct) * 64 + lane];
+#pragma unroll
+      for (int st = 0; st < ST; ++st) {
+        Frag f = decode_word(gw[st][s], dummy0, dummy1, dummy2);
+#pragma unroll
+        for (int ct = 0; ct < 2 * NCB; ++ct)
+          acc[st][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f.g0, bf[ct], acc[st][ct], 0, 0, 0);
+#pragma unroll
+        for (int ct = 2 * NCB; ct < NTB; ++ct)
+          acc[st][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f.miss, bf[ct], acc[st][ct], 0, 0, 0);
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        int idx = t + 256 * q;
+        if (idx < BLK) Bs[((kb + 1 - b0) & 1) * BLK + idx] = pre[q];
+      }
+#pragma unroll
+      for (int st = 0; st < ST; ++st)
+#pragma unroll
+        for (int s = 0; s < KS; ++s) gw[st][s] = gn[st][s];
+    }
+    __syncthreads();
+  }
+  (void)dummy0; (void)dummy1; (void)dummy2;
+  // lane-local combination: output (sample 4g + r of tile st, column 16 cb + l16)
+#pragma unroll
+  for (int st = 0; st < ST; ++st)
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+      const int col = 16 * cb + l16;
+      if (col >= l) continue;
+      const double w0 = csc[col], w1 = csc[col] / 254.0, m0 = csc[l + col], m1 = csc[l + col] / 254.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint64_t k = sbase + 16 * st + 4 * g + r;
+        if (k >= n) continue;
+        double v = w0 * (double)acc[st][cb][r] + w1 * (double)acc[st][NCB + cb][r] +
+                   m0 * (double)acc[st][2 * NCB + cb][r] + m1 * (double)acc[st][3 * NCB + cb][r];
+        part[((uint64_t)blockIdx.y * n + k) * ldp + col] = v;
+      }
+    }
+}
+
+__global__ void k_fwd_reduce(const double* __restrict__ part, uint64_t n, int l, uint64_t ldp, int nchunk,
+                             const double* __restrict__ vhat, double* W, uint64_t ldw) {
+  uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * (uint64_t)l) return;
+  uint64_t k = e / l;
+  int c = (int)(e % l);
+  double s = 0.0;
+  for (int q = 0; q < nchunk; ++q) s += part[((uint64_t)q * n + k) * ldp + c];
+  W[k * ldw + c] = s - vhat[c];
+}
+
+void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_t n, uint64_t pe, const double* Qz,
+                        uint64_t ldq, uint64_t l, const double* mu, const double* ssnp, double* W, uint64_t ldw) {
+  hipStream_t s = ctx->stream;
+  const int NCB = (int)cdiv(l, 16);
+  SLFMM_CHECK(NCB >= 1 && NCB <= 2, SLFMM_E_ARG, "int8 forward pass supports K + oversampling <= 32");
+  const int KS = 4, NTB = 4 * NCB;
+  double* csc = ctx->buf<double>("fw_csc", 2 * l);
+  k_colabsmax<<<(unsigned)l, 256, 0, s>>>(Qz, ldq, pe, ssnp, nullptr, csc);
+  k_colabsmax<<<(unsigned)l, 256, 0, s>>>(Qz, ldq, pe, ssnp, mu, csc + l);
+  const uint64_t nblk = cdiv(pe, 64 * KS);
+  const uint64_t total = nblk * KS * NTB * 1024;
+  int8_t* Bp = ctx->buf<int8_t>("fw_pack", total);
+  k_pack_cols<<<(unsigned)std::min<uint64_t>(cdiv(total, 256), 16384), 256, 0, s>>>(Qz, ldq, pe, (int)l, ssnp, mu,
+                                                                                  csc, NCB, KS, total, Bp);
+  double* vhat = ctx->buf<double>("fw_vhat", l);
+  k_vhat<<<(unsigned)l, 256, 0, s>>>(Qz, ldq, pe, ssnp, mu, csc, vhat);
+  const uint64_t sblocks = cdiv(n, 256);
+  uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(nblk, cdiv(1024, sblocks)));
+  const uint64_t per = cdiv(nblk, nchunk);
+  nchunk = cdiv(nblk, per);
+  const uint64_t ldp = l;
+  double* part = ctx->buf<double>("fw_part", nchunk * n * ldp);
+  dim3 grid((unsigned)sblocks, (unsigned)nchunk);
+  if (NCB == 1)
+    k_fwd_i8<1><<<grid, 256, 0, s>>>(YT, ldt, n, 0, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp), csc,
+                                     (int)l, part, ldp);
+  else
+    k_fwd_i8<2><<<grid, 256, 0, s>>>(YT, ldt, n, 0, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp), csc,
+                                     (int)l, part, ldp);
+  k_fwd_reduce<<<(unsigned)cdiv(n * l, 256), 256, 0, s>>>(part, n, (int)l, ldp, (int)nchunk, vhat, W, ldw);
+  SLFMM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Layout probe (test-only): one v_mfma_i32_16x16x64_i8 with the fragment
+// mapping the kernels above assume.  A is 16 x 64, B is 64 x 16, row-major.
+// ---------------------------------------------------------------------------
+__global__ void k_probe_mfma_i8(const int8_t* __restrict__ Afrag, const int8_t* __restrict__ B, int* D) {
+  __shared__ __align__(16) unsigned char s_a[1024];
+  const int lane = threadIdx.x, g = lane >> 4, l16 = lane & 15;
+  chunk_to_lds(reinterpret_cast<const unsigned char*>(Afrag), s_a, lane);   // the kernels' LDS-DMA path
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  v4i a = reinterpret_cast<const v4i*>(s_a)[lane];
+  v4i b;
+  for (int v = 0; v < 4; ++v) {
+    uint32_t wb = 0;
+    for (int c = 0; c < 4; ++c) wb |= (uint32_t)(uint8_t)B[(16 * g + 4 * c + v) * 16 + l16] << (8 * c);
+    b[v] = (int)wb;
+  }
+  v4i d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, v4i{0, 0, 0, 0}, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) D[(4 * g + r) * 16 + l16] = d[r];
+}
+
+void probe_mfma_i8(slfmm_ctx* ctx, const int8_t* hA, const int8_t* hB, int32_t* hD) {
+  hipStream_t s = ctx->stream;
+  // A (16 x 64, row-major) -> one fragment chunk: lane l, byte 4v+c = A[l%16][16(l/16) + 4c + v]
+  std::vector<int8_t> frag(1024);
+  for (int l = 0; l < 64; ++l)
+    for (int v = 0; v < 4; ++v)
+      for (int c = 0; c < 4; ++c) frag[l * 16 + 4 * v + c] = hA[(l & 15) * 64 + 16 * (l >> 4) + 4 * c + v];
+  int8_t* dA = ctx->buf<int8_t>("probe_A", 1024);
+  int8_t* dB = ctx->buf<int8_t>("probe_B", 1024);
+  int* dD = ctx->buf<int>("probe_D", 256);
+  SLFMM_HIP(hipMemcpyAsync(dA, frag.data(), 1024, hipMemcpyHostToDevice, s));
+  SLFMM_HIP(hipMemcpyAsync(dB, hB, 1024, hipMemcpyHostToDevice, s));
+  k_probe_mfma_i8<<<1, 64, 0, s>>>(dA, dB, dD);
+  SLFMM_HIP(hipGetLastError());
+  SLFMM_HIP(hipMemcpyAsync(hD, dD, 1024, hipMemcpyDeviceToHost, s));
+  SLFMM_HIP(hipStreamSynchronize(s));   // frag is a host buffer read by the async copy above
+}
+
 }  // namespace slfmm

@@ -1,0 +1,97 @@
+// Per-SNP association epilogue shared by the FP64 and the mixed-precision
+// genotype-stream kernels (included by assoc.hip and assoc_i8.hip).
+#pragma once
+#include "internal.h"
+
+namespace slfmm {
+
+struct EpiArgs {
+  uint64_t n, p;
+  int d, K, m, fallback, norm;
+  double df;
+  const double* plan;
+  const int32_t* n_miss;
+  const int32_t* sum_g;
+  const int32_t* n_hom;
+  double* beta;   // p x d
+  double* tstat;  // p x d
+  double* r2;     // p x 3
+};
+
+// compute_scale (src/bed.rs:382-395); contraction-free in this file too.
+__device__ __forceinline__ double epi_scale(double mean, int norm) {
+  double pp = __dmul_rn(mean, 0.5);
+  double twopq = __dmul_rn(__dmul_rn(2.0, pp), __dsub_rn(1.0, pp));
+  if (norm == SLFMM_NORM_CENTER_ONLY) return twopq < 1e-20 ? 0.0 : 1.0;
+  double den = sqrt(twopq);
+  return den > 1e-10 ? 1.0 / den : 0.0;
+}
+
+template <int MAXM>
+__device__ void snp_epilogue(const EpiArgs& a, const double* sPlan, uint64_t j, const double (&fu)[MAXM],
+                             int64_t nmiss, int64_t sg, int64_t nh) {
+  const int d = a.d, m = a.m;
+  const int64_t nobs = (int64_t)a.n - nmiss;
+  const double mean = nobs > 0 ? (double)sg / (double)nobs : 0.0;
+  const double sc = epi_scale(mean, a.norm);
+  const int64_t sg2 = sg + 2 * nh;
+  const double tss = nobs > 0 ? sc * sc * ((double)(nobs * sg2 - sg * sg) / (double)nobs) : 0.0;
+  double fy[MAXM];
+#pragma unroll
+  for (int r = 0; r < MAXM; ++r) fy[r] = r < m ? sc * fu[r] : 0.0;
+  const double* P = sPlan;
+  const double* Vd = P + d * m;
+  const double* Rm = Vd + d;
+  const double* Fm = Rm + d * m;
+  const double* L = Fm + d * m;
+  double proj = 0.0;
+  if (!a.fallback) {
+#pragma unroll
+    for (int r = 0; r < MAXM; ++r) proj += fy[r] * fy[r];
+  } else {
+    for (int c = 0; c < m; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < MAXM; ++r) if (r < m) s += L[r * m + c] * fy[r];
+      proj += s * s;
+    }
+  }
+  double rss = tss - proj;
+  if (rss < 0.0) rss = 0.0;
+  const double sigma2 = rss / a.df;
+  double sscov = 0.0;
+  for (int jj = 0; jj < d; ++jj) {
+    double coef = 0.0, b = 0.0, fc = 0.0;
+#pragma unroll
+    for (int r = 0; r < MAXM; ++r)
+      if (r < m) {
+        coef += P[jj * m + r] * fy[r];
+        b += Rm[jj * m + r] * fy[r];
+        fc += Fm[jj * m + r] * fy[r];
+      }
+    sscov += fc * fc;
+    // t_test (src/testing.rs:1036-1048)
+    double se = sqrt(sigma2 * Vd[jj]);
+    double tv = 0.0;
+    if (se >= 1e-300 && isfinite(se)) {
+      tv = coef / se;
+      if (!isfinite(tv)) tv = 0.0;
+    }
+    a.tstat[j * d + jj] = tv;
+    a.beta[j * d + jj] = b;
+  }
+  double r0 = 0.0, r1 = 0.0, r2v = 0.0;
+  if (tss > 1e-300) {                    // src/testing.rs:394-404
+    double sslat = tss - rss - sscov;
+    if (sslat < 0.0) sslat = 0.0;
+    r0 = sscov / tss;
+    r1 = sslat / tss;
+    r2v = rss / tss;
+  }
+  a.r2[j * 3 + 0] = r0;
+  a.r2[j * 3 + 1] = r1;
+  a.r2[j * 3 + 2] = r2v;
+}
+
+
+}  // namespace slfmm

@@ -9,7 +9,8 @@
 // Codes past the last sample and bytes past ceil(n/4) are zero (hom-ref) so
 // that whole-word arithmetic never needs a tail mask: a zero code adds
 // nothing to any sum, and counts of observed calls are n - n_missing.
-// 64 zero bytes follow the last row (kernels may read one word block past).
+// kSlack zero bytes follow the last row (kernels load whole 16-byte segments
+// of a sample block and may run past the row stride).
 #include "internal.h"
 
 namespace slfmm {
@@ -18,7 +19,7 @@ uint64_t geno_ld(uint64_t n) { return rup(cdiv(n, 4), 16); }
 
 void geno_alloc(slfmm_ctx* ctx, slfmm_geno* g) {
   g->ld = geno_ld(g->n);
-  size_t bytes = (size_t)g->p * g->ld + 64;
+  size_t bytes = (size_t)g->p * g->ld + kSlack;
   void* p = nullptr;
   if (hipMalloc(&p, bytes) != hipSuccess)
     throw Error(SLFMM_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes for genotypes failed");

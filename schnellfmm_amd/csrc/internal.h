@@ -36,6 +36,11 @@ constexpr int kBins = 100000;        // TsqHistogram bins (src/testing.rs:125)
 constexpr double kBinWidth = 0.001;  // (src/testing.rs:126)
 constexpr int kMaxRows = 64;         // max functionals per SNP handled by one launch row-block
 constexpr int kRowBlk = 32;          // rows per FP64 functional row-block
+// Zero bytes allocated after the last packed row: kernels load whole
+// 16-byte segments of a 256-sample block and may run past ld by < 128 bytes.
+constexpr uint64_t kSlack = 128;
+constexpr int kMaxM = 64;   // functionals per SNP (K + d, or K + oversampling)
+constexpr int kMaxD = 16;   // traits
 
 inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 inline uint64_t rup(uint64_t a, uint64_t b) { return cdiv(a, b) * b; }
@@ -131,6 +136,9 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
                const double* d_u, uint64_t k, double lambda, const slfmm_config& cfg,
                double* d_beta, double* d_t, double* d_r2, double* d_p, uint64_t* d_hist,
                double* d_gif_pt, double* d_gif);
+
+// ---- assoc_i8.hip: single-MFMA lane-layout probe (test-only entry point) ----
+void probe_mfma_i8(slfmm_ctx* ctx, const int8_t* hA, const int8_t* hB, int32_t* hD);
 
 // ---- rsvd.hip ---------------------------------------------------------------
 void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_t n_idx,

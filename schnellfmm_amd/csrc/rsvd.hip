@@ -17,8 +17,7 @@ void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t
                      const double* A, uint64_t lda, int nrb, const double* mu, double* F, uint64_t ldF,
                      const double* outscale);
 void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
-                     const double* d_B, uint64_t l, const int32_t* nmiss, const int32_t* sumg,
-                     const double* mu, const double* sc, double* Z, uint64_t ldZ);
+                     const double* d_B, uint64_t l, int norm, double* Z, uint64_t ldZ);
 void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* packedT, uint64_t ldt, uint64_t n, uint64_t p,
                         const double* Qz, uint64_t ldq, uint64_t l, const double* mu, const double* sc,
                         double* W, uint64_t ldw);
@@ -211,8 +210,8 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
       SLFMM_CHECK(h_idx[i] >= 0 && (uint64_t)h_idx[i] < g->p, SLFMM_E_ARG, "estimation SNP index out of range");
     int64_t* didx = ctx->buf<int64_t>("rs_idx", n_idx);
     SLFMM_HIP(hipMemcpyAsync(didx, h_idx, n_idx * 8, hipMemcpyHostToDevice, s));
-    uint8_t* Yg = ctx->buf<uint8_t>("rs_Y", n_idx * g->ld + 64);
-    SLFMM_HIP(hipMemsetAsync(Yg + n_idx * g->ld, 0, 64, s));
+    uint8_t* Yg = ctx->buf<uint8_t>("rs_Y", n_idx * g->ld + kSlack);
+    SLFMM_HIP(hipMemsetAsync(Yg + n_idx * g->ld, 0, kSlack, s));
     geno_gather(ctx, g, didx, n_idx, Yg);
     Y = Yg;
     pe = n_idx;
@@ -231,8 +230,8 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   uint64_t ldt = 0;
   if (mixed) {   // sample-major copy for the forward passes
     ldt = rup(cdiv(pe, 4), 16);
-    YT = ctx->buf<uint8_t>("rs_YT", n * ldt + 64);
-    SLFMM_HIP(hipMemsetAsync(YT, 0, n * ldt + 64, s));
+    YT = ctx->buf<uint8_t>("rs_YT", n * ldt + kSlack);
+    SLFMM_HIP(hipMemsetAsync(YT, 0, n * ldt + kSlack, s));
     launch_transpose(s, Y, g->ld, n, pe, YT, ldt);
   }
 
@@ -252,7 +251,7 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   auto project = [&]() {   // Z = Y^T B, scaled per SNP
     slfmm_timed tm(ctx, "rsvd_project");
     if (mixed) {
-      rsvd_project_i8(ctx, Y, g->ld, n, pe, B, l, nmiss, sumg, mu, sc, Z, lda);
+      rsvd_project_i8(ctx, Y, g->ld, n, pe, B, l, cfg.norm, Z, lda);
     } else {
       k_pad_rows<<<(unsigned)cdiv(npad * lda, 256), 256, 0, s>>>(B, l, n, (int)l, A, lda, npad);
       launch_func_f64(s, Y, g->ld, pe, npad, A, lda, nrb, mu, Z, lda, sc);

@@ -1,0 +1,116 @@
+"""GPU parity of the mixed-precision path (int8 MFMA on exact genotypes,
+multi-term int8 dense operands, exact int32 accumulation, FP64 epilogue)
+against the CPU oracle.  Tolerances: |delta log10 p| <= 1e-3 (the metric's
+bound); t within 2e-4 absolute; U_hat principal angles <= 1e-4."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from schnellfmm_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def S():
+    import schnellfmm_amd as S
+    return S
+
+
+def test_mfma_i8_lane_layout(S):
+    """One v_mfma_i32_16x16x64_i8 with the kernels' fragment mapping equals A @ B exactly."""
+    from schnellfmm_amd._lib import lib
+    rng = np.random.default_rng(0)
+    ctx = S.api.default_context()
+    for _ in range(3):
+        a = rng.integers(-127, 128, size=(16, 64), dtype=np.int8)
+        b = rng.integers(-127, 128, size=(64, 16), dtype=np.int8)
+        d = np.zeros((16, 16), np.int32)
+        ctx.check(lib().slfmm_debug_mfma_i8(ctx.h, a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p),
+                                            d.ctypes.data_as(C.c_void_p)))
+        assert np.array_equal(d, a.astype(np.int64) @ b.astype(np.int64))
+
+
+def _panel(n, p, seed, d=2, miss=0.02, n_causal=10, effect=0.8, **kw):
+    pan = synth.balding_nichols(n, p, n_pops=4, miss_rate=miss, d=d, n_causal=n_causal, effect=effect,
+                                seed=seed, **kw)
+    return pan, O.encode_bed(pan.g), O.bytes_per_snp(n)
+
+
+@pytest.mark.parametrize("n,p,K,d,norm,miss", [
+    (300, 3000, 5, 2, 1, 0.02),
+    (257, 1037, 4, 1, 0, 0.10),     # ragged sample and SNP tails, heavy missingness
+    (1100, 2000, 12, 1, 1, 0.05),
+    (640, 1500, 20, 4, 1, 0.02),    # K + d = 24 (headline shape)
+])
+def test_i8_association_vs_oracle(S, n, p, K, d, norm, miss):
+    pan, packed, bps = _panel(n, p, seed=n + p, d=d, miss=miss)
+    xs = O.center_covariates(pan.x)
+    pre = O.precompute(xs, 1e-5)
+    omega = np.random.default_rng(1).standard_normal((n, K + 10))
+    u = O.estimate_factors_streaming(packed, bps, n, np.arange(p), pre, K, 10, 2, 500, norm, O.Impute(),
+                                     omega=omega)
+    ref = O.test_associations_fused(packed, bps, n, p, xs, u, pre, K, 500, norm, O.Impute())
+    g = S.BedFile.from_packed(packed, n, p)
+    got = S.test_associations_fused(g, xs, u, S.Lfmm2Config(k=K, norm=norm, precision=S.PREC_I8))
+    assert np.max(np.abs(got.tstat - ref.t)) < 2e-4
+    assert np.allclose(got.beta, ref.beta, rtol=1e-3, atol=1e-4 * np.max(np.abs(ref.beta)))
+    assert np.max(np.abs(got.r2 - ref.r2)) < 1e-4
+    dlp = np.abs(O.log10_pvalue(got.tstat, got.gif_per_trait) - O.log10_pvalue(ref.t, ref.gif_per_trait))
+    assert np.max(dlp) <= 1e-3
+
+
+def test_i8_association_edge_snps(S):
+    n, p = 130, 9
+    g = np.random.default_rng(2).integers(0, 3, size=(n, p)).astype(np.uint8)
+    g[:, 1] = 2
+    g[:, 2] = 255
+    g[:, 3] = 0
+    g[::2, 4] = 255
+    g[:, 5] = 1
+    g[0, 5] = 255
+    packed = O.encode_bed(g)
+    rng = np.random.default_rng(1)
+    xs = O.center_covariates(rng.standard_normal((n, 2)))
+    u = rng.standard_normal((n, 3))
+    pre = O.precompute(xs, 1e-5)
+    for norm in (0, 1):
+        ref = O.test_associations_fused(packed, O.bytes_per_snp(n), n, p, xs, u, pre, 3, 4, norm, O.Impute())
+        G = S.BedFile.from_packed(packed, n, p)
+        got = S.test_associations_fused(G, xs, u, S.Lfmm2Config(k=3, norm=norm, precision=S.PREC_I8))
+        assert np.max(np.abs(got.tstat - ref.t)) < 2e-4
+        assert np.all(got.tstat[[1, 2, 3, 5]] == 0.0)
+        assert np.max(np.abs(got.r2 - ref.r2)) < 1e-4
+
+
+def test_i8_rsvd_vs_oracle(S):
+    n, p, K, os_ = 400, 3000, 5, 10
+    pan, packed, bps = _panel(n, p, seed=31, d=2)
+    xs = O.center_covariates(pan.x)
+    pre = O.precompute(xs, 1e-5)
+    b0 = pre.m.T @ np.random.default_rng(5).standard_normal((n, K + os_))
+    for norm, rate in ((1, None), (0, 0.5)):
+        idx = np.arange(p) if rate is None else O.subset_indices(rate, p)
+        ref = O.estimate_factors_streaming(packed, bps, n, idx, pre, K, os_, 2, 700, norm, O.Impute(), b0=b0)
+        g = S.BedFile.from_packed(packed, n, p)
+        got = S.estimate_factors_streaming(g, rate, xs, S.Lfmm2Config(k=K, oversampling=os_, norm=norm,
+                                                                      precision=S.PREC_I8), b0=b0)
+        assert np.max(O.principal_angles(got, ref)) < 1e-4
+
+
+def test_i8_fit_end_to_end_log10p_bound(S):
+    n, p, K = 700, 6000, 8
+    pan, packed, bps = _panel(n, p, seed=41, d=2, n_causal=30, effect=1.2)
+    xs = O.center_covariates(pan.x)
+    pre = O.precompute(xs, 1e-5)
+    b0 = pre.m.T @ np.random.default_rng(9).standard_normal((n, K + 10))
+    cfg_o = O.Config(k=K, norm=O.EIGENSTRAT, chunk_size=1000, n_power_iter=2)
+    u_ref, ref, _, _ = O.fit_lfmm2(packed, bps, n, p, pan.x, cfg_o, O.subset_indices(0.5, p), b0=b0)
+    g = S.BedFile.from_packed(packed, n, p)
+    got = S.fit_lfmm2(g, 0.5, g, pan.x, S.Lfmm2Config(k=K, norm=S.EIGENSTRAT, precision=S.PREC_I8), b0=b0)
+    assert np.max(O.principal_angles(got.u_hat, u_ref)) < 1e-4
+    dlp = np.abs(O.log10_pvalue(got.tstat, got.gif_per_trait) - O.log10_pvalue(ref.t, ref.gif_per_trait))
+    assert np.max(dlp) <= 1e-3
+    assert np.max(np.abs(ref.t)) > 5       # the panel has real signal
