@@ -94,6 +94,9 @@ void slfmm_nmf_config_default(slfmm_nmf_config* cfg);
 const char* slfmm_version(void);
 
 int slfmm_create(int device, slfmm_ctx** out);
+/* Destroys a context.  Panels (slfmm_geno) created on it may be freed before
+ * or after: a context with live panels is released by its last
+ * slfmm_geno_free.  No other call may use the context after this. */
 void slfmm_destroy(slfmm_ctx* ctx);
 const char* slfmm_last_error(const slfmm_ctx* ctx);
 int slfmm_synchronize(slfmm_ctx* ctx);

@@ -60,6 +60,28 @@ void slfmm_ctx::collect() {
   }
 }
 
+int* slfmm_ctx::check_slot(const std::string& warn, int code, const std::string& e) {
+  SLFMM_CHECK(checks.size() < (size_t)kMaxChecks, SLFMM_E_ARG, "too many solver status slots in one call");
+  if (!status) status = static_cast<int*>(scratch("__status", kMaxChecks * sizeof(int)));
+  if (checks.empty()) SLFMM_HIP(hipMemsetAsync(status, 0, kMaxChecks * sizeof(int), stream));
+  checks.push_back({warn, code, e});
+  return status + (checks.size() - 1);
+}
+
+std::vector<int> slfmm_ctx::finish_checks() {
+  std::vector<Check> cs;
+  cs.swap(checks);
+  std::vector<int> v(cs.size(), 0);
+  if (cs.empty()) return v;
+  SLFMM_HIP(hipMemcpyAsync(v.data(), status, cs.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
+  SLFMM_HIP(hipStreamSynchronize(stream));
+  for (size_t i = 0; i < cs.size(); ++i) {
+    if (v[i] == 1 && !cs[i].warn.empty()) fprintf(stderr, "%s\n", cs[i].warn.c_str());
+    if (v[i] >= 2 && cs[i].code != SLFMM_OK) throw Error(cs[i].code, cs[i].err);
+  }
+  return v;
+}
+
 slfmm_ctx::~slfmm_ctx() {
   for (auto& kv : timers)
     for (auto& ev : kv.second.pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
@@ -73,8 +95,12 @@ namespace {
 template <class F>
 int guarded(slfmm_ctx* ctx, F&& f) {
   try {
-    if (ctx) SLFMM_HIP(hipSetDevice(ctx->device));
+    if (ctx) {
+      SLFMM_HIP(hipSetDevice(ctx->device));
+      ctx->checks.clear();
+    }
     f();
+    if (ctx) ctx->finish_checks();
     if (ctx) ctx->err.clear();
     return SLFMM_OK;
   } catch (const Error& e) {
@@ -232,6 +258,10 @@ int slfmm_create(int device, slfmm_ctx** out) {
 void slfmm_destroy(slfmm_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
+  if (ctx->live_genos > 0) {   // panels still bound: the last slfmm_geno_free deletes it
+    ctx->closed = true;
+    return;
+  }
   delete ctx;
 }
 
@@ -291,20 +321,23 @@ int slfmm_geno_wrap_device(slfmm_ctx* ctx, void* d_packed, uint64_t n, uint64_t 
     SLFMM_CHECK(ld % 16 == 0 && ld >= cdiv(n, 4), SLFMM_E_ARG, "row stride must be a multiple of 16 >= ceil(n/4)");
     auto* g = new slfmm_geno();
     g->ctx = ctx; g->d = (uint8_t*)d_packed; g->n = n; g->p = n_snps; g->ld = ld; g->owned = false;
+    ctx->live_genos += 1;
     *out = g;
   });
 }
 
 void slfmm_geno_free(slfmm_geno* g) {
   if (!g) return;
+  slfmm_ctx* c = g->ctx;
   if (g->owned && g->d) {
-    if (g->ctx) {
-      (void)hipSetDevice(g->ctx->device);
-      (void)hipStreamSynchronize(g->ctx->stream);
+    if (c) {
+      (void)hipSetDevice(c->device);
+      (void)hipStreamSynchronize(c->stream);
     }
     (void)hipFree(g->d);
   }
   delete g;
+  if (c && --c->live_genos == 0 && c->closed) delete c;   // context destroyed before its last panel
 }
 
 int slfmm_geno_info(const slfmm_geno* g, uint64_t* n, uint64_t* p, uint64_t* ld) {
@@ -371,13 +404,15 @@ int slfmm_center_covariates(const double* x, uint64_t n, uint64_t d, int32_t sca
 int slfmm_safe_inv(slfmm_ctx* ctx, const double* a, uint64_t m, double* out, int32_t* regularized) {
   return guarded(ctx, [&] {
     SLFMM_CHECK(ctx && a && out, SLFMM_E_ARG, "NULL argument");
+    SLFMM_CHECK(m >= 1 && m <= 64, SLFMM_E_ARG, "safe_inv: order must be 1..64");
     double* da = to_dev(ctx, "api_inv_a", a, m * m);
     double* dout = ctx->buf<double>("api_inv_o", m * m);
-    int inf = safe_inv_dev(ctx, da, m, dout);
-    if (regularized) *regularized = inf == 1;
-    if (inf == 1) fprintf(stderr, "Warning: matrix is singular, adding epsilon diagonal regularization\n");
-    SLFMM_CHECK(inf != 2, SLFMM_E_SINGULAR, "inversion failed even with epsilon regularization");
+    safe_inv_dev(ctx, da, m, dout,
+                 ctx->check_slot("Warning: matrix is singular, adding epsilon diagonal regularization",
+                                 SLFMM_E_SINGULAR, "inversion failed even with epsilon regularization"));
     to_host(ctx, out, dout, m * m);
+    const std::vector<int> st = ctx->finish_checks();
+    if (regularized) *regularized = st[0] == 1;
     SLFMM_HIP(hipStreamSynchronize(ctx->stream));
   });
 }

@@ -92,107 +92,133 @@ void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t
 }
 
 // ---------------------------------------------------------------------------
-// Plan: small matrices of the per-SNP epilogue, computed by one thread (the
-// inputs are at most 65 x 65).
-//   layout of `plan`: P (d x m) | Vd (d) | Rm (d x m) | Fm (d x m) | L (m x m)
+// Plan: small matrices of the per-SNP epilogue (inputs at most 65 x 65),
+// one workgroup.  Layout of `plan` (plan_len(d, m) doubles):
+//   P (d x m) | Vd (d) | Rm (d x m) | Fm (d x m) | L (m x m) | fallback flag
+// The basis falls back to [U_c|X] itself (T = I, RSS by the quadratic form
+// L L^T = E^T Ginv E) when C^T C needed regularisation or when CholeskyQR
+// of [U_c|X] failed; both outcomes are read from device status words.
 // ---------------------------------------------------------------------------
-__global__ void k_plan(const double* __restrict__ Gfull, const double* __restrict__ Ginv,
-                       const double* __restrict__ UtUinv, const double* __restrict__ XtXinv,
-                       const double* __restrict__ ridge, const double* __restrict__ T, int d, int K,
-                       int fallback, double* plan, double* work, int* info) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ int plan_fallback(const int* ginv_st, const int* cq_st) {
+  return (*ginv_st != 0 || *cq_st >= 2) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(256) k_plan(const double* __restrict__ Gfull, const double* __restrict__ Ginv,
+                                              const double* __restrict__ UtUinv, const double* __restrict__ XtXinv,
+                                              const double* __restrict__ ridge, const double* __restrict__ T, int d,
+                                              int K, const int* ginv_st, const int* cq_st, double* plan,
+                                              double* work) {
+  __shared__ double sT[kMaxM * kMaxM];
+  __shared__ double sMid[kMaxD * kMaxM];
+  __shared__ double sD[kMaxD * kMaxM];
+  __shared__ double sLx[kMaxD * kMaxD];
+  const int t = threadIdx.x;
   const int m = K + d, mc = 1 + d + K;
+  const int fb = plan_fallback(ginv_st, cq_st);
+  for (int e = t; e < m * m; e += 256) sT[e] = fb ? ((e / m == e % m) ? 1.0 : 0.0) : T[e];
+  __syncthreads();
+  // E (mc x m): s_full = E f, basis order [U_c (K) | X (d)], s_basis = T^T f
+  auto E = [&](int c, int r) -> double {
+    if (c == 0) return 0.0;
+    if (c <= d) return sT[r * m + K + (c - 1)];
+    return sT[r * m + (c - 1 - d)];
+  };
   double* P = plan;
   double* Vd = P + d * m;
   double* Rm = Vd + d;
   double* Fm = Rm + d * m;
   double* L = Fm + d * m;
-  double* E = work;                 // mc x m
-  double* Mid = E + mc * m;         // d x K
-  double* D = Mid + d * K;          // d x m
-  double* Lx = D + d * m;           // d x d
-  double* S = Lx + d * d;           // m x m
-  // E: s_full = E f.  Basis order [U_c (K) | X (d)], s_basis = T^T f.
-  for (int r = 0; r < m; ++r) E[0 * m + r] = 0.0;
-  for (int j = 0; j < d; ++j)
-    for (int r = 0; r < m; ++r) E[(1 + j) * m + r] = T[r * m + (K + j)];
-  for (int u = 0; u < K; ++u)
-    for (int r = 0; r < m; ++r) E[(1 + d + u) * m + r] = T[r * m + u];
-  for (int j = 0; j < d; ++j) {
-    for (int r = 0; r < m; ++r) {
-      double s = 0.0;
-      for (int c = 0; c < mc; ++c) s += Ginv[(1 + j) * mc + c] * E[c * m + r];
-      P[j * m + r] = s;
-    }
-    Vd[j] = Ginv[(1 + j) * mc + (1 + j)];
+  double* flag = L + m * m;
+  for (int e = t; e < d * m; e += 256) {
+    const int j = e / m, r = e % m;
+    double s = 0.0;
+    for (int c = 0; c < mc; ++c) s += Ginv[(1 + j) * mc + c] * E(c, r);
+    P[e] = s;
   }
-  // Mid = X^T U (U^T U)^{-1};  D = E_X - Mid E_U;  Rm = ridge D
-  for (int j = 0; j < d; ++j)
-    for (int u = 0; u < K; ++u) {
-      double s = 0.0;
-      for (int v = 0; v < K; ++v) s += Gfull[(1 + j) * mc + (1 + d + v)] * UtUinv[v * K + u];
-      Mid[j * K + u] = s;
-    }
-  for (int j = 0; j < d; ++j)
-    for (int r = 0; r < m; ++r) {
-      double s = E[(1 + j) * m + r];
-      for (int u = 0; u < K; ++u) s -= Mid[j * K + u] * E[(1 + d + u) * m + r];
-      D[j * m + r] = s;
-    }
-  for (int j = 0; j < d; ++j)
-    for (int r = 0; r < m; ++r) {
-      double s = 0.0;
-      for (int i = 0; i < d; ++i) s += ridge[j * d + i] * D[i * m + r];
-      Rm[j * m + r] = s;
-    }
-  // Lx: Cholesky (lower) of (X^T X)^{-1};  Fm = Lx^T E_X
-  int bad = 0;
-  for (int j = 0; j < d; ++j) {
-    double s = XtXinv[j * d + j];
-    for (int k = 0; k < j; ++k) s -= Lx[j * d + k] * Lx[j * d + k];
-    if (!(s > 0.0)) { bad = 1; s = 0.0; }
-    double ljj = sqrt(s);
-    Lx[j * d + j] = ljj;
-    for (int i = j + 1; i < d; ++i) {
-      double v = XtXinv[i * d + j];
-      for (int k = 0; k < j; ++k) v -= Lx[i * d + k] * Lx[j * d + k];
-      Lx[i * d + j] = ljj > 0.0 ? v / ljj : 0.0;
-    }
-    for (int i = 0; i < j; ++i) Lx[i * d + j] = 0.0;
+  if (t < d) Vd[t] = Ginv[(1 + t) * mc + (1 + t)];
+  // Mid = X^T U (U^T U)^{-1}
+  for (int e = t; e < d * K; e += 256) {
+    const int j = e / K, u = e % K;
+    double s = 0.0;
+    for (int v = 0; v < K; ++v) s += Gfull[(1 + j) * mc + (1 + d + v)] * UtUinv[v * K + u];
+    sMid[e] = s;
   }
-  for (int c = 0; c < d; ++c)
-    for (int r = 0; r < m; ++r) {
-      double s = 0.0;
-      for (int i = 0; i < d; ++i) s += Lx[i * d + c] * E[(1 + i) * m + r];
-      Fm[c * m + r] = s;
+  __syncthreads();
+  // D = E_X - Mid E_U
+  for (int e = t; e < d * m; e += 256) {
+    const int j = e / m, r = e % m;
+    double s = E(1 + j, r);
+    for (int u = 0; u < K; ++u) s -= sMid[j * K + u] * E(1 + d + u, r);
+    sD[e] = s;
+  }
+  if (t == 0) {   // Lx: lower Cholesky factor of (X^T X)^{-1}
+    for (int j = 0; j < d; ++j) {
+      double s = XtXinv[j * d + j];
+      for (int k = 0; k < j; ++k) s -= sLx[j * d + k] * sLx[j * d + k];
+      const double ljj = s > 0.0 ? sqrt(s) : 0.0;
+      sLx[j * d + j] = ljj;
+      for (int i = j + 1; i < d; ++i) {
+        double v = XtXinv[i * d + j];
+        for (int k = 0; k < j; ++k) v -= sLx[i * d + k] * sLx[j * d + k];
+        sLx[i * d + j] = ljj > 0.0 ? v / ljj : 0.0;
+      }
+      for (int i = 0; i < j; ++i) sLx[i * d + j] = 0.0;
     }
-  // Fallback RSS quadratic form S = E^T Ginv E = L L^T
-  if (fallback) {
-    for (int a = 0; a < m; ++a)
-      for (int b = 0; b < m; ++b) {
-        double s = 0.0;
-        for (int c = 0; c < mc; ++c) {
-          double gc = 0.0;
-          for (int e = 0; e < mc; ++e) gc += Ginv[c * mc + e] * E[e * m + b];
-          s += E[c * m + a] * gc;
+  }
+  __syncthreads();
+  for (int e = t; e < d * m; e += 256) {
+    const int j = e / m, r = e % m;
+    double s = 0.0;
+    for (int i = 0; i < d; ++i) s += ridge[j * d + i] * sD[i * m + r];
+    Rm[e] = s;
+    double f = 0.0;   // Fm = Lx^T E_X
+    for (int i = 0; i < d; ++i) f += sLx[i * d + j] * E(1 + i, r);
+    Fm[e] = f;
+  }
+  if (!fb) {
+    for (int e = t; e < m * m; e += 256) L[e] = 0.0;
+  } else {
+    // S = E^T Ginv E = L L^T  (GE = Ginv E staged in work)
+    double* GE = work;          // mc x m
+    double* S = work + mc * m;  // m x m
+    for (int e = t; e < mc * m; e += 256) {
+      const int c = e / m, b = e % m;
+      double g = 0.0;
+      for (int q = 0; q < mc; ++q) g += Ginv[c * mc + q] * E(q, b);
+      GE[e] = g;
+    }
+    __syncthreads();
+    for (int e = t; e < m * m; e += 256) {
+      const int a = e / m, b = e % m;
+      double v = 0.0;
+      for (int c = 0; c < mc; ++c) v += E(c, a) * GE[c * m + b];
+      S[e] = v;
+    }
+    __syncthreads();
+    if (t == 0) {
+      for (int j = 0; j < m; ++j) {
+        double v = S[j * m + j];
+        for (int k = 0; k < j; ++k) v -= L[j * m + k] * L[j * m + k];
+        const double ljj = v > 0.0 ? sqrt(v) : 0.0;
+        L[j * m + j] = ljj;
+        for (int i = j + 1; i < m; ++i) {
+          double w = S[i * m + j];
+          for (int k = 0; k < j; ++k) w -= L[i * m + k] * L[j * m + k];
+          L[i * m + j] = ljj > 0.0 ? w / ljj : 0.0;
         }
-        S[a * m + b] = s;
+        for (int i = 0; i < j; ++i) L[i * m + j] = 0.0;
       }
-    for (int j = 0; j < m; ++j) {
-      double s = S[j * m + j];
-      for (int k = 0; k < j; ++k) s -= L[j * m + k] * L[j * m + k];
-      if (!(s > 0.0)) { bad |= 2; s = 0.0; }
-      double ljj = sqrt(s);
-      L[j * m + j] = ljj;
-      for (int i = j + 1; i < m; ++i) {
-        double v = S[i * m + j];
-        for (int k = 0; k < j; ++k) v -= L[i * m + k] * L[j * m + k];
-        L[i * m + j] = ljj > 0.0 ? v / ljj : 0.0;
-      }
-      for (int i = 0; i < j; ++i) L[i * m + j] = 0.0;
     }
   }
-  *info = bad;
+  if (t == 0) *flag = (double)fb;
+}
+
+// Q <- [U_c|X] when the plan falls back to it.
+__global__ void k_select_basis(const int* ginv_st, const int* cq_st, const double* __restrict__ cc,
+                               double* __restrict__ Q, uint64_t count) {
+  if (!plan_fallback(ginv_st, cq_st)) return;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (uint64_t)gridDim.x * blockDim.x)
+    Q[e] = cc[e];
 }
 
 // ---------------------------------------------------------------------------
@@ -201,8 +227,8 @@ __global__ void k_plan(const double* __restrict__ Gfull, const double* __restric
 template <int MAXM>
 __global__ void __launch_bounds__(128) k_epilogue(EpiArgs a, const double* __restrict__ F, uint64_t ldF) {
   extern __shared__ double sPlan[];
-  const int plan_len = a.d * a.m * 3 + a.d + (a.fallback ? a.m * a.m : 0);
-  for (int e = threadIdx.x; e < plan_len; e += blockDim.x) sPlan[e] = a.plan[e];
+  const int len = plan_len(a.d, a.m);
+  for (int e = threadIdx.x; e < len; e += blockDim.x) sPlan[e] = a.plan[e];
   __syncthreads();
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.p) return;
@@ -213,75 +239,125 @@ __global__ void __launch_bounds__(128) k_epilogue(EpiArgs a, const double* __res
 }
 
 // ---------------------------------------------------------------------------
-// t^2 histogram (src/testing.rs:136-149): LDS sub-histogram for bins below
-// 16384, global atomics beyond, integer adds only (order independent).
+// t^2 histogram (src/testing.rs:136-149) in two integer passes:
+//  k_hist_part: each workgroup counts a SNP range into LDS bins [0, LB) per
+//    trait (values beyond go straight to the global histogram by atomics,
+//    they are rare), then stores its LDS counts as a partial row;
+//  k_hist_sum:  sums the partial rows per bin and emits per-chunk totals
+//    (1024 bins per chunk) for the median search.
+// Integer adds only, so the counts do not depend on scheduling.
 // ---------------------------------------------------------------------------
-constexpr int kLdsBins = 16384;
+constexpr int kChunkBins = 1024;
+constexpr int kHistChunks = (kBins + kChunkBins - 1) / kChunkBins;
 
-__global__ void __launch_bounds__(256) k_hist(const double* __restrict__ tstat, uint64_t p, int d,
-                                              uint64_t snps_per_blk, unsigned long long* hist) {
-  __shared__ uint32_t h[kLdsBins];
-  const int trait = blockIdx.y;
-  for (int b = threadIdx.x; b < kLdsBins; b += 256) h[b] = 0;
+__global__ void __launch_bounds__(1024) k_hist_part(const double* __restrict__ tstat, uint64_t p, int d,
+                                                    uint64_t snps_per_blk, int LB, uint32_t* __restrict__ part,
+                                                    unsigned long long* __restrict__ hist) {
+  extern __shared__ uint32_t h[];   // d x LB
+  const int t = threadIdx.x;
+  for (int b = t; b < d * LB; b += 1024) h[b] = 0;
   __syncthreads();
-  const uint64_t j0 = (uint64_t)blockIdx.x * snps_per_blk, j1 = std::min(p, j0 + snps_per_blk);
-  unsigned long long* hg = hist + (uint64_t)trait * kBins;
-  for (uint64_t j = j0 + threadIdx.x; j < j1; j += 256) {
-    double tv = tstat[j * d + trait];
-    double tsq = tv * tv;
+  const uint64_t j0 = (uint64_t)blockIdx.x * snps_per_blk;
+  const uint64_t j1 = std::min(p, j0 + snps_per_blk);
+  for (uint64_t e = j0 * d + t; e < j1 * d; e += 1024) {
+    const double tv = tstat[e];
+    const double tsq = tv * tv;
     if (!isfinite(tsq)) continue;
-    double q = tsq / kBinWidth;
-    int bin = q >= (double)(kBins - 1) ? kBins - 1 : (int)q;
-    if (bin < kLdsBins) atomicAdd(&h[bin], 1u);
-    else atomicAdd(&hg[bin], 1ull);
+    const int tr = (int)(e % d);
+    const double q = tsq / kBinWidth;
+    const int bin = q >= (double)(kBins - 1) ? kBins - 1 : (int)q;
+    if (bin < LB) atomicAdd(&h[tr * LB + bin], 1u);
+    else atomicAdd(&hist[(uint64_t)tr * kBins + bin], 1ull);
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < kLdsBins; b += 256)
-    if (h[b]) atomicAdd(&hg[b], (unsigned long long)h[b]);
+  uint32_t* row = part + (uint64_t)blockIdx.x * d * LB;
+  for (int b = t; b < d * LB; b += 1024) row[b] = h[b];
+}
+
+__global__ void __launch_bounds__(256) k_hist_sum(const uint32_t* __restrict__ part, int nblk, int d, int LB,
+                                                  unsigned long long* __restrict__ hist,
+                                                  unsigned long long* __restrict__ chunk_tot) {
+  __shared__ unsigned long long red[256];
+  const int t = threadIdx.x, tr = blockIdx.y, ch = blockIdx.x;
+  unsigned long long local = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int b = ch * kChunkBins + i * 256 + t;
+    if (b >= kBins) continue;
+    unsigned long long v;
+    if (b < LB) {
+      v = 0;
+      for (int g = 0; g < nblk; ++g) v += part[((uint64_t)g * d + tr) * LB + b];
+      hist[(uint64_t)tr * kBins + b] = v;
+    } else {
+      v = hist[(uint64_t)tr * kBins + b];
+    }
+    local += v;
+  }
+  red[t] = local;
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if (t < s2) red[t] += red[t + s2];
+    __syncthreads();
+  }
+  if (t == 0) chunk_tot[tr * kHistChunks + ch] = red[0];
 }
 
 // Median walk + GIF (src/testing.rs:159-224), one workgroup for all traits.
 // For even N the reference ends on the first bin whose cumulative count
-// exceeds N/2, i.e. the bin of sorted position N/2.
-__global__ void __launch_bounds__(256) k_gif(const unsigned long long* __restrict__ hist, int d, double* gif_pt,
-                                             double* gif_avg) {
-  __shared__ unsigned long long seg[256];
-  __shared__ unsigned long long tot;
+// exceeds N/2, i.e. the bin of sorted position N/2; for odd N the bin of
+// position (N-1)/2.  The chunk holding a position is found from the chunk
+// totals, then the bin inside it by a workgroup scan.
+__global__ void __launch_bounds__(256) k_gif(const unsigned long long* __restrict__ hist,
+                                             const unsigned long long* __restrict__ chunk_tot, int d,
+                                             double* gif_pt, double* gif_avg) {
+  __shared__ unsigned long long cum[kHistChunks + 1];
+  __shared__ unsigned long long scan[256];
   __shared__ int fbin[2];
   __shared__ double raw[kMaxD];
   const int t = threadIdx.x;
-  const int per = (kBins + 255) / 256;
   for (int tr = 0; tr < d; ++tr) {
     const unsigned long long* h = hist + (uint64_t)tr * kBins;
-    unsigned long long s = 0;
-    const int b0 = t * per, b1 = min(kBins, b0 + per);
-    for (int b = b0; b < b1; ++b) s += h[b];
-    seg[t] = s;
-    __syncthreads();
     if (t == 0) {
       unsigned long long c = 0;
-      for (int i = 0; i < 256; ++i) { unsigned long long v = seg[i]; seg[i] = c; c += v; }
-      tot = c;
+      for (int i = 0; i < kHistChunks; ++i) { cum[i] = c; c += chunk_tot[tr * kHistChunks + i]; }
+      cum[kHistChunks] = c;
       fbin[0] = fbin[1] = -1;
     }
     __syncthreads();
-    const unsigned long long N = tot;
-    if (N > 0) {
-      const bool even = (N % 2) == 0;
-      const unsigned long long target = even ? N / 2 - 1 : N / 2;
-      unsigned long long c = seg[t];
-      for (int b = b0; b < b1; ++b) {
-        unsigned long long c2 = c + h[b];
-        if (c <= target && c2 > target) fbin[0] = b;
-        if (c <= target + 1 && c2 > target + 1) fbin[1] = b;
+    const unsigned long long N = cum[kHistChunks];
+    const bool even = (N % 2) == 0;
+    const int npos = N == 0 ? 0 : (even ? 2 : 1);
+    for (int q = 0; q < npos; ++q) {
+      const unsigned long long target = (even ? N / 2 - 1 : N / 2) + q;
+      int ch = 0;   // chunk whose cumulative range holds target (uniform)
+      while (!(cum[ch] <= target && target < cum[ch + 1])) ++ch;
+      unsigned long long v[4], s = 0;
+      const int b0 = ch * kChunkBins + 4 * t;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        v[i] = (b0 + i < kBins) ? h[b0 + i] : 0ull;
+        s += v[i];
+      }
+      scan[t] = s;
+      __syncthreads();
+      if (t == 0) {
+        unsigned long long c = cum[ch];
+        for (int i = 0; i < 256; ++i) { unsigned long long x = scan[i]; scan[i] = c; c += x; }
+      }
+      __syncthreads();
+      unsigned long long c = scan[t];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const unsigned long long c2 = c + v[i];
+        if (c <= target && c2 > target) fbin[q] = b0 + i;
         c = c2;
       }
+      __syncthreads();
     }
-    __syncthreads();
     if (t == 0) {
       double med = 0.0;
       if (N > 0) {
-        const bool even = (N % 2) == 0;
         const double w = kBinWidth;
         if (!even) med = ((double)fbin[0] + 0.5) * w;
         else med = (((double)fbin[0] + 0.5) * w + ((double)fbin[1] + 0.5) * w) / 2.0;
@@ -343,9 +419,9 @@ __global__ void k_pad_operand(const double* __restrict__ Q, uint64_t ldq, uint64
 }
 
 struct AssocPlan {
-  int d = 0, K = 0, m = 0, fallback = 0;
+  int d = 0, K = 0, m = 0;
   double* Q = nullptr;      // n x m orthonormal basis (or [U_c|X] itself)
-  double* plan = nullptr;   // P | Vd | Rm | Fm | L
+  double* plan = nullptr;   // P | Vd | Rm | Fm | L | fallback flag
 };
 
 // Builds the basis Q and the epilogue plan; xs, u, ridge are device pointers.
@@ -355,10 +431,10 @@ static AssocPlan make_plan(slfmm_ctx* ctx, uint64_t n, const double* d_xs, uint6
   AssocPlan pl;
   pl.d = (int)d; pl.K = (int)K; pl.m = (int)(K + d);
   const uint64_t m = K + d, mc = 1 + d + K;
-  SLFMM_CHECK(m <= (uint64_t)kMaxM && d <= (uint64_t)kMaxD, SLFMM_E_ARG,
-              "K + d must be <= 64 and d <= 16");
+  SLFMM_CHECK(m <= (uint64_t)kMaxM && mc <= (uint64_t)kMaxM && d <= (uint64_t)kMaxD, SLFMM_E_ARG,
+              "1 + d + K must be <= 64 and d <= 16");
   double* umean = ctx->buf<double>("as_umean", std::max<uint64_t>(K, 1));
-  if (K) launch_colmeans(s, d_u, K, n, K, umean);
+  if (K) colmeans(ctx, d_u, K, n, K, umean);
   double* cc = ctx->buf<double>("as_cc", n * m);
   double* cf = ctx->buf<double>("as_cf", n * mc);
   k_build_design<<<(unsigned)cdiv(n, 256), 256, 0, s>>>(d_xs, d_u, umean, n, (int)d, (int)K, cc, m, cf, mc);
@@ -366,43 +442,36 @@ static AssocPlan make_plan(slfmm_ctx* ctx, uint64_t n, const double* d_xs, uint6
   double* Gfull = ctx->buf<double>("as_Gfull", mc * mc);
   ts_gram(ctx, cf, mc, mc, cf, mc, mc, n, Gfull);
   double* Ginv = ctx->buf<double>("as_Ginv", mc * mc);
-  const int ginv_inf = safe_inv_dev(ctx, Gfull, mc, Ginv);
-  if (ginv_inf == 1) fprintf(stderr, "Warning: C^T C  where C = [1 | X | U_hat] is singular, adding epsilon diagonal regularization\n");
-  SLFMM_CHECK(ginv_inf != 2, SLFMM_E_SINGULAR, "C^T C inversion failed even with epsilon regularization");
-  int inf = 0;
+  int* st_ginv = ctx->check_slot(
+      "Warning: C^T C  where C = [1 | X | U_hat] is singular, adding epsilon diagonal regularization",
+      SLFMM_E_SINGULAR, "C^T C inversion failed even with epsilon regularization");
+  safe_inv_dev(ctx, Gfull, mc, Ginv, st_ginv);
   double* UtU = ctx->buf<double>("as_UtU", std::max<uint64_t>(K * K, 1));
   double* UtUinv = ctx->buf<double>("as_UtUinv", std::max<uint64_t>(K * K, 1));
   if (K) {
-    k_extract_block<<<(unsigned)cdiv(K * K, 256), 256, 0, s>>>(Gfull, (int)mc, (int)(1 + d), (int)(1 + d), (int)K, (int)K, UtU);
-    inf = safe_inv_dev(ctx, UtU, K, UtUinv);
-    if (inf == 1) fprintf(stderr, "Warning: U_hat^T U_hat is singular, adding epsilon diagonal regularization\n");
-    SLFMM_CHECK(inf != 2, SLFMM_E_SINGULAR, "U_hat^T U_hat inversion failed even with epsilon regularization");
+    k_extract_block<<<(unsigned)cdiv(K * K, 256), 256, 0, s>>>(Gfull, (int)mc, (int)(1 + d), (int)(1 + d), (int)K,
+                                                               (int)K, UtU);
+    safe_inv_dev(ctx, UtU, K, UtUinv,
+                 ctx->check_slot("Warning: U_hat^T U_hat is singular, adding epsilon diagonal regularization",
+                                 SLFMM_E_SINGULAR, "U_hat^T U_hat inversion failed even with epsilon regularization"));
   }
   double* XtX = ctx->buf<double>("as_XtX", d * d);
   double* XtXinv = ctx->buf<double>("as_XtXinv", d * d);
   k_extract_block<<<(unsigned)cdiv(d * d, 256), 256, 0, s>>>(Gfull, (int)mc, 1, 1, (int)d, (int)d, XtX);
-  inf = safe_inv_dev(ctx, XtX, d, XtXinv);
-  if (inf == 1) fprintf(stderr, "Warning: X^T X for sequential R^2 is singular, adding epsilon diagonal regularization\n");
-  SLFMM_CHECK(inf != 2, SLFMM_E_SINGULAR, "X^T X inversion failed even with epsilon regularization");
-  // Orthonormal basis of [U_c | X]; keep [U_c|X] if it is rank deficient or
-  // if C^T C needed regularisation (then the reference's own Gram form rules).
+  safe_inv_dev(ctx, XtX, d, XtXinv,
+               ctx->check_slot("Warning: X^T X for sequential R^2 is singular, adding epsilon diagonal regularization",
+                               SLFMM_E_SINGULAR, "X^T X inversion failed even with epsilon regularization"));
+  // Orthonormal basis of [U_c | X] = Q T (CholeskyQR2); the plan falls back
+  // to [U_c|X] itself if that fails or if C^T C needed regularisation.
   pl.Q = ctx->buf<double>("as_Q", n * m);
   double* T = ctx->buf<double>("as_T", m * m);
   SLFMM_HIP(hipMemcpyAsync(pl.Q, cc, n * m * 8, hipMemcpyDeviceToDevice, s));
-  bool ok = (ginv_inf == 0) && cholqr2(ctx, pl.Q, m, m, n, T);
-  if (!ok) {
-    pl.fallback = 1;
-    SLFMM_HIP(hipMemcpyAsync(pl.Q, cc, n * m * 8, hipMemcpyDeviceToDevice, s));
-    std::vector<double> I(m * m, 0.0);
-    for (uint64_t i = 0; i < m; ++i) I[i * m + i] = 1.0;
-    SLFMM_HIP(hipMemcpyAsync(T, I.data(), m * m * 8, hipMemcpyHostToDevice, s));
-    SLFMM_HIP(hipStreamSynchronize(s));
-  }
-  const uint64_t plan_len = d * m * 3 + d + m * m;
-  pl.plan = ctx->buf<double>("as_plan", plan_len);
-  double* work = ctx->buf<double>("as_work", mc * m + d * K + d * m + d * d + m * m + 16);
-  int* dinfo = ctx->buf<int>("plan_info", 1);
-  k_plan<<<1, 64, 0, s>>>(Gfull, Ginv, UtUinv, XtXinv, d_ridge, T, (int)d, (int)K, pl.fallback, pl.plan, work, dinfo);
+  int* st_cq = ctx->check_slot("", SLFMM_OK, "");
+  cholqr2(ctx, pl.Q, m, m, n, T, st_cq);
+  k_select_basis<<<(unsigned)std::min<uint64_t>(cdiv(n * m, 256), 2048), 256, 0, s>>>(st_ginv, st_cq, cc, pl.Q, n * m);
+  pl.plan = ctx->buf<double>("as_plan", plan_len((int)d, (int)m));
+  double* work = ctx->buf<double>("as_work", mc * m + m * m);
+  k_plan<<<1, 256, 0, s>>>(Gfull, Ginv, UtUinv, XtXinv, d_ridge, T, (int)d, (int)K, st_ginv, st_cq, pl.plan, work);
   SLFMM_HIP(hipGetLastError());
   return pl;
 }
@@ -420,18 +489,16 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
                   std::to_string(d) + "+" + std::to_string(k) + "=" + std::to_string(1 + d + k) +
                   ". Need n > 1 + d + K for valid t-tests. Reduce K or add more samples.");
   SLFMM_CHECK(lambda > 0.0, SLFMM_E_LAMBDA, "Ridge penalty λ must be positive");
+  SLFMM_CHECK(d >= 1 && d <= (uint64_t)kMaxD, SLFMM_E_ARG, "need 1 <= d <= 16 covariates");
   // ridge_inv = (X^T X + lambda I)^{-1}  (src/precompute.rs:73-78)
-  double* xtx = ctx->buf<double>("as_xtxr", d * d);
+  double* xtx = ctx->buf<double>("as_xtx", d * d);
+  double* xtxr = ctx->buf<double>("as_xtxr", d * d);
   ts_gram(ctx, d_xs, d, d, d_xs, d, d, n, xtx);
-  std::vector<double> h(d * d);
-  SLFMM_HIP(hipMemcpyAsync(h.data(), xtx, d * d * 8, hipMemcpyDeviceToHost, s));
-  SLFMM_HIP(hipStreamSynchronize(s));
-  for (uint64_t j = 0; j < d; ++j) h[j * d + j] += lambda;
-  SLFMM_HIP(hipMemcpyAsync(xtx, h.data(), d * d * 8, hipMemcpyHostToDevice, s));
+  add_diag(s, xtx, d, lambda, xtxr);
   double* ridge = ctx->buf<double>("as_ridge", d * d);
-  int inf = safe_inv_dev(ctx, xtx, d, ridge);
-  if (inf == 1) fprintf(stderr, "Warning: X^T X + λI (ridge) is singular, adding epsilon diagonal regularization\n");
-  SLFMM_CHECK(inf != 2, SLFMM_E_SINGULAR, "X^T X + λI (ridge) inversion failed even with epsilon regularization");
+  safe_inv_dev(ctx, xtxr, d, ridge,
+               ctx->check_slot("Warning: X^T X + λI (ridge) is singular, adding epsilon diagonal regularization",
+                               SLFMM_E_SINGULAR, "X^T X + λI (ridge) inversion failed even with epsilon regularization"));
 
   AssocPlan pl = make_plan(ctx, n, d_xs, d, d_u, k, ridge);
   const uint64_t m = pl.m;
@@ -439,12 +506,11 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
   int32_t* sumg = ctx->buf<int32_t>("as_sumg", p);
   int32_t* nhom = ctx->buf<int32_t>("as_nhom", p);
   EpiArgs ea;
-  ea.n = n; ea.p = p; ea.d = (int)d; ea.K = (int)k; ea.m = (int)m; ea.fallback = pl.fallback;
+  ea.n = n; ea.p = p; ea.d = (int)d; ea.K = (int)k; ea.m = (int)m;
   ea.norm = cfg.norm; ea.df = (double)(n - 1 - d - k); ea.plan = pl.plan;
   ea.n_miss = nmiss; ea.sum_g = sumg; ea.n_hom = nhom;
   ea.beta = d_beta; ea.tstat = d_t; ea.r2 = d_r2;
   if (cfg.precision == SLFMM_PREC_I8) {
-    slfmm_timed tm(ctx, "assoc_stream");
     launch_func_i8(ctx, g, pl.Q, m, ea, true);
   } else {
     launch_stats(s, g->d, g->ld, n, p, nmiss, sumg, nhom);
@@ -461,8 +527,7 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
       slfmm_timed tm(ctx, "assoc_stream");
       launch_func_f64(s, g->d, g->ld, p, npad, A, lda, nrb, mu, F, lda, nullptr);
     }
-    const int plan_len = (int)(d * m * 3 + d + (pl.fallback ? m * m : 0));
-    const size_t shm = plan_len * sizeof(double);
+    const size_t shm = plan_len((int)d, (int)m) * sizeof(double);
     if (m <= 32) k_epilogue<32><<<(unsigned)cdiv(p, 128), 128, shm, s>>>(ea, F, lda);
     else k_epilogue<64><<<(unsigned)cdiv(p, 128), 128, shm, s>>>(ea, F, lda);
     SLFMM_HIP(hipGetLastError());
@@ -470,13 +535,17 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
   // histogram, GIF, calibration
   unsigned long long* hist = d_hist ? (unsigned long long*)d_hist
                                     : ctx->buf<unsigned long long>("as_hist", d * (uint64_t)kBins);
+  unsigned long long* ctot = ctx->buf<unsigned long long>("as_hist_chunks", d * (uint64_t)kHistChunks);
   SLFMM_HIP(hipMemsetAsync(hist, 0, d * (uint64_t)kBins * 8, s));
-  const uint64_t spb = 1ull << 16;
-  if (p) {
-    k_hist<<<dim3((unsigned)cdiv(p, spb), (unsigned)d), 256, 0, s>>>(d_t, p, (int)d, spb, hist);
-    SLFMM_HIP(hipGetLastError());
-  }
-  k_gif<<<1, 256, 0, s>>>(hist, (int)d, d_gif_pt, d_gif);
+  const int LB = (int)std::min<uint64_t>(16384, 32768 / d);   // LDS bins per trait (128 KB)
+  const uint64_t nblk = std::max<uint64_t>(1, std::min<uint64_t>(256, cdiv(p, 4096)));
+  const uint64_t spb = std::max<uint64_t>(1, cdiv(p, nblk));
+  uint32_t* part = ctx->buf<uint32_t>("as_hist_part", nblk * d * LB);
+  SLFMM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_part),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)(d * LB * sizeof(uint32_t))));
+  k_hist_part<<<(unsigned)nblk, 1024, d * LB * sizeof(uint32_t), s>>>(d_t, p, (int)d, spb, LB, part, hist);
+  k_hist_sum<<<dim3(kHistChunks, (unsigned)d), 256, 0, s>>>(part, (int)nblk, (int)d, LB, hist, ctot);
+  k_gif<<<1, 256, 0, s>>>(hist, ctot, (int)d, d_gif_pt, d_gif);
   if (p) k_calibrate<<<(unsigned)cdiv(p * d, 256), 256, 0, s>>>(d_t, p, (int)d, d_gif_pt, d_p);
   SLFMM_HIP(hipGetLastError());
 }

@@ -23,6 +23,7 @@
 // and the dense operand is pre-permuted in memory into exactly this order
 // (one 16-byte fragment per lane), so the same K index meets on both sides.
 #include "internal.h"
+#include "colreduce.h"
 #include "epilogue.h"
 
 namespace slfmm {
@@ -46,30 +47,34 @@ __device__ __forceinline__ int quant_term(double q, int t) {
   return clamp127(rint((r1 - md) * 254.0));
 }
 
-// Column max |A[k][c]| over rows, one workgroup per column.
-__global__ void __launch_bounds__(256) k_colabsmax(const double* __restrict__ A, uint64_t lda, uint64_t rows,
-                                                   const double* __restrict__ rowscale, const double* __restrict__ rowmul,
-                                                   double* out) {
-  __shared__ double red[256];
-  const int c = blockIdx.x;
-  double m = 0.0;
-  for (uint64_t r = threadIdx.x; r < rows; r += 256) {
-    double v = A[r * lda + c];
-    if (rowscale) v *= rowscale[r];
-    if (rowmul) v *= rowmul[r];
-    m = fmax(m, fabs(v));
+// Row layout of a packed operand with m functionals: columns i < k3 carry
+// 2 int8 terms, columns i >= k3 carry 3, on the g0 side (rows 0 .. 16 NT0);
+// every column carries 2 terms on the miss side (rows 16 NT0 ..).
+struct RowMap {
+  int m, k3, NT0, NT1;
+  __host__ __device__ static RowMap make(int m, int k3) {
+    RowMap r{m, k3, 0, 0};
+    r.NT0 = (2 * k3 + 3 * (m - k3) + 15) / 16;
+    r.NT1 = (2 * m + 15) / 16;
+    return r;
   }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
+  // (column, term) of packed row `row`, column -1 for padding
+  __device__ int2 info(int row) const {
+    if (row < 16 * NT0) {
+      if (row < 2 * k3) return make_int2(row / 2, row % 2);
+      const int r = row - 2 * k3;
+      return r < 3 * (m - k3) ? make_int2(k3 + r / 3, r % 3) : make_int2(-1, 0);
+    }
+    const int r = row - 16 * NT0;
+    return r < 2 * m ? make_int2(r / 2, r % 2) : make_int2(-1, 0);
   }
-  if (threadIdx.x == 0) out[c] = red[0] > 0.0 ? red[0] / 127.0 : 1.0;
-}
+  __host__ __device__ int ng(int i) const { return i < k3 ? 2 : 3; }
+  __host__ __device__ int rg(int i, int t) const { return i < k3 ? 2 * i + t : 2 * k3 + 3 * (i - k3) + t; }
+  __host__ __device__ int rm(int i, int t) const { return 16 * NT0 + 2 * i + t; }
+};
 
-// Apack for the SNP-side kernels: rows = functional terms (rowinfo), K = samples.
-__global__ void k_pack_rows(const double* __restrict__ Q, uint64_t ldq, uint64_t n, const int2* __restrict__ rowinfo,
+// Apack for the SNP-side kernels: rows = functional terms, K = samples.
+__global__ void k_pack_rows(const double* __restrict__ Q, uint64_t ldq, uint64_t n, RowMap rmap,
                             const double* __restrict__ sc, int NT, int KS, uint64_t total, int8_t* __restrict__ out) {
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
     const int j = (int)(e & 15), lane = (int)((e >> 4) & 63);
@@ -80,28 +85,29 @@ __global__ void k_pack_rows(const double* __restrict__ Q, uint64_t ldq, uint64_t
     const uint64_t blk = ks / KS;
     const int v = j >> 2, c = j & 3, g = lane >> 4;
     const uint64_t sample = blk * 64 * KS + 16 * KS * g + 16 * s + 4 * c + v;
-    const int2 ri = rowinfo[16 * rt + (lane & 15)];
+    const int2 ri = rmap.info(16 * rt + (lane & 15));
     int val = 0;
     if (ri.x >= 0 && sample < n) val = quant_term(Q[sample * ldq + ri.x] / sc[ri.x], ri.y);
     out[e] = (int8_t)val;
   }
 }
 
-// O_t = sum_k term_t(Q[k][i]) over the n samples (exact integers).
-__global__ void __launch_bounds__(256) k_term_sums(const double* __restrict__ Q, uint64_t ldq, uint64_t n,
-                                                   const double* __restrict__ sc, int nterm, double* out) {
-  __shared__ long long red[256];
-  const int i = blockIdx.x, t = blockIdx.y;
-  long long s = 0;
-  for (uint64_t k = threadIdx.x; k < n; k += 256) s += quant_term(Q[k * ldq + i] / sc[i], t);
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
+// O_t = sum_k term_t(Q[k][i]) over the n samples (exact integers in FP64).
+struct FnTerms {
+  const double* Q; uint64_t ldq; const double* sc;
+  __device__ void operator()(uint64_t r, int c, double (&v)[3]) const {
+    const double q = Q[r * ldq + c] / sc[c];
+    v[0] = (double)quant_term(q, 0);
+    v[1] = (double)quant_term(q, 1);
+    v[2] = (double)quant_term(q, 2);
   }
-  if (threadIdx.x == 0) out[i * nterm + t] = (double)red[0];
-}
+};
+struct GStore3 {
+  double* out;
+  __device__ void operator()(int c, const double (&v)[3]) const {
+    out[c * 3 + 0] = v[0]; out[c * 3 + 1] = v[1]; out[c * 3 + 2] = v[2];
+  }
+};
 
 // Row recombination table of one packed operand (device copy).
 struct Recomb {
@@ -113,6 +119,20 @@ struct Recomb {
   double O[kMaxM][3];      // sum over samples of each term
 };
 
+__global__ void k_recomb(RowMap rmap, const double* __restrict__ sc, const double* __restrict__ O, Recomb* rc) {
+  const int i = threadIdx.x;
+  if (i == 0) rc->m = rmap.m;
+  if (i >= rmap.m) return;
+  rc->ng[i] = rmap.ng(i);
+  for (int t = 0; t < 3; ++t) {
+    rc->rg[i][t] = t < rmap.ng(i) ? rmap.rg(i, t) : -1;
+    rc->w[i][t] = sc[i] * (t == 0 ? 1.0 : (t == 1 ? 1.0 / 254.0 : 1.0 / 64516.0));
+    rc->O[i][t] = O[i * 3 + t];
+  }
+  rc->rm[i][0] = rmap.rm(i, 0);
+  rc->rm[i][1] = rmap.rm(i, 1);
+}
+
 struct Operand {
   int8_t* pack = nullptr;
   int NT0 = 0, NT1 = 0, KS = 4;
@@ -122,52 +142,30 @@ struct Operand {
 
 static int ks_for(int NT) { return NT <= 8 ? 4 : 2; }   // k-steps per staged block of the SNP-side kernels
 
-// Quantise Q (n x m, ldq) into the packed fragment layout; terms_g per column.
+// Quantise Q (n x m, ldq) into the packed fragment layout; columns from k3
+// on carry 3 terms on the g0 side.  Device work only (no host round trip).
 static Operand prepare_operand(slfmm_ctx* ctx, const char* tag, const double* Q, uint64_t ldq, uint64_t n, int m,
-                               const std::vector<int>& terms_g) {
+                               int k3) {
   hipStream_t s = ctx->stream;
+  SLFMM_CHECK(m >= 1 && m <= kMaxM && k3 >= 0 && k3 <= m, SLFMM_E_ARG, "operand shape");
   Operand op;
   std::string T(tag);
-  double* sc = ctx->buf<double>(T + "_sc", m);
-  k_colabsmax<<<m, 256, 0, s>>>(Q, ldq, n, nullptr, nullptr, sc);
-  // rows: g0 block (terms_g[i] rows per functional), then miss block (2 rows each)
-  std::vector<int2> rows;
-  Recomb h{};
-  h.m = m;
-  for (int i = 0; i < m; ++i) {
-    h.ng[i] = terms_g[i];
-    for (int t = 0; t < 3; ++t) h.rg[i][t] = -1;
-    for (int t = 0; t < terms_g[i]; ++t) { h.rg[i][t] = (int)rows.size(); rows.push_back(make_int2(i, t)); }
-  }
-  const int R0 = (int)rows.size();
-  op.NT0 = (int)cdiv(R0, 16);
-  rows.resize(16 * op.NT0, make_int2(-1, 0));
-  for (int i = 0; i < m; ++i)
-    for (int t = 0; t < 2; ++t) { h.rm[i][t] = (int)rows.size(); rows.push_back(make_int2(i, t)); }
-  op.NT1 = (int)cdiv(rows.size() - 16 * op.NT0, 16);
-  rows.resize(16 * (op.NT0 + op.NT1), make_int2(-1, 0));
+  const RowMap rmap = RowMap::make(m, k3);
+  op.NT0 = rmap.NT0;
+  op.NT1 = rmap.NT1;
   const int NT = op.NT0 + op.NT1;
   op.KS = ks_for(NT);
   op.nblk = cdiv(n, 64 * op.KS);
-  int2* drows = ctx->buf<int2>(T + "_rows", rows.size());
-  SLFMM_HIP(hipMemcpyAsync(drows, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+  double* sc = ctx->buf<double>(T + "_sc", m);
+  colabsmax_scale(ctx, Q, ldq, n, m, nullptr, nullptr, sc);
+  double* O = ctx->buf<double>(T + "_O", m * 3);
+  colreduce<3, kColSum>(ctx, FnTerms{Q, ldq, sc}, GStore3{O}, n, m);
+  op.rc = ctx->buf<Recomb>(T + "_rc", 1);
+  k_recomb<<<1, 64, 0, s>>>(rmap, sc, O, op.rc);
   const uint64_t total = op.nblk * op.KS * NT * 1024;
   op.pack = ctx->buf<int8_t>(T + "_pack", total);
-  k_pack_rows<<<(unsigned)std::min<uint64_t>(cdiv(total, 256), 16384), 256, 0, s>>>(Q, ldq, n, drows, sc, NT, op.KS,
+  k_pack_rows<<<(unsigned)std::min<uint64_t>(cdiv(total, 256), 16384), 256, 0, s>>>(Q, ldq, n, rmap, sc, NT, op.KS,
                                                                                   total, op.pack);
-  double* O = ctx->buf<double>(T + "_O", m * 3);
-  k_term_sums<<<dim3(m, 3), 256, 0, s>>>(Q, ldq, n, sc, 3, O);
-  std::vector<double> hsc(m), hO(m * 3);
-  SLFMM_HIP(hipMemcpyAsync(hsc.data(), sc, m * 8, hipMemcpyDeviceToHost, s));
-  SLFMM_HIP(hipMemcpyAsync(hO.data(), O, m * 3 * 8, hipMemcpyDeviceToHost, s));
-  SLFMM_HIP(hipStreamSynchronize(s));
-  for (int i = 0; i < m; ++i)
-    for (int t = 0; t < 3; ++t) {
-      h.w[i][t] = hsc[i] * (t == 0 ? 1.0 : (t == 1 ? 1.0 / 254.0 : 1.0 / 64516.0));
-      h.O[i][t] = hO[i * 3 + t];
-    }
-  op.rc = ctx->buf<Recomb>(T + "_rc", 1);
-  SLFMM_HIP(hipMemcpyAsync(op.rc, &h, sizeof(Recomb), hipMemcpyHostToDevice, s));
   SLFMM_HIP(hipGetLastError());
   return op;
 }
@@ -243,7 +241,7 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
   constexpr int CH = KS * NT;                              // 1 KB operand chunks per block
   constexpr int LDS_A = 2 * CH * 1024;
   constexpr int LDS_RED = 4 * NT * 1024;                   // one SNP tile of int32 sums per wave
-  constexpr int PLAN_MAX = MODE == 0 ? 3 * kMaxD * MAXM + kMaxD + MAXM * MAXM : 0;
+  constexpr int PLAN_MAX = MODE == 0 ? 3 * kMaxD * MAXM + kMaxD + MAXM * MAXM + 1 : 0;
   constexpr int LDS_E = LDS_RED + PLAN_MAX * 8;
   constexpr int LDS = LDS_A > LDS_E ? LDS_A : LDS_E;
   __shared__ __align__(16) unsigned char lds[LDS];
@@ -368,8 +366,8 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
   }
   if constexpr (MODE == 0) {
     double* plan = reinterpret_cast<double*>(lds + LDS_RED);
-    const int plan_len = a.ea.d * a.ea.m * 3 + a.ea.d + (a.ea.fallback ? a.ea.m * a.ea.m : 0);
-    for (int e = t; e < plan_len; e += 256) plan[e] = a.ea.plan[e];
+    const int len = plan_len(a.ea.d, a.ea.m);
+    for (int e = t; e < len; e += 256) plan[e] = a.ea.plan[e];
     __syncthreads();
     if (j < a.p) snp_epilogue<MAXM>(a.ea, plan, j, fy, nmiss, sg, nhom);
   }
@@ -618,8 +616,8 @@ __global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
   }
   if constexpr (MODE == 0) {
     double* plan = reinterpret_cast<double*>(A0);
-    const int plan_len = a.ea.d * a.ea.m * 3 + a.ea.d + (a.ea.fallback ? a.ea.m * a.ea.m : 0);
-    for (int e = t; e < plan_len; e += 512) plan[e] = a.ea.plan[e];
+    const int len = plan_len(a.ea.d, a.ea.m);
+    for (int e = t; e < len; e += 512) plan[e] = a.ea.plan[e];
     __syncthreads();
     if (j < a.p) snp_epilogue<MAXM>(a.ea, plan, j, fy, nmiss, sg, nhom);
   }
@@ -680,23 +678,21 @@ void launch_func_i8(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_Q, uint
   // Latent directions: 2 int8 terms; covariate directions (the t-statistic
   // numerators, last d columns of the basis): 3 terms while K + d <= 32.
   SLFMM_CHECK(m <= 32, SLFMM_E_ARG, "the int8 association path supports K + d <= 32 (use SLFMM_PREC_FP64)");
-  std::vector<int> terms(m, 2);
-  for (int i = ea.K; i < (int)m; ++i) terms[i] = 3;
-  Operand op = prepare_operand(ctx, "as_i8", d_Q, m, g->n, (int)m, terms);
-  const uint64_t plan_bytes = 8 * (ea.d * m * 3 + ea.d + (ea.fallback ? m * m : 0));
+  Operand op = prepare_operand(ctx, "as_i8", d_Q, m, g->n, (int)m, ea.K);
+  const uint64_t plan_bytes = 8 * (uint64_t)plan_len(ea.d, (int)m);
   SLFMM_CHECK(op.NT0 + op.NT1 > 8 || plan_bytes <= (uint64_t)4 * (op.NT0 + op.NT1) * 1024, SLFMM_E_ARG,
               "association plan does not fit the int8 kernel's LDS (use SLFMM_PREC_FP64)");
   GtArgs a{};
   a.packed = g->d; a.ld = g->ld; a.p = g->p; a.n = g->n; a.nblk = (int)op.nblk; a.m = (int)m; a.norm = ea.norm;
   a.A = reinterpret_cast<const v4i*>(op.pack); a.rc = op.rc; a.ea = ea;
+  slfmm_timed tm(ctx, "assoc_stream");
   launch_gt(ctx->stream, op, a, 0, (int)m);
 }
 
 // RSVD sketch / backward pass: Z[j][c] = s_j * (B^T y_j)_c for every SNP row.
 void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p, const double* d_B,
                      uint64_t l, int norm, double* Z, uint64_t ldZ) {
-  std::vector<int> terms(l, 2);
-  Operand op = prepare_operand(ctx, "rs_i8", d_B, l, n, (int)l, terms);
+  Operand op = prepare_operand(ctx, "rs_i8", d_B, l, n, (int)l, (int)l);
   GtArgs a{};
   a.packed = packed; a.ld = ld; a.p = p; a.n = n; a.nblk = (int)op.nblk; a.m = (int)l; a.norm = norm;
   a.A = reinterpret_cast<const v4i*>(op.pack); a.rc = op.rc; a.Z = Z; a.ldZ = ldZ;
@@ -735,26 +731,18 @@ __global__ void k_pack_cols(const double* __restrict__ Qz, uint64_t ldq, uint64_
   }
 }
 
-// vhat_c = sum_j mu_j * Qt^_jc (2-term value of Qt), one workgroup per column.
-__global__ void __launch_bounds__(256) k_vhat(const double* __restrict__ Qz, uint64_t ldq, uint64_t pe,
-                                              const double* __restrict__ ssnp, const double* __restrict__ mu,
-                                              const double* __restrict__ csc, double* vhat) {
-  __shared__ double red[256];
-  const int c = blockIdx.x;
-  double acc = 0.0;
-  for (uint64_t jj = threadIdx.x; jj < pe; jj += 256) {
-    double q = Qz[jj * ldq + c] * ssnp[jj] / csc[c];
-    double val = csc[c] * ((double)quant_term(q, 0) + (double)quant_term(q, 1) / 254.0);
-    acc += mu[jj] * val;
+// vhat_c = sum_j mu_j * Qt^_jc (the 2-term value of Qt), column reduction.
+struct FnVhat {
+  const double* Qz; uint64_t ldq; const double* ssnp; const double* mu; const double* csc;
+  __device__ void operator()(uint64_t r, int c, double (&v)[1]) const {
+    const double q = Qz[r * ldq + c] * ssnp[r] / csc[c];
+    v[0] = mu[r] * (csc[c] * ((double)quant_term(q, 0) + (double)quant_term(q, 1) / 254.0));
   }
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) vhat[c] = red[0];
-}
+};
+struct GStore1 {
+  double* out;
+  __device__ void operator()(int c, const double (&v)[1]) const { out[c] = v[0]; }
+};
 
 template <int NCB>
 __global__ void __launch_bounds__(256) k_fwd_i8(const uint8_t* __restrict__ YT, uint64_t ldt, uint64_t n, int blk0,
@@ -884,15 +872,14 @@ void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_
   SLFMM_CHECK(NCB >= 1 && NCB <= 2, SLFMM_E_ARG, "int8 forward pass supports K + oversampling <= 32");
   const int KS = 4, NTB = 4 * NCB;
   double* csc = ctx->buf<double>("fw_csc", 2 * l);
-  k_colabsmax<<<(unsigned)l, 256, 0, s>>>(Qz, ldq, pe, ssnp, nullptr, csc);
-  k_colabsmax<<<(unsigned)l, 256, 0, s>>>(Qz, ldq, pe, ssnp, mu, csc + l);
+  colabsmax_scale(ctx, Qz, ldq, pe, l, ssnp, mu, csc);   // csc[c]: Qt, csc[l + c]: mu Qt
   const uint64_t nblk = cdiv(pe, 64 * KS);
   const uint64_t total = nblk * KS * NTB * 1024;
   int8_t* Bp = ctx->buf<int8_t>("fw_pack", total);
   k_pack_cols<<<(unsigned)std::min<uint64_t>(cdiv(total, 256), 16384), 256, 0, s>>>(Qz, ldq, pe, (int)l, ssnp, mu,
                                                                                   csc, NCB, KS, total, Bp);
   double* vhat = ctx->buf<double>("fw_vhat", l);
-  k_vhat<<<(unsigned)l, 256, 0, s>>>(Qz, ldq, pe, ssnp, mu, csc, vhat);
+  colreduce<1, kColSum>(ctx, FnVhat{Qz, ldq, ssnp, mu, csc}, GStore1{vhat}, pe, (int)l);
   const uint64_t sblocks = cdiv(n, 256);
   uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(nblk, cdiv(1024, sblocks)));
   const uint64_t per = cdiv(nblk, nchunk);

@@ -1,9 +1,12 @@
-// Dense FP64 helpers that stay on the GPU: tall-skinny Gram / GEMM with
-// deterministic reductions, and one-workgroup solvers for the small
-// matrices of the pipeline (m <= 64): Cholesky (CholeskyQR2 of the sketch),
-// LU inverse with the reference's safe_inv fallback, and one-sided Jacobi
-// (small SVD / symmetric eigendecomposition).
+// Dense FP64 helpers that stay on the GPU: tall-skinny Gram / GEMM and
+// column reductions with deterministic reductions, and one-workgroup
+// solvers for the small matrices of the pipeline (m <= 64): Cholesky
+// (CholeskyQR2 of the sketch), LU inverse with the reference's safe_inv
+// fallback, and one-sided Jacobi (small SVD / symmetric eigendecomposition).
+// Nothing here synchronises with the host: solver outcomes go to device
+// status words that the caller reads once per API call.
 #include "internal.h"
+#include "colreduce.h"
 
 namespace slfmm {
 
@@ -20,10 +23,40 @@ void launch_fill(hipStream_t s, double* p, double v, uint64_t count) {
   SLFMM_HIP(hipGetLastError());
 }
 
+// ---- copies ----------------------------------------------------------------
+__global__ void k_copy2d(const double* __restrict__ src, uint64_t lds, double* __restrict__ dst, uint64_t ldd,
+                         uint64_t rows, uint64_t cols, const int* gate) {
+  if (gate && *gate != 1) return;
+  const uint64_t total = rows * cols;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x)
+    dst[(e / cols) * ldd + e % cols] = src[(e / cols) * lds + e % cols];
+}
+
+void copy2d(hipStream_t s, const double* src, uint64_t lds, double* dst, uint64_t ldd, uint64_t rows, uint64_t cols,
+            const int* gate) {
+  if (!rows || !cols) return;
+  k_copy2d<<<(unsigned)std::min<uint64_t>(cdiv(rows * cols, 256), 4096), 256, 0, s>>>(src, lds, dst, ldd, rows, cols,
+                                                                                       gate);
+  SLFMM_HIP(hipGetLastError());
+}
+
+__global__ void k_add_diag(const double* __restrict__ In, int m, double lambda, double* __restrict__ out) {
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x) out[e] = In[e] + ((e / m == e % m) ? lambda : 0.0);
+}
+
+void add_diag(hipStream_t s, const double* In, uint64_t m, double lambda, double* out) {
+  k_add_diag<<<1, 256, 0, s>>>(In, (int)m, lambda, out);
+  SLFMM_HIP(hipGetLastError());
+}
+
 // ---- tall-skinny Gram: out = A^T B ------------------------------------------
+// Pass 1: per-workgroup partial Gram of a row range (32-row tiles in LDS).
+// Pass 2: one wave per output entry sums the partials in a fixed order.
 __global__ void __launch_bounds__(256) k_gram_partial(const double* __restrict__ A, uint64_t lda, int a,
                                                       const double* __restrict__ B, uint64_t ldb, int b,
-                                                      uint64_t rows, uint64_t rpb, double* __restrict__ part) {
+                                                      uint64_t rows, uint64_t rpb, double* __restrict__ part,
+                                                      const int* gate) {
+  if (gate && *gate != 1) return;
   __shared__ double sa[32][kSm + 1];
   __shared__ double sb[32][kSm + 1];
   const int t = threadIdx.x;
@@ -60,25 +93,30 @@ __global__ void __launch_bounds__(256) k_gram_partial(const double* __restrict__
   }
 }
 
-__global__ void k_gram_reduce(const double* __restrict__ part, int nblk, int nout, double* out) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nout) return;
+__global__ void __launch_bounds__(256) k_gram_reduce(const double* __restrict__ part, int nblk, int nout,
+                                                     double* __restrict__ out, const int* gate) {
+  if (gate && *gate != 1) return;
+  const int lane = threadIdx.x & 63;
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (e >= nout) return;   // uniform per wave
   double s = 0.0;
-  for (int g = 0; g < nblk; ++g) s += part[(uint64_t)g * nout + e];
-  out[e] = s;
+  for (int g = lane; g < nblk; g += 64) s += part[(uint64_t)g * nout + e];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) out[e] = s;
 }
 
 void ts_gram(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t a, const double* B,
-             uint64_t ldb, uint64_t b, uint64_t rows, double* out) {
+             uint64_t ldb, uint64_t b, uint64_t rows, double* out, const int* gate) {
   SLFMM_CHECK(a <= (uint64_t)kSm && b <= (uint64_t)kSm, SLFMM_E_ARG, "ts_gram: too many columns");
   hipStream_t s = ctx->stream;
   if (rows == 0) { launch_fill(s, out, 0.0, a * b); return; }
-  uint64_t nblk = std::min<uint64_t>(cdiv(rows, 128), 2048);
+  uint64_t nblk = std::min<uint64_t>(cdiv(rows, 128), 1024);
   uint64_t rpb = cdiv(rows, nblk);
   nblk = cdiv(rows, rpb);
   double* part = ctx->buf<double>("gram_part", nblk * a * b);
-  k_gram_partial<<<(unsigned)nblk, 256, 0, s>>>(A, lda, (int)a, B, ldb, (int)b, rows, rpb, part);
-  k_gram_reduce<<<(unsigned)cdiv(a * b, 256), 256, 0, s>>>(part, (int)nblk, (int)(a * b), out);
+  k_gram_partial<<<(unsigned)nblk, 256, 0, s>>>(A, lda, (int)a, B, ldb, (int)b, rows, rpb, part, gate);
+  k_gram_reduce<<<(unsigned)cdiv(a * b, 4), 256, 0, s>>>(part, (int)nblk, (int)(a * b), out, gate);
   SLFMM_HIP(hipGetLastError());
 }
 
@@ -86,7 +124,8 @@ void ts_gram(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t a, const do
 __global__ void __launch_bounds__(256) k_ts_gemm(const double* __restrict__ A, uint64_t lda, int m,
                                                  const double* __restrict__ M, uint64_t ldm, int q,
                                                  uint64_t rows, double alpha, double beta, double* out,
-                                                 uint64_t ldo) {
+                                                 uint64_t ldo, const int* gate) {
+  if (gate && *gate != 1) return;
   __shared__ double sm[kSm * kSm];
   for (int e = threadIdx.x; e < m * q; e += 256) sm[e] = M[(uint64_t)(e / q) * ldm + (e % q)];
   __syncthreads();
@@ -105,135 +144,164 @@ __global__ void __launch_bounds__(256) k_ts_gemm(const double* __restrict__ A, u
 
 void ts_gemm(hipStream_t s, const double* A, uint64_t lda, uint64_t m, const double* M,
              uint64_t ldm, uint64_t q, uint64_t rows, double alpha, double beta, double* out,
-             uint64_t ldo) {
+             uint64_t ldo, const int* gate) {
   if (rows == 0 || q == 0) return;
   SLFMM_CHECK(m <= (uint64_t)kSm && q <= (uint64_t)kSm, SLFMM_E_ARG, "ts_gemm: too many columns");
   uint64_t blocks = std::min<uint64_t>(cdiv(rows * q, 256), 8192);
-  k_ts_gemm<<<(unsigned)blocks, 256, 0, s>>>(A, lda, (int)m, M, ldm, (int)q, rows, alpha, beta, out, ldo);
+  k_ts_gemm<<<(unsigned)blocks, 256, 0, s>>>(A, lda, (int)m, M, ldm, (int)q, rows, alpha, beta, out, ldo, gate);
   SLFMM_HIP(hipGetLastError());
 }
 
-// Column means, one thread per column, sequential (deterministic) sums.
-__global__ void k_colmeans(const double* A, uint64_t lda, uint64_t rows, int m, double* means) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m) return;
-  double s = 0.0;
-  for (uint64_t r = 0; r < rows; ++r) s += A[r * lda + c];
-  means[c] = rows ? s / (double)rows : 0.0;
+// ---- column reductions --------------------------------------------------------
+namespace {
+struct FnVal {
+  const double* A; uint64_t lda;
+  __device__ void operator()(uint64_t r, int c, double (&v)[1]) const { v[0] = A[r * lda + c]; }
+};
+struct GMean {
+  double* out; uint64_t rows;
+  __device__ void operator()(int c, const double (&v)[1]) const { out[c] = rows ? v[0] / (double)rows : 0.0; }
+};
+template <int NV>
+struct FnAbsScaled {
+  const double* A; uint64_t lda; const double* rs; const double* rm2;
+  __device__ void operator()(uint64_t r, int c, double (&v)[NV]) const {
+    double x = A[r * lda + c];
+    if (rs) x *= rs[r];
+    v[0] = fabs(x);
+    if constexpr (NV == 2) v[1] = fabs(x * rm2[r]);
+  }
+};
+template <int NV>
+struct GScale127 {
+  double* out; int m;
+  __device__ void operator()(int c, const double (&v)[NV]) const {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k * m + c] = v[k] > 0.0 ? v[k] / 127.0 : 1.0;
+  }
+};
+}  // namespace
+
+void colmeans(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t rows, uint64_t m, double* means) {
+  colreduce<1, kColSum>(ctx, FnVal{A, lda}, GMean{means, rows}, rows, (int)m);
 }
 
-void launch_colmeans(hipStream_t s, const double* A, uint64_t lda, uint64_t rows, uint64_t m, double* means) {
-  k_colmeans<<<1, 64, 0, s>>>(A, lda, rows, (int)m, means);
-  SLFMM_HIP(hipGetLastError());
+void colabsmax_scale(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t rows, uint64_t m, const double* rs,
+                     const double* rm2, double* out) {
+  if (rm2) colreduce<2, kColMax>(ctx, FnAbsScaled<2>{A, lda, rs, rm2}, GScale127<2>{out, (int)m}, rows, (int)m);
+  else colreduce<1, kColMax>(ctx, FnAbsScaled<1>{A, lda, rs, nullptr}, GScale127<1>{out, (int)m}, rows, (int)m);
 }
 
 // ---- one-workgroup Cholesky:  G + shift*I = R^T R,  outputs R and R^{-1} ---
-__global__ void __launch_bounds__(256) k_chol(const double* __restrict__ G, int m, double shift, double tol,
-                                              double* R, double* Rinv, int* info) {
+// Attempt 0 is plain Cholesky with a relative pivot test of 1e-13 against
+// the original diagonal.  If it breaks down and allow_shift is set, it is
+// redone on G + s I with s = 11 (rows m + m (m + 1)) u tr(G) (shifted
+// CholeskyQR).  R^{-1} is formed in place: its strictly upper part, stored
+// transposed, fills the strictly lower triangle of the factor.
+// Status: mode 0 writes 0 (plain), 1 (shifted) or 2 (failed); mode 1
+// writes only 2, on failure.
+__global__ void __launch_bounds__(256) k_chol(const double* __restrict__ G, int m, double rows, int allow_shift,
+                                              double* R, double* Rinv, int* st, int mode, const int* gate) {
+  if (gate && *gate != 1) return;
   extern __shared__ double A[];   // m*m
   __shared__ double diag0[kSm];
+  __shared__ double dinv[kSm];
   __shared__ int fail;
+  __shared__ double shift;
   const int t = threadIdx.x;
-  for (int e = t; e < m * m; e += 256) A[e] = G[e] + ((e / m == e % m) ? shift : 0.0);
-  if (t == 0) fail = 0;
-  __syncthreads();
-  if (t < m) diag0[t] = A[t * m + t];
-  __syncthreads();
-  for (int j = 0; j < m; ++j) {
+  if (t == 0) shift = 0.0;
+  int attempt = 0;
+  for (;; ++attempt) {
+    __syncthreads();
+    const double sh = shift;
+    for (int e = t; e < m * m; e += 256) A[e] = G[e] + ((e / m == e % m) ? sh : 0.0);
+    if (t == 0) fail = 0;
+    __syncthreads();
+    if (t < m) diag0[t] = A[t * m + t];
+    __syncthreads();
+    const double tol = attempt == 0 ? 1e-13 : 0.0;
+    for (int j = 0; j < m; ++j) {
+      if (t == 0) {
+        double dj = A[j * m + j];
+        if (!(dj > tol * diag0[j]) || !(dj < 1e308)) fail = j + 1;
+        else A[j * m + j] = sqrt(dj);
+      }
+      __syncthreads();
+      if (fail) break;
+      const double rjj = A[j * m + j];
+      for (int i = j + 1 + t; i < m; i += 256) A[j * m + i] /= rjj;
+      __syncthreads();
+      const int w = m - j - 1;
+      for (int e = t; e < w * w; e += 256) {
+        int i = j + 1 + e / w, l = j + 1 + e % w;
+        if (l >= i) A[i * m + l] -= A[j * m + i] * A[j * m + l];
+      }
+      __syncthreads();
+    }
+    if (!fail || attempt == 1 || !allow_shift) break;
     if (t == 0) {
-      double dj = A[j * m + j];
-      if (!(dj > tol * diag0[j]) || !(dj < 1e308)) fail = j + 1;
-      else A[j * m + j] = sqrt(dj);
+      double tr = 0.0;
+      for (int i = 0; i < m; ++i) tr += G[i * m + i];
+      shift = 11.0 * (rows * m + (double)m * (m + 1)) * 1.1102230246251565e-16 * tr;
     }
-    __syncthreads();
-    if (fail) break;
-    const double rjj = A[j * m + j];
-    for (int i = j + 1 + t; i < m; i += 256) A[j * m + i] /= rjj;
-    __syncthreads();
-    const int w = m - j - 1;
-    for (int e = t; e < w * w; e += 256) {
-      int i = j + 1 + e / w, l = j + 1 + e % w;
-      if (l >= i) A[i * m + l] -= A[j * m + i] * A[j * m + l];
-    }
-    __syncthreads();
   }
   if (fail) {
-    if (t == 0) *info = fail;
+    if (t == 0) *st = 2;
     return;
   }
-  if (t < m) {   // back substitution, column t of R^{-1} (written straight to Rinv)
+  if (t < m) dinv[t] = 1.0 / A[t * m + t];
+  __syncthreads();
+  if (t < m) {   // column c of R^{-1}: X[i][c] = -(sum_{i<k<=c} R[i][k] X[k][c]) / R[i][i]
     const int c = t;
-    for (int i = m - 1; i >= 0; --i) {
-      double s = (i == c) ? 1.0 : 0.0;
-      for (int k = i + 1; k <= c; ++k) s -= A[i * m + k] * Rinv[k * m + c];
-      Rinv[i * m + c] = (i <= c) ? s / A[i * m + i] : 0.0;
+    for (int i = c - 1; i >= 0; --i) {
+      double s = A[i * m + c] * dinv[c];
+      for (int k = i + 1; k < c; ++k) s += A[i * m + k] * A[c * m + k];
+      A[c * m + i] = -s * dinv[i];
     }
   }
   __syncthreads();
   for (int e = t; e < m * m; e += 256) {
     int i = e / m, l = e % m;
     R[e] = (l >= i) ? A[e] : 0.0;
+    Rinv[e] = (l > i) ? A[l * m + i] : (l == i ? dinv[i] : 0.0);
   }
-  if (t == 0) *info = 0;
+  if (t == 0 && mode == 0) *st = attempt;
 }
 
-// one chol call; returns host-visible info
-static int chol_dev(slfmm_ctx* ctx, const double* G, uint64_t m, double shift, double tol, double* R, double* Rinv) {
-  int* dinfo = ctx->buf<int>("chol_info", 1);
-  k_chol<<<1, 256, m * m * sizeof(double), ctx->stream>>>(G, (int)m, shift, tol, R, Rinv, dinfo);
+static void launch_chol(hipStream_t s, const double* G, uint64_t m, uint64_t rows, bool allow_shift, double* R,
+                        double* Rinv, int* st, int mode, const int* gate) {
+  k_chol<<<1, 256, m * m * sizeof(double), s>>>(G, (int)m, (double)rows, allow_shift ? 1 : 0, R, Rinv, st, mode,
+                                                gate);
   SLFMM_HIP(hipGetLastError());
-  int info = 0;
-  SLFMM_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  SLFMM_HIP(hipStreamSynchronize(ctx->stream));
-  return info;
 }
 
-bool cholqr2(slfmm_ctx* ctx, double* A, uint64_t lda, uint64_t m, uint64_t rows, double* d_R) {
-  SLFMM_CHECK(m <= (uint64_t)kSm, SLFMM_E_ARG, "cholqr2: too many columns");
+void cholqr2(slfmm_ctx* ctx, double* A, uint64_t lda, uint64_t m, uint64_t rows, double* d_R, int* st) {
+  SLFMM_CHECK(m >= 1 && m <= (uint64_t)kSm, SLFMM_E_ARG, "cholqr2: 1..64 columns");
   hipStream_t s = ctx->stream;
   double* G = ctx->buf<double>("cqr_G", m * m);
   double* R1 = ctx->buf<double>("cqr_R1", m * m);
   double* R1i = ctx->buf<double>("cqr_R1i", m * m);
   double* R2 = ctx->buf<double>("cqr_R2", m * m);
   double* R2i = ctx->buf<double>("cqr_R2i", m * m);
-  double* Racc = ctx->buf<double>("cqr_Racc", m * m);
   double* Rtmp = ctx->buf<double>("cqr_Rtmp", m * m);
+  double* Rout = d_R ? d_R : ctx->buf<double>("cqr_Racc", m * m);
   double* T = ctx->buf<double>("cqr_T", rows * m);
-  // pass 1 (shifted if plain Cholesky breaks down)
+  // pass 1 (shifted if plain Cholesky breaks down): A -> T
   ts_gram(ctx, A, lda, m, A, lda, m, rows, G);
-  int info = chol_dev(ctx, G, m, 0.0, 1e-13, R1, R1i);
-  int passes = 2;
-  if (info) {
-    // shifted CholeskyQR: G + s I with s ~ u * ||A||_F^2 * (rows*m), then two clean passes
-    std::vector<double> hG(m * m);
-    SLFMM_HIP(hipMemcpyAsync(hG.data(), G, m * m * 8, hipMemcpyDeviceToHost, s));
-    SLFMM_HIP(hipStreamSynchronize(s));
-    double tr = 0.0;
-    for (uint64_t i = 0; i < m; ++i) tr += hG[i * m + i];
-    double shift = 11.0 * (double)(rows * m + m * (m + 1)) * 1.1102230246251565e-16 * tr;
-    info = chol_dev(ctx, G, m, shift, 0.0, R1, R1i);
-    if (info) return false;
-    passes = 3;
-  }
+  launch_chol(s, G, m, rows, true, R1, R1i, st, 0, nullptr);
   ts_gemm(s, A, lda, m, R1i, m, m, rows, 1.0, 0.0, T, m);
-  SLFMM_HIP(hipMemcpyAsync(Racc, R1, m * m * 8, hipMemcpyDeviceToDevice, s));
-  for (int ps = 1; ps < passes; ++ps) {
-    ts_gram(ctx, T, m, m, T, m, m, rows, G);
-    info = chol_dev(ctx, G, m, 0.0, 1e-13, R2, R2i);
-    if (info) return false;
-    double* dst = (ps == passes - 1) ? nullptr : T;
-    if (dst) {
-      // T <- T R2^{-1}  (through A as a temporary)
-      ts_gemm(s, T, m, m, R2i, m, m, rows, 1.0, 0.0, A, lda);
-      SLFMM_HIP(hipMemcpy2DAsync(T, m * 8, A, lda * 8, m * 8, rows, hipMemcpyDeviceToDevice, s));
-    } else {
-      ts_gemm(s, T, m, m, R2i, m, m, rows, 1.0, 0.0, A, lda);
-    }
-    ts_gemm(s, R2, m, m, Racc, m, m, m, 1.0, 0.0, Rtmp, m);
-    SLFMM_HIP(hipMemcpyAsync(Racc, Rtmp, m * m * 8, hipMemcpyDeviceToDevice, s));
-  }
-  if (d_R) SLFMM_HIP(hipMemcpyAsync(d_R, Racc, m * m * 8, hipMemcpyDeviceToDevice, s));
-  return true;
+  // pass 2: T -> A
+  ts_gram(ctx, T, m, m, T, m, m, rows, G);
+  launch_chol(s, G, m, rows, false, R2, R2i, st, 1, nullptr);
+  ts_gemm(s, T, m, m, R2i, m, m, rows, 1.0, 0.0, A, lda);
+  ts_gemm(s, R2, m, m, R1, m, m, m, 1.0, 0.0, Rout, m);
+  // pass 3, only after a shifted first pass (*st == 1)
+  ts_gram(ctx, A, lda, m, A, lda, m, rows, G, st);
+  launch_chol(s, G, m, rows, false, R2, R2i, st, 1, st);
+  ts_gemm(s, A, lda, m, R2i, m, m, rows, 1.0, 0.0, T, m, st);
+  copy2d(s, T, m, A, lda, rows, m, st);
+  ts_gemm(s, R2, m, m, Rout, m, m, m, 1.0, 0.0, Rtmp, m, st);
+  copy2d(s, Rtmp, m, Rout, m, m, m, st);
 }
 
 // ---- one-workgroup LU inverse with the safe_inv fallback -------------------
@@ -302,15 +370,10 @@ __global__ void __launch_bounds__(256) k_safe_inv(const double* __restrict__ In,
   if (t == 0) *info = 2;
 }
 
-int safe_inv_dev(slfmm_ctx* ctx, const double* d_a, uint64_t m, double* d_out) {
+void safe_inv_dev(slfmm_ctx* ctx, const double* d_a, uint64_t m, double* d_out, int* d_st) {
   SLFMM_CHECK(m >= 1 && m <= (uint64_t)kSm, SLFMM_E_ARG, "safe_inv: order must be 1..64");
-  int* dinfo = ctx->buf<int>("inv_info", 1);
-  k_safe_inv<<<1, 256, m * m * sizeof(double), ctx->stream>>>(d_a, (int)m, d_out, dinfo);
+  k_safe_inv<<<1, 256, m * m * sizeof(double), ctx->stream>>>(d_a, (int)m, d_out, d_st);
   SLFMM_HIP(hipGetLastError());
-  int info = 0;
-  SLFMM_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  SLFMM_HIP(hipStreamSynchronize(ctx->stream));
-  return info;
 }
 
 // ---- one-workgroup one-sided Jacobi -----------------------------------------

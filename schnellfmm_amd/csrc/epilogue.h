@@ -5,9 +5,13 @@
 
 namespace slfmm {
 
+// Plan layout (doubles): P (d x m) | Vd (d) | Rm (d x m) | Fm (d x m) |
+// L (m x m) | fallback flag (1.0: RSS by the quadratic form L L^T).
+__host__ __device__ inline int plan_len(int d, int m) { return d * m * 3 + d + m * m + 1; }
+
 struct EpiArgs {
   uint64_t n, p;
-  int d, K, m, fallback, norm;
+  int d, K, m, norm;
   double df;
   const double* plan;
   const int32_t* n_miss;
@@ -44,8 +48,9 @@ __device__ void snp_epilogue(const EpiArgs& a, const double* sPlan, uint64_t j, 
   const double* Rm = Vd + d;
   const double* Fm = Rm + d * m;
   const double* L = Fm + d * m;
+  const bool fallback = L[m * m] != 0.0;
   double proj = 0.0;
-  if (!a.fallback) {
+  if (!fallback) {
 #pragma unroll
     for (int r = 0; r < MAXM; ++r) proj += fy[r] * fy[r];
   } else {

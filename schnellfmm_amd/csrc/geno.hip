@@ -26,6 +26,7 @@ void geno_alloc(slfmm_ctx* ctx, slfmm_geno* g) {
   g->d = (uint8_t*)p;
   g->owned = true;
   g->ctx = ctx;
+  ctx->live_genos += 1;
   SLFMM_HIP(hipMemsetAsync(g->d, 0, bytes, ctx->stream));
 }
 

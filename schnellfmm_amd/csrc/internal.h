@@ -51,6 +51,11 @@ inline uint64_t rup(uint64_t a, uint64_t b) { return cdiv(a, b) * b; }
 struct slfmm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Panels bound to this context that are still alive.  slfmm_destroy on a
+  // context with live panels only marks it closed; the last slfmm_geno_free
+  // deletes it, so the two may be called in either order.
+  int live_genos = 0;
+  bool closed = false;
   std::string err;
   // Named, grow-only device scratch buffers reused across calls.
   std::map<std::string, std::pair<void*, size_t>> pool;
@@ -66,6 +71,20 @@ struct slfmm_ctx {
   std::pair<hipEvent_t, hipEvent_t> tic(const std::string& name);
   void toc(const std::string& name, std::pair<hipEvent_t, hipEvent_t> ev);
   void collect();
+  // Device status words of the current API call.  Kernels that can meet a
+  // singular matrix write 0 (ok), 1 (regularised / shifted) or 2 (failed)
+  // into their slot; the host reads every slot once, after the call's work
+  // is queued (one synchronisation instead of one per small solve).
+  struct Check { std::string warn; int code; std::string err; };
+  static constexpr int kMaxChecks = 4096;
+  int* status = nullptr;
+  std::vector<Check> checks;
+  // warn is printed to stderr for value 1 (if non-empty); value 2 raises
+  // Error(code, err) unless code is SLFMM_OK.
+  int* check_slot(const std::string& warn, int code, const std::string& err);
+  // Reads the slots (synchronises the stream), prints warnings in order and
+  // throws the first error; returns the values.  Clears the slot list.
+  std::vector<int> finish_checks();
   ~slfmm_ctx();
 };
 
@@ -111,21 +130,32 @@ void launch_transpose(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_
                       uint8_t* out, uint64_t ldt);
 
 // ---- dense.hip: tall-skinny FP64 helpers and one-workgroup small solvers ---
+// Kernels taking `const int* gate` do nothing unless gate is null or *gate == 1.
 // out (a x b) = A^T B, A rows x a (lda), B rows x b (ldb); deterministic.
 void ts_gram(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t a, const double* B,
-             uint64_t ldb, uint64_t b, uint64_t rows, double* out);
+             uint64_t ldb, uint64_t b, uint64_t rows, double* out, const int* gate = nullptr);
 // out (rows x q, ldo) = alpha * A (rows x m, lda) * M (m x q, ldm) + beta * out
 void ts_gemm(hipStream_t s, const double* A, uint64_t lda, uint64_t m, const double* M,
              uint64_t ldm, uint64_t q, uint64_t rows, double alpha, double beta, double* out,
-             uint64_t ldo);
+             uint64_t ldo, const int* gate = nullptr);
 void launch_fill(hipStream_t s, double* p, double v, uint64_t count);
-void launch_colmeans(hipStream_t s, const double* A, uint64_t lda, uint64_t rows, uint64_t m,
-                     double* means);
-// In-place thin QR of A (rows x m, lda) by CholeskyQR2; R (m x m upper) to
-// d_R if non-null.  Returns false (and leaves A unchanged) if not full rank.
-bool cholqr2(slfmm_ctx* ctx, double* A, uint64_t lda, uint64_t m, uint64_t rows, double* d_R);
-// safe_inv on device (m <= 64): returns 0 ok, 1 regularised, 2 failed.
-int safe_inv_dev(slfmm_ctx* ctx, const double* d_a, uint64_t m, double* d_out);
+// dst (rows x cols, ldd) <- src (rows x cols, lds)
+void copy2d(hipStream_t s, const double* src, uint64_t lds, double* dst, uint64_t ldd, uint64_t rows,
+            uint64_t cols, const int* gate = nullptr);
+// Column reductions of a tall matrix (rows x m, lda), deterministic order.
+void colmeans(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t rows, uint64_t m, double* means);
+// out[c] = max_r |A[r][c] * rs[r] * rm[r]| / 127 (1 if zero); rs, rm optional.
+// With rm2 given, also out[m + c] for the row multiplier rs * rm2.
+void colabsmax_scale(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t rows, uint64_t m,
+                     const double* rs, const double* rm2, double* out);
+// In-place thin QR of A (rows x m, lda) by CholeskyQR2 (a shifted first
+// pass, then two clean ones, if plain Cholesky breaks down); R (m x m upper)
+// to d_R if non-null.  *d_st <- 0 ok, 1 ok after the shift, 2 not full rank.
+void cholqr2(slfmm_ctx* ctx, double* A, uint64_t lda, uint64_t m, uint64_t rows, double* d_R, int* d_st);
+// safe_inv on device (m <= 64): *d_st <- 0 ok, 1 regularised, 2 failed.
+void safe_inv_dev(slfmm_ctx* ctx, const double* d_a, uint64_t m, double* d_out, int* d_st);
+// out (m x m) <- In + lambda I
+void add_diag(hipStream_t s, const double* In, uint64_t m, double lambda, double* out);
 // Symmetric/one-sided Jacobi on an m x m matrix (m <= 64): columns of
 // A V orthogonal; outputs V (m x m) and column norms, sorted descending.
 void jacobi_svd_small(slfmm_ctx* ctx, const double* d_a, uint64_t m, double* d_v, double* d_sv);

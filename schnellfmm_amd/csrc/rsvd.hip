@@ -169,7 +169,7 @@ void precompute_dev(slfmm_ctx* ctx, const double* d_xs, uint64_t n, uint64_t d, 
               "Ridge penalty λ must be positive, got " + std::to_string(lambda) +
                   ". λ=0 makes the D_λ diagonal singular (d_λ[j] = 0/(0+σ²) = 0, d_λ_inv = ∞). "
                   "Use a small positive value like 1e-10.");
-  SLFMM_CHECK(d >= 1 && d <= 64, SLFMM_E_ARG, "need 1 <= d <= 64 covariates");
+  SLFMM_CHECK(d >= 1 && d <= (uint64_t)kMaxD, SLFMM_E_ARG, "need 1 <= d <= 16 covariates");
   hipStream_t s = ctx->stream;
   double* G = ctx->buf<double>("pc_G", d * d);
   ts_gram(ctx, d_xs, d, d, d_xs, d, d, n, G);
@@ -179,9 +179,9 @@ void precompute_dev(slfmm_ctx* ctx, const double* d_xs, uint64_t n, uint64_t d, 
   double* M = ctx->buf<double>("pc_M", d * d);
   k_prec_finish<<<1, 64, 0, s>>>(ev, V, (int)d, lambda, d_sigma, d_dlam, M, G);
   ts_gemm(s, d_xs, d, d, M, d, d, n, 1.0, 0.0, d_qd, d);
-  int inf = safe_inv_dev(ctx, G, d, d_ridge);
-  if (inf == 1) fprintf(stderr, "Warning: X^T X + λI (ridge) is singular, adding epsilon diagonal regularization\n");
-  SLFMM_CHECK(inf != 2, SLFMM_E_SINGULAR, "X^T X + λI (ridge) inversion failed even with epsilon regularization");
+  safe_inv_dev(ctx, G, d, d_ridge,
+               ctx->check_slot("Warning: X^T X + λI (ridge) is singular, adding epsilon diagonal regularization",
+                               SLFMM_E_SINGULAR, "X^T X + λI (ridge) inversion failed even with epsilon regularization"));
 }
 
 // ---- randomized SVD ---------------------------------------------------------------
@@ -258,7 +258,8 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
     }
   };
   auto orth_z = [&]() {
-    SLFMM_CHECK(cholqr2(ctx, Z, lda, l, pe, nullptr), SLFMM_E_NUMERIC, "QR of the RSVD sketch failed (rank deficient)");
+    cholqr2(ctx, Z, lda, l, pe, nullptr,
+            ctx->check_slot("", SLFMM_E_NUMERIC, "QR of the RSVD sketch failed (rank deficient)"));
   };
   auto forward = [&]() {   // W = N Y Q_z
     {
@@ -272,7 +273,7 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   orth_z();
   for (uint64_t it = 0; it < cfg.n_power_iter; ++it) {  // src/rsvd.rs:79-133
     forward();
-    SLFMM_CHECK(cholqr2(ctx, W, l, l, n, nullptr), SLFMM_E_NUMERIC, "QR of the power-iteration block failed");
+    cholqr2(ctx, W, l, l, n, nullptr, ctx->check_slot("", SLFMM_E_NUMERIC, "QR of the power-iteration block failed"));
     SLFMM_HIP(hipMemcpyAsync(B, W, n * l * 8, hipMemcpyDeviceToDevice, s));
     apply_N(ctx, Qd, d, dlam, B, l, l, n, false);
     project();
@@ -283,7 +284,7 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   double* Rt = ctx->buf<double>("rs_Rt", l * l);
   double* Vs = ctx->buf<double>("rs_V", l * l);
   double* sv = ctx->buf<double>("rs_sv", l);
-  SLFMM_CHECK(cholqr2(ctx, W, l, l, n, R), SLFMM_E_NUMERIC, "QR of the final projection failed");
+  cholqr2(ctx, W, l, l, n, R, ctx->check_slot("", SLFMM_E_NUMERIC, "QR of the final projection failed"));
   k_transpose_small<<<(unsigned)cdiv(l * l, 256), 256, 0, s>>>(R, (int)l, Rt);
   jacobi_svd_small(ctx, Rt, l, Vs, sv);                // left singular vectors of R
   ts_gemm(s, W, l, l, Vs, l, K, n, 1.0, 0.0, d_uhat, K);

@@ -1,7 +1,9 @@
 """GPU parity of the mixed-precision path (int8 MFMA on exact genotypes,
 multi-term int8 dense operands, exact int32 accumulation, FP64 epilogue)
 against the CPU oracle.  Tolerances: |delta log10 p| <= 1e-3 (the metric's
-bound); t within 2e-4 absolute; U_hat principal angles <= 1e-4."""
+bound); t within 2e-4 absolute; U_hat: separated directions within 2e-5 rad,
+captured variance within 1e-6 (noise-band directions are bounded by the
+quantisation error over the spectral gap, see the RSVD test)."""
 import ctypes as C
 
 import numpy as np
@@ -85,7 +87,44 @@ def test_i8_association_edge_snps(S):
         assert np.max(np.abs(got.r2 - ref.r2)) < 1e-4
 
 
+def test_i8_association_singular_design(S):
+    """Duplicated latent column (integer design, exactly singular C^T C): the
+    fallback basis [U_c|X] is quantised column by column, so the duplicate
+    columns stay identical and the quadratic-form RSS cancels the null
+    direction exactly."""
+    n, p = 300, 400
+    pan, packed, bps = _panel(n, p, seed=17, d=2)
+    rng = np.random.default_rng(3)
+    xs = rng.integers(-3, 4, size=(n, 2)).astype(np.float64)
+    xs[-1] -= xs.sum(axis=0)
+    u = rng.integers(-3, 4, size=(n, 3)).astype(np.float64)
+    u[:, 2] = u[:, 0]
+    pre = O.precompute(xs, 1e-5)
+    ref = O.test_associations_fused(packed, bps, n, p, xs, u, pre, 3, 100, 1, O.Impute())
+    g = S.BedFile.from_packed(packed, n, p)
+    got = S.test_associations_fused(g, xs, u, S.Lfmm2Config(k=3, norm=1, precision=S.PREC_I8))
+    assert np.max(np.abs(got.tstat - ref.t)) < 2e-4
+    assert np.max(np.abs(got.r2 - ref.r2)) < 1e-4
+
+
+def _captured(packed, bps, n, idx, norm, u):
+    """Variance of the standardised estimation panel captured by span(u)."""
+    y = O.decode_chunk(packed, bps, n, idx, norm)
+    q, _ = np.linalg.qr(u)
+    return float(np.sum((q.T @ y) ** 2))
+
+
 def test_i8_rsvd_vs_oracle(S):
+    """Mixed-precision RSVD against the oracle driven by the same sketch.
+
+    The panel has three axes of population structure (singular values ~240)
+    above a flat noise band (~71, gaps ~0.2); K = 5 asks for two directions
+    inside that band.  Those are defined only up to ~eps / gap: two FP64 runs
+    that differ only in the random sketch disagree on them by ~1.4 rad and on
+    the captured variance by ~1.6e-3.  The int8 operand error (2 terms, ~2e-5
+    relative) moves them by < 1e-3 rad (CPU emulation of the quantised
+    pipeline: 8e-5 and 3.3e-4), so the bounds are: separated directions
+    < 2e-5 rad, every direction < 1e-3 rad, captured variance within 1e-6."""
     n, p, K, os_ = 400, 3000, 5, 10
     pan, packed, bps = _panel(n, p, seed=31, d=2)
     xs = O.center_covariates(pan.x)
@@ -97,20 +136,29 @@ def test_i8_rsvd_vs_oracle(S):
         g = S.BedFile.from_packed(packed, n, p)
         got = S.estimate_factors_streaming(g, rate, xs, S.Lfmm2Config(k=K, oversampling=os_, norm=norm,
                                                                       precision=S.PREC_I8), b0=b0)
-        assert np.max(O.principal_angles(got, ref)) < 1e-4
+        ang = np.sort(O.principal_angles(got, ref))
+        assert np.max(ang[:3]) < 2e-5          # the three separated structure axes
+        assert np.max(ang) < 1e-3              # noise-band directions
+        e_got, e_ref = _captured(packed, bps, n, idx, norm, got), _captured(packed, bps, n, idx, norm, ref)
+        assert abs(e_got / e_ref - 1.0) < 1e-6
 
 
 def test_i8_fit_end_to_end_log10p_bound(S):
+    """Whole fit in mixed precision: |delta log10 p| <= 1e-3 against the
+    oracle (the headline accuracy bound), captured variance within 1e-6."""
     n, p, K = 700, 6000, 8
     pan, packed, bps = _panel(n, p, seed=41, d=2, n_causal=30, effect=1.2)
     xs = O.center_covariates(pan.x)
     pre = O.precompute(xs, 1e-5)
     b0 = pre.m.T @ np.random.default_rng(9).standard_normal((n, K + 10))
     cfg_o = O.Config(k=K, norm=O.EIGENSTRAT, chunk_size=1000, n_power_iter=2)
-    u_ref, ref, _, _ = O.fit_lfmm2(packed, bps, n, p, pan.x, cfg_o, O.subset_indices(0.5, p), b0=b0)
+    idx = O.subset_indices(0.5, p)
+    u_ref, ref, _, _ = O.fit_lfmm2(packed, bps, n, p, pan.x, cfg_o, idx, b0=b0)
     g = S.BedFile.from_packed(packed, n, p)
     got = S.fit_lfmm2(g, 0.5, g, pan.x, S.Lfmm2Config(k=K, norm=S.EIGENSTRAT, precision=S.PREC_I8), b0=b0)
-    assert np.max(O.principal_angles(got.u_hat, u_ref)) < 1e-4
+    e_got = _captured(packed, bps, n, idx, O.EIGENSTRAT, got.u_hat)
+    e_ref = _captured(packed, bps, n, idx, O.EIGENSTRAT, u_ref)
+    assert abs(e_got / e_ref - 1.0) < 1e-6
     dlp = np.abs(O.log10_pvalue(got.tstat, got.gif_per_trait) - O.log10_pvalue(ref.t, ref.gif_per_trait))
     assert np.max(dlp) <= 1e-3
     assert np.max(np.abs(ref.t)) > 5       # the panel has real signal

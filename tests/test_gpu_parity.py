@@ -150,6 +150,37 @@ def test_association_monomorphic_and_all_missing(S):
         assert np.all(got.tstat[[0, 1, 2, 4]] == 0.0)
 
 
+def _integer_singular_design(n, seed):
+    """Zero-mean integer covariates and integer factors with the last factor
+    a copy of the first: every entry of C^T C is an exact integer whatever
+    the summation order, so both sides meet an exactly zero pivot."""
+    rng = np.random.default_rng(seed)
+    xs = rng.integers(-3, 4, size=(n, 2)).astype(np.float64)
+    xs[-1] -= xs.sum(axis=0)
+    u = rng.integers(-3, 4, size=(n, 3)).astype(np.float64)
+    u[:, 2] = u[:, 0]
+    return xs, u
+
+
+def test_association_singular_design_falls_back(S, capfd):
+    """Duplicated latent column: C^T C and U^T U get the safe_inv epsilon
+    (src/testing.rs:996-1024) and the basis falls back to [U_c|X] with the
+    RSS quadratic form; the warnings go to stderr."""
+    n, p = 300, 400
+    pan, packed, bps = _panel(n=n, p=p, seed=17, d=2)
+    xs, u = _integer_singular_design(n, 3)
+    pre = O.precompute(xs, 1e-5)
+    ref = O.test_associations_fused(packed, bps, n, p, xs, u, pre, 3, 100, 1, O.Impute())
+    g = S.BedFile.from_packed(packed, n, p)
+    got = S.test_associations_fused(g, xs, u, S.Lfmm2Config(k=3, norm=1))
+    err = capfd.readouterr().err
+    assert "C^T C  where C = [1 | X | U_hat] is singular" in err
+    assert "U_hat^T U_hat is singular" in err
+    assert np.allclose(got.tstat, ref.t, rtol=1e-6, atol=1e-7)
+    assert np.allclose(got.r2, ref.r2, rtol=1e-6, atol=1e-7)
+    assert np.allclose(got.beta, ref.beta, rtol=1e-6, atol=1e-9)
+
+
 def test_rsvd_fp64_vs_oracle_injected_sketch(S):
     n, p, K, os_ = 250, 1500, 4, 10
     pan, packed, bps = _panel(n=n, p=p, seed=11, d=2)
