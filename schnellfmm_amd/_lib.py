@@ -9,7 +9,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libslfmm.so")
+# SLFMM_LIBRARY may name another build of the same library (e.g. one built
+# with different tuning macros); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get("SLFMM_LIBRARY") or os.path.join(HERE, "libslfmm.so")
 
 u64, i64, i32, u32, dbl = C.c_uint64, C.c_int64, C.c_int32, C.c_uint32, C.c_double
 vp = C.c_void_p

@@ -50,20 +50,24 @@ __device__ __forceinline__ int quant_term(double q, int t) {
 // Row layout of a packed operand with m functionals: columns i < k3 carry
 // 2 int8 terms, columns i >= k3 carry 3, on the g0 side (rows 0 .. 16 NT0);
 // every column carries 2 terms on the miss side (rows 16 NT0 ..).
+// The g0 block ends with one all-ones row, whose accumulator is the dosage
+// sum of the SNP over its observed calls.
 struct RowMap {
   int m, k3, NT0, NT1;
   __host__ __device__ static RowMap make(int m, int k3) {
     RowMap r{m, k3, 0, 0};
-    r.NT0 = (2 * k3 + 3 * (m - k3) + 15) / 16;
+    r.NT0 = (2 * k3 + 3 * (m - k3) + 1 + 15) / 16;
     r.NT1 = (2 * m + 15) / 16;
     return r;
   }
-  // (column, term) of packed row `row`, column -1 for padding
+  __host__ __device__ int ones() const { return 2 * k3 + 3 * (m - k3); }
+  // (column, term) of packed row `row`: column -1 for padding, -2 for the ones row
   __device__ int2 info(int row) const {
     if (row < 16 * NT0) {
       if (row < 2 * k3) return make_int2(row / 2, row % 2);
       const int r = row - 2 * k3;
-      return r < 3 * (m - k3) ? make_int2(k3 + r / 3, r % 3) : make_int2(-1, 0);
+      if (r < 3 * (m - k3)) return make_int2(k3 + r / 3, r % 3);
+      return row == ones() ? make_int2(-2, 0) : make_int2(-1, 0);
     }
     const int r = row - 16 * NT0;
     return r < 2 * m ? make_int2(r / 2, r % 2) : make_int2(-1, 0);
@@ -87,7 +91,10 @@ __global__ void k_pack_rows(const double* __restrict__ Q, uint64_t ldq, uint64_t
     const uint64_t sample = blk * 64 * KS + 16 * KS * g + 16 * s + 4 * c + v;
     const int2 ri = rmap.info(16 * rt + (lane & 15));
     int val = 0;
-    if (ri.x >= 0 && sample < n) val = quant_term(Q[sample * ldq + ri.x] / sc[ri.x], ri.y);
+    if (sample < n) {
+      if (ri.x >= 0) val = quant_term(Q[sample * ldq + ri.x] / sc[ri.x], ri.y);
+      else if (ri.x == -2) val = 1;
+    }
     out[e] = (int8_t)val;
   }
 }
@@ -112,25 +119,34 @@ struct GStore3 {
 // Row recombination table of one packed operand (device copy).
 struct Recomb {
   int m;
+  int ones;                // row of the all-ones operand row (dosage sum)
   int ng[kMaxM];           // terms on the g0 side (2 or 3)
   int rg[kMaxM][3];        // their rows
   int rm[kMaxM][2];        // rows of the two terms on the miss side
-  double w[kMaxM][3];      // sc_i / 254^t
-  double O[kMaxM][3];      // sum over samples of each term
+  // With X = sum_t G_t 254^(ng-1-t), Y = M_0 254 + M_1 (exact integers):
+  //   f_i = wg_i X + mu (wm_i Y - C_i),   C_i = sum_t sc_i 254^-t O_t
+  double wg[kMaxM];        // sc_i / 254^(ng-1)
+  double wm[kMaxM];        // sc_i / 254
+  double C[kMaxM];
 };
 
 __global__ void k_recomb(RowMap rmap, const double* __restrict__ sc, const double* __restrict__ O, Recomb* rc) {
   const int i = threadIdx.x;
-  if (i == 0) rc->m = rmap.m;
-  if (i >= rmap.m) return;
-  rc->ng[i] = rmap.ng(i);
-  for (int t = 0; t < 3; ++t) {
-    rc->rg[i][t] = t < rmap.ng(i) ? rmap.rg(i, t) : -1;
-    rc->w[i][t] = sc[i] * (t == 0 ? 1.0 : (t == 1 ? 1.0 / 254.0 : 1.0 / 64516.0));
-    rc->O[i][t] = O[i * 3 + t];
+  if (i == 0) {
+    rc->m = rmap.m;
+    rc->ones = rmap.ones();
   }
+  if (i >= rmap.m) return;
+  const int ng = rmap.ng(i);
+  rc->ng[i] = ng;
+  for (int t = 0; t < 3; ++t) rc->rg[i][t] = t < ng ? rmap.rg(i, t) : -1;
   rc->rm[i][0] = rmap.rm(i, 0);
   rc->rm[i][1] = rmap.rm(i, 1);
+  rc->wg[i] = sc[i] * (ng == 3 ? 1.0 / 64516.0 : 1.0 / 254.0);
+  rc->wm[i] = sc[i] / 254.0;
+  double c = O[i * 3 + 0] + O[i * 3 + 1] / 254.0;
+  if (ng == 3) c += O[i * 3 + 2] / 64516.0;
+  rc->C[i] = sc[i] * c;
 }
 
 struct Operand {
@@ -177,21 +193,25 @@ struct Frag {
   v4i g0, miss;
 };
 
-__device__ __forceinline__ Frag decode_word(uint32_t w, uint32_t& c_hom, uint32_t& c_miss, uint32_t& c_hi) {
-  const uint32_t lo = w & 0x55555555u, hi = (w >> 1) & 0x55555555u, hb = lo & hi;
-  const uint32_t g0w = hi + hb, mw = lo ^ hb;
-  c_hom += __popc(hb);
-  c_miss += __popc(mw);
-  c_hi += __popc(hi);
+// One word = 16 two-bit codes of one SNP.  Register v of a fragment takes
+// the codes at bit 8c + 2v (c = 0..3): one shift and mask leave one code per
+// byte, and v_perm_b32 maps each byte through a 4-entry table (selector
+// bytes 0..3 pick bytes of the second operand).  Counts: code 11 (hom-alt)
+// and code 01 (missing); the dosage sum comes from the all-ones operand row.
+constexpr uint32_t kPermG0 = 0x02010000u;     // 00->0, 01->0, 10->1, 11->2
+constexpr uint32_t kPermMiss = 0x00000100u;   // 01->1, else 0
+
+__device__ __forceinline__ Frag decode_word(uint32_t w, uint32_t& c_hom, uint32_t& c_miss) {
+  const uint32_t sh = w >> 1;
+  c_hom += __popc(w & sh & 0x55555555u);
+  c_miss += __popc(w & ~sh & 0x55555555u);
   Frag f;
-  f.g0[0] = (int)(g0w & 0x03030303u);
-  f.g0[1] = (int)((g0w >> 2) & 0x03030303u);
-  f.g0[2] = (int)((g0w >> 4) & 0x03030303u);
-  f.g0[3] = (int)((g0w >> 6) & 0x03030303u);
-  f.miss[0] = (int)(mw & 0x01010101u);
-  f.miss[1] = (int)((mw >> 2) & 0x01010101u);
-  f.miss[2] = (int)((mw >> 4) & 0x01010101u);
-  f.miss[3] = (int)((mw >> 6) & 0x01010101u);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const uint32_t cv = (w >> (2 * v)) & 0x03030303u;
+    f.g0[v] = (int)__builtin_amdgcn_perm(0u, kPermG0, cv);
+    f.miss[v] = (int)__builtin_amdgcn_perm(0u, kPermMiss, cv);
+  }
   return f;
 }
 
@@ -233,6 +253,64 @@ __device__ __forceinline__ void chunk_to_lds(const unsigned char* src_chunk, uns
   __builtin_amdgcn_global_load_lds((const void*)(src_chunk + 16 * lane), (lds_ptr_t)(dst_chunk), 16, 0, 0);
 }
 
+// ---------------------------------------------------------------------------
+// Epilogue of the SNP-side kernels.  After the block loop, lane (g, l16) of
+// a wave holds rows 16 rt + 4 g + r of SNP column l16 of every tile st.  Per
+// tile the wave stages its int32 row sums in `red` ([16 NT rows][16 SNPs]);
+// then every lane recombines the functionals i = g + 4 q of SNP
+// j = base + 16 st + l16 (all 64 lanes busy), and either stores them scaled
+// (MODE 1, RSVD sketch) or feeds the split per-SNP epilogue (MODE 0).
+// ---------------------------------------------------------------------------
+constexpr int kEpiNQ = 8;   // functionals per lane group (K + d <= 32)
+
+template <int ST, int NT, int MODE>
+__device__ __forceinline__ void gt_epilogue(const GtArgs& a, const v4i (&acc)[ST][NT], uint32_t (&c_hom)[ST],
+                                            uint32_t (&c_miss)[ST], int* red, const double* plan, uint64_t base,
+                                            int g, int l16) {
+#pragma unroll
+  for (int st = 0; st < ST; ++st) {
+    c_hom[st] += __shfl_xor(c_hom[st], 16); c_hom[st] += __shfl_xor(c_hom[st], 32);
+    c_miss[st] += __shfl_xor(c_miss[st], 16); c_miss[st] += __shfl_xor(c_miss[st], 32);
+  }
+  const Recomb* rc = a.rc;
+#pragma unroll
+  for (int st = 0; st < ST; ++st) {
+#pragma unroll
+    for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[(16 * rt + 4 * g + r) * 16 + l16] = acc[st][rt][r];
+    __syncthreads();
+    const uint64_t j = base + 16 * st + l16;
+    const bool valid = j < a.p;
+    const int64_t nmiss = c_miss[st], nhom = c_hom[st];
+    const int64_t nobs = (int64_t)a.n - nmiss;
+    const int64_t sg = red[rc->ones * 16 + l16];   // dosage sum (all-ones operand row)
+    const double mu = nobs > 0 ? (double)sg / (double)nobs : 0.0;
+    double fq[kEpiNQ];
+#pragma unroll
+    for (int q = 0; q < kEpiNQ; ++q) {
+      const int i = g + 4 * q;
+      fq[q] = 0.0;
+      if (i < a.m) {
+        const int ng = rc->ng[i];
+        int64_t X = 0;
+        for (int tt = 0; tt < ng; ++tt) X = X * 254 + red[rc->rg[i][tt] * 16 + l16];
+        const int64_t Y = (int64_t)red[rc->rm[i][0] * 16 + l16] * 254 + red[rc->rm[i][1] * 16 + l16];
+        fq[q] = rc->wg[i] * (double)X + mu * (rc->wm[i] * (double)Y - rc->C[i]);
+      }
+    }
+    __syncthreads();   // red is rewritten by the next tile
+    if constexpr (MODE == 1) {
+      const double scale = epi_scale(mu, a.norm);
+#pragma unroll
+      for (int q = 0; q < kEpiNQ; ++q)
+        if (valid && g + 4 * q < a.m) a.Z[j * a.ldZ + g + 4 * q] = scale * fq[q];
+    } else {
+      snp_epilogue_split<kEpiNQ>(a.ea, plan, j, valid, g, fq, nmiss, sg, nhom);
+    }
+  }
+}
+
 template <int NT0, int NT1, int MAXM, int MODE>
 __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
   constexpr int NT = NT0 + NT1;
@@ -264,9 +342,9 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
   for (int st = 0; st < ST; ++st)
 #pragma unroll
     for (int rt = 0; rt < NT; ++rt) acc[st][rt] = v4i{0, 0, 0, 0};
-  uint32_t c_hom[ST], c_miss[ST], c_hi[ST];
+  uint32_t c_hom[ST], c_miss[ST];
 #pragma unroll
-  for (int st = 0; st < ST; ++st) c_hom[st] = c_miss[st] = c_hi[st] = 0;
+  for (int st = 0; st < ST; ++st) c_hom[st] = c_miss[st] = 0;
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
 
@@ -293,7 +371,7 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
     for (int s = 0; s < KS; ++s) {
       Frag f[ST];
 #pragma unroll
-      for (int st = 0; st < ST; ++st) f[st] = decode_word(gw[st][s], c_hom[st], c_miss[st], c_hi[st]);
+      for (int st = 0; st < ST; ++st) f[st] = decode_word(gw[st][s], c_hom[st], c_miss[st]);
 #pragma unroll
       for (int rt = 0; rt < NT; ++rt) {
         const v4i af = Ab[(s * NT + rt) * 64 + lane];
@@ -312,87 +390,29 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
     __syncthreads();
   }
 
-  // ---- epilogue: lane (g, l16) owns SNP base + 16 g + l16 -------------------
-#pragma unroll
-  for (int st = 0; st < ST; ++st) {
-    c_hom[st] += __shfl_xor(c_hom[st], 16); c_hom[st] += __shfl_xor(c_hom[st], 32);
-    c_miss[st] += __shfl_xor(c_miss[st], 16); c_miss[st] += __shfl_xor(c_miss[st], 32);
-    c_hi[st] += __shfl_xor(c_hi[st], 16); c_hi[st] += __shfl_xor(c_hi[st], 32);
-  }
-  uint32_t o_miss = c_miss[0], o_hom = c_hom[0], o_hi = c_hi[0];
-#pragma unroll
-  for (int st = 1; st < ST; ++st) {
-    o_miss = (g == st) ? c_miss[st] : o_miss;
-    o_hom = (g == st) ? c_hom[st] : o_hom;
-    o_hi = (g == st) ? c_hi[st] : o_hi;
-  }
-  const int64_t nmiss = o_miss, nhom = o_hom;
-  const int64_t sg = (int64_t)o_hi + nhom;
-  const int64_t nobs = (int64_t)a.n - nmiss;
-  const double mu = nobs > 0 ? (double)sg / (double)nobs : 0.0;
-  const uint64_t j = base + 16 * g + l16;
-  const double scale = MODE == 1 ? epi_scale(mu, a.norm) : 1.0;
-  int* red = reinterpret_cast<int*>(lds) + wave * (NT * 256);   // [16*NT rows][16 SNPs]
-  const Recomb* rc = a.rc;
-  double fy[MODE == 0 ? MAXM : 1];
-#pragma unroll
-  for (int st = 0; st < ST; ++st) {
-#pragma unroll
-    for (int rt = 0; rt < NT; ++rt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) red[(16 * rt + 4 * g + r) * 16 + l16] = acc[st][rt][r];
-    __syncthreads();
-    if (g == st) {
-#pragma unroll
-      for (int i = 0; i < (MODE == 0 ? MAXM : kMaxM); ++i) {
-        if (i < a.m) {
-          double fg = 0.0, fm = 0.0;
-          const int ng = rc->ng[i];
-          for (int tt = 0; tt < ng; ++tt)
-            fg += rc->w[i][tt] * ((double)red[rc->rg[i][tt] * 16 + l16] - mu * rc->O[i][tt]);
-          fm = rc->w[i][0] * (double)red[rc->rm[i][0] * 16 + l16] + rc->w[i][1] * (double)red[rc->rm[i][1] * 16 + l16];
-          const double f = fg + mu * fm;
-          if constexpr (MODE == 1) {
-            if (j < a.p) a.Z[j * a.ldZ + i] = scale * f;
-          } else {
-            fy[i] = f;
-          }
-        } else if constexpr (MODE == 0) {
-          fy[i] = 0.0;
-        }
-      }
-    }
-    __syncthreads();
-  }
+  // ---- epilogue --------------------------------------------------------------
+  const double* plan = nullptr;
   if constexpr (MODE == 0) {
-    double* plan = reinterpret_cast<double*>(lds + LDS_RED);
+    double* sp = reinterpret_cast<double*>(lds + LDS_RED);
     const int len = plan_len(a.ea.d, a.ea.m);
-    for (int e = t; e < len; e += 256) plan[e] = a.ea.plan[e];
-    __syncthreads();
-    if (j < a.p) snp_epilogue<MAXM>(a.ea, plan, j, fy, nmiss, sg, nhom);
+    for (int e = t; e < len; e += 256) sp[e] = a.ea.plan[e];
+    plan = sp;
   }
+  gt_epilogue<ST, NT, MODE>(a, acc, c_hom, c_miss, reinterpret_cast<int*>(lds) + wave * (NT * 256), plan, base, g,
+                            l16);
 }
 
 // ---------------------------------------------------------------------------
-// Pipelined variant for NT <= 8: 8 waves x 64 SNPs per workgroup, blocks of
+// Pipelined variant for NT <= 8: 8 waves x 3 tiles of 16 SNPs per workgroup, blocks of
 // 4 k-steps (256 samples, 64 bytes per SNP row).  Genotype segments are
 // copied by LDS-DMA into a 3-slot ring (blocks kb+1 and kb+2 in flight
-// while kb is computed), the operand block into a 2-slot ring.  At the top
-// of each block the wave waits for all but its 4 newest loads (the genotype
-// block kb+1), so block kb's genotypes and operand have landed.
+// while kb is computed), the operand blocks into a 3-slot ring when it fits
+// the 160 KB of LDS (NT <= 7), else a 2-slot ring.  The block loop is
+// unrolled over the ring period so every slot is a distinct LDS object.
 // ---------------------------------------------------------------------------
 constexpr int kPipeWaves = 8;
 constexpr int kPipeTiles = 3;   // SNP tiles of 16 per wave
 
-// One 4-k-step block of the pipelined kernel with statically chosen ring
-// slots (separate LDS objects, so the compiler can tell the slot being read
-// from the slots being written by LDS-DMA and inserts no drain in between).
-// Reads ST 16-byte words, 1024 bytes apart, from LDS and waits for them in
-// the same asm block.  The genotype ring is filled by LDS-DMA; the ordering
-// against those copies is the explicit vmcnt wait plus barrier that precedes
-// every read, so the compiler must not add its own full drain of the DMA
-// counter (its alias tracking of DMA'd LDS objects is incomplete across the
-// unrolled ring and falls back to vmcnt(0)).
 template <int ST>
 __device__ __forceinline__ void lds_read_tiles(const unsigned char* p, v4i (&q)[ST]) {
   const uint32_t addr = (uint32_t)(uintptr_t)(lds_ptr_t)p;
@@ -413,23 +433,47 @@ __device__ __forceinline__ void lds_read_tiles(const unsigned char* p, v4i (&q)[
   }
 }
 
-template <int NT0, int NT, int ST>
+// Cache policy of the genotype stream's LDS-DMA: non-temporal, so the
+// stream does not evict the operand blocks that every workgroup re-reads.
+#ifndef SLFMM_GENO_CPOL
+#define SLFMM_GENO_CPOL 0
+#endif
+constexpr int kGenoCPol = SLFMM_GENO_CPOL;
+
+// vmcnt wait with a compile-time count (vmcnt[3:0]; expcnt and lgkmcnt at
+// their maxima, i.e. not waited on).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N <= 15, "vmcnt immediate");
+  __builtin_amdgcn_s_waitcnt(0x0F70 | N);
+}
+
+// One block of the pipelined kernel.  Operand blocks go through an AR-slot
+// ring (block kb + AR - 1 is issued while kb is computed), genotype blocks
+// through a 3-slot ring (block kb + 2).  Every wave issues the same number
+// of loads in every block (blocks past the end are replaced by dummy copies
+// of block 0 into the slot being filled, which nobody reads), so a wave can
+// wait for exactly the loads of the block it is about to read.
+template <int NT0, int NT, int ST, int AR>
 __device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, int wave, int lane, int g,
                                            const uint8_t* row0, uint64_t tile_stride, uint64_t jl,
                                            const uint8_t* lastrow, const unsigned char* Ag,
                                            const unsigned char* a_rd, unsigned char* a_wr,
                                            const unsigned char* g_rd, unsigned char* g_wr,
-                                           v4i (&acc)[ST][NT], uint32_t (&c_hom)[ST], uint32_t (&c_miss)[ST],
-                                           uint32_t (&c_hi)[ST]) {
+                                           v4i (&acc)[ST][NT], uint32_t (&c_hom)[ST], uint32_t (&c_miss)[ST]) {
   constexpr int KS = 4;
   constexpr int W = kPipeWaves;
   constexpr int CH = KS * NT;
-  // block kb's operand and genotypes have landed once at most the ST newest
-  // loads of this wave (genotype block kb+1, one load per SNP tile) are
-  // still in flight.  s_waitcnt immediate: vmcnt[3:0] = ST, expcnt and
-  // lgkmcnt at their maxima (no wait on them).
-  static_assert(ST >= 1 && ST <= 15, "vmcnt immediate");
-  __builtin_amdgcn_s_waitcnt(0x0F70 | ST);
+  constexpr int CA_LO = CH / W, CA_HI = (CH + W - 1) / W;   // operand chunks per wave
+  // Loads still allowed in flight: those issued in the previous block for
+  // later blocks (AR == 2: genotypes of kb+1; AR == 3: operand and
+  // genotypes of kb+1).
+  if constexpr (AR == 2) {
+    wait_vmcnt<ST>();
+  } else {
+    if (wave < CH % W) wait_vmcnt<CA_HI + ST>();
+    else wait_vmcnt<CA_LO + ST>();
+  }
   __builtin_amdgcn_s_barrier();
   const v4i* Ab = reinterpret_cast<const v4i*>(a_rd);
   uint32_t gw[ST][KS];
@@ -442,6 +486,7 @@ __device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, int wave, in
       for (int s = 0; s < KS; ++s) gw[st][s] = (uint32_t)q[st][s];
   }
   if (kb == a.nblk - 1) {   // bytes at or past the row stride belong to the next row
+    asm volatile("" ::: "memory");   // keep this a branch: only the last block pays for the mask
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
       const int64_t lim = (int64_t)a.ld - ((int64_t)kb * 64 + 16 * g + 4 * s);
@@ -450,28 +495,33 @@ __device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, int wave, in
       for (int st = 0; st < ST; ++st) gw[st][s] &= mask;
     }
   }
-  // the ring slots written below are not the ones read above
-  if (kb + 1 < a.nblk) {
-    const unsigned char* src = Ag + (uint64_t)(kb + 1) * CH * 1024;
+  // the ring slots written below are not the ones read in this block
+  if constexpr (AR == 2) {
+    if (kb + 1 < a.nblk) {
+      const unsigned char* src = Ag + (uint64_t)(kb + 1) * CH * 1024;
+      for (int c = wave; c < CH; c += W) chunk_to_lds(src + (uint64_t)c * 1024, a_wr + c * 1024, lane);
+    }
+  } else {
+    const int ab = kb + AR - 1 < a.nblk ? kb + AR - 1 : 0;
+    const unsigned char* src = Ag + (uint64_t)ab * CH * 1024;
     for (int c = wave; c < CH; c += W) chunk_to_lds(src + (uint64_t)c * 1024, a_wr + c * 1024, lane);
   }
   {
-    // past the last block a dummy copy of block 0 keeps the count of
-    // in-flight genotype loads per wave at exactly ST
     const int src = kb + 2 < a.nblk ? kb + 2 : 0;
     unsigned char* dst = g_wr + wave * (ST * 1024);
 #pragma unroll
     for (int st = 0; st < ST; ++st) {
       // rows past the last SNP are clamped onto it: read, never stored
       const uint8_t* rp = (jl + 16 * st < a.p) ? row0 + st * tile_stride : lastrow;
-      __builtin_amdgcn_global_load_lds((const void*)(rp + (uint64_t)src * 64), (lds_ptr_t)(dst + st * 1024), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(rp + (uint64_t)src * 64), (lds_ptr_t)(dst + st * 1024), 16, 0,
+                                       kGenoCPol);
     }
   }
 #pragma unroll
   for (int s = 0; s < KS; ++s) {
     Frag f[ST];
 #pragma unroll
-    for (int st = 0; st < ST; ++st) f[st] = decode_word(gw[st][s], c_hom[st], c_miss[st], c_hi[st]);
+    for (int st = 0; st < ST; ++st) f[st] = decode_word(gw[st][s], c_hom[st], c_miss[st]);
 #pragma unroll
     for (int rt = 0; rt < NT; rt += 2) {
       const v4i af0 = Ab[(s * NT + rt) * 64 + lane];
@@ -487,13 +537,6 @@ __device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, int wave, in
   }
 }
 
-// ---------------------------------------------------------------------------
-// Pipelined variant for NT <= 8: 8 waves x 64 SNPs per workgroup, blocks of
-// 4 k-steps (256 samples, 64 bytes per SNP row).  Genotype segments are
-// copied by LDS-DMA into a 3-slot ring (blocks kb+1 and kb+2 in flight
-// while kb is computed), the operand block into a 2-slot ring; the block
-// loop is unrolled by 6 so every slot is a compile-time LDS object.
-// ---------------------------------------------------------------------------
 template <int NT0, int NT1, int MAXM, int MODE>
 __global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
   constexpr int NT = NT0 + NT1;
@@ -503,9 +546,15 @@ __global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
   constexpr int CH = KS * NT;
   constexpr int A_SLOT = CH * 1024;
   constexpr int G_SLOT = W * ST * 1024;
+#ifndef SLFMM_OPERAND_RING
+#define SLFMM_OPERAND_RING 2
+#endif
+  // operand ring depth: 3 slots when they fit beside the genotype ring
+  constexpr int AR = SLFMM_OPERAND_RING >= 3 && 3 * A_SLOT + 3 * G_SLOT <= 160 * 1024 ? 3 : 2;
   static_assert(NT * 1024 * W <= 3 * G_SLOT, "epilogue sums of all waves must fit the genotype ring");
   __shared__ __align__(16) unsigned char A0[A_SLOT];
   __shared__ __align__(16) unsigned char A1[A_SLOT];
+  __shared__ __align__(16) unsigned char A2[AR == 3 ? A_SLOT : 16];
   __shared__ __align__(16) unsigned char G0[G_SLOT];
   __shared__ __align__(16) unsigned char G1[G_SLOT];
   __shared__ __align__(16) unsigned char G2[G_SLOT];
@@ -522,15 +571,22 @@ __global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
   for (int st = 0; st < ST; ++st)
 #pragma unroll
     for (int rt = 0; rt < NT; ++rt) acc[st][rt] = v4i{0, 0, 0, 0};
-  uint32_t c_hom[ST], c_miss[ST], c_hi[ST];
+  uint32_t c_hom[ST], c_miss[ST];
 #pragma unroll
-  for (int st = 0; st < ST; ++st) c_hom[st] = c_miss[st] = c_hi[st] = 0;
-  // prologue: operand block 0 -> A0, genotype blocks 0 -> G0 and 1 -> G1
+  for (int st = 0; st < ST; ++st) c_hom[st] = c_miss[st] = 0;
+  // Prologue, in the order the blocks issue them (operand, then genotypes):
+  // block 0 -> A0, G0, then block 1 -> (A1,) G1.  The first block's wait
+  // leaves exactly the block-1 loads in flight.
   for (int c = wave; c < CH; c += W) chunk_to_lds(Ag + (uint64_t)c * 1024, A0 + c * 1024, lane);
 #pragma unroll
   for (int st = 0; st < ST; ++st) {
     const uint8_t* rp = (jl + 16 * st < a.p) ? row0 + st * tile_stride : lastrow;
-    __builtin_amdgcn_global_load_lds((const void*)(rp), (lds_ptr_t)(G0 + wave * (ST * 1024) + st * 1024), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void*)(rp), (lds_ptr_t)(G0 + wave * (ST * 1024) + st * 1024), 16, 0,
+                                     kGenoCPol);
+  }
+  if constexpr (AR == 3) {
+    const unsigned char* src = Ag + (uint64_t)(a.nblk > 1 ? 1 : 0) * CH * 1024;
+    for (int c = wave; c < CH; c += W) chunk_to_lds(src + (uint64_t)c * 1024, A1 + c * 1024, lane);
   }
   {
     const int src = a.nblk > 1 ? 1 : 0;
@@ -538,89 +594,46 @@ __global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
     for (int st = 0; st < ST; ++st) {
       const uint8_t* rp = (jl + 16 * st < a.p) ? row0 + st * tile_stride : lastrow;
       __builtin_amdgcn_global_load_lds((const void*)(rp + (uint64_t)src * 64),
-                                       (lds_ptr_t)(G1 + wave * (ST * 1024) + st * 1024), 16, 0, 0);
+                                       (lds_ptr_t)(G1 + wave * (ST * 1024) + st * 1024), 16, 0, kGenoCPol);
     }
   }
-#define SLFMM_PIPE(PH, AR, AW, GR, GW)                                                              \
-  if (kb + PH < a.nblk)                                                                             \
-    pipe_block<NT0, NT, ST>(kb + PH, a, wave, lane, g, row0, tile_stride, jl, lastrow, Ag, AR, AW, GR, GW, acc, c_hom, \
-                            c_miss, c_hi);
-  for (int kb = 0; kb < a.nblk; kb += 6) {
-    SLFMM_PIPE(0, A0, A1, G0, G2)
-    SLFMM_PIPE(1, A1, A0, G1, G0)
-    SLFMM_PIPE(2, A0, A1, G2, G1)
-    SLFMM_PIPE(3, A1, A0, G0, G2)
-    SLFMM_PIPE(4, A0, A1, G1, G0)
-    SLFMM_PIPE(5, A1, A0, G2, G1)
+#define SLFMM_PIPE(PH, AR_, AW_, GR_, GW_)                                                               \
+  if (kb + PH < a.nblk)                                                                                 \
+    pipe_block<NT0, NT, ST, AR>(kb + PH, a, wave, lane, g, row0, tile_stride, jl, lastrow, Ag, AR_, AW_, GR_, GW_, \
+                                acc, c_hom, c_miss);
+  if constexpr (AR == 3) {
+    for (int kb = 0; kb < a.nblk; kb += 3) {
+      SLFMM_PIPE(0, A0, A2, G0, G2)
+      SLFMM_PIPE(1, A1, A0, G1, G0)
+      SLFMM_PIPE(2, A2, A1, G2, G1)
+    }
+  } else {
+    for (int kb = 0; kb < a.nblk; kb += 6) {
+      SLFMM_PIPE(0, A0, A1, G0, G2)
+      SLFMM_PIPE(1, A1, A0, G1, G0)
+      SLFMM_PIPE(2, A0, A1, G2, G1)
+      SLFMM_PIPE(3, A1, A0, G0, G2)
+      SLFMM_PIPE(4, A0, A1, G1, G0)
+      SLFMM_PIPE(5, A1, A0, G2, G1)
+    }
   }
 #undef SLFMM_PIPE
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
 
-  // ---- epilogue: lane (g, l16) owns SNP base + 16 g + l16 -------------------
-#pragma unroll
-  for (int st = 0; st < ST; ++st) {
-    c_hom[st] += __shfl_xor(c_hom[st], 16); c_hom[st] += __shfl_xor(c_hom[st], 32);
-    c_miss[st] += __shfl_xor(c_miss[st], 16); c_miss[st] += __shfl_xor(c_miss[st], 32);
-    c_hi[st] += __shfl_xor(c_hi[st], 16); c_hi[st] += __shfl_xor(c_hi[st], 32);
-  }
-  uint32_t o_miss = c_miss[0], o_hom = c_hom[0], o_hi = c_hi[0];
-#pragma unroll
-  for (int st = 1; st < ST; ++st) {
-    o_miss = (g == st) ? c_miss[st] : o_miss;
-    o_hom = (g == st) ? c_hom[st] : o_hom;
-    o_hi = (g == st) ? c_hi[st] : o_hi;
-  }
-  const int64_t nmiss = o_miss, nhom = o_hom;
-  const int64_t sg = (int64_t)o_hi + nhom;
-  const int64_t nobs = (int64_t)a.n - nmiss;
-  const double mu = nobs > 0 ? (double)sg / (double)nobs : 0.0;
-  const bool owner = g < ST;
-  const uint64_t j = owner ? base + 16 * g + l16 : a.p;
-  const double scale = MODE == 1 ? epi_scale(mu, a.norm) : 1.0;
-  // per-wave int32 sums of one SNP tile: waves spread over the three genotype slots
+  // ---- epilogue: per-wave int32 sums of one SNP tile; waves spread over the
+  // three genotype slots, the plan goes to A0 --------------------------------
   constexpr int WPS = G_SLOT / (NT * 1024);   // waves per slot
   unsigned char* gslot = wave < WPS ? G0 : (wave < 2 * WPS ? G1 : G2);
   int* red = reinterpret_cast<int*>(gslot) + (wave % WPS) * (NT * 256);
-  const Recomb* rc = a.rc;
-  double fy[MODE == 0 ? MAXM : 1];
-#pragma unroll
-  for (int st = 0; st < ST; ++st) {
-#pragma unroll
-    for (int rt = 0; rt < NT; ++rt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) red[(16 * rt + 4 * g + r) * 16 + l16] = acc[st][rt][r];
-    __syncthreads();
-    if (g == st) {   // lane group st owns this tile's SNPs
-#pragma unroll
-      for (int i = 0; i < (MODE == 0 ? MAXM : kMaxM); ++i) {
-        if (i < a.m) {
-          double fg = 0.0;
-          const int ng = rc->ng[i];
-          for (int tt = 0; tt < ng; ++tt)
-            fg += rc->w[i][tt] * ((double)red[rc->rg[i][tt] * 16 + l16] - mu * rc->O[i][tt]);
-          const double fm = rc->w[i][0] * (double)red[rc->rm[i][0] * 16 + l16] +
-                            rc->w[i][1] * (double)red[rc->rm[i][1] * 16 + l16];
-          const double f = fg + mu * fm;
-          if constexpr (MODE == 1) {
-            if (j < a.p) a.Z[j * a.ldZ + i] = scale * f;
-          } else {
-            fy[i] = f;
-          }
-        } else if constexpr (MODE == 0) {
-          fy[i] = 0.0;
-        }
-      }
-    }
-    __syncthreads();
-  }
+  const double* plan = nullptr;
   if constexpr (MODE == 0) {
-    double* plan = reinterpret_cast<double*>(A0);
+    double* sp = reinterpret_cast<double*>(A0);
     const int len = plan_len(a.ea.d, a.ea.m);
-    for (int e = t; e < len; e += 512) plan[e] = a.ea.plan[e];
-    __syncthreads();
-    if (j < a.p) snp_epilogue<MAXM>(a.ea, plan, j, fy, nmiss, sg, nhom);
+    for (int e = t; e < len; e += 512) sp[e] = a.ea.plan[e];
+    plan = sp;
   }
+  gt_epilogue<ST, NT, MODE>(a, acc, c_hom, c_miss, red, plan, base, g, l16);
 }
 
 #define SLFMM_GT_CASES(X) \
@@ -658,15 +671,6 @@ static void launch_gt(hipStream_t s, const Operand& op, GtArgs& a, int mode, int
     }
     SLFMM_GT_CASES(X)
 #undef X
-    if (!done && mode == 1 && op.NT0 == op.NT1) {
-      switch (op.NT0) {
-        case 5: k_gt_i8<5, 5, 1, 1><<<grid, 256, 0, s>>>(a); done = true; break;
-        case 6: k_gt_i8<6, 6, 1, 1><<<grid, 256, 0, s>>>(a); done = true; break;
-        case 7: k_gt_i8<7, 7, 1, 1><<<grid, 256, 0, s>>>(a); done = true; break;
-        case 8: k_gt_i8<8, 8, 1, 1><<<grid, 256, 0, s>>>(a); done = true; break;
-        default: break;
-      }
-    }
   }
   SLFMM_CHECK(done, SLFMM_E_ARG, "no int8 kernel for this tile shape (K + d too large)");
   SLFMM_HIP(hipGetLastError());
@@ -692,6 +696,7 @@ void launch_func_i8(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_Q, uint
 // RSVD sketch / backward pass: Z[j][c] = s_j * (B^T y_j)_c for every SNP row.
 void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p, const double* d_B,
                      uint64_t l, int norm, double* Z, uint64_t ldZ) {
+  SLFMM_CHECK(l >= 1 && l <= 32, SLFMM_E_ARG, "the int8 RSVD path supports K + oversampling <= 32");
   Operand op = prepare_operand(ctx, "rs_i8", d_B, l, n, (int)l, (int)l);
   GtArgs a{};
   a.packed = packed; a.ld = ld; a.p = p; a.n = n; a.nblk = (int)op.nblk; a.m = (int)l; a.norm = norm;
@@ -812,7 +817,7 @@ __global__ void __launch_bounds__(256) k_fwd_i8(const uint8_t* __restrict__ YT, 
       for (int ct = 0; ct < NTB; ++ct) bf[ct] = Bb[(s * NTB + ct) * 64 + lane];
 #pragma unroll
       for (int st = 0; st < ST; ++st) {
-        Frag f = decode_word(gw[st][s], dummy0, dummy1, dummy2);
+        Frag f = decode_word(gw[st][s], dummy0, dummy1);
 #pragma unroll
         for (int ct = 0; ct < 2 * NCB; ++ct)
           acc[st][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f.g0, bf[ct], acc[st][ct], 0, 0, 0);
