@@ -175,6 +175,22 @@ int slfmm_fit_lfmm2(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset
                     const slfmm_config* cfg, const double* b0, double* u_hat,
                     slfmm_assoc_out* out, uint32_t flags);
 
+/* ---- L3: NMF imputation (src/nmf.rs) -------------------------------------- */
+
+/* nmf_impute_estimation (src/nmf.rs:80-249) in FP64 on the estimation SNPs
+ * subset_idx of g (NULL = all SNPs): w (n x k), h (k x p_est; the
+ * reference's h_full), cv_mae (n_iter: held-out MAE of W H per iteration),
+ * cv_mae_mean (1: MAE of column-mean imputation in the last iteration),
+ * mask_count (1: held-out entries summed over the iterations).  omega
+ * (n x (k + 10)) may be injected; NULL draws it from cfg->seed.  The CV
+ * hold-out draw of entry (sample i, column c of chunk q) in iteration t is
+ * element c * n + i of the library's uniform stream seeded cfg->seed + t + q
+ * (the reference draws from ChaCha8).  w and h follow `flags`; the CV
+ * outputs are host memory. */
+int slfmm_nmf_estimate(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* subset_idx, uint64_t n_idx,
+                       const slfmm_nmf_config* cfg, const double* omega, double* w, double* h,
+                       double* cv_mae, double* cv_mae_mean, uint64_t* mask_count, uint32_t flags);
+
 /* ---- Kernel timing (no reference counterpart; used by the benchmark) ---- */
 
 /* Enable HIP-event timing of the genotype-stream kernels on the context's

@@ -262,6 +262,52 @@ def estimate_factors(y_est: BedFile, subset, x, config: Lfmm2Config, b0=None) ->
     return estimate_factors_streaming(y_est, subset, center_covariates(x, config.scale_cov), config, b0)
 
 
+@dataclass
+class NmfParams:
+    """NmfConfig (src/nmf.rs:15-50)."""
+    k: int = 5
+    n_iter: int = 10
+    eps: float = 1e-16
+    cv_rate: float = 5e-4
+    chunk_size: int = 10_000
+    seed: int = 42
+
+    def to_c(self) -> NmfConfig:
+        c = NmfConfig()
+        lib().slfmm_nmf_config_default(C.byref(c))
+        c.k, c.n_iter, c.eps, c.cv_rate = self.k, self.n_iter, self.eps, self.cv_rate
+        c.chunk_size, c.seed = self.chunk_size, self.seed
+        return c
+
+
+@dataclass
+class NmfEstResult:
+    """NmfEstResult (src/nmf.rs:53-66)."""
+    w: np.ndarray
+    h_full: np.ndarray
+    cv_mae_per_iter: np.ndarray
+    cv_mae_mean_impute: float
+    total_mask_count: int
+
+
+def nmf_impute_estimation(bed: BedFile, subset, config: NmfParams, omega=None) -> NmfEstResult:
+    """src/nmf.rs:80-249 on the GPU (FP64).  omega (n x (k + 10)) may be
+    injected; otherwise it is drawn from config.seed."""
+    ctx = bed.ctx
+    idx = None if subset is None or (isinstance(subset, str) and subset == "all") else subset_indices(subset, bed.n_snps)
+    pe = bed.n_snps if idx is None else len(idx)
+    cfg = config.to_c()
+    w = np.empty((bed.n_samples, config.k))
+    h = np.empty((config.k, pe))
+    cv = np.empty(max(config.n_iter, 1))
+    mean = C.c_double()
+    cnt = C.c_uint64()
+    om = None if omega is None else _f64(omega)
+    ctx.check(lib().slfmm_nmf_estimate(ctx.h, bed.h, _ptr(idx), 0 if idx is None else len(idx), C.byref(cfg),
+                                       _ptr(om), _ptr(w), _ptr(h), _ptr(cv), C.byref(mean), C.byref(cnt), 0))
+    return NmfEstResult(w, h, cv[:config.n_iter].copy(), float(mean.value), int(cnt.value))
+
+
 def _assoc_arrays(p: int, d: int, with_hist: bool):
     return (np.empty((p, d)), np.empty((p, d)), np.empty((p, 3)), np.empty((p, d)),
             np.empty((d, N_BINS), np.uint64) if with_hist else None, np.empty(d), np.empty(1))

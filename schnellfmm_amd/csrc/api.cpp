@@ -508,6 +508,41 @@ int slfmm_fit_lfmm2(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset
   });
 }
 
+int slfmm_nmf_estimate(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* subset_idx, uint64_t n_idx,
+                       const slfmm_nmf_config* cfg, const double* omega, double* w, double* h,
+                       double* cv_mae, double* cv_mae_mean, uint64_t* mask_count, uint32_t flags) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && g && cfg && w && h, SLFMM_E_ARG, "NULL argument");
+    SLFMM_CHECK(cfg->k >= 1 && cfg->k <= 16, SLFMM_E_ARG, "NMF rank k must be 1..16");
+    const uint64_t n = g->n, K = cfg->k;
+    uint64_t pe = 0;
+    const uint8_t* Y = gather_subset(ctx, g, subset_idx, subset_idx ? n_idx : g->p, "nmf_Y", &pe);
+    const double* dom = omega ? to_dev(ctx, "api_nmf_omega", omega, n * (K + 10)) : nullptr;
+    const bool dev = flags & SLFMM_OUT_DEVICE;
+    double* dW = dev ? w : ctx->buf<double>("api_nmf_w", n * K);
+    double* dH = dev ? h : ctx->buf<double>("api_nmf_h", pe * K);
+    double* dHt = ctx->buf<double>("api_nmf_ht", pe * K);
+    const uint64_t it = cfg->n_iter;
+    double* dcv = ctx->buf<double>("api_nmf_cv", 3 * std::max<uint64_t>(it, 1));
+    nmf_run(ctx, Y, g->ld, n, pe, *cfg, dom, dW, dHt, dcv);
+    nmf_transpose_h(ctx->stream, dHt, pe, (int)K, dH);
+    if (!dev) {
+      to_host(ctx, w, dW, n * K);
+      to_host(ctx, h, dH, pe * K);
+    }
+    std::vector<double> hcv(3 * std::max<uint64_t>(it, 1), 0.0);
+    if (it) to_host(ctx, hcv.data(), dcv, 3 * it);
+    SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < it; ++i) {
+      if (cv_mae) cv_mae[i] = hcv[3 * i];
+      total += (uint64_t)hcv[3 * i + 2];
+    }
+    if (cv_mae_mean) *cv_mae_mean = it ? hcv[3 * (it - 1) + 1] : 0.0;
+    if (mask_count) *mask_count = total;
+  });
+}
+
 int slfmm_debug_mfma_i8(slfmm_ctx* ctx, const int8_t* a, const int8_t* b, int32_t* d) {
   return guarded(ctx, [&] {
     SLFMM_CHECK(ctx && a && b && d, SLFMM_E_ARG, "NULL argument");

@@ -418,6 +418,12 @@ __global__ void k_pad_operand(const double* __restrict__ Q, uint64_t ldq, uint64
   A[e] = (i < n && c < m) ? Q[i * ldq + c] : 0.0;
 }
 
+void pad_operand(hipStream_t s, const double* Q, uint64_t ldq, uint64_t n, int m, double* A, uint64_t lda,
+                 uint64_t npad) {
+  k_pad_operand<<<(unsigned)cdiv(npad * lda, 256), 256, 0, s>>>(Q, ldq, n, m, A, lda, npad);
+  SLFMM_HIP(hipGetLastError());
+}
+
 struct AssocPlan {
   int d = 0, K = 0, m = 0;
   double* Q = nullptr;      // n x m orthonormal basis (or [U_c|X] itself)

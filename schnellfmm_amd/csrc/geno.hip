@@ -12,6 +12,7 @@
 // kSlack zero bytes follow the last row (kernels load whole 16-byte segments
 // of a sample block and may run past the row stride).
 #include "internal.h"
+#include "rng.h"
 
 namespace slfmm {
 
@@ -269,19 +270,7 @@ void launch_transpose(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_
 // ---------------------------------------------------------------------------
 // Counter-based random numbers (SplitMix64 finaliser over a seeded counter).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t mix64(uint64_t seed, uint64_t i) {
-  uint64_t z = seed * 0xD1B54A32D192ED03ull + (i + 1) * 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-__device__ __forceinline__ double u01(uint64_t seed, uint64_t i) {
-  return (double)(mix64(seed, i) >> 11) * (1.0 / 9007199254740992.0);
-}
-__device__ double normal01(uint64_t seed, uint64_t i) {
-  double u1 = 1.0 - u01(seed, 2 * i), u2 = u01(seed, 2 * i + 1);
-  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
-}
+
 
 // Marsaglia-Tsang gamma(a) with a bounded number of proposals.
 __device__ double gamma_draw(double a, uint64_t seed, uint64_t base) {

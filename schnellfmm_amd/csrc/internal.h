@@ -42,8 +42,8 @@ constexpr uint64_t kSlack = 128;
 constexpr int kMaxM = 64;   // functionals per SNP (K + d, or K + oversampling)
 constexpr int kMaxD = 16;   // traits
 
-inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
-inline uint64_t rup(uint64_t a, uint64_t b) { return cdiv(a, b) * b; }
+__host__ __device__ inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline uint64_t rup(uint64_t a, uint64_t b) { return cdiv(a, b) * b; }
 
 }  // namespace slfmm
 
@@ -174,8 +174,21 @@ void probe_mfma_i8(slfmm_ctx* ctx, const int8_t* hA, const int8_t* hB, int32_t* 
 void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_t n_idx,
               const double* d_xs, uint64_t d, const slfmm_config& cfg, const double* d_b0,
               double* d_uhat);
+// N(0,1) matrix (n x l, row stride ld) from the library's counter-based stream.
+void launch_normal(hipStream_t s, double* out, uint64_t n, uint64_t l, uint64_t ld, uint64_t seed);
+// Rows h_idx of g as a contiguous block (g's buffer itself for the identity).
+const uint8_t* gather_subset(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_t n_idx,
+                             const char* tag, uint64_t* pe);
 // Thin precompute on the device: sigma(d), dlam(d), Qd (n x d), ridge_inv (d x d).
 void precompute_dev(slfmm_ctx* ctx, const double* d_xs, uint64_t n, uint64_t d, double lambda,
                     double* d_sigma, double* d_dlam, double* d_qd, double* d_ridge);
+
+// ---- nmf.hip ------------------------------------------------------------------
+// NMF estimation (src/nmf.rs:80-249) on packed rows Y (pe SNPs): device
+// outputs W (n x K), Ht (pe x K) and cv (3 per iteration: NMF MAE, mean MAE,
+// held-out count); omega (n x (K + 10)) is drawn from cfg.seed when null.
+void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const slfmm_nmf_config& cfg,
+             const double* d_omega, double* W, double* Ht, double* cv);
+void nmf_transpose_h(hipStream_t s, const double* Ht, uint64_t pe, int K, double* H);
 
 }  // namespace slfmm

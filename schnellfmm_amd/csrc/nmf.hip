@@ -1,0 +1,408 @@
+// NMF imputation of the estimation panel on the GPU (FP64, the reference's
+// arithmetic): src/nmf.rs:80-455.
+//
+// Layout: W is n x K (row-major), H is kept transposed as Ht (p_est x K), so
+// the K factor loadings of one SNP are contiguous.  Every pass reads the raw
+// genotype codes of the estimation rows directly; missing calls are filled
+// inside the kernels with the current reconstruction W_k . H_j (the
+// reference's NmfRawInRam fill, src/parallel.rs:290-321), never stored.
+//
+//   forward   num_W[k] = sum_j y_kj H_j      (src/nmf.rs:131-147)
+//   W update  W *= num_W / (W HH^T) where the denominator > eps, clamp,
+//             normalise columns (src/nmf.rs:150-168, :259-271)
+//   backward  per SNP: num_H = W^T y_j, H_j *= num_H / (W^T W H_j) where the
+//             denominator > eps, clamp (src/nmf.rs:171-221), normalise
+//   CV error  masked MAE against W H and against the column mean, one
+//             uniform draw per entry (src/nmf.rs:379-455)
+#include "internal.h"
+#include "colreduce.h"
+#include "rng.h"
+
+namespace slfmm {
+
+constexpr int kNmfMaxK = 16;
+
+__device__ __forceinline__ uint32_t code_of(uint32_t word, int pos) { return (word >> (2 * pos)) & 3u; }
+__device__ __forceinline__ double dosage_of(uint32_t code) { return code == 0u ? 0.0 : (code == 2u ? 1.0 : 2.0); }
+
+// ---- forward pass: per-chunk partials of num_W (n x K) -----------------------
+// One workgroup per (256-sample tile, SNP chunk); thread = sample.
+__global__ void __launch_bounds__(256) k_nmf_fwd(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t n,
+                                                 uint64_t p, uint64_t snps_per_chunk, const double* __restrict__ W,
+                                                 const double* __restrict__ Ht, int K, double* __restrict__ part) {
+  __shared__ double sh[32][kNmfMaxK + 1];
+  __shared__ uint32_t sw[32][16];
+  const int t = threadIdx.x;
+  const uint64_t k = (uint64_t)blockIdx.x * 256 + t;
+  const uint64_t j0 = (uint64_t)blockIdx.y * snps_per_chunk;
+  const uint64_t j1 = std::min(p, j0 + snps_per_chunk);
+  double w[kNmfMaxK], acc[kNmfMaxK];
+#pragma unroll
+  for (int r = 0; r < kNmfMaxK; ++r) {
+    w[r] = (k < n && r < K) ? W[k * K + r] : 0.0;
+    acc[r] = 0.0;
+  }
+  for (uint64_t jb = j0; jb < j1; jb += 32) {
+    const int nj = (int)std::min<uint64_t>(32, j1 - jb);
+    __syncthreads();
+    for (int e = t; e < 32 * kNmfMaxK; e += 256) {
+      const int jj = e / kNmfMaxK, r = e % kNmfMaxK;
+      sh[jj][r] = (jj < nj && r < K) ? Ht[(jb + jj) * K + r] : 0.0;
+    }
+    for (int e = t; e < 32 * 16; e += 256) {
+      const int jj = e >> 4, wd = e & 15;
+      sw[jj][wd] = jj < nj ? reinterpret_cast<const uint32_t*>(packed + (jb + jj) * ld + (uint64_t)blockIdx.x * 64)[wd]
+                           : 0u;
+    }
+    __syncthreads();
+    for (int jj = 0; jj < nj; ++jj) {
+      const uint32_t code = code_of(sw[jj][t >> 4], t & 15);
+      double v;
+      if (code == 1u) {   // missing: current reconstruction
+        v = 0.0;
+#pragma unroll
+        for (int r = 0; r < kNmfMaxK; ++r) v += w[r] * sh[jj][r];
+      } else {
+        v = dosage_of(code);
+      }
+#pragma unroll
+      for (int r = 0; r < kNmfMaxK; ++r) acc[r] += v * sh[jj][r];
+    }
+  }
+  if (k < n)
+    for (int r = 0; r < K; ++r) part[((uint64_t)blockIdx.y * n + k) * K + r] = acc[r];
+}
+
+// num_W = sum of the chunk partials in chunk order
+__global__ void k_nmf_sum_chunks(const double* __restrict__ part, uint64_t count, int nchunk, double* out) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += part[(uint64_t)c * count + e];
+  out[e] = s;
+}
+
+// W update, one thread per sample row (src/nmf.rs:150-167)
+__global__ void k_nmf_wupd(double* __restrict__ W, const double* __restrict__ num, const double* __restrict__ hh,
+                           uint64_t n, int K, double eps) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double w[kNmfMaxK], nw[kNmfMaxK];
+  for (int r = 0; r < K; ++r) w[r] = W[k * K + r];
+  for (int r = 0; r < K; ++r) {
+    double d = 0.0;
+    for (int s = 0; s < K; ++s) d += w[s] * hh[s * K + r];
+    double v = w[r];
+    if (d > eps) v *= num[k * K + r] / d;
+    nw[r] = v > eps ? v : eps;
+  }
+  for (int r = 0; r < K; ++r) W[k * K + r] = nw[r];
+}
+
+// normalize_w (src/nmf.rs:259-271): W[:, r] *= 1 / s_r, H[r, :] *= s_r for
+// column sums s_r > eps.
+__global__ void k_nmf_scale(double* __restrict__ W, uint64_t n, double* __restrict__ Ht, uint64_t p, int K,
+                            const double* __restrict__ colsum, double eps) {
+  const uint64_t total = (n + p) * (uint64_t)K;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % K);
+    const double s = colsum[r];
+    if (!(s > eps)) continue;
+    if (e < n * (uint64_t)K) W[e] *= 1.0 / s;
+    else Ht[e - n * (uint64_t)K] *= s;
+  }
+}
+
+// ---- backward pass with the H update fused (src/nmf.rs:171-221) --------------
+// One thread per SNP; samples in blocks of 64 with their W rows in LDS.
+__global__ void __launch_bounds__(256) k_nmf_bwd(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t n,
+                                                 uint64_t p, const double* __restrict__ W,
+                                                 const double* __restrict__ WtW, int K, double eps, double* Ht) {
+  __shared__ double sW[64][kNmfMaxK + 1];
+  __shared__ double sG[kNmfMaxK * kNmfMaxK];
+  const int t = threadIdx.x;
+  const uint64_t j = (uint64_t)blockIdx.x * 256 + t;
+  const bool ok = j < p;
+  for (int e = t; e < K * K; e += 256) sG[e] = WtW[e];
+  double h[kNmfMaxK], acc[kNmfMaxK];
+#pragma unroll
+  for (int r = 0; r < kNmfMaxK; ++r) {
+    h[r] = (ok && r < K) ? Ht[j * K + r] : 0.0;
+    acc[r] = 0.0;
+  }
+  const uint4* row = reinterpret_cast<const uint4*>(packed + (ok ? j : 0) * ld);
+  const uint64_t nkb = cdiv(n, 64);
+  for (uint64_t kb = 0; kb < nkb; ++kb) {
+    __syncthreads();
+    for (int e = t; e < 64 * kNmfMaxK; e += 256) {
+      const int kk = e / kNmfMaxK, r = e % kNmfMaxK;
+      const uint64_t k = kb * 64 + kk;
+      sW[kk][r] = (k < n && r < K) ? W[k * K + r] : 0.0;
+    }
+    const uint4 q = ok ? row[kb] : make_uint4(0, 0, 0, 0);
+    __syncthreads();
+    const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
+    for (int kk = 0; kk < 64; ++kk) {
+      const uint32_t code = code_of(wd[kk >> 4], kk & 15);
+      double v;
+      if (code == 1u) {   // missing: fill with W_k . H_j (pre-update H)
+        v = 0.0;
+#pragma unroll
+        for (int r = 0; r < kNmfMaxK; ++r) v += sW[kk][r] * h[r];
+      } else {
+        v = dosage_of(code);
+      }
+#pragma unroll
+      for (int r = 0; r < kNmfMaxK; ++r) acc[r] += v * sW[kk][r];
+    }
+  }
+  if (!ok) return;
+  double nh[kNmfMaxK];
+  for (int r = 0; r < K; ++r) {
+    double d = 0.0;
+    for (int s = 0; s < K; ++s) d += sG[r * K + s] * h[s];
+    double v = h[r];
+    if (d > eps) v *= acc[r] / d;
+    nh[r] = v > eps ? v : eps;
+  }
+  for (int r = 0; r < K; ++r) Ht[j * K + r] = nh[r];
+}
+
+// ---- CV error (src/nmf.rs:379-455) -------------------------------------------
+// Entry (sample k, SNP j) of chunk seq = j / chunk draws element
+// (j % chunk) * n + k of the stream seeded mask_seed + seq (column, then
+// row, every entry drawn); it is held out if the draw is < cv_rate and the
+// call is observed.  Per-workgroup sums, combined in a fixed order.
+__global__ void __launch_bounds__(256) k_nmf_cv(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t n,
+                                                uint64_t p, const double* __restrict__ W,
+                                                const double* __restrict__ Ht, int K,
+                                                const int32_t* __restrict__ nmiss, const int32_t* __restrict__ sumg,
+                                                uint64_t mask_seed, uint64_t chunk, double cv_rate,
+                                                double* __restrict__ part) {
+  __shared__ double r0[256], r1[256];
+  __shared__ unsigned long long r2[256];
+  const int t = threadIdx.x;
+  const uint64_t j = (uint64_t)blockIdx.x * 256 + t;
+  double e_nmf = 0.0, e_mean = 0.0;
+  unsigned long long cnt = 0;
+  if (j < p) {
+    const int64_t nobs = (int64_t)n - nmiss[j];
+    const double mu = nobs > 0 ? (double)sumg[j] / (double)nobs : 0.0;
+    const uint64_t seed = mask_seed + j / chunk;
+    const uint64_t base = (j % chunk) * n;
+    const uint4* row = reinterpret_cast<const uint4*>(packed + j * ld);
+    const uint64_t nkb = cdiv(n, 64);
+    for (uint64_t kb = 0; kb < nkb; ++kb) {
+      const uint4 q = row[kb];
+      const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
+      const int lim = (int)std::min<uint64_t>(64, n - kb * 64);
+      for (int kk = 0; kk < lim; ++kk) {
+        const uint64_t k = kb * 64 + kk;
+        const double u = u01(seed, base + k);
+        const uint32_t code = code_of(wd[kk >> 4], kk & 15);
+        if (u < cv_rate && code != 1u) {
+          double rec = 0.0;
+          for (int r = 0; r < K; ++r) rec += W[k * K + r] * Ht[j * K + r];
+          const double g = dosage_of(code);
+          e_nmf += fabs(g - rec);
+          e_mean += fabs(g - mu);
+          cnt += 1;
+        }
+      }
+    }
+  }
+  r0[t] = e_nmf; r1[t] = e_mean; r2[t] = cnt;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) { r0[t] += r0[t + s]; r1[t] += r1[t + s]; r2[t] += r2[t + s]; }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[blockIdx.x * 3 + 0] = r0[0];
+    part[blockIdx.x * 3 + 1] = r1[0];
+    part[blockIdx.x * 3 + 2] = (double)r2[0];
+  }
+}
+
+// out[0..2] = (nmf MAE, mean-imputation MAE, held-out count) of one pass
+__global__ void k_nmf_cv_final(const double* __restrict__ part, int nblk, double* out) {
+  if (threadIdx.x != 0) return;
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int i = 0; i < nblk; ++i) { a += part[i * 3]; b += part[i * 3 + 1]; c += part[i * 3 + 2]; }
+  out[0] = c > 0.0 ? a / c : 0.0;
+  out[1] = c > 0.0 ? b / c : 0.0;
+  out[2] = c;
+}
+
+// ---- initialisation helpers (src/nmf.rs:279-362) -------------------------------
+// Z[j][c] += mu_j * colsum[c]: turns centred functionals into functionals of
+// the mean-filled raw column (g on observed calls, mu on missing ones).
+__global__ void k_add_mean_term(double* __restrict__ Z, uint64_t ldz, uint64_t p, int m, const double* __restrict__ mu,
+                                const double* __restrict__ colsum) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= p * (uint64_t)m) return;
+  const uint64_t j = e / m;
+  const int c = (int)(e % m);
+  Z[j * ldz + c] += mu[j] * colsum[c];
+}
+
+__global__ void k_abs(double* __restrict__ a, uint64_t count) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < count) a[e] = fabs(a[e]);
+}
+
+// Ht[j][r] = max(F[j][r], lo)  (F with row stride ldf)
+__global__ void k_relu_rows(const double* __restrict__ F, uint64_t ldf, uint64_t p, int K, double lo,
+                            double* __restrict__ Ht) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= p * (uint64_t)K) return;
+  const double v = F[(e / K) * ldf + e % K];
+  Ht[e] = v > lo ? v : lo;
+}
+
+__global__ void k_clamp_min(double* __restrict__ a, uint64_t count, double lo) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < count && !(a[e] > lo)) a[e] = lo;
+}
+
+__global__ void k_transpose_kp(const double* __restrict__ Ht, uint64_t p, int K, double* __restrict__ H) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= p * (uint64_t)K) return;
+  const uint64_t j = e / K;
+  const int r = (int)(e % K);
+  H[(uint64_t)r * p + j] = Ht[e];
+}
+
+void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t p, uint64_t npad,
+                     const double* A, uint64_t lda, int nrb, const double* mu, double* F, uint64_t ldF,
+                     const double* outscale);   // assoc.hip
+void pad_operand(hipStream_t s, const double* Q, uint64_t ldq, uint64_t n, int m, double* A, uint64_t lda,
+                 uint64_t npad);                 // assoc.hip
+
+namespace {
+struct FnColSum {
+  const double* A; uint64_t lda;
+  __device__ void operator()(uint64_t r, int c, double (&v)[1]) const { v[0] = A[r * lda + c]; }
+};
+struct GStore {
+  double* out;
+  __device__ void operator()(int c, const double (&v)[1]) const { out[c] = v[0]; }
+};
+}  // namespace
+
+// Functionals of the mean-filled raw columns of Y (pe rows) against the
+// n x m operand Op: out[j][c] = sum_k Op[k][c] y_kj, y = g or mu_j.
+static void raw_functionals(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const double* mu,
+                            const double* Op, uint64_t ldo, uint64_t m, double* out, uint64_t ldout) {
+  hipStream_t s = ctx->stream;
+  const uint64_t npad = rup(n, 64);
+  const int nrb = (int)cdiv(m, kRowBlk);
+  const uint64_t lda = (uint64_t)nrb * kRowBlk;
+  SLFMM_CHECK(ldout >= lda, SLFMM_E_ARG, "raw_functionals: output stride");
+  double* A = ctx->buf<double>("nmf_A", npad * lda);
+  pad_operand(s, Op, ldo, n, (int)m, A, lda, npad);
+  launch_func_f64(s, Y, ld, pe, npad, A, lda, nrb, mu, out, ldout, nullptr);
+  double* cs = ctx->buf<double>("nmf_cs", m);
+  colreduce<1, kColSum>(ctx, FnColSum{Op, ldo}, GStore{cs}, n, (int)m);
+  k_add_mean_term<<<(unsigned)cdiv(pe * m, 256), 256, 0, s>>>(out, ldout, pe, (int)m, mu, cs);
+  SLFMM_HIP(hipGetLastError());
+}
+
+static void normalize_w(slfmm_ctx* ctx, double* W, uint64_t n, double* Ht, uint64_t pe, int K, double eps) {
+  double* cs = ctx->buf<double>("nmf_wsum", K);
+  colreduce<1, kColSum>(ctx, FnColSum{W, (uint64_t)K}, GStore{cs}, n, K);
+  k_nmf_scale<<<(unsigned)std::min<uint64_t>(cdiv((n + pe) * K, 256), 8192), 256, 0, ctx->stream>>>(W, n, Ht, pe, K,
+                                                                                                    cs, eps);
+  SLFMM_HIP(hipGetLastError());
+}
+
+// NMF estimation on the rows Y (pe SNPs, row stride ld, kSlack after the
+// last row).  W (n x K), Ht (pe x K) and cv (3 x n_iter: nmf MAE, mean MAE,
+// count per iteration) are device outputs; omega (n x (K + 10)) may be
+// injected, else it is drawn from seed.
+void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const slfmm_nmf_config& cfg,
+             const double* d_omega, double* W, double* Ht, double* cv) {
+  hipStream_t s = ctx->stream;
+  const int K = (int)cfg.k;
+  const uint64_t l2 = cfg.k + 10;
+  const double eps = cfg.eps;
+  SLFMM_CHECK(K >= 1 && K <= kNmfMaxK, SLFMM_E_ARG, "NMF rank k must be 1..16");
+  SLFMM_CHECK(pe >= 1 && n >= 1, SLFMM_E_ARG, "empty estimation panel");
+  SLFMM_CHECK(cfg.chunk_size >= 1, SLFMM_E_ARG, "chunk_size must be positive");
+  int32_t* nmiss = ctx->buf<int32_t>("nmf_nmiss", pe);
+  int32_t* sumg = ctx->buf<int32_t>("nmf_sumg", pe);
+  int32_t* nhom = ctx->buf<int32_t>("nmf_nhom", pe);
+  double* mu = ctx->buf<double>("nmf_mu", pe);
+  double* sc = ctx->buf<double>("nmf_sc", pe);
+  launch_stats(s, Y, ld, n, pe, nmiss, sumg, nhom);
+  launch_mean_scale(s, nmiss, sumg, n, pe, SLFMM_NORM_CENTER_ONLY, mu, sc);
+
+  // -- init_from_random_probe: Z = Y_meanfill^T Omega', V = right singular
+  //    vectors of Z, W = |Omega' V_k|, H = max(0, W^+ Y_meanfill)
+  double* Om = ctx->buf<double>("nmf_omega", n * l2);
+  if (d_omega) SLFMM_HIP(hipMemcpyAsync(Om, d_omega, n * l2 * 8, hipMemcpyDeviceToDevice, s));
+  else launch_normal(s, Om, n, l2, l2, cfg.seed);
+  const uint64_t ldz = rup(l2, kRowBlk);
+  double* Z = ctx->buf<double>("nmf_Z", pe * ldz);
+  raw_functionals(ctx, Y, ld, n, pe, mu, Om, l2, l2, Z, ldz);
+  double* G = ctx->buf<double>("nmf_G", l2 * l2);
+  ts_gram(ctx, Z, ldz, l2, Z, ldz, l2, pe, G);
+  double* V = ctx->buf<double>("nmf_V", l2 * l2);
+  double* sv = ctx->buf<double>("nmf_sv", l2);
+  jacobi_svd_small(ctx, G, l2, V, sv);
+  ts_gemm(s, Om, l2, l2, V, l2, K, n, 1.0, 0.0, W, K);
+  k_abs<<<(unsigned)cdiv(n * K, 256), 256, 0, s>>>(W, n * K);
+  double* WtW = ctx->buf<double>("nmf_WtW", K * K);
+  double* Winv = ctx->buf<double>("nmf_Winv", K * K);
+  ts_gram(ctx, W, K, K, W, K, K, n, WtW);
+  safe_inv_dev(ctx, WtW, K, Winv,
+               ctx->check_slot("Warning: W_init^T W_init is singular, adding epsilon diagonal regularization",
+                               SLFMM_E_SINGULAR, "W_init^T W_init inversion failed even with epsilon regularization"));
+  double* Wp = ctx->buf<double>("nmf_Wp", n * K);   // (W^+)^T = W (W^T W)^-1
+  ts_gemm(s, W, K, K, Winv, K, K, n, 1.0, 0.0, Wp, K);
+  const uint64_t ldf = rup(K, kRowBlk);
+  double* F = ctx->buf<double>("nmf_F", pe * ldf);
+  raw_functionals(ctx, Y, ld, n, pe, mu, Wp, K, K, F, ldf);
+  k_relu_rows<<<(unsigned)cdiv(pe * K, 256), 256, 0, s>>>(F, ldf, pe, K, 0.0, Ht);
+  k_clamp_min<<<(unsigned)cdiv(n * K, 256), 256, 0, s>>>(W, n * K, eps);
+  k_clamp_min<<<(unsigned)cdiv(pe * K, 256), 256, 0, s>>>(Ht, pe * K, eps);
+  SLFMM_HIP(hipGetLastError());
+  normalize_w(ctx, W, n, Ht, pe, K, eps);
+
+  // -- iterations
+  const uint64_t ntile = cdiv(n, 256);
+  uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(cdiv(pe, 256), cdiv(2048, ntile)));
+  const uint64_t spc = cdiv(pe, nchunk);
+  nchunk = cdiv(pe, spc);
+  double* part = ctx->buf<double>("nmf_part", nchunk * n * K);
+  double* num = ctx->buf<double>("nmf_num", n * K);
+  double* hh = ctx->buf<double>("nmf_hh", K * K);
+  const unsigned cvblk = (unsigned)cdiv(pe, 256);
+  double* cvpart = ctx->buf<double>("nmf_cvpart", (uint64_t)cvblk * 3);
+  for (uint64_t it = 0; it < cfg.n_iter; ++it) {
+    // forward: W update from the current reconstruction
+    k_nmf_fwd<<<dim3((unsigned)ntile, (unsigned)nchunk), 256, 0, s>>>(Y, ld, n, pe, spc, W, Ht, K, part);
+    k_nmf_sum_chunks<<<(unsigned)cdiv(n * K, 256), 256, 0, s>>>(part, n * K, (int)nchunk, num);
+    ts_gram(ctx, Ht, K, K, Ht, K, K, pe, hh);
+    k_nmf_wupd<<<(unsigned)cdiv(n, 256), 256, 0, s>>>(W, num, hh, n, K, eps);
+    SLFMM_HIP(hipGetLastError());
+    normalize_w(ctx, W, n, Ht, pe, K, eps);
+    // backward: H update in place, fill from the new W and the old H
+    ts_gram(ctx, W, K, K, W, K, K, n, WtW);
+    k_nmf_bwd<<<(unsigned)cdiv(pe, 256), 256, 0, s>>>(Y, ld, n, pe, W, WtW, K, eps, Ht);
+    SLFMM_HIP(hipGetLastError());
+    normalize_w(ctx, W, n, Ht, pe, K, eps);
+    // held-out error of this iteration
+    k_nmf_cv<<<cvblk, 256, 0, s>>>(Y, ld, n, pe, W, Ht, K, nmiss, sumg, cfg.seed + it, cfg.chunk_size, cfg.cv_rate,
+                                   cvpart);
+    k_nmf_cv_final<<<1, 64, 0, s>>>(cvpart, (int)cvblk, cv + 3 * it);
+    SLFMM_HIP(hipGetLastError());
+  }
+}
+
+void nmf_transpose_h(hipStream_t s, const double* Ht, uint64_t pe, int K, double* H) {
+  k_transpose_kp<<<(unsigned)cdiv(pe * K, 256), 256, 0, s>>>(Ht, pe, K, H);
+  SLFMM_HIP(hipGetLastError());
+}
+
+}  // namespace slfmm

@@ -10,6 +10,7 @@
 //   recovery   U_hat = N^{-1} U_{NW}[:, :K] (reference: Q D^{-1} U_small[:, :K])
 // so U_hat equals the reference's up to column signs, with no n x n array.
 #include "internal.h"
+#include "rng.h"
 
 namespace slfmm {
 
@@ -114,20 +115,13 @@ static void apply_N(slfmm_ctx* ctx, const double* Qd, uint64_t d, const double* 
 __global__ void k_normal(double* out, uint64_t n, int l, uint64_t ld, uint64_t seed) {
   uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n * (uint64_t)l) return;
-  uint64_t z0, z1;
-  {
-    auto mix = [](uint64_t sd, uint64_t i) {
-      uint64_t z = sd * 0xD1B54A32D192ED03ull + (i + 1) * 0x9E3779B97F4A7C15ull;
-      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-      return z ^ (z >> 31);
-    };
-    z0 = mix(seed, 2 * e);
-    z1 = mix(seed, 2 * e + 1);
-  }
-  double u1 = 1.0 - (double)(z0 >> 11) * (1.0 / 9007199254740992.0);
-  double u2 = (double)(z1 >> 11) * (1.0 / 9007199254740992.0);
-  out[(e / l) * ld + (e % l)] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  out[(e / l) * ld + (e % l)] = normal01(seed, e);
+}
+
+void launch_normal(hipStream_t s, double* out, uint64_t n, uint64_t l, uint64_t ld, uint64_t seed) {
+  if (!n || !l) return;
+  k_normal<<<(unsigned)cdiv(n * l, 256), 256, 0, s>>>(out, n, (int)l, ld, seed);
+  SLFMM_HIP(hipGetLastError());
 }
 
 __global__ void k_pad_rows(const double* __restrict__ src, uint64_t lds, uint64_t rows, int cols,
@@ -184,6 +178,31 @@ void precompute_dev(slfmm_ctx* ctx, const double* d_xs, uint64_t n, uint64_t d, 
                                SLFMM_E_SINGULAR, "X^T X + λI (ridge) inversion failed even with epsilon regularization"));
 }
 
+// Rows h_idx[0..n_idx) of g as a contiguous packed block (kSlack zero bytes
+// after the last row); g's own buffer when the subset is every SNP in order.
+const uint8_t* gather_subset(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_t n_idx,
+                             const char* tag, uint64_t* pe) {
+  hipStream_t s = ctx->stream;
+  bool identity = (h_idx == nullptr);
+  if (h_idx) {
+    identity = (n_idx == g->p);
+    for (uint64_t i = 0; identity && i < n_idx; ++i) identity = (h_idx[i] == (int64_t)i);
+  }
+  if (identity) {
+    *pe = g->p;
+    return g->d;
+  }
+  for (uint64_t i = 0; i < n_idx; ++i)
+    SLFMM_CHECK(h_idx[i] >= 0 && (uint64_t)h_idx[i] < g->p, SLFMM_E_ARG, "estimation SNP index out of range");
+  int64_t* didx = ctx->buf<int64_t>(std::string(tag) + "_idx", n_idx);
+  SLFMM_HIP(hipMemcpyAsync(didx, h_idx, n_idx * 8, hipMemcpyHostToDevice, s));
+  uint8_t* Yg = ctx->buf<uint8_t>(tag, n_idx * g->ld + kSlack);
+  SLFMM_HIP(hipMemsetAsync(Yg + n_idx * g->ld, 0, kSlack, s));
+  geno_gather(ctx, g, didx, n_idx, Yg);
+  *pe = n_idx;
+  return Yg;
+}
+
 // ---- randomized SVD ---------------------------------------------------------------
 void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_t n_idx, const double* d_xs,
               uint64_t d, const slfmm_config& cfg, const double* d_b0, double* d_uhat) {
@@ -198,24 +217,8 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   precompute_dev(ctx, d_xs, n, d, cfg.lambda, sigma, dlam, Qd, ridge);
 
   // estimation SNPs as a contiguous packed block
-  const uint8_t* Y = g->d;
-  uint64_t pe = g->p;
-  bool identity = (h_idx == nullptr);
-  if (h_idx) {
-    identity = (n_idx == g->p);
-    for (uint64_t i = 0; identity && i < n_idx; ++i) identity = (h_idx[i] == (int64_t)i);
-  }
-  if (!identity) {
-    for (uint64_t i = 0; i < n_idx; ++i)
-      SLFMM_CHECK(h_idx[i] >= 0 && (uint64_t)h_idx[i] < g->p, SLFMM_E_ARG, "estimation SNP index out of range");
-    int64_t* didx = ctx->buf<int64_t>("rs_idx", n_idx);
-    SLFMM_HIP(hipMemcpyAsync(didx, h_idx, n_idx * 8, hipMemcpyHostToDevice, s));
-    uint8_t* Yg = ctx->buf<uint8_t>("rs_Y", n_idx * g->ld + kSlack);
-    SLFMM_HIP(hipMemsetAsync(Yg + n_idx * g->ld, 0, kSlack, s));
-    geno_gather(ctx, g, didx, n_idx, Yg);
-    Y = Yg;
-    pe = n_idx;
-  }
+  uint64_t pe = 0;
+  const uint8_t* Y = gather_subset(ctx, g, h_idx, n_idx, "rs_Y", &pe);
   SLFMM_CHECK(pe >= l, SLFMM_E_ARG, "need at least K + oversampling estimation SNPs");
   int32_t* nmiss = ctx->buf<int32_t>("rs_nmiss", pe);
   int32_t* sumg = ctx->buf<int32_t>("rs_sumg", pe);
@@ -245,7 +248,7 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   if (d_b0) {
     SLFMM_HIP(hipMemcpyAsync(B, d_b0, n * l * 8, hipMemcpyDeviceToDevice, s));
   } else {
-    k_normal<<<(unsigned)cdiv(n * l, 256), 256, 0, s>>>(B, n, (int)l, l, cfg.seed);
+    launch_normal(s, B, n, l, l, cfg.seed);
     apply_N(ctx, Qd, d, dlam, B, l, l, n, false);
   }
   auto project = [&]() {   // Z = Y^T B, scaled per SNP
