@@ -191,6 +191,19 @@ int slfmm_nmf_estimate(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* subse
                        const slfmm_nmf_config* cfg, const double* omega, double* w, double* h,
                        double* cv_mae, double* cv_mae_mean, uint64_t* mask_count, uint32_t flags);
 
+/* fit_lfmm2 with NMF imputation (src/lib.rs:162-263, ImputeConfig::Nmf*),
+ * FP64: NMF estimation on est's subset, W^+ = safe_inv(W^T W) W^T, the RSVD
+ * on the NMF-filled estimation rows (NmfInRam) and the association on every
+ * SNP of g with NMF on-the-fly fill (NmfOnTheFly).  nmf_omega
+ * (n x (nmf k + 10)) may be injected; nmf_cv_mae (n_iter, host) receives the
+ * estimation CV error per iteration; gwas_cv (3, host: NMF MAE, mean-fill
+ * MAE, held-out count) enables the association-phase CV with rate
+ * nmf->cv_rate and seed cfg->seed (NULL disables it). */
+int slfmm_fit_lfmm2_nmf(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset_idx, uint64_t n_idx,
+                        const slfmm_geno* g, const double* x, uint64_t d, const slfmm_config* cfg,
+                        const slfmm_nmf_config* nmf, const double* b0, const double* nmf_omega, double* u_hat,
+                        slfmm_assoc_out* out, double* nmf_cv_mae, double* gwas_cv, uint32_t flags);
+
 /* ---- Kernel timing (no reference counterpart; used by the benchmark) ---- */
 
 /* Enable HIP-event timing of the genotype-stream kernels on the context's

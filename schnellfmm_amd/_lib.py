@@ -58,6 +58,7 @@ SIGNATURES = {
     "slfmm_test_associations": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, u32]),
     "slfmm_fit_lfmm2": (C.c_int, [vp, vp, vp, u64, vp, vp, u64, vp, vp, vp, vp, u32]),
     "slfmm_nmf_estimate": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp, u32]),
+    "slfmm_fit_lfmm2_nmf": (C.c_int, [vp, vp, vp, u64, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp, vp, u32]),
     "slfmm_debug_mfma_i8": (C.c_int, [vp, vp, vp, vp]),
     "slfmm_profile_enable": (C.c_int, [vp, i32]),
     "slfmm_profile_read": (C.c_int, [vp, C.c_char_p, vp, vp, i32]),

@@ -238,8 +238,8 @@ class TestResults:
     r2: np.ndarray
     pvalue: np.ndarray
     hist: Optional[np.ndarray] = None
-    nmf_cv: Optional[list] = None
-    nmf_cv_mean: Optional[float] = None
+    nmf_cv: Optional[np.ndarray] = None          # estimation CV MAE per NMF iteration
+    gwas_cv: Optional[tuple] = None              # (NMF MAE, mean-fill MAE, held-out count)
 
     __test__ = False   # not a pytest class
 
@@ -354,3 +354,31 @@ def fit_lfmm2(y_est: BedFile, est_subset, y_full: BedFile, x, config: Lfmm2Confi
                                     _ptr(x), d, C.byref(cfg), _ptr(b), _ptr(u), C.byref(out), 0))
     beta, t, r2, p, hist, gpt, gif = arrs
     return TestResults(u, float(gif[0]), gpt, beta, t, r2, p, hist)
+
+
+def fit_lfmm2_nmf(y_est: BedFile, est_subset, y_full: BedFile, x, config: Lfmm2Config, nmf: NmfParams,
+                  b0=None, nmf_omega=None, gwas_cv: bool = True, with_hist: bool = False) -> TestResults:
+    """Full pipeline with NMF imputation, src/lib.rs:162-263 (FP64)."""
+    ctx = y_full.ctx
+    x = _f64(x)
+    idx = None if est_subset is None or (isinstance(est_subset, str) and est_subset == "all") \
+        else subset_indices(est_subset, y_est.n_snps)
+    d = x.shape[1]
+    arrs = _assoc_arrays(y_full.n_snps, d, with_hist)
+    out = _out_struct(arrs)
+    u = np.empty((y_full.n_samples, config.k))
+    cfg = config.to_c()
+    ncfg = nmf.to_c()
+    b = None if b0 is None else _f64(b0)
+    om = None if nmf_omega is None else _f64(nmf_omega)
+    cv_iter = np.empty(max(nmf.n_iter, 1))
+    gcv = np.zeros(3) if gwas_cv else None
+    ctx.check(lib().slfmm_fit_lfmm2_nmf(ctx.h, y_est.h, _ptr(idx), 0 if idx is None else len(idx), y_full.h,
+                                        _ptr(x), d, C.byref(cfg), C.byref(ncfg), _ptr(b), _ptr(om), _ptr(u),
+                                        C.byref(out), _ptr(cv_iter), _ptr(gcv), 0))
+    beta, t, r2, p, hist, gpt, gif = arrs
+    res = TestResults(u, float(gif[0]), gpt, beta, t, r2, p, hist)
+    res.nmf_cv = cv_iter[:nmf.n_iter].copy()
+    if gcv is not None:
+        res.gwas_cv = (float(gcv[0]), float(gcv[1]), int(gcv[2]))
+    return res

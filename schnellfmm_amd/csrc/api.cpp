@@ -543,6 +543,85 @@ int slfmm_nmf_estimate(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* subse
   });
 }
 
+int slfmm_fit_lfmm2_nmf(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset_idx, uint64_t n_idx,
+                        const slfmm_geno* g, const double* x, uint64_t d, const slfmm_config* cfg,
+                        const slfmm_nmf_config* nmf, const double* b0, const double* nmf_omega, double* u_hat,
+                        slfmm_assoc_out* out, double* nmf_cv_mae, double* gwas_cv, uint32_t flags) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && est && g && x && out && cfg && nmf, SLFMM_E_ARG, "NULL argument");
+    SLFMM_CHECK(est->n == g->n, SLFMM_E_ARG, "estimation and test panels must have the same samples");
+    slfmm_config c = *cfg;
+    c.impute = SLFMM_IMPUTE_MEAN;   // the fills are passed explicitly below
+    check_cfg(&c);
+    SLFMM_CHECK(c.precision == SLFMM_PREC_FP64, SLFMM_E_ARG, "NMF imputation runs on the FP64 path (use SLFMM_PREC_FP64)");
+    SLFMM_CHECK(nmf->k >= 1 && nmf->k <= 16, SLFMM_E_ARG, "NMF rank k must be 1..16");
+    hipStream_t s = ctx->stream;
+    const uint64_t n = g->n, K = c.k, l = K + c.oversampling, kn = nmf->k;
+    std::vector<double> xs(n * d);
+    center_host(x, n, d, c.scale_cov, xs.data());                       // src/lib.rs:170
+    double* dx = to_dev(ctx, "api_xs", xs.data(), n * d);
+    const double* db0 = b0 ? to_dev(ctx, "api_b0", b0, n * l) : nullptr;
+    const double* dom = nmf_omega ? to_dev(ctx, "api_nmf_omega", nmf_omega, n * (kn + 10)) : nullptr;
+    // NMF estimation on the estimation rows (src/lib.rs:186-198)
+    uint64_t pe = 0;
+    const int64_t* idx = subset_idx;
+    const uint64_t nidx = subset_idx ? n_idx : est->p;
+    const uint8_t* Y = gather_subset(ctx, est, idx, nidx, "nmf_Y", &pe);
+    double* W = ctx->buf<double>("fit_nmf_w", n * kn);
+    double* Ht = ctx->buf<double>("fit_nmf_ht", pe * kn);
+    double* dcv = ctx->buf<double>("fit_nmf_cv", 3 * std::max<uint64_t>(nmf->n_iter, 1));
+    nmf_run(ctx, Y, est->ld, n, pe, *nmf, dom, W, Ht, dcv);
+    // W^+ = safe_inv(W^T W) W^T, kept as its transpose Wp = W (W^T W)^-1 (src/lib.rs:199-206)
+    double* WtW = ctx->buf<double>("fit_nmf_wtw", kn * kn);
+    double* Winv = ctx->buf<double>("fit_nmf_winv", kn * kn);
+    double* Wp = ctx->buf<double>("fit_nmf_wp", n * kn);
+    ts_gram(ctx, W, kn, kn, W, kn, kn, n, WtW);
+    safe_inv_dev(ctx, WtW, kn, Winv,
+                 ctx->check_slot("Warning: W^T W for NMF pinv is singular, adding epsilon diagonal regularization",
+                                 SLFMM_E_SINGULAR, "W^T W for NMF pinv inversion failed even with epsilon regularization"));
+    ts_gemm(s, W, kn, kn, Winv, kn, kn, n, 1.0, 0.0, Wp, kn);
+    // RSVD on the NMF-filled estimation rows (NmfInRam, src/lib.rs:223-231)
+    const bool dev = flags & SLFMM_OUT_DEVICE;
+    double* du = (dev && u_hat) ? u_hat : ctx->buf<double>("api_uhat", n * K);
+    FillSpec fill;
+    fill.W = W; fill.Ht = Ht; fill.K = (int)kn; fill.n = n;
+    rsvd_run(ctx, est, idx, nidx, dx, d, c, db0, du, &fill);
+    // association with NMF on-the-fly fill and the GWAS-phase CV (src/lib.rs:233-254)
+    NmfAssoc na;
+    na.W = W; na.Wp = Wp; na.K = (int)kn;
+    double* dgcv = ctx->buf<double>("fit_gwas_cv", 3);
+    if (gwas_cv) {
+      na.cv_rate = nmf->cv_rate; na.cv_seed = c.seed; na.chunk = c.chunk_size; na.cv_out = dgcv;
+    }
+    const uint64_t p = g->p;
+    double* beta = dev ? out->beta : ctx->buf<double>("api_beta", p * d);
+    double* t = dev ? out->tstat : ctx->buf<double>("api_t", p * d);
+    double* r2 = dev ? out->r2 : ctx->buf<double>("api_r2", p * 3);
+    double* pv = dev ? out->pvalue : ctx->buf<double>("api_p", p * d);
+    uint64_t* hist = (dev && out->hist) ? out->hist : (out->hist ? ctx->buf<uint64_t>("api_hist", d * kBins) : nullptr);
+    double* gpt = dev ? out->gif_per_trait : ctx->buf<double>("api_gpt", d);
+    double* gif = dev ? out->gif : ctx->buf<double>("api_gif", 1);
+    SLFMM_CHECK(beta && t && r2 && pv && gpt && gif, SLFMM_E_ARG, "NULL output pointer");
+    assoc_run(ctx, g, dx, d, du, K, c.lambda, c, beta, t, r2, pv, hist, gpt, gif, &na);
+    if (!dev) {
+      to_host(ctx, out->beta, beta, p * d);
+      to_host(ctx, out->tstat, t, p * d);
+      to_host(ctx, out->r2, r2, p * 3);
+      to_host(ctx, out->pvalue, pv, p * d);
+      if (out->hist) to_host(ctx, out->hist, hist, d * kBins);
+      to_host(ctx, out->gif_per_trait, gpt, d);
+      to_host(ctx, out->gif, gif, 1);
+      if (u_hat) to_host(ctx, u_hat, du, n * K);
+    }
+    std::vector<double> hcv(3 * std::max<uint64_t>(nmf->n_iter, 1), 0.0);
+    if (nmf->n_iter) to_host(ctx, hcv.data(), dcv, 3 * nmf->n_iter);
+    if (gwas_cv) to_host(ctx, gwas_cv, dgcv, 3);
+    SLFMM_HIP(hipStreamSynchronize(s));
+    if (nmf_cv_mae)
+      for (uint64_t i = 0; i < nmf->n_iter; ++i) nmf_cv_mae[i] = hcv[3 * i];
+  });
+}
+
 int slfmm_debug_mfma_i8(slfmm_ctx* ctx, const int8_t* a, const int8_t* b, int32_t* d) {
   return guarded(ctx, [&] {
     SLFMM_CHECK(ctx && a && b && d, SLFMM_E_ARG, "NULL argument");

@@ -28,11 +28,16 @@ namespace slfmm {
 // wave w -> rows 16*(w&1)..+16 of the row block, SNP group w>>1; lane ->
 // 2 SNPs.  A k-block of 64 samples of A is staged in LDS.
 // ---------------------------------------------------------------------------
+// With FILL, missing calls take the value W_k . H_j - mu_j (NMF imputation
+// after centring): the 64 W rows of the block are staged beside A, and each
+// lane keeps the K loadings of its two SNPs in registers.
+template <bool FILL>
 __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t p,
                                                   uint64_t npad, const double* __restrict__ A, uint64_t lda,
                                                   const double* __restrict__ mu, double* __restrict__ F,
-                                                  uint64_t ldF, const double* __restrict__ outscale) {
+                                                  uint64_t ldF, const double* __restrict__ outscale, FillSpec fs) {
   __shared__ double As[64][33];
+  __shared__ double Ws[FILL ? 64 : 1][kNmfMaxK + 1];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int half = w & 1, grp = w >> 1;
   const int rb = blockIdx.y;
@@ -42,6 +47,14 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
   const double m0 = ok0 ? mu[j0] : 0.0, m1 = ok1 ? mu[j1] : 0.0;
   const double l00 = -m0, l02 = 1.0 - m0, l03 = 2.0 - m0;
   const double l10 = -m1, l12 = 1.0 - m1, l13 = 2.0 - m1;
+  double h0[FILL ? kNmfMaxK : 1], h1[FILL ? kNmfMaxK : 1];
+  if constexpr (FILL) {
+#pragma unroll
+    for (int r = 0; r < kNmfMaxK; ++r) {
+      h0[r] = (ok0 && r < fs.K) ? fs.Ht[j0 * fs.K + r] : 0.0;
+      h1[r] = (ok1 && r < fs.K) ? fs.Ht[j1 * fs.K + r] : 0.0;
+    }
+  }
   double acc0[16], acc1[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) { acc0[r] = 0.0; acc1[r] = 0.0; }
@@ -54,6 +67,13 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
       int kk = e >> 5, r = e & 31;
       As[kk][r] = A[(kb * 64 + kk) * lda + rb * 32 + r];
     }
+    if constexpr (FILL) {
+      for (int e = t; e < 64 * kNmfMaxK; e += 256) {
+        const int kk = e / kNmfMaxK, r = e % kNmfMaxK;
+        const uint64_t k = kb * 64 + kk;
+        Ws[kk][r] = (k < fs.n && r < fs.K) ? fs.W[k * fs.K + r] : 0.0;
+      }
+    }
     uint4 q0 = ok0 ? row0[kb] : make_uint4(0, 0, 0, 0);
     uint4 q1 = ok1 ? row1[kb] : make_uint4(0, 0, 0, 0);
     __syncthreads();
@@ -62,8 +82,21 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
     for (int kk = 0; kk < 64; ++kk) {
       uint32_t c0 = (w0[kk >> 4] >> (2 * (kk & 15))) & 3u;
       uint32_t c1 = (w1[kk >> 4] >> (2 * (kk & 15))) & 3u;
-      double v0 = (c0 & 2u) ? ((c0 & 1u) ? l03 : l02) : ((c0 & 1u) ? 0.0 : l00);
-      double v1 = (c1 & 2u) ? ((c1 & 1u) ? l13 : l12) : ((c1 & 1u) ? 0.0 : l10);
+      double miss0 = 0.0, miss1 = 0.0;
+      if constexpr (FILL) {
+        if (c0 == 1u || c1 == 1u) {
+          double f0 = 0.0, f1 = 0.0;
+#pragma unroll
+          for (int r = 0; r < kNmfMaxK; ++r) {
+            f0 += Ws[kk][r] * h0[r];
+            f1 += Ws[kk][r] * h1[r];
+          }
+          miss0 = f0 - m0;
+          miss1 = f1 - m1;
+        }
+      }
+      double v0 = (c0 & 2u) ? ((c0 & 1u) ? l03 : l02) : ((c0 & 1u) ? miss0 : l00);
+      double v1 = (c1 & 2u) ? ((c1 & 1u) ? l13 : l12) : ((c1 & 1u) ? miss1 : l10);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         double a = As[kk][half * 16 + r];
@@ -81,13 +114,106 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
     for (int r = 0; r < 16; ++r) F[j1 * ldF + rb * 32 + half * 16 + r] = acc1[r] * s1;
 }
 
-// Host launcher shared with the RSVD projection (Z = Y^T B).
+// Host launcher shared with the RSVD projection (Z = Y^T B) and the NMF
+// functionals; `fill` (optional) selects NMF imputation of missing calls.
 void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t p, uint64_t npad,
                      const double* A, uint64_t lda, int nrb, const double* mu, double* F, uint64_t ldF,
-                     const double* outscale) {
+                     const double* outscale, const FillSpec* fill) {
   if (p == 0) return;
   dim3 grid((unsigned)cdiv(p, 256), (unsigned)nrb);
-  k_func_f64<<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale);
+  if (fill && fill->W)
+    k_func_f64<true><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, *fill);
+  else
+    k_func_f64<false><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, FillSpec{});
+  SLFMM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Moments of NMF-filled rows (decode with fill, src/bed.rs:211-235): one
+// thread per row; S1 = sum of the fills, S2 = sum of their squares, and with
+// Wp the sum of the Wp rows over the missing calls.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_fill_stats(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t rows,
+                                                    FillSpec fs, const double* __restrict__ Wp,
+                                                    double* __restrict__ S1, double* __restrict__ S2,
+                                                    double* __restrict__ SWp) {
+  __shared__ double sW[64][kNmfMaxK + 1];
+  __shared__ double sP[64][kNmfMaxK + 1];
+  const int t = threadIdx.x;
+  const uint64_t j = (uint64_t)blockIdx.x * 256 + t;
+  const bool ok = j < rows;
+  const int K = fs.K;
+  double h[kNmfMaxK], sp[kNmfMaxK];
+#pragma unroll
+  for (int r = 0; r < kNmfMaxK; ++r) {
+    h[r] = (ok && r < K) ? fs.Ht[j * K + r] : 0.0;
+    sp[r] = 0.0;
+  }
+  double s1 = 0.0, s2 = 0.0;
+  const uint4* row = reinterpret_cast<const uint4*>(packed + (ok ? j : 0) * ld);
+  const uint64_t nkb = cdiv(fs.n, 64);
+  for (uint64_t kb = 0; kb < nkb; ++kb) {
+    __syncthreads();
+    for (int e = t; e < 64 * kNmfMaxK; e += 256) {
+      const int kk = e / kNmfMaxK, r = e % kNmfMaxK;
+      const uint64_t k = kb * 64 + kk;
+      const bool in = k < fs.n && r < K;
+      sW[kk][r] = in ? fs.W[k * K + r] : 0.0;
+      if (Wp) sP[kk][r] = in ? Wp[k * K + r] : 0.0;
+    }
+    const uint4 q = ok ? row[kb] : make_uint4(0, 0, 0, 0);
+    __syncthreads();
+    const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
+    for (int kk = 0; kk < 64; ++kk) {
+      if (((wd[kk >> 4] >> (2 * (kk & 15))) & 3u) != 1u) continue;
+      double f = 0.0;
+#pragma unroll
+      for (int r = 0; r < kNmfMaxK; ++r) f += sW[kk][r] * h[r];
+      s1 += f;
+      s2 += f * f;
+      if (Wp)
+#pragma unroll
+        for (int r = 0; r < kNmfMaxK; ++r) sp[r] += sP[kk][r];
+    }
+  }
+  if (!ok) return;
+  S1[j] = s1;
+  S2[j] = s2;
+  if (SWp)
+    for (int r = 0; r < K; ++r) SWp[j * K + r] = sp[r];
+}
+
+__global__ void k_fill_moments(const int32_t* __restrict__ nmiss, const int32_t* __restrict__ sumg,
+                               const int32_t* __restrict__ nhom, const double* __restrict__ S1,
+                               const double* __restrict__ S2, uint64_t n, uint64_t rows, int norm,
+                               double* __restrict__ mu, double* __restrict__ sc, double* __restrict__ tss) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= rows) return;
+  const double nm = (double)nmiss[j], sg = (double)sumg[j], nobs = (double)n - nm;
+  const double sg2 = sg + 2.0 * (double)nhom[j];   // sum of g^2 over observed calls
+  const double m = n ? (sg + S1[j]) / (double)n : 0.0;
+  const double s = epi_scale(m, norm);
+  const double obs = sg2 - 2.0 * m * sg + nobs * m * m;
+  const double fil = S2[j] - 2.0 * m * S1[j] + nm * m * m;
+  mu[j] = m;
+  sc[j] = s;
+  if (tss) tss[j] = s * s * (obs + fil);
+}
+
+void fill_moments(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t rows, const FillSpec& fs, int norm,
+                  const double* Wp, double* mu, double* sc, double* tss, double* SWp) {
+  hipStream_t s = ctx->stream;
+  SLFMM_CHECK(fs.K >= 1 && fs.K <= kNmfMaxK, SLFMM_E_ARG, "NMF rank must be 1..16");
+  int32_t* nmiss = ctx->buf<int32_t>("fl_nmiss", rows);
+  int32_t* sumg = ctx->buf<int32_t>("fl_sumg", rows);
+  int32_t* nhom = ctx->buf<int32_t>("fl_nhom", rows);
+  double* S1 = ctx->buf<double>("fl_s1", rows);
+  double* S2 = ctx->buf<double>("fl_s2", rows);
+  launch_stats(s, packed, ld, fs.n, rows, nmiss, sumg, nhom);
+  if (rows) {
+    k_fill_stats<<<(unsigned)cdiv(rows, 256), 256, 0, s>>>(packed, ld, rows, fs, Wp, S1, S2, SWp);
+    k_fill_moments<<<(unsigned)cdiv(rows, 256), 256, 0, s>>>(nmiss, sumg, nhom, S1, S2, fs.n, rows, norm, mu, sc, tss);
+  }
   SLFMM_HIP(hipGetLastError());
 }
 
@@ -487,7 +613,7 @@ void launch_func_i8(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_Q, uint
 
 void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t d, const double* d_u,
                uint64_t k, double lambda, const slfmm_config& cfg, double* d_beta, double* d_t,
-               double* d_r2, double* d_p, uint64_t* d_hist, double* d_gif_pt, double* d_gif) {
+               double* d_r2, double* d_p, uint64_t* d_hist, double* d_gif_pt, double* d_gif, const NmfAssoc* nmf) {
   hipStream_t s = ctx->stream;
   const uint64_t n = g->n, p = g->p;
   SLFMM_CHECK(n > 1 + d + k, SLFMM_E_DF,
@@ -516,13 +642,39 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
   ea.norm = cfg.norm; ea.df = (double)(n - 1 - d - k); ea.plan = pl.plan;
   ea.n_miss = nmiss; ea.sum_g = sumg; ea.n_hom = nhom;
   ea.beta = d_beta; ea.tstat = d_t; ea.r2 = d_r2;
+  const bool with_nmf = nmf && nmf->W;
   if (cfg.precision == SLFMM_PREC_I8) {
+    SLFMM_CHECK(!with_nmf, SLFMM_E_ARG, "NMF imputation runs on the FP64 path (use SLFMM_PREC_FP64)");
     launch_func_i8(ctx, g, pl.Q, m, ea, true);
   } else {
     launch_stats(s, g->d, g->ld, n, p, nmiss, sumg, nhom);
     double* mu = ctx->buf<double>("as_mu", p);
     double* sc = ctx->buf<double>("as_sc", p);
     launch_mean_scale(s, nmiss, sumg, n, p, cfg.norm, mu, sc);
+    FillSpec fs;
+    if (with_nmf) {
+      // NMF on-the-fly imputation (src/parallel.rs:252-282): H_j = max(0, W^+ y_j)
+      // of the mean-filled raw column, missing calls filled with W_k . H_j
+      const int K = nmf->K;
+      const uint64_t ldh = rup(K, kRowBlk);
+      double* Hraw = ctx->buf<double>("as_nmf_hraw", p * ldh);
+      double* Ht = ctx->buf<double>("as_nmf_ht", p * K);
+      raw_functionals(ctx, g->d, g->ld, n, p, mu, nmf->Wp, K, K, Hraw, ldh);
+      relu_rows(s, Hraw, ldh, p, K, Ht);
+      fs.W = nmf->W; fs.Ht = Ht; fs.K = K; fs.n = n;
+      double* fsc = ctx->buf<double>("as_nmf_sc", p);
+      double* ftss = ctx->buf<double>("as_nmf_tss", p);
+      double* fmu = ctx->buf<double>("as_nmf_mu", p);
+      const bool cv = nmf->cv_rate > 0.0 && nmf->cv_out;
+      double* SWp = cv ? ctx->buf<double>("as_nmf_swp", p * K) : nullptr;
+      fill_moments(ctx, g->d, g->ld, p, fs, cfg.norm, cv ? nmf->Wp : nullptr, fmu, fsc, ftss, SWp);
+      if (cv)
+        gwas_cv(ctx, g->d, g->ld, n, p, nmf->W, nmf->Wp, K, Hraw, ldh, SWp, nmiss, sumg, nmf->cv_seed, nmf->chunk,
+                nmf->cv_rate, nmf->cv_out);
+      mu = fmu;
+      ea.fill_sc = fsc;
+      ea.fill_tss = ftss;
+    }
     const uint64_t npad = rup(n, 64);
     const int nrb = (int)cdiv(m, kRowBlk);
     const uint64_t lda = (uint64_t)nrb * kRowBlk;
@@ -531,7 +683,7 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
     double* F = ctx->buf<double>("as_F", p * lda);
     {
       slfmm_timed tm(ctx, "assoc_stream");
-      launch_func_f64(s, g->d, g->ld, p, npad, A, lda, nrb, mu, F, lda, nullptr);
+      launch_func_f64(s, g->d, g->ld, p, npad, A, lda, nrb, mu, F, lda, nullptr, with_nmf ? &fs : nullptr);
     }
     const size_t shm = plan_len((int)d, (int)m) * sizeof(double);
     if (m <= 32) k_epilogue<32><<<(unsigned)cdiv(p, 128), 128, shm, s>>>(ea, F, lda);

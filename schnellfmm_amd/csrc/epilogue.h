@@ -20,6 +20,9 @@ struct EpiArgs {
   double* beta;   // p x d
   double* tstat;  // p x d
   double* r2;     // p x 3
+  // NMF-filled columns: per-SNP scale and TSS replace the integer-count ones
+  const double* fill_sc = nullptr;
+  const double* fill_tss = nullptr;
 };
 
 // compute_scale (src/bed.rs:382-395); contraction-free in this file too.
@@ -37,9 +40,13 @@ __device__ void snp_epilogue(const EpiArgs& a, const double* sPlan, uint64_t j, 
   const int d = a.d, m = a.m;
   const int64_t nobs = (int64_t)a.n - nmiss;
   const double mean = nobs > 0 ? (double)sg / (double)nobs : 0.0;
-  const double sc = epi_scale(mean, a.norm);
+  double sc = epi_scale(mean, a.norm);
   const int64_t sg2 = sg + 2 * nh;
-  const double tss = nobs > 0 ? sc * sc * ((double)(nobs * sg2 - sg * sg) / (double)nobs) : 0.0;
+  double tss = nobs > 0 ? sc * sc * ((double)(nobs * sg2 - sg * sg) / (double)nobs) : 0.0;
+  if (a.fill_sc) {
+    sc = a.fill_sc[j];
+    tss = a.fill_tss[j];
+  }
   double fy[MAXM];
 #pragma unroll
   for (int r = 0; r < MAXM; ++r) fy[r] = r < m ? sc * fu[r] : 0.0;

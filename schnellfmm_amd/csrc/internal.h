@@ -40,6 +40,7 @@ constexpr int kRowBlk = 32;          // rows per FP64 functional row-block
 // 16-byte segments of a 256-sample block and may run past ld by < 128 bytes.
 constexpr uint64_t kSlack = 128;
 constexpr int kMaxM = 64;   // functionals per SNP (K + d, or K + oversampling)
+constexpr int kNmfMaxK = 16; // NMF components
 constexpr int kMaxD = 16;   // traits
 
 __host__ __device__ inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
@@ -160,20 +161,57 @@ void add_diag(hipStream_t s, const double* In, uint64_t m, double lambda, double
 // A V orthogonal; outputs V (m x m) and column norms, sorted descending.
 void jacobi_svd_small(slfmm_ctx* ctx, const double* d_a, uint64_t m, double* d_v, double* d_sv);
 
+// Missing calls of a streamed panel filled with W_k . H_j (NMF imputation,
+// src/parallel.rs:233-321): W is n x K, Ht holds H_j for every streamed row
+// j (rows x K); both device pointers.  W == nullptr means mean imputation.
+struct FillSpec {
+  const double* W = nullptr;
+  const double* Ht = nullptr;
+  int K = 0;
+  uint64_t n = 0;
+};
+
 // ---- assoc.hip --------------------------------------------------------------
 struct AssocPlan;   // defined in assoc.hip
+// FP64 functionals F[j][c] = sum_k A[k][c] v_kj of the centred rows (v = g - mu,
+// missing calls 0, or W_k . H_j - mu with a fill); A is npad x lda (lda a
+// multiple of 32, nrb = lda / 32 row blocks); F rows scaled by outscale.
+void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t p, uint64_t npad,
+                     const double* A, uint64_t lda, int nrb, const double* mu, double* F, uint64_t ldF,
+                     const double* outscale, const FillSpec* fill = nullptr);
+void pad_operand(hipStream_t s, const double* Q, uint64_t ldq, uint64_t n, int m, double* A, uint64_t lda,
+                 uint64_t npad);
+// Per-row moments of a filled panel: mu = (sum_g + S1) / n over all calls,
+// sc = compute_scale(mu, norm), tss = sc^2 sum_k (y_k - mu)^2 (y filled).
+// With Wp (n x K) also SWp[j] = sum over missing k of Wp[k] (rows x K).
+void fill_moments(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t rows, const FillSpec& fs,
+                  int norm, const double* Wp, double* mu, double* sc, double* tss, double* SWp);
+// Association with NMF on-the-fly imputation (src/parallel.rs:252-282): the
+// caller passes W, the pseudo-inverse rows Wp = W (W^T W)^-1 and the GWAS
+// CV settings; cv_out (3) receives (NMF MAE, mean MAE, count) if non-null.
+struct NmfAssoc {
+  const double* W = nullptr;
+  const double* Wp = nullptr;
+  int K = 0;
+  double cv_rate = 0.0;   // 0: no GWAS-phase CV
+  uint64_t cv_seed = 0;
+  uint64_t chunk = 10000;
+  double* cv_out = nullptr;
+};
 void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t d,
                const double* d_u, uint64_t k, double lambda, const slfmm_config& cfg,
                double* d_beta, double* d_t, double* d_r2, double* d_p, uint64_t* d_hist,
-               double* d_gif_pt, double* d_gif);
+               double* d_gif_pt, double* d_gif, const NmfAssoc* nmf = nullptr);
 
 // ---- assoc_i8.hip: single-MFMA lane-layout probe (test-only entry point) ----
 void probe_mfma_i8(slfmm_ctx* ctx, const int8_t* hA, const int8_t* hB, int32_t* hD);
 
 // ---- rsvd.hip ---------------------------------------------------------------
+// fill (optional): NMF in-RAM imputation of the estimation rows, with
+// fill.Ht indexed by estimation row (src/parallel.rs:284-299).
 void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_t n_idx,
               const double* d_xs, uint64_t d, const slfmm_config& cfg, const double* d_b0,
-              double* d_uhat);
+              double* d_uhat, const FillSpec* fill = nullptr);
 // N(0,1) matrix (n x l, row stride ld) from the library's counter-based stream.
 void launch_normal(hipStream_t s, double* out, uint64_t n, uint64_t l, uint64_t ld, uint64_t seed);
 // Rows h_idx of g as a contiguous block (g's buffer itself for the identity).
@@ -190,5 +228,15 @@ void precompute_dev(slfmm_ctx* ctx, const double* d_xs, uint64_t n, uint64_t d, 
 void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const slfmm_nmf_config& cfg,
              const double* d_omega, double* W, double* Ht, double* cv);
 void nmf_transpose_h(hipStream_t s, const double* Ht, uint64_t pe, int K, double* H);
+// Ht[j][r] = max(F[j][r], 0)
+void relu_rows(hipStream_t s, const double* F, uint64_t ldf, uint64_t p, int K, double* Ht);
+// GWAS-phase CV (src/testing.rs:423-487); out3 = (NMF MAE, mean MAE, count).
+void gwas_cv(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p, const double* W,
+             const double* Wp, int K, const double* Hraw, uint64_t ldh, const double* SWp, const int32_t* nmiss,
+             const int32_t* sumg, uint64_t seed, uint64_t chunk, double cv_rate, double* out3);
+// out[j][c] = sum_k Op[k][c] y_kj for the mean-filled raw rows (y = g, or
+// the row mean on missing calls); ldout >= rup(m, 32).
+void raw_functionals(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const double* mu,
+                     const double* Op, uint64_t ldo, uint64_t m, double* out, uint64_t ldout);
 
 }  // namespace slfmm

@@ -20,8 +20,6 @@
 
 namespace slfmm {
 
-constexpr int kNmfMaxK = 16;
-
 __device__ __forceinline__ uint32_t code_of(uint32_t word, int pos) { return (word >> (2 * pos)) & 3u; }
 __device__ __forceinline__ double dosage_of(uint32_t code) { return code == 0u ? 0.0 : (code == 2u ? 1.0 : 2.0); }
 
@@ -273,11 +271,6 @@ __global__ void k_transpose_kp(const double* __restrict__ Ht, uint64_t p, int K,
   H[(uint64_t)r * p + j] = Ht[e];
 }
 
-void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t p, uint64_t npad,
-                     const double* A, uint64_t lda, int nrb, const double* mu, double* F, uint64_t ldF,
-                     const double* outscale);   // assoc.hip
-void pad_operand(hipStream_t s, const double* Q, uint64_t ldq, uint64_t n, int m, double* A, uint64_t lda,
-                 uint64_t npad);                 // assoc.hip
 
 namespace {
 struct FnColSum {
@@ -292,7 +285,7 @@ struct GStore {
 
 // Functionals of the mean-filled raw columns of Y (pe rows) against the
 // n x m operand Op: out[j][c] = sum_k Op[k][c] y_kj, y = g or mu_j.
-static void raw_functionals(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const double* mu,
+void raw_functionals(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const double* mu,
                             const double* Op, uint64_t ldo, uint64_t m, double* out, uint64_t ldout) {
   hipStream_t s = ctx->stream;
   const uint64_t npad = rup(n, 64);
@@ -398,6 +391,147 @@ void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t
     k_nmf_cv_final<<<1, 64, 0, s>>>(cvpart, (int)cvblk, cv + 3 * it);
     SLFMM_HIP(hipGetLastError());
   }
+}
+
+void relu_rows(hipStream_t s, const double* F, uint64_t ldf, uint64_t p, int K, double* Ht) {
+  if (!p) return;
+  k_relu_rows<<<(unsigned)cdiv(p * K, 256), 256, 0, s>>>(F, ldf, p, K, 0.0, Ht);
+  SLFMM_HIP(hipGetLastError());
+}
+
+// ---- GWAS-phase CV of the association pass (src/testing.rs:423-487) ---------
+// Chunk q (chunk_size consecutive SNPs) draws one uniform per OBSERVED call
+// from the stream seeded seed + q, in column-then-row order; a draw below
+// cv_rate holds the call out.  Per SNP the held-out column is mean-filled
+// (leave-out mean mu_m), H = max(0, W^+ y_m) and the held-out calls are
+// predicted by W H.  W^+ y_m follows from the precomputed W^+ y_mf (mean
+// fill with the full mean mu) by the corrections on the changed entries:
+//   held-out k: (mu_m - g_k) Wp_k;   missing k: (mu_m - mu) Wp_k (summed: SWp).
+__global__ void k_chunk_obs_base(const int32_t* __restrict__ nmiss, uint64_t n, uint64_t p, uint64_t chunk,
+                                 uint64_t* __restrict__ base) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t j0 = q * chunk;
+  if (j0 >= p) return;
+  const uint64_t j1 = std::min(p, j0 + chunk);
+  uint64_t b = 0;
+  for (uint64_t j = j0; j < j1; ++j) {
+    base[j] = b;
+    b += n - (uint64_t)nmiss[j];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_gwas_cv(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t n,
+                                                 uint64_t p, const double* __restrict__ W,
+                                                 const double* __restrict__ Wp, int K,
+                                                 const double* __restrict__ Hraw, uint64_t ldh,
+                                                 const double* __restrict__ SWp, const int32_t* __restrict__ nmiss,
+                                                 const int32_t* __restrict__ sumg, const uint64_t* __restrict__ base,
+                                                 uint64_t seed, uint64_t chunk, double cv_rate,
+                                                 double* __restrict__ part) {
+  __shared__ double r0[256], r1[256];
+  __shared__ unsigned long long r2[256];
+  const int t = threadIdx.x;
+  const uint64_t j = (uint64_t)blockIdx.x * 256 + t;
+  double e1 = 0.0, e2 = 0.0;
+  unsigned long long cnt = 0;
+  if (j < p) {
+    const uint64_t sd = seed + j / chunk;
+    const uint64_t b0 = base[j];
+    const uint4* row = reinterpret_cast<const uint4*>(packed + j * ld);
+    const uint64_t nkb = cdiv(n, 64);
+    // pass A: held-out count and dosage sum
+    uint64_t rank = 0, m = 0;
+    double sgm = 0.0;
+    for (uint64_t kb = 0; kb < nkb; ++kb) {
+      const uint4 q = row[kb];
+      const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
+      const int lim = (int)std::min<uint64_t>(64, n - kb * 64);
+      for (int kk = 0; kk < lim; ++kk) {
+        const uint32_t code = code_of(wd[kk >> 4], kk & 15);
+        if (code == 1u) continue;
+        if (u01(sd, b0 + rank) < cv_rate) { ++m; sgm += dosage_of(code); }
+        ++rank;
+      }
+    }
+    if (m > 0) {
+      const int64_t nobs = (int64_t)n - nmiss[j];
+      const double mu = nobs > 0 ? (double)sumg[j] / (double)nobs : 0.0;
+      const int64_t keep = nobs - (int64_t)m;
+      const double mum = keep > 0 ? ((double)sumg[j] - sgm) / (double)keep : 0.0;
+      double h[kNmfMaxK];
+      for (int r = 0; r < K; ++r) h[r] = Hraw[j * ldh + r] + (mum - mu) * SWp[j * K + r];
+      // pass B: corrections of the held-out calls
+      rank = 0;
+      for (uint64_t kb = 0; kb < nkb; ++kb) {
+        const uint4 q = row[kb];
+        const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
+        const int lim = (int)std::min<uint64_t>(64, n - kb * 64);
+        for (int kk = 0; kk < lim; ++kk) {
+          const uint32_t code = code_of(wd[kk >> 4], kk & 15);
+          if (code == 1u) continue;
+          if (u01(sd, b0 + rank) < cv_rate) {
+            const uint64_t k = kb * 64 + kk;
+            const double dlt = mum - dosage_of(code);
+            for (int r = 0; r < K; ++r) h[r] += Wp[k * K + r] * dlt;
+          }
+          ++rank;
+        }
+      }
+      for (int r = 0; r < K; ++r) h[r] = h[r] > 0.0 ? h[r] : 0.0;
+      // pass C: errors of the held-out calls
+      rank = 0;
+      for (uint64_t kb = 0; kb < nkb; ++kb) {
+        const uint4 q = row[kb];
+        const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
+        const int lim = (int)std::min<uint64_t>(64, n - kb * 64);
+        for (int kk = 0; kk < lim; ++kk) {
+          const uint32_t code = code_of(wd[kk >> 4], kk & 15);
+          if (code == 1u) continue;
+          if (u01(sd, b0 + rank) < cv_rate) {
+            const uint64_t k = kb * 64 + kk;
+            double pred = 0.0;
+            for (int r = 0; r < K; ++r) pred += W[k * K + r] * h[r];
+            const double g = dosage_of(code);
+            e1 += fabs(g - pred);
+            e2 += fabs(g - mum);
+            ++cnt;
+          }
+          ++rank;
+        }
+      }
+    }
+  }
+  r0[t] = e1; r1[t] = e2; r2[t] = cnt;
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if (t < s2) { r0[t] += r0[t + s2]; r1[t] += r1[t + s2]; r2[t] += r2[t + s2]; }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[blockIdx.x * 3 + 0] = r0[0];
+    part[blockIdx.x * 3 + 1] = r1[0];
+    part[blockIdx.x * 3 + 2] = (double)r2[0];
+  }
+}
+
+void gwas_cv(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p, const double* W,
+             const double* Wp, int K, const double* Hraw, uint64_t ldh, const double* SWp, const int32_t* nmiss,
+             const int32_t* sumg, uint64_t seed, uint64_t chunk, double cv_rate, double* out3) {
+  hipStream_t s = ctx->stream;
+  SLFMM_CHECK(chunk >= 1, SLFMM_E_ARG, "chunk_size must be positive");
+  if (!p) {
+    launch_fill(s, out3, 0.0, 3);
+    return;
+  }
+  uint64_t* base = ctx->buf<uint64_t>("gcv_base", p);
+  const uint64_t nq = cdiv(p, chunk);
+  k_chunk_obs_base<<<(unsigned)cdiv(nq, 64), 64, 0, s>>>(nmiss, n, p, chunk, base);
+  const unsigned nblk = (unsigned)cdiv(p, 256);
+  double* part = ctx->buf<double>("gcv_part", (uint64_t)nblk * 3);
+  k_gwas_cv<<<nblk, 256, 0, s>>>(packed, ld, n, p, W, Wp, K, Hraw, ldh, SWp, nmiss, sumg, base, seed, chunk, cv_rate,
+                                 part);
+  k_nmf_cv_final<<<1, 64, 0, s>>>(part, (int)nblk, out3);
+  SLFMM_HIP(hipGetLastError());
 }
 
 void nmf_transpose_h(hipStream_t s, const double* Ht, uint64_t pe, int K, double* H) {

@@ -14,9 +14,6 @@
 
 namespace slfmm {
 
-void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t p, uint64_t npad,
-                     const double* A, uint64_t lda, int nrb, const double* mu, double* F, uint64_t ldF,
-                     const double* outscale);
 void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
                      const double* d_B, uint64_t l, int norm, double* Z, uint64_t ldZ);
 void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* packedT, uint64_t ldt, uint64_t n, uint64_t p,
@@ -25,21 +22,29 @@ void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* packedT, uint64_t ldt, ui
 
 // ---- forward pass: W[k][c] = sum_j v_jk * sc_j * Qz[j][c] --------------------
 // One workgroup per (256-sample tile, SNP chunk, 32-column block); partial
-// sums per chunk are reduced in a fixed order afterwards.
+// sums per chunk are reduced in a fixed order afterwards.  With FILL the
+// missing calls take W_k . H_j - mu_j (NMF in-RAM imputation).
+template <bool FILL>
 __global__ void __launch_bounds__(256) k_acc_f64(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t n,
                                                  uint64_t p, uint64_t snps_per_chunk, const double* __restrict__ Qz,
                                                  uint64_t ldq, int l, const double* __restrict__ mu,
                                                  const double* __restrict__ sc, double* __restrict__ part,
-                                                 uint64_t ldp) {
+                                                 uint64_t ldp, FillSpec fs) {
   __shared__ double sq[32][33];
   __shared__ double smu[32];
   __shared__ uint32_t sw[32][16];
+  __shared__ double sh[FILL ? 32 : 1][kNmfMaxK + 1];
   const int t = threadIdx.x;
   const uint64_t k = (uint64_t)blockIdx.x * 256 + t;
   const uint64_t j0 = (uint64_t)blockIdx.y * snps_per_chunk;
   const uint64_t j1 = std::min(p, j0 + snps_per_chunk);
   const int cb = blockIdx.z;
   const int ncol = min(32, l - cb * 32);
+  double wk[FILL ? kNmfMaxK : 1];
+  if constexpr (FILL) {
+#pragma unroll
+    for (int r = 0; r < kNmfMaxK; ++r) wk[r] = (k < n && r < fs.K) ? fs.W[k * fs.K + r] : 0.0;
+  }
   double acc[32];
 #pragma unroll
   for (int c = 0; c < 32; ++c) acc[c] = 0.0;
@@ -55,11 +60,26 @@ __global__ void __launch_bounds__(256) k_acc_f64(const uint8_t* __restrict__ pac
       int jj = e >> 4, wd = e & 15;
       sw[jj][wd] = jj < nj ? reinterpret_cast<const uint32_t*>(packed + (jb + jj) * ld + (uint64_t)blockIdx.x * 64)[wd] : 0u;
     }
+    if constexpr (FILL) {
+      for (int e = t; e < 32 * kNmfMaxK; e += 256) {
+        const int jj = e / kNmfMaxK, r = e % kNmfMaxK;
+        sh[jj][r] = (jj < nj && r < fs.K) ? fs.Ht[(jb + jj) * fs.K + r] : 0.0;
+      }
+    }
     __syncthreads();
     for (int jj = 0; jj < nj; ++jj) {
       uint32_t code = (sw[jj][t >> 4] >> (2 * (t & 15))) & 3u;
       double m = smu[jj];
-      double v = (code & 2u) ? ((code & 1u) ? 2.0 - m : 1.0 - m) : ((code & 1u) ? 0.0 : -m);
+      double miss = 0.0;
+      if constexpr (FILL) {
+        if (code == 1u) {
+          double f = 0.0;
+#pragma unroll
+          for (int r = 0; r < kNmfMaxK; ++r) f += wk[r] * sh[jj][r];
+          miss = f - m;
+        }
+      }
+      double v = (code & 2u) ? ((code & 1u) ? 2.0 - m : 1.0 - m) : ((code & 1u) ? miss : -m);
 #pragma unroll
       for (int c = 0; c < 32; ++c) acc[c] += v * sq[jj][c];
     }
@@ -81,7 +101,7 @@ __global__ void k_acc_reduce(const double* __restrict__ part, uint64_t n, int l,
 
 static void accumulate_f64(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
                            const double* Qz, uint64_t ldq, uint64_t l, const double* mu, const double* sc,
-                           double* W, uint64_t ldw) {
+                           double* W, uint64_t ldw, const FillSpec* fill) {
   hipStream_t s = ctx->stream;
   const uint64_t ntile = cdiv(n, 256), ncb = cdiv(l, 32);
   uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(cdiv(p, 256), cdiv(2048, ntile * ncb)));
@@ -89,8 +109,11 @@ static void accumulate_f64(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, u
   nchunk = cdiv(p, spc);
   const uint64_t ldp = ncb * 32;
   double* part = ctx->buf<double>("rs_accpart", nchunk * n * ldp);
-  k_acc_f64<<<dim3((unsigned)ntile, (unsigned)nchunk, (unsigned)ncb), 256, 0, s>>>(packed, ld, n, p, spc, Qz, ldq,
-                                                                                  (int)l, mu, sc, part, ldp);
+  const dim3 grid((unsigned)ntile, (unsigned)nchunk, (unsigned)ncb);
+  if (fill && fill->W)
+    k_acc_f64<true><<<grid, 256, 0, s>>>(packed, ld, n, p, spc, Qz, ldq, (int)l, mu, sc, part, ldp, *fill);
+  else
+    k_acc_f64<false><<<grid, 256, 0, s>>>(packed, ld, n, p, spc, Qz, ldq, (int)l, mu, sc, part, ldp, FillSpec{});
   k_acc_reduce<<<(unsigned)cdiv(n * l, 256), 256, 0, s>>>(part, n, (int)l, ldp, (int)nchunk, W, ldw);
   SLFMM_HIP(hipGetLastError());
 }
@@ -205,7 +228,7 @@ const uint8_t* gather_subset(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t*
 
 // ---- randomized SVD ---------------------------------------------------------------
 void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_t n_idx, const double* d_xs,
-              uint64_t d, const slfmm_config& cfg, const double* d_b0, double* d_uhat) {
+              uint64_t d, const slfmm_config& cfg, const double* d_b0, double* d_uhat, const FillSpec* fill) {
   hipStream_t s = ctx->stream;
   const uint64_t n = g->n, K = cfg.k, l = cfg.k + cfg.oversampling;
   SLFMM_CHECK(K >= 1 && l <= 64, SLFMM_E_ARG, "need 1 <= K and K + oversampling <= 64");
@@ -225,8 +248,15 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   int32_t* nhom = ctx->buf<int32_t>("rs_nhom", pe);
   double* mu = ctx->buf<double>("rs_mu", pe);
   double* sc = ctx->buf<double>("rs_sc", pe);
-  launch_stats(s, Y, g->ld, n, pe, nmiss, sumg, nhom);
-  launch_mean_scale(s, nmiss, sumg, n, pe, cfg.norm, mu, sc);
+  const bool filled = fill && fill->W;
+  SLFMM_CHECK(!(filled && cfg.precision == SLFMM_PREC_I8), SLFMM_E_ARG,
+              "NMF imputation runs on the FP64 path (use SLFMM_PREC_FP64)");
+  if (filled) {   // NMF in-RAM fill of the estimation rows: moments of the filled rows
+    fill_moments(ctx, Y, g->ld, pe, *fill, cfg.norm, nullptr, mu, sc, nullptr, nullptr);
+  } else {
+    launch_stats(s, Y, g->ld, n, pe, nmiss, sumg, nhom);
+    launch_mean_scale(s, nmiss, sumg, n, pe, cfg.norm, mu, sc);
+  }
 
   const bool mixed = cfg.precision == SLFMM_PREC_I8;
   uint8_t* YT = nullptr;
@@ -257,7 +287,7 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
       rsvd_project_i8(ctx, Y, g->ld, n, pe, B, l, cfg.norm, Z, lda);
     } else {
       k_pad_rows<<<(unsigned)cdiv(npad * lda, 256), 256, 0, s>>>(B, l, n, (int)l, A, lda, npad);
-      launch_func_f64(s, Y, g->ld, pe, npad, A, lda, nrb, mu, Z, lda, sc);
+      launch_func_f64(s, Y, g->ld, pe, npad, A, lda, nrb, mu, Z, lda, sc, fill);
     }
   };
   auto orth_z = [&]() {
@@ -268,7 +298,7 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
     {
       slfmm_timed tm(ctx, "rsvd_forward");
       if (mixed) rsvd_accumulate_i8(ctx, YT, ldt, n, pe, Z, lda, l, mu, sc, W, l);
-      else accumulate_f64(ctx, Y, g->ld, n, pe, Z, lda, l, mu, sc, W, l);
+      else accumulate_f64(ctx, Y, g->ld, n, pe, Z, lda, l, mu, sc, W, l, fill);
     }
     apply_N(ctx, Qd, d, dlam, W, l, l, n, false);
   };

@@ -63,3 +63,45 @@ def test_nmf_rank_limit(S):
     g = S.BedFile.from_packed(packed, 40, 30)
     with pytest.raises(S.api.SlfmmError):
         S.nmf_impute_estimation(g, None, S.NmfParams(k=17, n_iter=1))
+
+
+def test_fit_lfmm2_nmf_vs_oracle(S):
+    """fit_lfmm2 with NMF imputation (src/lib.rs:162-263): NmfInRam fill in the
+    RSVD, NmfOnTheFly fill in the association, GWAS-phase CV
+    (src/testing.rs:423-487).  FP64: U_hat angles < 1e-8, t / beta / r2
+    within 1e-8 relative, CV errors within 1e-9, counts exact."""
+    n, p, K, d = 300, 2000, 3, 2
+    pan = synth.balding_nichols(n, p, n_pops=3, miss_rate=0.10, d=d, n_causal=20, effect=1.0, seed=77)
+    packed, bps = O.encode_bed(pan.g), O.bytes_per_snp(n)
+    xs = O.center_covariates(pan.x)
+    pre = O.precompute(xs, 1e-5)
+    b0 = pre.m.T @ np.random.default_rng(4).standard_normal((n, K + 10))
+    kn, seed_nmf, cv_rate = 3, 7, 0.02
+    omega = synth.normal_matrix(seed_nmf, n, kn + 10)
+    cfg_o = O.Config(k=K, norm=O.EIGENSTRAT, chunk_size=500, n_power_iter=2, seed=42)
+    est = O.subset_indices(0.5, p)
+    nmf_o = dict(k=kn, n_iter=3, eps=1e-16, cv_rate=cv_rate, seed=seed_nmf, omega=omega,
+                 uniform=synth.uniform_stream)
+    u_ref, ref, _, nres = O.fit_lfmm2(packed, bps, n, p, pan.x, cfg_o, est, b0=b0, nmf=nmf_o)
+    g = S.BedFile.from_packed(packed, n, p)
+    cfg = S.Lfmm2Config(k=K, norm=S.EIGENSTRAT, chunk_size=500, n_power_iter=2, seed=42)
+    # the reference CLI gives NmfConfig the LFMM chunk size (src/main.rs:183, :203)
+    nmf = S.NmfParams(k=kn, n_iter=3, cv_rate=cv_rate, seed=seed_nmf, chunk_size=500)
+    got = S.fit_lfmm2_nmf(g, 0.5, g, pan.x, cfg, nmf, b0=b0, nmf_omega=omega)
+    assert np.max(O.principal_angles(got.u_hat, u_ref)) < 1e-8
+    assert _close(got.tstat, ref.t, 1e-8)
+    assert _close(got.beta, ref.beta, 1e-8)
+    assert _close(got.r2, ref.r2, 1e-8)
+    assert np.allclose(got.gif_per_trait, ref.gif_per_trait, rtol=1e-12)
+    assert np.allclose(got.nmf_cv, nres.cv_mae_per_iter, rtol=1e-9, atol=0)
+    assert got.gwas_cv[2] == ref.cv[2] > 0
+    assert np.allclose(got.gwas_cv[:2], ref.cv[:2], rtol=1e-9, atol=0)
+    assert np.max(np.abs(ref.t)) > 4   # the panel has real signal
+
+
+def test_nmf_rejects_mixed_precision(S):
+    n, p = 64, 40
+    pan = synth.balding_nichols(n, p, n_pops=2, miss_rate=0.05, d=1, n_causal=0, seed=3)
+    g = S.BedFile.from_packed(O.encode_bed(pan.g), n, p)
+    with pytest.raises(S.api.SlfmmError, match="FP64"):
+        S.fit_lfmm2_nmf(g, None, g, pan.x, S.Lfmm2Config(k=2, precision=S.PREC_I8), S.NmfParams(k=2, n_iter=1))
