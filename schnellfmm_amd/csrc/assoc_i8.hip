@@ -256,35 +256,48 @@ __device__ __forceinline__ void chunk_to_lds(const unsigned char* src_chunk, uns
 // ---------------------------------------------------------------------------
 // Epilogue of the SNP-side kernels.  After the block loop, lane (g, l16) of
 // a wave holds rows 16 rt + 4 g + r of SNP column l16 of every tile st.  Per
-// tile the wave stages its int32 row sums in `red` ([16 NT rows][16 SNPs]);
-// then every lane recombines the functionals i = g + 4 q of SNP
-// j = base + 16 st + l16 (all 64 lanes busy), and either stores them scaled
-// (MODE 1, RSVD sketch) or feeds the split per-SNP epilogue (MODE 0).
+// tile the wave stages its int32 row sums in its own `red` area ([16 NT
+// rows][16 SNPs]); then every lane recombines the functionals i = g + 4 q of
+// SNP j = base + 16 st + l16 (all 64 lanes busy), and either stores them
+// scaled (MODE 1, RSVD sketch) or feeds the split per-SNP epilogue (MODE 0).
+// `red` is private to the wave and a wave's LDS accesses complete in order,
+// so the tiles need no workgroup barrier.  The recombination table is read
+// from its LDS copy.
 // ---------------------------------------------------------------------------
 constexpr int kEpiNQ = 8;   // functionals per lane group (K + d <= 32)
 
+// Copies the recombination table to LDS (all threads of the workgroup take
+// part; the caller synchronises before reading it).
+__device__ __forceinline__ const Recomb* stage_recomb(const Recomb* src, unsigned char* dst, int t, int nthreads) {
+  static_assert(sizeof(Recomb) % 4 == 0, "table is copied as words");
+  const int* s = reinterpret_cast<const int*>(src);
+  int* d = reinterpret_cast<int*>(dst);
+  for (int e = t; e < (int)(sizeof(Recomb) / 4); e += nthreads) d[e] = s[e];
+  return reinterpret_cast<const Recomb*>(dst);
+}
+
 template <int ST, int NT, int MODE>
 __device__ __forceinline__ void gt_epilogue(const GtArgs& a, const v4i (&acc)[ST][NT], uint32_t (&c_hom)[ST],
-                                            uint32_t (&c_miss)[ST], int* red, const double* plan, uint64_t base,
-                                            int g, int l16) {
+                                            uint32_t (&c_miss)[ST], int* red, const double* plan, const Recomb* rc,
+                                            uint64_t base, int g, int l16) {
 #pragma unroll
   for (int st = 0; st < ST; ++st) {
     c_hom[st] += __shfl_xor(c_hom[st], 16); c_hom[st] += __shfl_xor(c_hom[st], 32);
     c_miss[st] += __shfl_xor(c_miss[st], 16); c_miss[st] += __shfl_xor(c_miss[st], 32);
   }
-  const Recomb* rc = a.rc;
+  const int ones = rc->ones;
 #pragma unroll
   for (int st = 0; st < ST; ++st) {
 #pragma unroll
     for (int rt = 0; rt < NT; ++rt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) red[(16 * rt + 4 * g + r) * 16 + l16] = acc[st][rt][r];
-    __syncthreads();
+    asm volatile("" ::: "memory");   // the wave's sums are written before any lane reads them
     const uint64_t j = base + 16 * st + l16;
     const bool valid = j < a.p;
     const int64_t nmiss = c_miss[st], nhom = c_hom[st];
     const int64_t nobs = (int64_t)a.n - nmiss;
-    const int64_t sg = red[rc->ones * 16 + l16];   // dosage sum (all-ones operand row)
+    const int64_t sg = red[ones * 16 + l16];   // dosage sum (all-ones operand row)
     const double mu = nobs > 0 ? (double)sg / (double)nobs : 0.0;
     double fq[kEpiNQ];
 #pragma unroll
@@ -293,13 +306,13 @@ __device__ __forceinline__ void gt_epilogue(const GtArgs& a, const v4i (&acc)[ST
       fq[q] = 0.0;
       if (i < a.m) {
         const int ng = rc->ng[i];
-        int64_t X = 0;
-        for (int tt = 0; tt < ng; ++tt) X = X * 254 + red[rc->rg[i][tt] * 16 + l16];
+        int64_t X = (int64_t)red[rc->rg[i][0] * 16 + l16] * 254 + red[rc->rg[i][1] * 16 + l16];
+        if (ng == 3) X = X * 254 + red[rc->rg[i][2] * 16 + l16];
         const int64_t Y = (int64_t)red[rc->rm[i][0] * 16 + l16] * 254 + red[rc->rm[i][1] * 16 + l16];
         fq[q] = rc->wg[i] * (double)X + mu * (rc->wm[i] * (double)Y - rc->C[i]);
       }
     }
-    __syncthreads();   // red is rewritten by the next tile
+    asm volatile("" ::: "memory");   // this tile's reads precede the next tile's writes
     if constexpr (MODE == 1) {
       const double scale = epi_scale(mu, a.norm);
 #pragma unroll
@@ -320,7 +333,7 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
   constexpr int LDS_A = 2 * CH * 1024;
   constexpr int LDS_RED = 4 * NT * 1024;                   // one SNP tile of int32 sums per wave
   constexpr int PLAN_MAX = MODE == 0 ? 3 * kMaxD * MAXM + kMaxD + MAXM * MAXM + 1 : 0;
-  constexpr int LDS_E = LDS_RED + PLAN_MAX * 8;
+  constexpr int LDS_E = LDS_RED + PLAN_MAX * 8 + (int)sizeof(Recomb);
   constexpr int LDS = LDS_A > LDS_E ? LDS_A : LDS_E;
   __shared__ __align__(16) unsigned char lds[LDS];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, l16 = lane & 15;
@@ -398,20 +411,23 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
     for (int e = t; e < len; e += 256) sp[e] = a.ea.plan[e];
     plan = sp;
   }
-  gt_epilogue<ST, NT, MODE>(a, acc, c_hom, c_miss, reinterpret_cast<int*>(lds) + wave * (NT * 256), plan, base, g,
-                            l16);
+  const Recomb* rc = stage_recomb(a.rc, lds + LDS_RED + PLAN_MAX * 8, t, 256);
+  __syncthreads();
+  gt_epilogue<ST, NT, MODE>(a, acc, c_hom, c_miss, reinterpret_cast<int*>(lds) + wave * (NT * 256), plan, rc, base,
+                            g, l16);
 }
 
 // ---------------------------------------------------------------------------
-// Pipelined variant for NT <= 8: 8 waves x 3 tiles of 16 SNPs per workgroup, blocks of
-// 4 k-steps (256 samples, 64 bytes per SNP row).  Genotype segments are
-// copied by LDS-DMA into a 3-slot ring (blocks kb+1 and kb+2 in flight
-// while kb is computed), the operand blocks into a 3-slot ring when it fits
-// the 160 KB of LDS (NT <= 7), else a 2-slot ring.  The block loop is
-// unrolled over the ring period so every slot is a distinct LDS object.
+// Pipelined kernel for NT <= 8: 8 waves x 3 tiles of 16 SNPs per workgroup,
+// blocks of 4 k-steps (256 samples, 64 bytes per SNP row).  Both streams go
+// to LDS by buffer-descriptor DMA (buffer_load_dwordx4 ... lds): the
+// genotype segments into a 3-slot ring (blocks kb+1 and kb+2 in flight while
+// kb is computed), the operand blocks into a 2-slot ring.  The block loop is
+// unrolled over the ring period (6) so every slot is a distinct LDS object.
 // ---------------------------------------------------------------------------
 constexpr int kPipeWaves = 8;
 constexpr int kPipeTiles = 3;   // SNP tiles of 16 per wave
+constexpr int kOperandPrefetch = 6;   // operand fragments read ahead of their MFMAs
 
 template <int ST>
 __device__ __forceinline__ void lds_read_tiles(const unsigned char* p, v4i (&q)[ST]) {
@@ -433,13 +449,6 @@ __device__ __forceinline__ void lds_read_tiles(const unsigned char* p, v4i (&q)[
   }
 }
 
-// Cache policy of the genotype stream's LDS-DMA: non-temporal, so the
-// stream does not evict the operand blocks that every workgroup re-reads.
-#ifndef SLFMM_GENO_CPOL
-#define SLFMM_GENO_CPOL 0
-#endif
-constexpr int kGenoCPol = SLFMM_GENO_CPOL;
-
 // vmcnt wait with a compile-time count (vmcnt[3:0]; expcnt and lgkmcnt at
 // their maxima, i.e. not waited on).
 template <int N>
@@ -448,32 +457,38 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt(0x0F70 | N);
 }
 
-// One block of the pipelined kernel.  Operand blocks go through an AR-slot
-// ring (block kb + AR - 1 is issued while kb is computed), genotype blocks
-// through a 3-slot ring (block kb + 2).  Every wave issues the same number
-// of loads in every block (blocks past the end are replaced by dummy copies
-// of block 0 into the slot being filled, which nobody reads), so a wave can
-// wait for exactly the loads of the block it is about to read.
-template <int NT0, int NT, int ST, int AR>
-__device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, int wave, int lane, int g,
-                                           const uint8_t* row0, uint64_t tile_stride, uint64_t jl,
-                                           const uint8_t* lastrow, const unsigned char* Ag,
+// Raw buffer descriptor over `bytes` bytes at `base` (stride 0, byte
+// offsets; loads at or past `bytes` read zeros).
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ __forceinline__ rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+// LDS-DMA through a buffer descriptor: lane l's 16 bytes at byte offset voff
+// land at dst + 16 l.  (The buffer form keeps the compiler's LDS-read waits
+// partial: after a global_load_lds it waits for every outstanding LDS read.)
+__device__ __forceinline__ void buf_to_lds(rsrc_t rs, uint32_t voff, unsigned char* dst) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_ptr_t)dst, 16, voff, 0, 0, 0);
+}
+
+struct PipeCtx {
+  rsrc_t geno;        // this workgroup's SNP rows; rows past the last SNP read zeros
+  rsrc_t opnd;        // the operand blocks
+  uint32_t gvoff;     // byte offset of this lane's 16-byte piece of its tile-0 row
+  uint32_t tstride;   // 16 rows
+};
+
+// One block of the pipelined kernel.  Operand block kb+1 and genotype block
+// kb+2 are issued while kb is computed, in that order, so the wait at the top
+// of block kb leaves only the genotypes of kb+1 in flight.
+template <int NT0, int NT, int ST>
+__device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, const PipeCtx& pc, int wave, int lane, int g,
                                            const unsigned char* a_rd, unsigned char* a_wr,
                                            const unsigned char* g_rd, unsigned char* g_wr,
                                            v4i (&acc)[ST][NT], uint32_t (&c_hom)[ST], uint32_t (&c_miss)[ST]) {
   constexpr int KS = 4;
   constexpr int W = kPipeWaves;
   constexpr int CH = KS * NT;
-  constexpr int CA_LO = CH / W, CA_HI = (CH + W - 1) / W;   // operand chunks per wave
-  // Loads still allowed in flight: those issued in the previous block for
-  // later blocks (AR == 2: genotypes of kb+1; AR == 3: operand and
-  // genotypes of kb+1).
-  if constexpr (AR == 2) {
-    wait_vmcnt<ST>();
-  } else {
-    if (wave < CH % W) wait_vmcnt<CA_HI + ST>();
-    else wait_vmcnt<CA_LO + ST>();
-  }
+  wait_vmcnt<ST>();
   __builtin_amdgcn_s_barrier();
   const v4i* Ab = reinterpret_cast<const v4i*>(a_rd);
   uint32_t gw[ST][KS];
@@ -496,43 +511,39 @@ __device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, int wave, in
     }
   }
   // the ring slots written below are not the ones read in this block
-  if constexpr (AR == 2) {
-    if (kb + 1 < a.nblk) {
-      const unsigned char* src = Ag + (uint64_t)(kb + 1) * CH * 1024;
-      for (int c = wave; c < CH; c += W) chunk_to_lds(src + (uint64_t)c * 1024, a_wr + c * 1024, lane);
-    }
-  } else {
-    const int ab = kb + AR - 1 < a.nblk ? kb + AR - 1 : 0;
-    const unsigned char* src = Ag + (uint64_t)ab * CH * 1024;
-    for (int c = wave; c < CH; c += W) chunk_to_lds(src + (uint64_t)c * 1024, a_wr + c * 1024, lane);
+  if (kb + 1 < a.nblk) {
+    const uint32_t src = (uint32_t)(kb + 1) * CH * 1024 + 16 * lane;
+    for (int c = wave; c < CH; c += W) buf_to_lds(pc.opnd, src + c * 1024, a_wr + c * 1024);
   }
   {
-    const int src = kb + 2 < a.nblk ? kb + 2 : 0;
+    const uint32_t src = (uint32_t)(kb + 2 < a.nblk ? kb + 2 : 0) * 64;
     unsigned char* dst = g_wr + wave * (ST * 1024);
 #pragma unroll
-    for (int st = 0; st < ST; ++st) {
-      // rows past the last SNP are clamped onto it: read, never stored
-      const uint8_t* rp = (jl + 16 * st < a.p) ? row0 + st * tile_stride : lastrow;
-      __builtin_amdgcn_global_load_lds((const void*)(rp + (uint64_t)src * 64), (lds_ptr_t)(dst + st * 1024), 16, 0,
-                                       kGenoCPol);
-    }
+    for (int st = 0; st < ST; ++st) buf_to_lds(pc.geno, pc.gvoff + st * pc.tstride + src, dst + st * 1024);
   }
+  // operand fragment c = s NT + rt is read kOperandPrefetch fragments ahead;
+  // the genotype fragments of k-step s + 1 are decoded after the MFMAs of s
+  constexpr int D = kOperandPrefetch < CH ? kOperandPrefetch : CH;
+  v4i af[CH];
+#pragma unroll
+  for (int c = 0; c < D; ++c) af[c] = Ab[c * 64 + lane];
+  Frag fc[ST];
+#pragma unroll
+  for (int st = 0; st < ST; ++st) fc[st] = decode_word(gw[st][0], c_hom[st], c_miss[st]);
 #pragma unroll
   for (int s = 0; s < KS; ++s) {
-    Frag f[ST];
 #pragma unroll
-    for (int st = 0; st < ST; ++st) f[st] = decode_word(gw[st][s], c_hom[st], c_miss[st]);
+    for (int rt = 0; rt < NT; ++rt) {
+      const int c = s * NT + rt;
+      if (c + D < CH) af[c + D] = Ab[(c + D) * 64 + lane];
 #pragma unroll
-    for (int rt = 0; rt < NT; rt += 2) {
-      const v4i af0 = Ab[(s * NT + rt) * 64 + lane];
-      const v4i af1 = rt + 1 < NT ? Ab[(s * NT + rt + 1) * 64 + lane] : af0;
+      for (int st = 0; st < ST; ++st)
+        acc[st][rt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[c], rt < NT0 ? fc[st].g0 : fc[st].miss, acc[st][rt], 0,
+                                                            0, 0);
+    }
+    if (s + 1 < KS) {
 #pragma unroll
-      for (int st = 0; st < ST; ++st) {
-        acc[st][rt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af0, rt < NT0 ? f[st].g0 : f[st].miss, acc[st][rt], 0, 0, 0);
-        if (rt + 1 < NT)
-          acc[st][rt + 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af1, rt + 1 < NT0 ? f[st].g0 : f[st].miss,
-                                                                  acc[st][rt + 1], 0, 0, 0);
-      }
+      for (int st = 0; st < ST; ++st) fc[st] = decode_word(gw[st][s + 1], c_hom[st], c_miss[st]);
     }
   }
 }
@@ -546,26 +557,22 @@ __global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
   constexpr int CH = KS * NT;
   constexpr int A_SLOT = CH * 1024;
   constexpr int G_SLOT = W * ST * 1024;
-#ifndef SLFMM_OPERAND_RING
-#define SLFMM_OPERAND_RING 2
-#endif
-  // operand ring depth: 3 slots when they fit beside the genotype ring
-  constexpr int AR = SLFMM_OPERAND_RING >= 3 && 3 * A_SLOT + 3 * G_SLOT <= 160 * 1024 ? 3 : 2;
   static_assert(NT * 1024 * W <= 3 * G_SLOT, "epilogue sums of all waves must fit the genotype ring");
+  static_assert(sizeof(Recomb) <= A_SLOT, "recombination table must fit an operand slot");
   __shared__ __align__(16) unsigned char A0[A_SLOT];
   __shared__ __align__(16) unsigned char A1[A_SLOT];
-  __shared__ __align__(16) unsigned char A2[AR == 3 ? A_SLOT : 16];
   __shared__ __align__(16) unsigned char G0[G_SLOT];
   __shared__ __align__(16) unsigned char G1[G_SLOT];
   __shared__ __align__(16) unsigned char G2[G_SLOT];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, l16 = lane & 15;
-  const uint64_t base = ((uint64_t)blockIdx.x * W + wave) * (ST * 16);
-  const unsigned char* Ag = reinterpret_cast<const unsigned char*>(a.A);
-  // rows past the last SNP are clamped onto it: read, never stored
-  const uint64_t jl = base + l16;
-  const uint8_t* lastrow = a.packed + (a.p - 1) * a.ld + 16 * g;
-  const uint8_t* row0 = jl < a.p ? a.packed + jl * a.ld + 16 * g : lastrow;
-  const uint64_t tile_stride = (uint64_t)16 * a.ld;
+  const uint64_t wgbase = (uint64_t)blockIdx.x * W * ST * 16;
+  const uint64_t base = wgbase + (uint64_t)wave * (ST * 16);
+  PipeCtx pc;
+  const uint64_t gbytes = (a.p - wgbase) * a.ld;
+  pc.geno = make_rsrc(a.packed + wgbase * a.ld, (uint32_t)(gbytes < 0xFFFFFFFFull ? gbytes : 0xFFFFFFFFull));
+  pc.opnd = make_rsrc(a.A, (uint32_t)a.nblk * CH * 1024);
+  pc.gvoff = (uint32_t)((base - wgbase + l16) * a.ld + 16 * g);
+  pc.tstride = (uint32_t)(16 * a.ld);
   v4i acc[ST][NT];
 #pragma unroll
   for (int st = 0; st < ST; ++st)
@@ -574,55 +581,33 @@ __global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
   uint32_t c_hom[ST], c_miss[ST];
 #pragma unroll
   for (int st = 0; st < ST; ++st) c_hom[st] = c_miss[st] = 0;
-  // Prologue, in the order the blocks issue them (operand, then genotypes):
-  // block 0 -> A0, G0, then block 1 -> (A1,) G1.  The first block's wait
-  // leaves exactly the block-1 loads in flight.
-  for (int c = wave; c < CH; c += W) chunk_to_lds(Ag + (uint64_t)c * 1024, A0 + c * 1024, lane);
+  // Prologue in the order the blocks issue them: operand 0 and genotypes 0,
+  // then genotypes 1.  The first block's wait leaves the genotypes of block 1 in flight.
+  for (int c = wave; c < CH; c += W) buf_to_lds(pc.opnd, c * 1024 + 16 * lane, A0 + c * 1024);
 #pragma unroll
-  for (int st = 0; st < ST; ++st) {
-    const uint8_t* rp = (jl + 16 * st < a.p) ? row0 + st * tile_stride : lastrow;
-    __builtin_amdgcn_global_load_lds((const void*)(rp), (lds_ptr_t)(G0 + wave * (ST * 1024) + st * 1024), 16, 0,
-                                     kGenoCPol);
-  }
-  if constexpr (AR == 3) {
-    const unsigned char* src = Ag + (uint64_t)(a.nblk > 1 ? 1 : 0) * CH * 1024;
-    for (int c = wave; c < CH; c += W) chunk_to_lds(src + (uint64_t)c * 1024, A1 + c * 1024, lane);
-  }
+  for (int st = 0; st < ST; ++st) buf_to_lds(pc.geno, pc.gvoff + st * pc.tstride, G0 + wave * (ST * 1024) + st * 1024);
   {
-    const int src = a.nblk > 1 ? 1 : 0;
+    const uint32_t src = a.nblk > 1 ? 64 : 0;
 #pragma unroll
-    for (int st = 0; st < ST; ++st) {
-      const uint8_t* rp = (jl + 16 * st < a.p) ? row0 + st * tile_stride : lastrow;
-      __builtin_amdgcn_global_load_lds((const void*)(rp + (uint64_t)src * 64),
-                                       (lds_ptr_t)(G1 + wave * (ST * 1024) + st * 1024), 16, 0, kGenoCPol);
-    }
+    for (int st = 0; st < ST; ++st)
+      buf_to_lds(pc.geno, pc.gvoff + st * pc.tstride + src, G1 + wave * (ST * 1024) + st * 1024);
   }
-#define SLFMM_PIPE(PH, AR_, AW_, GR_, GW_)                                                               \
-  if (kb + PH < a.nblk)                                                                                 \
-    pipe_block<NT0, NT, ST, AR>(kb + PH, a, wave, lane, g, row0, tile_stride, jl, lastrow, Ag, AR_, AW_, GR_, GW_, \
-                                acc, c_hom, c_miss);
-  if constexpr (AR == 3) {
-    for (int kb = 0; kb < a.nblk; kb += 3) {
-      SLFMM_PIPE(0, A0, A2, G0, G2)
-      SLFMM_PIPE(1, A1, A0, G1, G0)
-      SLFMM_PIPE(2, A2, A1, G2, G1)
-    }
-  } else {
-    for (int kb = 0; kb < a.nblk; kb += 6) {
-      SLFMM_PIPE(0, A0, A1, G0, G2)
-      SLFMM_PIPE(1, A1, A0, G1, G0)
-      SLFMM_PIPE(2, A0, A1, G2, G1)
-      SLFMM_PIPE(3, A1, A0, G0, G2)
-      SLFMM_PIPE(4, A0, A1, G1, G0)
-      SLFMM_PIPE(5, A1, A0, G2, G1)
-    }
+#define SLFMM_PIPE(PH, AR_, AW_, GR_, GW_) \
+  if (kb + PH < a.nblk) pipe_block<NT0, NT, ST>(kb + PH, a, pc, wave, lane, g, AR_, AW_, GR_, GW_, acc, c_hom, c_miss);
+  for (int kb = 0; kb < a.nblk; kb += 6) {
+    SLFMM_PIPE(0, A0, A1, G0, G2)
+    SLFMM_PIPE(1, A1, A0, G1, G0)
+    SLFMM_PIPE(2, A0, A1, G2, G1)
+    SLFMM_PIPE(3, A1, A0, G0, G2)
+    SLFMM_PIPE(4, A0, A1, G1, G0)
+    SLFMM_PIPE(5, A1, A0, G2, G1)
   }
 #undef SLFMM_PIPE
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
 
   // ---- epilogue: per-wave int32 sums of one SNP tile; waves spread over the
-  // three genotype slots, the plan goes to A0 --------------------------------
+  // three genotype slots, the plan goes to A0, the recombination table to A1
   constexpr int WPS = G_SLOT / (NT * 1024);   // waves per slot
   unsigned char* gslot = wave < WPS ? G0 : (wave < 2 * WPS ? G1 : G2);
   int* red = reinterpret_cast<int*>(gslot) + (wave % WPS) * (NT * 256);
@@ -633,7 +618,9 @@ __global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
     for (int e = t; e < len; e += 512) sp[e] = a.ea.plan[e];
     plan = sp;
   }
-  gt_epilogue<ST, NT, MODE>(a, acc, c_hom, c_miss, red, plan, base, g, l16);
+  const Recomb* rc = stage_recomb(a.rc, A1, t, 512);
+  __syncthreads();
+  gt_epilogue<ST, NT, MODE>(a, acc, c_hom, c_miss, red, plan, rc, base, g, l16);
 }
 
 #define SLFMM_GT_CASES(X) \
@@ -656,7 +643,12 @@ static void launch_gt(hipStream_t s, const Operand& op, GtArgs& a, int mode, int
   const int NT = op.NT0 + op.NT1;
   bool done = false;
   if (NT <= 8) {
-    const unsigned grid = (unsigned)cdiv(a.p, 16 * kPipeTiles * kPipeWaves);
+    // buffer descriptors address a workgroup's rows and the operand with 32-bit byte offsets
+    const uint64_t rows = 16 * kPipeTiles * kPipeWaves;
+    SLFMM_CHECK(rows * a.ld + 64 * (uint64_t)a.nblk < 0xFFFFFFFFull &&
+                    (uint64_t)a.nblk * 4 * NT * 1024 < 0xFFFFFFFFull,
+                SLFMM_E_ARG, "panel rows too long for the int8 association kernel (use SLFMM_PREC_FP64)");
+    const unsigned grid = (unsigned)cdiv(a.p, rows);
 #define X(A0, A1)                                                                      \
     if (!done && key == A0 * 100 + A1) done = launch_pipe<A0, A1>(s, grid, a, mode, maxm);
     SLFMM_GT_CASES(X)
