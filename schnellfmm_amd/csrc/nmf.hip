@@ -24,52 +24,56 @@ __device__ __forceinline__ uint32_t code_of(uint32_t word, int pos) { return (wo
 __device__ __forceinline__ double dosage_of(uint32_t code) { return code == 0u ? 0.0 : (code == 2u ? 1.0 : 2.0); }
 
 // ---- forward pass: per-chunk partials of num_W (n x K) -----------------------
-// One workgroup per (256-sample tile, SNP chunk); thread = sample.
+// One workgroup per (256-sample tile, SNP chunk); thread = sample.  The
+// loadings H_j of the SNP in hand are the same for every thread, so they come
+// in through scalar loads (one wide load per SNP: the rank K is a template
+// parameter).
+template <int K>
 __global__ void __launch_bounds__(256) k_nmf_fwd(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t n,
                                                  uint64_t p, uint64_t snps_per_chunk, const double* __restrict__ W,
-                                                 const double* __restrict__ Ht, int K, double* __restrict__ part) {
-  __shared__ double sh[32][kNmfMaxK + 1];
+                                                 const double* __restrict__ Ht, double* __restrict__ part) {
+  constexpr int KM = K;
   __shared__ uint32_t sw[32][16];
   const int t = threadIdx.x;
   const uint64_t k = (uint64_t)blockIdx.x * 256 + t;
   const uint64_t j0 = (uint64_t)blockIdx.y * snps_per_chunk;
   const uint64_t j1 = std::min(p, j0 + snps_per_chunk);
-  double w[kNmfMaxK], acc[kNmfMaxK];
+  double w[KM], acc[KM];
 #pragma unroll
-  for (int r = 0; r < kNmfMaxK; ++r) {
-    w[r] = (k < n && r < K) ? W[k * K + r] : 0.0;
+  for (int r = 0; r < KM; ++r) {
+    w[r] = k < n ? W[k * K + r] : 0.0;
     acc[r] = 0.0;
   }
   for (uint64_t jb = j0; jb < j1; jb += 32) {
     const int nj = (int)std::min<uint64_t>(32, j1 - jb);
     __syncthreads();
-    for (int e = t; e < 32 * kNmfMaxK; e += 256) {
-      const int jj = e / kNmfMaxK, r = e % kNmfMaxK;
-      sh[jj][r] = (jj < nj && r < K) ? Ht[(jb + jj) * K + r] : 0.0;
-    }
     for (int e = t; e < 32 * 16; e += 256) {
       const int jj = e >> 4, wd = e & 15;
       sw[jj][wd] = jj < nj ? reinterpret_cast<const uint32_t*>(packed + (jb + jj) * ld + (uint64_t)blockIdx.x * 64)[wd]
                            : 0u;
     }
     __syncthreads();
+#pragma unroll 4
     for (int jj = 0; jj < nj; ++jj) {
+      const double* hj = Ht + (jb + jj) * K;
+      double h[KM];
+#pragma unroll
+      for (int r = 0; r < KM; ++r) h[r] = hj[r];
       const uint32_t code = code_of(sw[jj][t >> 4], t & 15);
-      double v;
-      if (code == 1u) {   // missing: current reconstruction
-        v = 0.0;
+      double f = 0.0;   // missing: current reconstruction
 #pragma unroll
-        for (int r = 0; r < kNmfMaxK; ++r) v += w[r] * sh[jj][r];
-      } else {
-        v = dosage_of(code);
-      }
+      for (int r = 0; r < KM; ++r) f += w[r] * h[r];
+      const double v = code == 1u ? f : dosage_of(code);
 #pragma unroll
-      for (int r = 0; r < kNmfMaxK; ++r) acc[r] += v * sh[jj][r];
+      for (int r = 0; r < KM; ++r) acc[r] += v * h[r];
     }
   }
   if (k < n)
     for (int r = 0; r < K; ++r) part[((uint64_t)blockIdx.y * n + k) * K + r] = acc[r];
 }
+
+#define SLFMM_NMF_RANKS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+static_assert(kNmfMaxK == 16, "rank instantiations cover 1 .. kNmfMaxK");
 
 // num_W = sum of the chunk partials in chunk order
 __global__ void k_nmf_sum_chunks(const double* __restrict__ part, uint64_t count, int nchunk, double* out) {
@@ -112,53 +116,48 @@ __global__ void k_nmf_scale(double* __restrict__ W, uint64_t n, double* __restri
 }
 
 // ---- backward pass with the H update fused (src/nmf.rs:171-221) --------------
-// One thread per SNP; samples in blocks of 64 with their W rows in LDS.
+// One thread per SNP; the W row of the sample in hand is the same for every
+// thread (scalar loads; the rank K is a template parameter).
+template <int K>
 __global__ void __launch_bounds__(256) k_nmf_bwd(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t n,
                                                  uint64_t p, const double* __restrict__ W,
-                                                 const double* __restrict__ WtW, int K, double eps, double* Ht) {
-  __shared__ double sW[64][kNmfMaxK + 1];
-  __shared__ double sG[kNmfMaxK * kNmfMaxK];
+                                                 const double* __restrict__ WtW, double eps, double* Ht) {
+  constexpr int KM = K;
   const int t = threadIdx.x;
   const uint64_t j = (uint64_t)blockIdx.x * 256 + t;
   const bool ok = j < p;
-  for (int e = t; e < K * K; e += 256) sG[e] = WtW[e];
-  double h[kNmfMaxK], acc[kNmfMaxK];
+  double h[KM], acc[KM];
 #pragma unroll
-  for (int r = 0; r < kNmfMaxK; ++r) {
-    h[r] = (ok && r < K) ? Ht[j * K + r] : 0.0;
+  for (int r = 0; r < KM; ++r) {
+    h[r] = ok ? Ht[j * K + r] : 0.0;
     acc[r] = 0.0;
   }
   const uint4* row = reinterpret_cast<const uint4*>(packed + (ok ? j : 0) * ld);
   const uint64_t nkb = cdiv(n, 64);
   for (uint64_t kb = 0; kb < nkb; ++kb) {
-    __syncthreads();
-    for (int e = t; e < 64 * kNmfMaxK; e += 256) {
-      const int kk = e / kNmfMaxK, r = e % kNmfMaxK;
-      const uint64_t k = kb * 64 + kk;
-      sW[kk][r] = (k < n && r < K) ? W[k * K + r] : 0.0;
-    }
     const uint4 q = ok ? row[kb] : make_uint4(0, 0, 0, 0);
-    __syncthreads();
     const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
-    for (int kk = 0; kk < 64; ++kk) {
+    const int lim = (int)std::min<uint64_t>(64, n - kb * 64);
+#pragma unroll 4
+    for (int kk = 0; kk < lim; ++kk) {
+      const double* wk = W + (kb * 64 + kk) * K;
+      double wr[KM];
+#pragma unroll
+      for (int r = 0; r < KM; ++r) wr[r] = wk[r];
       const uint32_t code = code_of(wd[kk >> 4], kk & 15);
-      double v;
-      if (code == 1u) {   // missing: fill with W_k . H_j (pre-update H)
-        v = 0.0;
+      double f = 0.0;   // missing: fill with W_k . H_j (pre-update H)
 #pragma unroll
-        for (int r = 0; r < kNmfMaxK; ++r) v += sW[kk][r] * h[r];
-      } else {
-        v = dosage_of(code);
-      }
+      for (int r = 0; r < KM; ++r) f += wr[r] * h[r];
+      const double v = code == 1u ? f : dosage_of(code);
 #pragma unroll
-      for (int r = 0; r < kNmfMaxK; ++r) acc[r] += v * sW[kk][r];
+      for (int r = 0; r < KM; ++r) acc[r] += v * wr[r];
     }
   }
   if (!ok) return;
-  double nh[kNmfMaxK];
+  double nh[KM];
   for (int r = 0; r < K; ++r) {
     double d = 0.0;
-    for (int s = 0; s < K; ++s) d += sG[r * K + s] * h[s];
+    for (int s2 = 0; s2 < K; ++s2) d += WtW[r * K + s2] * h[s2];
     double v = h[r];
     if (d > eps) v *= acc[r] / d;
     nh[r] = v > eps ? v : eps;
@@ -175,7 +174,7 @@ __global__ void __launch_bounds__(256) k_nmf_cv(const uint8_t* __restrict__ pack
                                                 uint64_t p, const double* __restrict__ W,
                                                 const double* __restrict__ Ht, int K,
                                                 const int32_t* __restrict__ nmiss, const int32_t* __restrict__ sumg,
-                                                uint64_t mask_seed, uint64_t chunk, double cv_rate,
+                                                uint64_t mask_seed, uint64_t chunk, uint64_t cv_thresh,
                                                 double* __restrict__ part) {
   __shared__ double r0[256], r1[256];
   __shared__ unsigned long long r2[256];
@@ -188,17 +187,19 @@ __global__ void __launch_bounds__(256) k_nmf_cv(const uint8_t* __restrict__ pack
     const double mu = nobs > 0 ? (double)sumg[j] / (double)nobs : 0.0;
     const uint64_t seed = mask_seed + j / chunk;
     const uint64_t base = (j % chunk) * n;
+    // draw base + k of the stream: u01 < cv_rate  <=>  (mix >> 11) < cv_thresh
+    uint64_t ctr = stream_counter(seed, base);
     const uint4* row = reinterpret_cast<const uint4*>(packed + j * ld);
     const uint64_t nkb = cdiv(n, 64);
     for (uint64_t kb = 0; kb < nkb; ++kb) {
       const uint4 q = row[kb];
       const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
       const int lim = (int)std::min<uint64_t>(64, n - kb * 64);
-      for (int kk = 0; kk < lim; ++kk) {
+      for (int kk = 0; kk < lim; ++kk, ctr += kStreamStep) {
         const uint64_t k = kb * 64 + kk;
-        const double u = u01(seed, base + k);
+        const bool drawn = (mix_counter(ctr) >> 11) < cv_thresh;
         const uint32_t code = code_of(wd[kk >> 4], kk & 15);
-        if (u < cv_rate && code != 1u) {
+        if (drawn && code != 1u) {
           double rec = 0.0;
           for (int r = 0; r < K; ++r) rec += W[k * K + r] * Ht[j * K + r];
           const double g = dosage_of(code);
@@ -222,14 +223,24 @@ __global__ void __launch_bounds__(256) k_nmf_cv(const uint8_t* __restrict__ pack
   }
 }
 
-// out[0..2] = (nmf MAE, mean-imputation MAE, held-out count) of one pass
-__global__ void k_nmf_cv_final(const double* __restrict__ part, int nblk, double* out) {
-  if (threadIdx.x != 0) return;
+// out[0..2] = (nmf MAE, mean-imputation MAE, held-out count) of one pass;
+// one workgroup of 256 threads, partials combined in a fixed order.
+__global__ void __launch_bounds__(256) k_nmf_cv_final(const double* __restrict__ part, int nblk, double* out) {
+  __shared__ double s0[256], s1[256], s2[256];
+  const int t = threadIdx.x;
   double a = 0.0, b = 0.0, c = 0.0;
-  for (int i = 0; i < nblk; ++i) { a += part[i * 3]; b += part[i * 3 + 1]; c += part[i * 3 + 2]; }
-  out[0] = c > 0.0 ? a / c : 0.0;
-  out[1] = c > 0.0 ? b / c : 0.0;
-  out[2] = c;
+  for (int i = t; i < nblk; i += 256) { a += part[i * 3]; b += part[i * 3 + 1]; c += part[i * 3 + 2]; }
+  s0[t] = a; s1[t] = b; s2[t] = c;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) { s0[t] += s0[t + h]; s1[t] += s1[t + h]; s2[t] += s2[t + h]; }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = s2[0] > 0.0 ? s0[0] / s2[0] : 0.0;
+    out[1] = s2[0] > 0.0 ? s1[0] / s2[0] : 0.0;
+    out[2] = s2[0];
+  }
 }
 
 // ---- initialisation helpers (src/nmf.rs:279-362) -------------------------------
@@ -374,7 +385,12 @@ void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t
   double* cvpart = ctx->buf<double>("nmf_cvpart", (uint64_t)cvblk * 3);
   for (uint64_t it = 0; it < cfg.n_iter; ++it) {
     // forward: W update from the current reconstruction
-    k_nmf_fwd<<<dim3((unsigned)ntile, (unsigned)nchunk), 256, 0, s>>>(Y, ld, n, pe, spc, W, Ht, K, part);
+    const dim3 fgrid((unsigned)ntile, (unsigned)nchunk);
+    switch (K) {
+#define SLFMM_FWD(R) case R: k_nmf_fwd<R><<<fgrid, 256, 0, s>>>(Y, ld, n, pe, spc, W, Ht, part); break;
+      SLFMM_NMF_RANKS(SLFMM_FWD)
+#undef SLFMM_FWD
+    }
     k_nmf_sum_chunks<<<(unsigned)cdiv(n * K, 256), 256, 0, s>>>(part, n * K, (int)nchunk, num);
     ts_gram(ctx, Ht, K, K, Ht, K, K, pe, hh);
     k_nmf_wupd<<<(unsigned)cdiv(n, 256), 256, 0, s>>>(W, num, hh, n, K, eps);
@@ -382,13 +398,18 @@ void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t
     normalize_w(ctx, W, n, Ht, pe, K, eps);
     // backward: H update in place, fill from the new W and the old H
     ts_gram(ctx, W, K, K, W, K, K, n, WtW);
-    k_nmf_bwd<<<(unsigned)cdiv(pe, 256), 256, 0, s>>>(Y, ld, n, pe, W, WtW, K, eps, Ht);
+    const unsigned bgrid = (unsigned)cdiv(pe, 256);
+    switch (K) {
+#define SLFMM_BWD(R) case R: k_nmf_bwd<R><<<bgrid, 256, 0, s>>>(Y, ld, n, pe, W, WtW, eps, Ht); break;
+      SLFMM_NMF_RANKS(SLFMM_BWD)
+#undef SLFMM_BWD
+    }
     SLFMM_HIP(hipGetLastError());
     normalize_w(ctx, W, n, Ht, pe, K, eps);
     // held-out error of this iteration
-    k_nmf_cv<<<cvblk, 256, 0, s>>>(Y, ld, n, pe, W, Ht, K, nmiss, sumg, cfg.seed + it, cfg.chunk_size, cfg.cv_rate,
-                                   cvpart);
-    k_nmf_cv_final<<<1, 64, 0, s>>>(cvpart, (int)cvblk, cv + 3 * it);
+    k_nmf_cv<<<cvblk, 256, 0, s>>>(Y, ld, n, pe, W, Ht, K, nmiss, sumg, cfg.seed + it, cfg.chunk_size,
+                                   u01_threshold(cfg.cv_rate), cvpart);
+    k_nmf_cv_final<<<1, 256, 0, s>>>(cvpart, (int)cvblk, cv + 3 * it);
     SLFMM_HIP(hipGetLastError());
   }
 }
@@ -530,7 +551,7 @@ void gwas_cv(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uin
   double* part = ctx->buf<double>("gcv_part", (uint64_t)nblk * 3);
   k_gwas_cv<<<nblk, 256, 0, s>>>(packed, ld, n, p, W, Wp, K, Hraw, ldh, SWp, nmiss, sumg, base, seed, chunk, cv_rate,
                                  part);
-  k_nmf_cv_final<<<1, 64, 0, s>>>(part, (int)nblk, out3);
+  k_nmf_cv_final<<<1, 256, 0, s>>>(part, (int)nblk, out3);
   SLFMM_HIP(hipGetLastError());
 }
 
