@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-snps", type=int, default=5000)
+    ap.add_argument("--workload", choices=["lfmm2", "nmf"], default="lfmm2",
+                    help="lfmm2: the headline fit (default); nmf: NMF imputation estimation and the "
+                         "NMF-imputed fit at n=5000, p=2M, nmf-k=8, 10 iterations, K=8, d=2 (FP64)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB on the device three times (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter calibration) and exit")
@@ -72,6 +75,10 @@ def main():
                                    "note (SNP-range sharding + histogram all-reduce), not run here",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.workload == "nmf":
+        nmf_workload(args)
+        return
 
     import torch
     import schnellfmm_amd as S
@@ -199,6 +206,62 @@ def main():
         "kernels_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kern.items()},
         "fp64_gpu": fp64,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def nmf_workload(args):
+    """NMF imputation (src/nmf.rs:80-455) and the NMF-imputed fit
+    (src/lib.rs:162-263) at n=5000, p=2M (10 % missing), nmf-k=8, 10
+    multiplicative-update iterations, cv-rate 5e-4, then LFMM2 with K=8, d=2.
+    Both run in FP64 on the GPU with the panel resident in HBM."""
+    import torch
+    import schnellfmm_amd as S
+
+    torch.cuda.init()
+    n, p = 5000, 2_000_000
+    ctx = S.Context(0)
+    geno = S.BedFile.synthetic(n, p, n_pops=8, fst=(0.02, 0.15), maf=(0.05, 0.5), miss_rate=0.10, seed=5,
+                               ctx=ctx)
+    torch.cuda.synchronize()
+    cfg = S.NmfParams(k=8, n_iter=10, cv_rate=5e-4, seed=42, chunk_size=10_000)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((n, 2))
+    lc = S.Lfmm2Config(k=8, norm=S.EIGENSTRAT, chunk_size=10_000, n_power_iter=2, seed=42)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, out
+
+    steps, warmup = max(args.steps, 1), max(args.warmup, 0)
+    dt_est, est = timed(lambda: S.nmf_impute_estimation(geno, None, cfg), steps, warmup)
+    log(f"[bench] nmf estimation: {dt_est * 1e3:.1f} ms")
+    dt_fit, fit = timed(lambda: S.fit_lfmm2_nmf(geno, None, geno, x, lc, cfg), max(steps // 2, 1), min(warmup, 1))
+    log(f"[bench] nmf fit: {dt_fit * 1e3:.1f} ms")
+    bytes_pass = p * math.ceil(n / 4)
+    passes = 2 + 3 * cfg.n_iter
+    line = {
+        "metric": "NMF imputation estimation SNP-iterations/s (n=5000, nmf-k=8)",
+        "value": p * cfg.n_iter / dt_est, "unit": "SNP-iterations/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": dt_est * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (Balding-Nichols, 10% missing, generated in HBM)",
+        "config": {"workload": "nmf_impute_estimation n=5000 p=2000000 nmf_k=8 n_iter=10 cv_rate=5e-4",
+                   "n": n, "p": p, "nmf_k": 8, "n_iter": cfg.n_iter},
+        "roofline": {"bound": "fp64", "achieved": 8.0 * n * 8 * p * cfg.n_iter / dt_est / 1e12,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": 8.0 * n * 8 * p * cfg.n_iter / dt_est / 1e12 / FP64_PEAK_TFLOPS,
+                     "traffic": None, "kernel": "nmf iteration (forward + backward + CV)",
+                     "genotype_bytes_per_pass": bytes_pass, "passes": passes},
+        "cv_mae_last": float(est.cv_mae_per_iter[-1]), "cv_mae_mean_impute": float(est.cv_mae_mean_impute),
+        "fit_ms": dt_fit * 1e3, "fit_snps_per_s": p / dt_fit, "fit_gif": float(fit.gif),
+        "gwas_cv": [float(v) for v in fit.gwas_cv],
+        "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 

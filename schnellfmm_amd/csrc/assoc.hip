@@ -28,16 +28,19 @@ namespace slfmm {
 // wave w -> rows 16*(w&1)..+16 of the row block, SNP group w>>1; lane ->
 // 2 SNPs.  A k-block of 64 samples of A is staged in LDS.
 // ---------------------------------------------------------------------------
-// With FILL, missing calls take the value W_k . H_j - mu_j (NMF imputation
-// after centring): the 64 W rows of the block are staged beside A, and each
-// lane keeps the K loadings of its two SNPs in registers.
-template <bool FILL>
+// With a fill rank FK > 0, missing calls take the value W_k . H_j - mu_j (NMF
+// imputation after centring, rank FK = fs.K): the 64 W rows of the block are
+// staged beside A, and each lane keeps the FK loadings of its two SNPs in
+// registers.
+template <int FK>
 __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t p,
                                                   uint64_t npad, const double* __restrict__ A, uint64_t lda,
                                                   const double* __restrict__ mu, double* __restrict__ F,
                                                   uint64_t ldF, const double* __restrict__ outscale, FillSpec fs) {
+  constexpr bool FILL = FK > 0;
+  constexpr int KR = FILL ? FK : 1;
   __shared__ double As[64][33];
-  __shared__ double Ws[FILL ? 64 : 1][kNmfMaxK + 1];
+  __shared__ double Ws[FILL ? 64 : 1][KR + 1];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int half = w & 1, grp = w >> 1;
   const int rb = blockIdx.y;
@@ -47,12 +50,12 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
   const double m0 = ok0 ? mu[j0] : 0.0, m1 = ok1 ? mu[j1] : 0.0;
   const double l00 = -m0, l02 = 1.0 - m0, l03 = 2.0 - m0;
   const double l10 = -m1, l12 = 1.0 - m1, l13 = 2.0 - m1;
-  double h0[FILL ? kNmfMaxK : 1], h1[FILL ? kNmfMaxK : 1];
+  double h0[KR], h1[KR];
   if constexpr (FILL) {
 #pragma unroll
-    for (int r = 0; r < kNmfMaxK; ++r) {
-      h0[r] = (ok0 && r < fs.K) ? fs.Ht[j0 * fs.K + r] : 0.0;
-      h1[r] = (ok1 && r < fs.K) ? fs.Ht[j1 * fs.K + r] : 0.0;
+    for (int r = 0; r < KR; ++r) {
+      h0[r] = ok0 ? fs.Ht[j0 * KR + r] : 0.0;
+      h1[r] = ok1 ? fs.Ht[j1 * KR + r] : 0.0;
     }
   }
   double acc0[16], acc1[16];
@@ -68,10 +71,10 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
       As[kk][r] = A[(kb * 64 + kk) * lda + rb * 32 + r];
     }
     if constexpr (FILL) {
-      for (int e = t; e < 64 * kNmfMaxK; e += 256) {
-        const int kk = e / kNmfMaxK, r = e % kNmfMaxK;
+      for (int e = t; e < 64 * KR; e += 256) {
+        const int kk = e / KR, r = e % KR;
         const uint64_t k = kb * 64 + kk;
-        Ws[kk][r] = (k < fs.n && r < fs.K) ? fs.W[k * fs.K + r] : 0.0;
+        Ws[kk][r] = k < fs.n ? fs.W[k * KR + r] : 0.0;
       }
     }
     uint4 q0 = ok0 ? row0[kb] : make_uint4(0, 0, 0, 0);
@@ -87,7 +90,7 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
         if (c0 == 1u || c1 == 1u) {
           double f0 = 0.0, f1 = 0.0;
 #pragma unroll
-          for (int r = 0; r < kNmfMaxK; ++r) {
+          for (int r = 0; r < KR; ++r) {
             f0 += Ws[kk][r] * h0[r];
             f1 += Ws[kk][r] * h1[r];
           }
@@ -121,10 +124,17 @@ void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t
                      const double* outscale, const FillSpec* fill) {
   if (p == 0) return;
   dim3 grid((unsigned)cdiv(p, 256), (unsigned)nrb);
-  if (fill && fill->W)
-    k_func_f64<true><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, *fill);
-  else
-    k_func_f64<false><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, FillSpec{});
+  if (fill && fill->W) {
+    SLFMM_CHECK(fill->K >= 1 && fill->K <= kNmfMaxK, SLFMM_E_ARG, "NMF rank must be 1..16");
+    switch (fill->K) {
+#define SLFMM_FUNC(R) \
+  case R: k_func_f64<R><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, *fill); break;
+      SLFMM_NMF_RANKS(SLFMM_FUNC)
+#undef SLFMM_FUNC
+    }
+  } else {
+    k_func_f64<0><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, FillSpec{});
+  }
   SLFMM_HIP(hipGetLastError());
 }
 

@@ -41,6 +41,9 @@ constexpr int kRowBlk = 32;          // rows per FP64 functional row-block
 constexpr uint64_t kSlack = 128;
 constexpr int kMaxM = 64;   // functionals per SNP (K + d, or K + oversampling)
 constexpr int kNmfMaxK = 16; // NMF components
+// every NMF rank 1 .. kNmfMaxK gets its own kernel instantiation
+#define SLFMM_NMF_RANKS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+static_assert(kNmfMaxK == 16, "rank instantiations cover 1 .. kNmfMaxK");
 constexpr int kMaxD = 16;   // traits
 
 __host__ __device__ inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }

@@ -72,9 +72,6 @@ __global__ void __launch_bounds__(256) k_nmf_fwd(const uint8_t* __restrict__ pac
     for (int r = 0; r < K; ++r) part[((uint64_t)blockIdx.y * n + k) * K + r] = acc[r];
 }
 
-#define SLFMM_NMF_RANKS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
-static_assert(kNmfMaxK == 16, "rank instantiations cover 1 .. kNmfMaxK");
-
 // num_W = sum of the chunk partials in chunk order
 __global__ void k_nmf_sum_chunks(const double* __restrict__ part, uint64_t count, int nchunk, double* out) {
   const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

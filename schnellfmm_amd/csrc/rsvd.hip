@@ -24,7 +24,7 @@ void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* packedT, uint64_t ldt, ui
 // One workgroup per (256-sample tile, SNP chunk, 32-column block); partial
 // sums per chunk are reduced in a fixed order afterwards.  With FILL the
 // missing calls take W_k . H_j - mu_j (NMF in-RAM imputation).
-template <bool FILL>
+template <int FK>
 __global__ void __launch_bounds__(256) k_acc_f64(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t n,
                                                  uint64_t p, uint64_t snps_per_chunk, const double* __restrict__ Qz,
                                                  uint64_t ldq, int l, const double* __restrict__ mu,
@@ -33,17 +33,19 @@ __global__ void __launch_bounds__(256) k_acc_f64(const uint8_t* __restrict__ pac
   __shared__ double sq[32][33];
   __shared__ double smu[32];
   __shared__ uint32_t sw[32][16];
-  __shared__ double sh[FILL ? 32 : 1][kNmfMaxK + 1];
+  constexpr bool FILL = FK > 0;
+  constexpr int KR = FILL ? FK : 1;   // fill rank (fs.K)
+  __shared__ double sh[FILL ? 32 : 1][KR + 1];
   const int t = threadIdx.x;
   const uint64_t k = (uint64_t)blockIdx.x * 256 + t;
   const uint64_t j0 = (uint64_t)blockIdx.y * snps_per_chunk;
   const uint64_t j1 = std::min(p, j0 + snps_per_chunk);
   const int cb = blockIdx.z;
   const int ncol = min(32, l - cb * 32);
-  double wk[FILL ? kNmfMaxK : 1];
+  double wk[KR];
   if constexpr (FILL) {
 #pragma unroll
-    for (int r = 0; r < kNmfMaxK; ++r) wk[r] = (k < n && r < fs.K) ? fs.W[k * fs.K + r] : 0.0;
+    for (int r = 0; r < KR; ++r) wk[r] = k < n ? fs.W[k * KR + r] : 0.0;
   }
   double acc[32];
 #pragma unroll
@@ -61,9 +63,9 @@ __global__ void __launch_bounds__(256) k_acc_f64(const uint8_t* __restrict__ pac
       sw[jj][wd] = jj < nj ? reinterpret_cast<const uint32_t*>(packed + (jb + jj) * ld + (uint64_t)blockIdx.x * 64)[wd] : 0u;
     }
     if constexpr (FILL) {
-      for (int e = t; e < 32 * kNmfMaxK; e += 256) {
-        const int jj = e / kNmfMaxK, r = e % kNmfMaxK;
-        sh[jj][r] = (jj < nj && r < fs.K) ? fs.Ht[(jb + jj) * fs.K + r] : 0.0;
+      for (int e = t; e < 32 * KR; e += 256) {
+        const int jj = e / KR, r = e % KR;
+        sh[jj][r] = jj < nj ? fs.Ht[(jb + jj) * KR + r] : 0.0;
       }
     }
     __syncthreads();
@@ -75,7 +77,7 @@ __global__ void __launch_bounds__(256) k_acc_f64(const uint8_t* __restrict__ pac
         if (code == 1u) {
           double f = 0.0;
 #pragma unroll
-          for (int r = 0; r < kNmfMaxK; ++r) f += wk[r] * sh[jj][r];
+          for (int r = 0; r < KR; ++r) f += wk[r] * sh[jj][r];
           miss = f - m;
         }
       }
@@ -110,10 +112,17 @@ static void accumulate_f64(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, u
   const uint64_t ldp = ncb * 32;
   double* part = ctx->buf<double>("rs_accpart", nchunk * n * ldp);
   const dim3 grid((unsigned)ntile, (unsigned)nchunk, (unsigned)ncb);
-  if (fill && fill->W)
-    k_acc_f64<true><<<grid, 256, 0, s>>>(packed, ld, n, p, spc, Qz, ldq, (int)l, mu, sc, part, ldp, *fill);
-  else
-    k_acc_f64<false><<<grid, 256, 0, s>>>(packed, ld, n, p, spc, Qz, ldq, (int)l, mu, sc, part, ldp, FillSpec{});
+  if (fill && fill->W) {
+    SLFMM_CHECK(fill->K >= 1 && fill->K <= kNmfMaxK, SLFMM_E_ARG, "NMF rank must be 1..16");
+    switch (fill->K) {
+#define SLFMM_ACC(R) \
+  case R: k_acc_f64<R><<<grid, 256, 0, s>>>(packed, ld, n, p, spc, Qz, ldq, (int)l, mu, sc, part, ldp, *fill); break;
+      SLFMM_NMF_RANKS(SLFMM_ACC)
+#undef SLFMM_ACC
+    }
+  } else {
+    k_acc_f64<0><<<grid, 256, 0, s>>>(packed, ld, n, p, spc, Qz, ldq, (int)l, mu, sc, part, ldp, FillSpec{});
+  }
   k_acc_reduce<<<(unsigned)cdiv(n * l, 256), 256, 0, s>>>(part, n, (int)l, ldp, (int)nchunk, W, ldw);
   SLFMM_HIP(hipGetLastError());
 }
