@@ -50,9 +50,11 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-snps", type=int, default=5000)
-    ap.add_argument("--workload", choices=["lfmm2", "nmf"], default="lfmm2",
+    ap.add_argument("--workload", choices=["lfmm2", "nmf", "batch"], default="lfmm2",
                     help="lfmm2: the headline fit (default); nmf: NMF imputation estimation and the "
-                         "NMF-imputed fit at n=5000, p=2M, nmf-k=8, 10 iterations, K=8, d=2 (FP64)")
+                         "NMF-imputed fit at n=5000, p=2M, nmf-k=8, 10 iterations, K=8, d=2 (FP64); "
+                         "batch: 64 independent fits of seeded uneven shapes, batched vs looped")
+    ap.add_argument("--streams", type=int, default=4, help="streams of the batched fit (--workload batch)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB on the device three times (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter calibration) and exit")
@@ -78,6 +80,9 @@ def main():
 
     if args.workload == "nmf":
         nmf_workload(args)
+        return
+    if args.workload == "batch":
+        batch_workload(args)
         return
 
     import torch
@@ -262,6 +267,93 @@ def nmf_workload(args):
         "fit_ms": dt_fit * 1e3, "fit_snps_per_s": p / dt_fit, "fit_gif": float(fit.gif),
         "gwas_cv": [float(v) for v in fit.gwas_cv],
         "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def batch_workload(args):
+    """64 independent LFMM2 problems of uneven size (gen seed 4): n ~ logU[200, 5000],
+    p ~ logU[2e4, 2e6], K ~ U{3..20}, d ~ U{1..8}, 2-10 populations, 0-5 %
+    missing, RSVD on a 10 % SNP subset, mixed precision, outputs in HBM.
+    Timed as one slfmm_fit_lfmm2_batch call and as a loop of slfmm_fit_lfmm2."""
+    import ctypes as C
+    import torch
+    import schnellfmm_amd as S
+    from schnellfmm_amd._lib import Problem, lib
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(4)
+    ctx = S.Context(0)
+    count = 64
+    arr = (Problem * count)()
+    keep = []
+    shapes = []
+    total_snps = 0
+    total_bytes = 0
+    for i in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(200), math.log(5000)))))
+        p = int(round(math.exp(rng.uniform(math.log(2e4), math.log(2e6)))))
+        K, d = int(rng.integers(3, 21)), int(rng.integers(1, 9))
+        pops, miss = int(rng.integers(2, 11)), float(rng.uniform(0.0, 0.05))
+        g = S.BedFile.synthetic(n, p, n_pops=pops, fst=(0.02, 0.15), maf=(0.05, 0.5), miss_rate=miss,
+                                seed=4000 + i, ctx=ctx)
+        x = np.ascontiguousarray(rng.standard_normal((n, d)))
+        idx = np.ascontiguousarray(S.subset_indices(0.1, p), dtype=np.int64)
+        t = {k: torch.empty(sh, dtype=torch.float64, device=dev) for k, sh in
+             (("beta", (p, d)), ("t", (p, d)), ("r2", (p, 3)), ("p", (p, d)), ("gpt", (d,)), ("gif", (1,)),
+              ("u", (n, K)))}
+        cfg = S.Lfmm2Config(k=K, oversampling=10, n_power_iter=2, norm=S.EIGENSTRAT, precision=S.PREC_I8,
+                            seed=42).to_c()
+        P = arr[i]
+        P.est, P.subset_idx, P.n_idx, P.g = g.h, idx.ctypes.data, len(idx), g.h
+        P.x, P.d, P.cfg, P.b0 = x.ctypes.data, d, cfg, None
+        P.u_hat = t["u"].data_ptr()
+        P.out.beta, P.out.tstat, P.out.r2, P.out.pvalue = (t[k].data_ptr() for k in ("beta", "t", "r2", "p"))
+        P.out.hist, P.out.gif_per_trait, P.out.gif = None, t["gpt"].data_ptr(), t["gif"].data_ptr()
+        keep.append((g, x, idx, t))
+        shapes.append((n, p, K, d))
+        total_snps += p
+        total_bytes += p * math.ceil(n / 4)
+    torch.cuda.synchronize()
+    log(f"[bench] batch: {count} problems, {total_snps} SNPs, {total_bytes / 1e9:.2f} GB of genotypes")
+
+    def run_loop():
+        for i in range(count):
+            P = arr[i]
+            ctx.check(lib().slfmm_fit_lfmm2(ctx.h, P.est, P.subset_idx, P.n_idx, P.g, P.x, P.d, C.byref(P.cfg),
+                                            None, P.u_hat, C.byref(P.out), 1))
+
+    def run_batch(ns):
+        ctx.check(lib().slfmm_fit_lfmm2_batch(ctx.h, arr, count, ns, 1))
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    steps, warmup = max(args.steps, 1), max(args.warmup, 1)
+    dt_loop = timed(run_loop, steps, warmup)
+    log(f"[bench] loop of single fits: {dt_loop * 1e3:.1f} ms")
+    dt_b = {}
+    for ns in sorted({1, args.streams}):
+        dt_b[ns] = timed(lambda: run_batch(ns), steps, warmup)
+        log(f"[bench] batched, {ns} stream(s): {dt_b[ns] * 1e3:.1f} ms")
+    dt = dt_b[args.streams]
+    line = {
+        "metric": "LFMM2 batch SNPs/s (64 problems of uneven size, mixed precision)",
+        "value": total_snps / dt, "unit": "SNPs/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "i8-mfma/i32-acc+f64", "data": "synthetic (Balding-Nichols, generated in HBM)",
+        "config": {"workload": "batch of 64 lfmm2 fits, gen seed 4", "problems": count, "snps": total_snps,
+                   "genotype_bytes": total_bytes, "streams": args.streams, "est_rate": 0.1},
+        "loop_ms": dt_loop * 1e3, "batch_ms_by_streams": {str(k): v * 1e3 for k, v in dt_b.items()},
+        "speedup_vs_loop": dt_loop / dt, "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 

@@ -191,6 +191,31 @@ int slfmm_nmf_estimate(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* subse
                        const slfmm_nmf_config* cfg, const double* omega, double* w, double* h,
                        double* cv_mae, double* cv_mae_mean, uint64_t* mask_count, uint32_t flags);
 
+/* One problem of a batched fit: the arguments of slfmm_fit_lfmm2. */
+typedef struct slfmm_problem {
+  const slfmm_geno* est;
+  const int64_t* subset_idx;
+  uint64_t n_idx;
+  const slfmm_geno* g;
+  const double* x;
+  uint64_t d;
+  slfmm_config cfg;
+  const double* b0;
+  double* u_hat;
+  slfmm_assoc_out out;
+} slfmm_problem;
+
+/* Many independent fit_lfmm2 problems (each src/lib.rs:162-263) in one call.
+ * No counterpart in the reference, which fits one problem per process; this
+ * is the batching pattern of the hot path (many small LFMM2 problems, uneven
+ * sizes).  Problem i runs on internal stream i % n_streams (1..8) of the
+ * context, so small problems overlap on the device, and the host synchronises
+ * once per stream at the end.  Each problem's results are those of
+ * slfmm_fit_lfmm2 on the same inputs; outputs follow `flags` as there.  An
+ * error names the problem it came from ("problem i: ..."). */
+int slfmm_fit_lfmm2_batch(slfmm_ctx* ctx, slfmm_problem* problems, uint64_t count, int n_streams,
+                          uint32_t flags);
+
 /* fit_lfmm2 with NMF imputation (src/lib.rs:162-263, ImputeConfig::Nmf*),
  * FP64: NMF estimation on est's subset, W^+ = safe_inv(W^T W) W^T, the RSVD
  * on the NMF-filled estimation rows (NmfInRam) and the association on every

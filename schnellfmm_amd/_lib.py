@@ -33,6 +33,12 @@ class AssocOut(C.Structure):
                 ("gif_per_trait", vp), ("gif", vp)]
 
 
+class Problem(C.Structure):
+    """slfmm_problem: the arguments of slfmm_fit_lfmm2 for one problem of a batch."""
+    _fields_ = [("est", vp), ("subset_idx", vp), ("n_idx", u64), ("g", vp), ("x", vp), ("d", u64),
+                ("cfg", Config), ("b0", vp), ("u_hat", vp), ("out", AssocOut)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "slfmm_config_default": (None, [vp]),
@@ -57,6 +63,7 @@ SIGNATURES = {
     "slfmm_estimate_factors": (C.c_int, [vp, vp, vp, u64, vp, u64, vp, vp, vp, u32]),
     "slfmm_test_associations": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, u32]),
     "slfmm_fit_lfmm2": (C.c_int, [vp, vp, vp, u64, vp, vp, u64, vp, vp, vp, vp, u32]),
+    "slfmm_fit_lfmm2_batch": (C.c_int, [vp, vp, u64, C.c_int, u32]),
     "slfmm_nmf_estimate": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp, u32]),
     "slfmm_fit_lfmm2_nmf": (C.c_int, [vp, vp, vp, u64, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp, vp, u32]),
     "slfmm_debug_mfma_i8": (C.c_int, [vp, vp, vp, vp]),

@@ -356,6 +356,53 @@ def fit_lfmm2(y_est: BedFile, est_subset, y_full: BedFile, x, config: Lfmm2Confi
     return TestResults(u, float(gif[0]), gpt, beta, t, r2, p, hist)
 
 
+@dataclass
+class Lfmm2Problem:
+    """One problem of fit_lfmm2_batch: the arguments of fit_lfmm2."""
+    y_est: BedFile
+    est_subset: object
+    y_full: BedFile
+    x: np.ndarray
+    config: Lfmm2Config
+    b0: Optional[np.ndarray] = None
+
+
+def fit_lfmm2_batch(problems, n_streams: int = 4, with_hist: bool = False):
+    """Many independent fit_lfmm2 problems in one call (slfmm_fit_lfmm2_batch):
+    problem i runs on stream i % n_streams of the context, so small problems
+    overlap on the GPU.  Returns one TestResults per problem, equal to what
+    fit_lfmm2 returns for the same inputs."""
+    from ._lib import Problem
+    problems = list(problems)
+    if not problems:
+        return []
+    ctx = problems[0].y_full.ctx
+    arr = (Problem * len(problems))()
+    keep, results = [], []
+    for i, pr in enumerate(problems):
+        x = _f64(pr.x)
+        idx = None if pr.est_subset is None or (isinstance(pr.est_subset, str) and pr.est_subset == "all") \
+            else subset_indices(pr.est_subset, pr.y_est.n_snps)
+        d = x.shape[1]
+        arrs = _assoc_arrays(pr.y_full.n_snps, d, with_hist)
+        u = np.empty((pr.y_full.n_samples, pr.config.k))
+        b = None if pr.b0 is None else _f64(pr.b0)
+        P = arr[i]
+        P.est, P.subset_idx, P.n_idx = pr.y_est.h, _ptr(idx), 0 if idx is None else len(idx)
+        P.g, P.x, P.d = pr.y_full.h, _ptr(x), d
+        P.cfg = pr.config.to_c()
+        P.b0, P.u_hat = _ptr(b), _ptr(u)
+        P.out = _out_struct(arrs)
+        keep.append((x, idx, b))
+        results.append((u, arrs))
+    ctx.check(lib().slfmm_fit_lfmm2_batch(ctx.h, arr, len(problems), n_streams, 0))
+    out = []
+    for u, arrs in results:
+        beta, t, r2, p, hist, gpt, gif = arrs
+        out.append(TestResults(u, float(gif[0]), gpt, beta, t, r2, p, hist))
+    return out
+
+
 def fit_lfmm2_nmf(y_est: BedFile, est_subset, y_full: BedFile, x, config: Lfmm2Config, nmf: NmfParams,
                   b0=None, nmf_omega=None, gwas_cv: bool = True, with_hist: bool = False) -> TestResults:
     """Full pipeline with NMF imputation, src/lib.rs:162-263 (FP64)."""

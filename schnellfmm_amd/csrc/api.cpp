@@ -64,7 +64,7 @@ int* slfmm_ctx::check_slot(const std::string& warn, int code, const std::string&
   SLFMM_CHECK(checks.size() < (size_t)kMaxChecks, SLFMM_E_ARG, "too many solver status slots in one call");
   if (!status) status = static_cast<int*>(scratch("__status", kMaxChecks * sizeof(int)));
   if (checks.empty()) SLFMM_HIP(hipMemsetAsync(status, 0, kMaxChecks * sizeof(int), stream));
-  checks.push_back({warn, code, e});
+  checks.push_back({warn, code, check_prefix + e});
   return status + (checks.size() - 1);
 }
 
@@ -83,6 +83,7 @@ std::vector<int> slfmm_ctx::finish_checks() {
 }
 
 slfmm_ctx::~slfmm_ctx() {
+  for (slfmm_ctx* w : workers) delete w;
   for (auto& kv : timers)
     for (auto& ev : kv.second.pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
   if (stream) (void)hipStreamSynchronize(stream);
@@ -505,6 +506,81 @@ int slfmm_fit_lfmm2(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset
     assoc_host(ctx, g, dx, d, du, cfg, out, flags);                    // src/lib.rs:254
     if (!dev && u_hat) to_host(ctx, u_hat, du, n * K);
     SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int slfmm_fit_lfmm2_batch(slfmm_ctx* ctx, slfmm_problem* probs, uint64_t count, int n_streams, uint32_t flags) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && (probs || count == 0), SLFMM_E_ARG, "NULL argument");
+    SLFMM_CHECK(n_streams >= 1 && n_streams <= 8, SLFMM_E_ARG, "n_streams must be 1..8");
+    for (uint64_t i = 0; i < count; ++i) {
+      const slfmm_problem& P = probs[i];
+      const std::string tag = "problem " + std::to_string(i) + ": ";
+      if (!(P.est && P.g && P.x && P.d >= 1)) throw Error(SLFMM_E_ARG, tag + "NULL argument");
+      if (P.est->n != P.g->n) throw Error(SLFMM_E_ARG, tag + "estimation and test panels must have the same samples");
+      try {
+        check_cfg(&P.cfg);
+      } catch (const Error& e) {
+        throw Error(e.code, tag + e.what());
+      }
+    }
+    while ((int)ctx->workers.size() < n_streams) {
+      slfmm_ctx* w = new slfmm_ctx();
+      w->device = ctx->device;
+      if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete w;
+        throw Error(SLFMM_E_HIP, "stream creation failed");
+      }
+      ctx->workers.push_back(w);
+    }
+    // panels and inputs queued on the context's own stream come first
+    SLFMM_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<slfmm_ctx*> ws(ctx->workers.begin(), ctx->workers.begin() + n_streams);
+    for (slfmm_ctx* w : ws) w->checks.clear();
+    std::vector<std::vector<double>> xs(count);   // host sources of the queued copies
+    const bool dev = flags & SLFMM_OUT_DEVICE;
+    auto drain = [&] {
+      for (slfmm_ctx* w : ws) (void)hipStreamSynchronize(w->stream);
+    };
+    for (uint64_t i = 0; i < count; ++i) {
+      slfmm_problem& P = probs[i];
+      slfmm_ctx* w = ws[i % n_streams];
+      w->check_prefix = "problem " + std::to_string(i) + ": ";
+      try {
+        const uint64_t n = P.g->n, K = P.cfg.k, l = K + P.cfg.oversampling, d = P.d;
+        xs[i].resize(n * d);
+        center_host(P.x, n, d, P.cfg.scale_cov, xs[i].data());
+        // the next problem on this stream reuses these buffers only after this
+        // problem's work (stream order)
+        double* dx = to_dev(w, "api_xs", xs[i].data(), n * d);
+        const double* db0 = P.b0 ? to_dev(w, "api_b0", P.b0, n * l) : nullptr;
+        double* du = (dev && P.u_hat) ? P.u_hat : w->buf<double>("api_uhat", n * K);
+        rsvd_run(w, P.est, P.subset_idx, P.subset_idx ? P.n_idx : P.est->p, dx, d, P.cfg, db0, du);
+        assoc_host(w, P.g, dx, d, du, &P.cfg, &P.out, flags);
+        if (!dev && P.u_hat) to_host(w, P.u_hat, du, n * K);
+      } catch (const Error& e) {
+        const std::string tag = w->check_prefix;
+        drain();
+        for (slfmm_ctx* v : ws) { v->checks.clear(); v->check_prefix.clear(); }
+        throw Error(e.code, tag + e.what());
+      } catch (...) {
+        drain();
+        for (slfmm_ctx* v : ws) { v->checks.clear(); v->check_prefix.clear(); }
+        throw;
+      }
+    }
+    std::string err;
+    int code = SLFMM_OK;
+    for (slfmm_ctx* w : ws) {
+      w->check_prefix.clear();
+      try {
+        w->finish_checks();   // synchronises the stream
+      } catch (const Error& e) {
+        if (code == SLFMM_OK) { code = e.code; err = e.what(); }
+      }
+    }
+    drain();
+    if (code != SLFMM_OK) throw Error(code, err);
   });
 }
 

@@ -86,6 +86,9 @@ struct slfmm_ctx {
   // warn is printed to stderr for value 1 (if non-empty); value 2 raises
   // Error(code, err) unless code is SLFMM_OK.
   int* check_slot(const std::string& warn, int code, const std::string& err);
+  std::string check_prefix;   // prepended to the error text of new slots (batched fits)
+  // Contexts of the extra streams of batched fits (same device, own scratch).
+  std::vector<slfmm_ctx*> workers;
   // Reads the slots (synchronises the stream), prints warnings in order and
   // throws the first error; returns the values.  Clears the slot list.
   std::vector<int> finish_checks();
