@@ -192,7 +192,7 @@ def main():
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n, p, K, d),
-                "kernel": "k_gt_i8p<4,3,32,0>", "avg_ms": avg_ms, "bytes_per_launch": bytes_per_launch}
+                "kernel": "k_gt_i8p<4,3,24,0>", "avg_ms": avg_ms, "bytes_per_launch": bytes_per_launch}
 
     cpu = None
     if args.cpu_baseline:

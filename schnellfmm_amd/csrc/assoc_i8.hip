@@ -136,7 +136,14 @@ __global__ void k_recomb(RowMap rmap, const double* __restrict__ sc, const doubl
     rc->m = rmap.m;
     rc->ones = rmap.ones();
   }
-  if (i >= rmap.m) return;
+  if (i >= kMaxM) return;
+  if (i >= rmap.m) {   // padding functionals: zero weights on valid rows, so f_i = 0
+    rc->ng[i] = 2;
+    for (int t = 0; t < 3; ++t) rc->rg[i][t] = 0;
+    rc->rm[i][0] = rc->rm[i][1] = 0;
+    rc->wg[i] = rc->wm[i] = rc->C[i] = 0.0;
+    return;
+  }
   const int ng = rmap.ng(i);
   rc->ng[i] = ng;
   for (int t = 0; t < 3; ++t) rc->rg[i][t] = t < ng ? rmap.rg(i, t) : -1;
@@ -177,7 +184,7 @@ static Operand prepare_operand(slfmm_ctx* ctx, const char* tag, const double* Q,
   double* O = ctx->buf<double>(T + "_O", m * 3);
   colreduce<3, kColSum>(ctx, FnTerms{Q, ldq, sc}, GStore3{O}, n, m);
   op.rc = ctx->buf<Recomb>(T + "_rc", 1);
-  k_recomb<<<1, 64, 0, s>>>(rmap, sc, O, op.rc);
+  k_recomb<<<1, kMaxM, 0, s>>>(rmap, sc, O, op.rc);
   const uint64_t total = op.nblk * op.KS * NT * 1024;
   op.pack = ctx->buf<int8_t>(T + "_pack", total);
   k_pack_rows<<<(unsigned)std::min<uint64_t>(cdiv(total, 256), 16384), 256, 0, s>>>(Q, ldq, n, rmap, sc, NT, op.KS,
@@ -264,7 +271,10 @@ __device__ __forceinline__ void chunk_to_lds(const unsigned char* src_chunk, uns
 // so the tiles need no workgroup barrier.  The recombination table is read
 // from its LDS copy.
 // ---------------------------------------------------------------------------
-constexpr int kEpiNQ = 8;   // functionals per lane group (K + d <= 32)
+// functionals per lane group: MAXM / 4 for the association (MAXM = 24 or 32),
+// 8 for the RSVD sketch (l <= 32)
+template <int MAXM, int MODE>
+constexpr int epi_nq() { return MODE == 1 ? 8 : MAXM / 4; }
 
 // Copies the recombination table to LDS (all threads of the workgroup take
 // part; the caller synchronises before reading it).
@@ -276,10 +286,11 @@ __device__ __forceinline__ const Recomb* stage_recomb(const Recomb* src, unsigne
   return reinterpret_cast<const Recomb*>(dst);
 }
 
-template <int ST, int NT, int MODE>
+template <int ST, int NT, int MAXM, int MODE>
 __device__ __forceinline__ void gt_epilogue(const GtArgs& a, const v4i (&acc)[ST][NT], uint32_t (&c_hom)[ST],
                                             uint32_t (&c_miss)[ST], int* red, const double* plan, const Recomb* rc,
                                             uint64_t base, int g, int l16) {
+  constexpr int NQ = epi_nq<MAXM, MODE>();
 #pragma unroll
   for (int st = 0; st < ST; ++st) {
     c_hom[st] += __shfl_xor(c_hom[st], 16); c_hom[st] += __shfl_xor(c_hom[st], 32);
@@ -299,27 +310,24 @@ __device__ __forceinline__ void gt_epilogue(const GtArgs& a, const v4i (&acc)[ST
     const int64_t nobs = (int64_t)a.n - nmiss;
     const int64_t sg = red[ones * 16 + l16];   // dosage sum (all-ones operand row)
     const double mu = nobs > 0 ? (double)sg / (double)nobs : 0.0;
-    double fq[kEpiNQ];
+    double fq[NQ];   // functionals past m have zero weights in the table: fq = 0
 #pragma unroll
-    for (int q = 0; q < kEpiNQ; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       const int i = g + 4 * q;
-      fq[q] = 0.0;
-      if (i < a.m) {
-        const int ng = rc->ng[i];
-        int64_t X = (int64_t)red[rc->rg[i][0] * 16 + l16] * 254 + red[rc->rg[i][1] * 16 + l16];
-        if (ng == 3) X = X * 254 + red[rc->rg[i][2] * 16 + l16];
-        const int64_t Y = (int64_t)red[rc->rm[i][0] * 16 + l16] * 254 + red[rc->rm[i][1] * 16 + l16];
-        fq[q] = rc->wg[i] * (double)X + mu * (rc->wm[i] * (double)Y - rc->C[i]);
-      }
+      const int ng = rc->ng[i];
+      int64_t X = (int64_t)red[rc->rg[i][0] * 16 + l16] * 254 + red[rc->rg[i][1] * 16 + l16];
+      if (ng == 3) X = X * 254 + red[rc->rg[i][2] * 16 + l16];
+      const int64_t Y = (int64_t)red[rc->rm[i][0] * 16 + l16] * 254 + red[rc->rm[i][1] * 16 + l16];
+      fq[q] = rc->wg[i] * (double)X + mu * (rc->wm[i] * (double)Y - rc->C[i]);
     }
     asm volatile("" ::: "memory");   // this tile's reads precede the next tile's writes
     if constexpr (MODE == 1) {
       const double scale = epi_scale(mu, a.norm);
 #pragma unroll
-      for (int q = 0; q < kEpiNQ; ++q)
+      for (int q = 0; q < NQ; ++q)
         if (valid && g + 4 * q < a.m) a.Z[j * a.ldZ + g + 4 * q] = scale * fq[q];
     } else {
-      snp_epilogue_split<kEpiNQ>(a.ea, plan, j, valid, g, fq, nmiss, sg, nhom);
+      snp_epilogue_split<NQ>(a.ea, plan, j, valid, g, fq, nmiss, sg, nhom);
     }
   }
 }
@@ -332,7 +340,7 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
   constexpr int CH = KS * NT;                              // 1 KB operand chunks per block
   constexpr int LDS_A = 2 * CH * 1024;
   constexpr int LDS_RED = 4 * NT * 1024;                   // one SNP tile of int32 sums per wave
-  constexpr int PLAN_MAX = MODE == 0 ? 3 * kMaxD * MAXM + kMaxD + MAXM * MAXM + 1 : 0;
+  constexpr int PLAN_MAX = MODE == 0 ? 3 * kMaxD * 32 + kMaxD + MAXM * MAXM + 1 : 0;   // split_plan_len bound
   constexpr int LDS_E = LDS_RED + PLAN_MAX * 8 + (int)sizeof(Recomb);
   constexpr int LDS = LDS_A > LDS_E ? LDS_A : LDS_E;
   __shared__ __align__(16) unsigned char lds[LDS];
@@ -407,14 +415,13 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
   const double* plan = nullptr;
   if constexpr (MODE == 0) {
     double* sp = reinterpret_cast<double*>(lds + LDS_RED);
-    const int len = plan_len(a.ea.d, a.ea.m);
-    for (int e = t; e < len; e += 256) sp[e] = a.ea.plan[e];
+    stage_split_plan(a.ea.plan, a.ea.d, a.ea.m, sp, t, 256);
     plan = sp;
   }
   const Recomb* rc = stage_recomb(a.rc, lds + LDS_RED + PLAN_MAX * 8, t, 256);
   __syncthreads();
-  gt_epilogue<ST, NT, MODE>(a, acc, c_hom, c_miss, reinterpret_cast<int*>(lds) + wave * (NT * 256), plan, rc, base,
-                            g, l16);
+  gt_epilogue<ST, NT, MAXM, MODE>(a, acc, c_hom, c_miss, reinterpret_cast<int*>(lds) + wave * (NT * 256), plan, rc,
+                                  base, g, l16);
 }
 
 // ---------------------------------------------------------------------------
@@ -614,13 +621,12 @@ __global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
   const double* plan = nullptr;
   if constexpr (MODE == 0) {
     double* sp = reinterpret_cast<double*>(A0);
-    const int len = plan_len(a.ea.d, a.ea.m);
-    for (int e = t; e < len; e += 512) sp[e] = a.ea.plan[e];
+    stage_split_plan(a.ea.plan, a.ea.d, a.ea.m, sp, t, 512);
     plan = sp;
   }
   const Recomb* rc = stage_recomb(a.rc, A1, t, 512);
   __syncthreads();
-  gt_epilogue<ST, NT, MODE>(a, acc, c_hom, c_miss, red, plan, rc, base, g, l16);
+  gt_epilogue<ST, NT, MAXM, MODE>(a, acc, c_hom, c_miss, red, plan, rc, base, g, l16);
 }
 
 #define SLFMM_GT_CASES(X) \
@@ -630,6 +636,7 @@ template <int A0, int A1>
 static bool launch_pipe(hipStream_t s, unsigned grid, GtArgs& a, int mode, int maxm) {
   if constexpr (A0 + A1 <= 8) {
     if (mode == 1) k_gt_i8p<A0, A1, 1, 1><<<grid, 512, 0, s>>>(a);
+    else if (maxm <= 24) k_gt_i8p<A0, A1, 24, 0><<<grid, 512, 0, s>>>(a);
     else if (maxm <= 32) k_gt_i8p<A0, A1, 32, 0><<<grid, 512, 0, s>>>(a);
     else return false;
     return true;
@@ -658,6 +665,7 @@ static void launch_gt(hipStream_t s, const Operand& op, GtArgs& a, int mode, int
 #define X(A0, A1)                                                                      \
     if (!done && key == A0 * 100 + A1 && A0 + A1 > 8) {                                \
       if (mode == 1) k_gt_i8<A0, A1, 1, 1><<<grid, 256, 0, s>>>(a);                    \
+      else if (maxm <= 24) k_gt_i8<A0, A1, 24, 0><<<grid, 256, 0, s>>>(a);             \
       else if (maxm <= 32) k_gt_i8<A0, A1, 32, 0><<<grid, 256, 0, s>>>(a);             \
       done = true;                                                                     \
     }
@@ -675,7 +683,7 @@ void launch_func_i8(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_Q, uint
   // numerators, last d columns of the basis): 3 terms while K + d <= 32.
   SLFMM_CHECK(m <= 32, SLFMM_E_ARG, "the int8 association path supports K + d <= 32 (use SLFMM_PREC_FP64)");
   Operand op = prepare_operand(ctx, "as_i8", d_Q, m, g->n, (int)m, ea.K);
-  const uint64_t plan_bytes = 8 * (uint64_t)plan_len(ea.d, (int)m);
+  const uint64_t plan_bytes = 8 * (uint64_t)split_plan_len(ea.d, (int)m);
   SLFMM_CHECK(op.NT0 + op.NT1 > 8 || plan_bytes <= (uint64_t)4 * (op.NT0 + op.NT1) * 1024, SLFMM_E_ARG,
               "association plan does not fit the int8 kernel's LDS (use SLFMM_PREC_FP64)");
   GtArgs a{};

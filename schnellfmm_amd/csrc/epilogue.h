@@ -9,6 +9,34 @@ namespace slfmm {
 // L (m x m) | fallback flag (1.0: RSS by the quadratic form L L^T).
 __host__ __device__ inline int plan_len(int d, int m) { return d * m * 3 + d + m * m + 1; }
 
+// LDS layout of the plan for snp_epilogue_split: P, Rm and Fm as
+// [dp][32] with dp = 4 ceil(d / 4) (zero rows past d, zero columns past m),
+// Vd [dp] (1 past d), then L [m][m] and the fallback flag as in the plan.
+__host__ __device__ inline int split_plan_len(int d, int m) {
+  const int dp = (d + 3) & ~3;
+  return 3 * dp * 32 + dp + m * m + 1;
+}
+
+// Copies the plan into the split layout (all threads of the workgroup take
+// part; the caller synchronises before reading it).
+__device__ inline void stage_split_plan(const double* __restrict__ plan, int d, int m, double* sp, int t, int nt) {
+  const int dp = (d + 3) & ~3;
+  const double* P = plan;
+  const double* Vd = P + d * m;
+  const double* Rm = Vd + d;
+  const double* Fm = Rm + d * m;
+  const double* L = Fm + d * m;
+  for (int e = t; e < dp * 32; e += nt) {
+    const int u = e >> 5, i = e & 31;
+    const bool in = u < d && i < m;
+    sp[e] = in ? P[u * m + i] : 0.0;
+    sp[dp * 32 + e] = in ? Rm[u * m + i] : 0.0;
+    sp[2 * dp * 32 + e] = in ? Fm[u * m + i] : 0.0;
+  }
+  for (int e = t; e < dp; e += nt) sp[3 * dp * 32 + e] = e < d ? Vd[e] : 1.0;
+  for (int e = t; e < m * m + 1; e += nt) sp[3 * dp * 32 + dp + e] = L[e];
+}
+
 struct EpiArgs {
   uint64_t n, p;
   int d, K, m, norm;
@@ -106,18 +134,39 @@ __device__ void snp_epilogue(const EpiArgs& a, const double* sPlan, uint64_t j, 
 }
 
 
-// Sum over the four lane groups of a wave (lanes l, l^16, l^32, l^48).  The
-// two butterfly steps give every lane the same bits: (x0 + x1) + (x2 + x3)
-// up to commutation of each addition.
-__device__ __forceinline__ double sum_lane_groups(double x) {
-  x += __shfl_xor(x, 16, 64);
-  return x + __shfl_xor(x, 32, 64);
+// x_lower + x_upper of a lane-swap pair: v_permlane16_swap (row r of 16 lanes
+// with row r ^ 1) or v_permlane32_swap (lanes l with l ^ 32), applied to both
+// words of the double.  Swapping a register with itself leaves the lower
+// partner's value in element 0 and the upper partner's in element 1, so every
+// lane adds the same two values in the same order.
+template <int HALF>
+__device__ __forceinline__ double add_swapped(double x) {
+  const uint64_t u = (uint64_t)__double_as_longlong(x);
+  const uint32_t lo = (uint32_t)u, hi = (uint32_t)(u >> 32);
+  uint32_t l0, l1, h0, h1;
+  if constexpr (HALF == 16) {
+    const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    l0 = a[0]; l1 = a[1]; h0 = b[0]; h1 = b[1];
+  } else {
+    const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    l0 = a[0]; l1 = a[1]; h0 = b[0]; h1 = b[1];
+  }
+  return __longlong_as_double((long long)(((uint64_t)h0 << 32) | l0)) +
+         __longlong_as_double((long long)(((uint64_t)h1 << 32) | l1));
 }
+
+// Sum over the four lane groups of a wave (lanes l, l^16, l^32, l^48):
+// (x0 + x1) + (x2 + x3), the same bits in every lane.  Every lane of the wave
+// must take part.
+__device__ __forceinline__ double sum_lane_groups(double x) { return add_swapped<32>(add_swapped<16>(x)); }
 
 // The same outputs as snp_epilogue, with the functionals of SNP j spread
 // over the four lane groups: lane group g holds fq[q] = f_i for
 // i = g + 4 q.  Dot products are formed per lane and summed over the lane
-// groups; trait jj is finished and stored by lane group jj % 4.  Every lane
+// groups; trait jj is finished and stored by lane group jj % 4, r2 fraction
+// r by lane group r.  Every lane
 // of the wave must call this (it shuffles); `valid` masks the stores.
 template <int NQ>
 __device__ void snp_epilogue_split(const EpiArgs& a, const double* sPlan, uint64_t j, bool valid, int g,
@@ -128,14 +177,15 @@ __device__ void snp_epilogue_split(const EpiArgs& a, const double* sPlan, uint64
   const double sc = epi_scale(mean, a.norm);
   const int64_t sg2 = sg + 2 * nh;
   const double tss = nobs > 0 ? sc * sc * ((double)(nobs * sg2 - sg * sg) / (double)nobs) : 0.0;
-  double fy[NQ];
+  double fy[NQ];   // zero for i >= m (fq is)
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) fy[q] = g + 4 * q < m ? sc * fq[q] : 0.0;
-  const double* P = sPlan;
-  const double* Vd = P + d * m;
-  const double* Rm = Vd + d;
-  const double* Fm = Rm + d * m;
-  const double* L = Fm + d * m;
+  for (int q = 0; q < NQ; ++q) fy[q] = sc * fq[q];
+  const int dp = (d + 3) & ~3;
+  const double* P = sPlan;   // split layout (stage_split_plan)
+  const double* Rm = P + dp * 32;
+  const double* Fm = Rm + dp * 32;
+  const double* Vd = Fm + dp * 32;
+  const double* L = Vd + dp;
   const bool fallback = L[m * m] != 0.0;
   double proj = 0.0;
   if (!fallback) {
@@ -156,22 +206,31 @@ __device__ void snp_epilogue_split(const EpiArgs& a, const double* sPlan, uint64
   if (rss < 0.0) rss = 0.0;
   const double sigma2 = rss / a.df;
   double sscov = 0.0;
-  for (int jj = 0; jj < d; ++jj) {
-    double coef = 0.0, b = 0.0, fc = 0.0;
+  for (int jb = 0; jb < d; jb += 4) {
+    // the sums of traits jb .. jb + 3 reach every lane; lane group g finishes trait jb + g
+    double cf[4], bt[4];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int i = g + 4 * q;
-      if (i < m) {
-        coef += P[jj * m + i] * fy[q];
-        b += Rm[jj * m + i] * fy[q];
-        fc += Fm[jj * m + i] * fy[q];
+    for (int u = 0; u < 4; ++u) {   // traits past d have zero rows: they add exact zeros
+      const int jj = jb + u;
+      double coef = 0.0, b = 0.0, fc = 0.0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int i = g + 4 * q;
+        coef += P[jj * 32 + i] * fy[q];
+        b += Rm[jj * 32 + i] * fy[q];
+        fc += Fm[jj * 32 + i] * fy[q];
       }
+      coef = sum_lane_groups(coef);
+      b = sum_lane_groups(b);
+      fc = sum_lane_groups(fc);
+      sscov += fc * fc;
+      cf[u] = coef;
+      bt[u] = b;
     }
-    coef = sum_lane_groups(coef);
-    b = sum_lane_groups(b);
-    fc = sum_lane_groups(fc);
-    sscov += fc * fc;
-    if (valid && (jj & 3) == g) {
+    const int jj = jb + g;
+    const double coef = g == 0 ? cf[0] : (g == 1 ? cf[1] : (g == 2 ? cf[2] : cf[3]));
+    const double b = g == 0 ? bt[0] : (g == 1 ? bt[1] : (g == 2 ? bt[2] : bt[3]));
+    if (valid && jj < d) {
       // t_test (src/testing.rs:1036-1048)
       const double se = sqrt(sigma2 * Vd[jj]);
       double tv = 0.0;
@@ -183,18 +242,14 @@ __device__ void snp_epilogue_split(const EpiArgs& a, const double* sPlan, uint64
       a.beta[j * d + jj] = b;
     }
   }
-  if (valid && g == 0) {
-    double r0 = 0.0, r1 = 0.0, r2v = 0.0;
-    if (tss > 1e-300) {                    // src/testing.rs:394-404
+  if (valid && g < 3) {   // lane group g writes r2 fraction g (src/testing.rs:394-404)
+    double v = 0.0;
+    if (tss > 1e-300) {
       double sslat = tss - rss - sscov;
       if (sslat < 0.0) sslat = 0.0;
-      r0 = sscov / tss;
-      r1 = sslat / tss;
-      r2v = rss / tss;
+      v = (g == 0 ? sscov : (g == 1 ? sslat : rss)) / tss;
     }
-    a.r2[j * 3 + 0] = r0;
-    a.r2[j * 3 + 1] = r1;
-    a.r2[j * 3 + 2] = r2v;
+    a.r2[j * 3 + g] = v;
   }
 }
 
