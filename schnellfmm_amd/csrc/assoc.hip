@@ -379,8 +379,9 @@ __global__ void __launch_bounds__(128) k_epilogue(EpiArgs a, const double* __res
 //  k_hist_part: each workgroup counts a SNP range into LDS bins [0, LB) per
 //    trait (values beyond go straight to the global histogram by atomics,
 //    they are rare), then stores its LDS counts as a partial row;
-//  k_hist_sum:  sums the partial rows per bin and emits per-chunk totals
-//    (1024 bins per chunk) for the median search.
+//  k_hist_sum:  adds the partial rows into the histogram, 16 slices of rows
+//    in parallel (64-bit integer atomics);
+//  k_hist_chunks: per-chunk totals (1024 bins per chunk) for the median search.
 // Integer adds only, so the counts do not depend on scheduling.
 // ---------------------------------------------------------------------------
 constexpr int kChunkBins = 1024;
@@ -410,25 +411,28 @@ __global__ void __launch_bounds__(1024) k_hist_part(const double* __restrict__ t
   for (int b = t; b < d * LB; b += 1024) row[b] = h[b];
 }
 
+// hist[tr][b] += sum of the partial rows g in [z G, z G + G) for bins b < LB
+// (integer atomics: the totals do not depend on the order).
 __global__ void __launch_bounds__(256) k_hist_sum(const uint32_t* __restrict__ part, int nblk, int d, int LB,
-                                                  unsigned long long* __restrict__ hist,
-                                                  unsigned long long* __restrict__ chunk_tot) {
+                                                  int rows_per_z, unsigned long long* __restrict__ hist) {
+  const int b = blockIdx.x * 256 + threadIdx.x, tr = blockIdx.y;
+  if (b >= LB) return;
+  const int g0 = blockIdx.z * rows_per_z, g1 = min(nblk, g0 + rows_per_z);
+  unsigned long long v = 0;
+  for (int g = g0; g < g1; ++g) v += part[((uint64_t)g * d + tr) * LB + b];
+  if (v) atomicAdd(&hist[(uint64_t)tr * kBins + b], v);
+}
+
+// Per-chunk totals of the final histogram (kChunkBins bins per chunk).
+__global__ void __launch_bounds__(256) k_hist_chunks(const unsigned long long* __restrict__ hist,
+                                                     unsigned long long* __restrict__ chunk_tot) {
   __shared__ unsigned long long red[256];
   const int t = threadIdx.x, tr = blockIdx.y, ch = blockIdx.x;
   unsigned long long local = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int b = ch * kChunkBins + i * 256 + t;
-    if (b >= kBins) continue;
-    unsigned long long v;
-    if (b < LB) {
-      v = 0;
-      for (int g = 0; g < nblk; ++g) v += part[((uint64_t)g * d + tr) * LB + b];
-      hist[(uint64_t)tr * kBins + b] = v;
-    } else {
-      v = hist[(uint64_t)tr * kBins + b];
-    }
-    local += v;
+    if (b < kBins) local += hist[(uint64_t)tr * kBins + b];
   }
   red[t] = local;
   __syncthreads();
@@ -712,7 +716,10 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
   SLFMM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_part),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)(d * LB * sizeof(uint32_t))));
   k_hist_part<<<(unsigned)nblk, 1024, d * LB * sizeof(uint32_t), s>>>(d_t, p, (int)d, spb, LB, part, hist);
-  k_hist_sum<<<dim3(kHistChunks, (unsigned)d), 256, 0, s>>>(part, (int)nblk, (int)d, LB, hist, ctot);
+  const int zs = 16, rows_z = (int)cdiv(nblk, zs);
+  k_hist_sum<<<dim3((unsigned)cdiv(LB, 256), (unsigned)d, (unsigned)cdiv(nblk, rows_z)), 256, 0, s>>>(
+      part, (int)nblk, (int)d, LB, rows_z, hist);
+  k_hist_chunks<<<dim3(kHistChunks, (unsigned)d), 256, 0, s>>>(hist, ctot);
   k_gif<<<1, 256, 0, s>>>(hist, ctot, (int)d, d_gif_pt, d_gif);
   if (p) k_calibrate<<<(unsigned)cdiv(p * d, 256), 256, 0, s>>>(d_t, p, (int)d, d_gif_pt, d_p);
   SLFMM_HIP(hipGetLastError());
