@@ -230,40 +230,44 @@ void launch_decode_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64
 // ---------------------------------------------------------------------------
 // Sample-major transpose: out row i (sample) holds the codes of SNPs
 // 0..p-1, 4 SNPs per byte (SNP j at byte j/4, bits 2*(j%4)).  Tile of
-// 64 SNPs x 64 samples through LDS.
+// 128 SNPs x 128 samples: 16-byte loads of the SNP rows into LDS words,
+// 16-byte stores of the sample rows.  Every byte [0, ldt) of rows i < n is
+// written (codes past p are 0); ldt must be a multiple of 16.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_transpose(const uint8_t* __restrict__ in, uint64_t ld, uint64_t n,
                                                    uint64_t p, uint8_t* __restrict__ out, uint64_t ldt) {
-  __shared__ uint8_t tile[64][17];   // 64 SNPs x 16 bytes (64 samples)
-  const uint64_t j0 = (uint64_t)blockIdx.x * 64;   // SNP tile
-  const uint64_t i0 = (uint64_t)blockIdx.y * 64;   // sample tile
+  __shared__ uint32_t tile[128][9];   // 128 SNPs x 8 words (128 samples)
+  const uint64_t j0 = (uint64_t)blockIdx.x * 128;   // SNP tile
+  const uint64_t i0 = (uint64_t)blockIdx.y * 128;   // sample tile
   const int t = threadIdx.x;
-  for (int e = t; e < 64 * 16; e += 256) {
-    int jj = e / 16, bb = e % 16;
-    uint64_t j = j0 + jj, b = i0 / 4 + bb;
-    tile[jj][bb] = (j < p && b < ld) ? in[j * ld + b] : 0;
+  {
+    const int r = t >> 1, h = t & 1;
+    const uint64_t j = j0 + r, off = i0 / 4 + 16 * h;
+    uint4 q = make_uint4(0, 0, 0, 0);
+    if (j < p && off + 16 <= ld) q = *reinterpret_cast<const uint4*>(in + j * ld + off);
+    tile[r][4 * h + 0] = q.x; tile[r][4 * h + 1] = q.y; tile[r][4 * h + 2] = q.z; tile[r][4 * h + 3] = q.w;
   }
   __syncthreads();
-  // output: sample ii in tile, SNP byte q (4 SNPs) -> 64 samples x 16 bytes
-  for (int e = t; e < 64 * 16; e += 256) {
-    int ii = e / 16, q = e % 16;
-    uint64_t i = i0 + ii;
-    if (i >= n) continue;
+  const int ii = t >> 1, h = t & 1;
+  const uint64_t i = i0 + ii;
+  const uint64_t ob = j0 / 4 + 16 * h;
+  if (i >= n || ob >= ldt) return;
+  uint32_t w[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {   // word 4h + u: SNPs 16 (4h + u) .. + 15 of the tile
     uint32_t v = 0;
-    for (int s = 0; s < 4; ++s) {
-      int jj = 4 * q + s;
-      uint32_t code = (tile[jj][ii >> 2] >> (2 * (ii & 3))) & 3u;
-      v |= code << (2 * s);
-    }
-    uint64_t ob = j0 / 4 + q;
-    if (ob < ldt) out[i * ldt + ob] = (uint8_t)v;
+#pragma unroll
+    for (int s2 = 0; s2 < 16; ++s2) v |= ((tile[16 * (4 * h + u) + s2][ii >> 4] >> (2 * (ii & 15))) & 3u) << (2 * s2);
+    w[u] = v;
   }
+  *reinterpret_cast<uint4*>(out + i * ldt + ob) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 void launch_transpose(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
                       uint8_t* out, uint64_t ldt) {
   if (p == 0 || n == 0) return;
-  k_transpose<<<dim3((unsigned)cdiv(p, 64), (unsigned)cdiv(n, 64)), 256, 0, s>>>(packed, ld, n, p, out, ldt);
+  SLFMM_CHECK(ldt % 16 == 0 && ld % 16 == 0, SLFMM_E_ARG, "transpose strides must be multiples of 16 bytes");
+  k_transpose<<<dim3((unsigned)cdiv(p, 128), (unsigned)cdiv(n, 128)), 256, 0, s>>>(packed, ld, n, p, out, ldt);
   SLFMM_HIP(hipGetLastError());
 }
 

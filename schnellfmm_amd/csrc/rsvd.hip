@@ -273,7 +273,7 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   if (mixed) {   // sample-major copy for the forward passes
     ldt = rup(cdiv(pe, 4), 16);
     YT = ctx->buf<uint8_t>("rs_YT", n * ldt + kSlack);
-    SLFMM_HIP(hipMemsetAsync(YT, 0, n * ldt + kSlack, s));
+    SLFMM_HIP(hipMemsetAsync(YT + n * ldt, 0, kSlack, s));   // the transpose writes every row byte
     launch_transpose(s, Y, g->ld, n, pe, YT, ldt);
   }
 
