@@ -425,7 +425,7 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Pipelined kernel for NT <= 8: 8 waves x 3 tiles of 16 SNPs per workgroup,
+// Pipelined kernel for NT <= 8: 8 waves x 4 tiles of 16 SNPs per workgroup,
 // blocks of 4 k-steps (256 samples, 64 bytes per SNP row).  Both streams go
 // to LDS by buffer-descriptor DMA (buffer_load_dwordx4 ... lds): the
 // genotype segments into a 3-slot ring (blocks kb+1 and kb+2 in flight while
@@ -433,7 +433,7 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
 // unrolled over the ring period (6) so every slot is a distinct LDS object.
 // ---------------------------------------------------------------------------
 constexpr int kPipeWaves = 8;
-constexpr int kPipeTiles = 3;   // SNP tiles of 16 per wave
+constexpr int kPipeTiles = 4;   // SNP tiles of 16 per wave
 constexpr int kOperandPrefetch = 6;   // operand fragments read ahead of their MFMAs
 
 template <int ST>
