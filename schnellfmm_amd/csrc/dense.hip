@@ -69,10 +69,13 @@ __global__ void __launch_bounds__(256) k_gram_partial(const double* __restrict__
   for (uint64_t rb = r0; rb < r1; rb += 32) {
     const int nr = (int)std::min<uint64_t>(32, r1 - rb);
     __syncthreads();
-    for (int e = t; e < 32 * kSm; e += 256) {
-      int rr = e / kSm, c = e % kSm;
-      sa[rr][c] = (rr < nr && c < a) ? A[(rb + rr) * lda + c] : 0.0;
-      sb[rr][c] = (rr < nr && c < b) ? B[(rb + rr) * ldb + c] : 0.0;
+    for (int e = t; e < 32 * a; e += 256) {
+      const int rr = e / a, c = e % a;
+      sa[rr][c] = rr < nr ? A[(rb + rr) * lda + c] : 0.0;
+    }
+    for (int e = t; e < 32 * b; e += 256) {
+      const int rr = e / b, c = e % b;
+      sb[rr][c] = rr < nr ? B[(rb + rr) * ldb + c] : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -111,7 +114,7 @@ void ts_gram(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t a, const do
   SLFMM_CHECK(a <= (uint64_t)kSm && b <= (uint64_t)kSm, SLFMM_E_ARG, "ts_gram: too many columns");
   hipStream_t s = ctx->stream;
   if (rows == 0) { launch_fill(s, out, 0.0, a * b); return; }
-  uint64_t nblk = std::min<uint64_t>(cdiv(rows, 128), 1024);
+  uint64_t nblk = std::min<uint64_t>(cdiv(rows, 64), 512);
   uint64_t rpb = cdiv(rows, nblk);
   nblk = cdiv(rows, rpb);
   double* part = ctx->buf<double>("gram_part", nblk * a * b);
@@ -147,7 +150,7 @@ void ts_gemm(hipStream_t s, const double* A, uint64_t lda, uint64_t m, const dou
              uint64_t ldo, const int* gate) {
   if (rows == 0 || q == 0) return;
   SLFMM_CHECK(m <= (uint64_t)kSm && q <= (uint64_t)kSm, SLFMM_E_ARG, "ts_gemm: too many columns");
-  uint64_t blocks = std::min<uint64_t>(cdiv(rows * q, 256), 8192);
+  uint64_t blocks = std::min<uint64_t>(cdiv(rows * q, 256), 1024);   // M is staged once per workgroup
   k_ts_gemm<<<(unsigned)blocks, 256, 0, s>>>(A, lda, (int)m, M, ldm, (int)q, rows, alpha, beta, out, ldo, gate);
   SLFMM_HIP(hipGetLastError());
 }
