@@ -711,15 +711,18 @@ void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_
 // SNPs; B side: Qt and mu*Qt as 2 int8 terms per column, pre-permuted.
 // Workgroup: 4 waves x 4 sample tiles (256 samples) x one SNP chunk.
 // ---------------------------------------------------------------------------
+// (fragment counts fit 32 bits and NCB, KS are compile-time, so the index
+// arithmetic needs no 64-bit division)
+template <int NCB, int KS>
 __global__ void k_pack_cols(const double* __restrict__ Qz, uint64_t ldq, uint64_t pe, int l,
                             const double* __restrict__ ssnp, const double* __restrict__ mu,
-                            const double* __restrict__ csc, int NCB, int KS, uint64_t total, int8_t* __restrict__ out) {
-  const int NTB = 4 * NCB;
-  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+                            const double* __restrict__ csc, uint32_t total, int8_t* __restrict__ out) {
+  constexpr int NTB = 4 * NCB;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
     const int j = (int)(e & 15), lane = (int)((e >> 4) & 63);
-    const uint64_t frag = e >> 10;
+    const uint32_t frag = e >> 10;
     const int ct = (int)(frag % NTB);
-    const uint64_t ks = frag / NTB;
+    const uint32_t ks = frag / NTB;
     const int s = (int)(ks % KS);
     const uint64_t blk = ks / KS;
     const int v = j >> 2, c = j & 3, g = lane >> 4;
@@ -749,18 +752,21 @@ struct GStore1 {
   __device__ void operator()(int c, const double (&v)[1]) const { out[c] = v[0]; }
 };
 
+// WV waves x 4 tiles of 16 samples per workgroup share each staged B block.
+constexpr int kFwdWaves = 8;
 template <int NCB>
-__global__ void __launch_bounds__(256) k_fwd_i8(const uint8_t* __restrict__ YT, uint64_t ldt, uint64_t n, int blk0,
-                                                int nblk_chunk, int nblk_total, const v4i* __restrict__ Bp,
-                                                const double* __restrict__ csc, int l, double* __restrict__ part,
-                                                uint64_t ldp) {
+__global__ void __launch_bounds__(64 * kFwdWaves) k_fwd_i8(const uint8_t* __restrict__ YT, uint64_t ldt, uint64_t n,
+                                                           int blk0, int nblk_chunk, int nblk_total,
+                                                           const v4i* __restrict__ Bp, const double* __restrict__ csc,
+                                                           int l, double* __restrict__ part, uint64_t ldp) {
   constexpr int NTB = 4 * NCB;
   constexpr int KS = 4;
   constexpr int ST = 4;
+  constexpr int NTH = 64 * kFwdWaves;
   constexpr int BLK = KS * NTB * 64;
   __shared__ v4i Bs[2 * BLK];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, l16 = lane & 15;
-  const uint64_t sbase = (uint64_t)blockIdx.x * 256 + wave * (ST * 16);
+  const uint64_t sbase = (uint64_t)blockIdx.x * (ST * 16 * kFwdWaves) + wave * (ST * 16);
   const int b0 = blk0 + blockIdx.y * nblk_chunk;
   const int b1 = min(nblk_total, b0 + nblk_chunk);
   const uint8_t* rowp[ST];
@@ -776,10 +782,10 @@ __global__ void __launch_bounds__(256) k_fwd_i8(const uint8_t* __restrict__ YT, 
 #pragma unroll
     for (int ct = 0; ct < NTB; ++ct) acc[st][ct] = v4i{0, 0, 0, 0};
   uint32_t dummy0 = 0, dummy1 = 0, dummy2 = 0;
-  constexpr int PER = (BLK + 255) / 256;
+  constexpr int PER = (BLK + NTH - 1) / NTH;
   if (b0 < b1) {
     for (int q = 0; q < PER; ++q) {
-      int idx = t + 256 * q;
+      int idx = t + NTH * q;
       if (idx < BLK) Bs[idx] = Bp[(uint64_t)b0 * BLK + idx];
     }
   }
@@ -794,7 +800,7 @@ __global__ void __launch_bounds__(256) k_fwd_i8(const uint8_t* __restrict__ YT, 
     if (more) {
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
-        int idx = t + 256 * q;
+        int idx = t + NTH * q;
         if (idx < BLK) pre[q] = Bp[(uint64_t)(kb + 1) * BLK + idx];
       }
 #pragma unroll
@@ -829,7 +835,7 @@ __global__ void __launch_bounds__(256) k_fwd_i8(const uint8_t* __restrict__ YT, 
     if (more) {
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
-        int idx = t + 256 * q;
+        int idx = t + NTH * q;
         if (idx < BLK) Bs[((kb + 1 - b0) & 1) * BLK + idx] = pre[q];
       }
 #pragma unroll
@@ -881,11 +887,15 @@ void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_
   const uint64_t nblk = cdiv(pe, 64 * KS);
   const uint64_t total = nblk * KS * NTB * 1024;
   int8_t* Bp = ctx->buf<int8_t>("fw_pack", total);
-  k_pack_cols<<<(unsigned)std::min<uint64_t>(cdiv(total, 256), 16384), 256, 0, s>>>(Qz, ldq, pe, (int)l, ssnp, mu,
-                                                                                  csc, NCB, KS, total, Bp);
+  SLFMM_CHECK(total < 0xFFFFFFFFull, SLFMM_E_ARG, "estimation subset too large for the int8 forward pass");
+  const unsigned pgrid = (unsigned)std::min<uint64_t>(cdiv(total, 256), 16384);
+  if (NCB == 1)
+    k_pack_cols<1, 4><<<pgrid, 256, 0, s>>>(Qz, ldq, pe, (int)l, ssnp, mu, csc, (uint32_t)total, Bp);
+  else
+    k_pack_cols<2, 4><<<pgrid, 256, 0, s>>>(Qz, ldq, pe, (int)l, ssnp, mu, csc, (uint32_t)total, Bp);
   double* vhat = ctx->buf<double>("fw_vhat", l);
   colreduce<1, kColSum>(ctx, FnVhat{Qz, ldq, ssnp, mu, csc}, GStore1{vhat}, pe, (int)l);
-  const uint64_t sblocks = cdiv(n, 256);
+  const uint64_t sblocks = cdiv(n, 64 * kFwdWaves);
   uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(nblk, cdiv(1024, sblocks)));
   const uint64_t per = cdiv(nblk, nchunk);
   nchunk = cdiv(nblk, per);
@@ -893,10 +903,10 @@ void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_
   double* part = ctx->buf<double>("fw_part", nchunk * n * ldp);
   dim3 grid((unsigned)sblocks, (unsigned)nchunk);
   if (NCB == 1)
-    k_fwd_i8<1><<<grid, 256, 0, s>>>(YT, ldt, n, 0, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp), csc,
+    k_fwd_i8<1><<<grid, 64 * kFwdWaves, 0, s>>>(YT, ldt, n, 0, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp), csc,
                                      (int)l, part, ldp);
   else
-    k_fwd_i8<2><<<grid, 256, 0, s>>>(YT, ldt, n, 0, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp), csc,
+    k_fwd_i8<2><<<grid, 64 * kFwdWaves, 0, s>>>(YT, ldt, n, 0, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp), csc,
                                      (int)l, part, ldp);
   k_fwd_reduce<<<(unsigned)cdiv(n * l, 256), 256, 0, s>>>(part, n, (int)l, ldp, (int)nchunk, vhat, W, ldw);
   SLFMM_HIP(hipGetLastError());
