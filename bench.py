@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--precision", choices=["auto", "i8", "fp64"], default="auto")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--cpu-snps", type=int, default=5000)
+    ap.add_argument("--cpu-snps", type=int, default=40000)
     ap.add_argument("--workload", choices=["lfmm2", "nmf", "batch"], default="lfmm2",
                     help="lfmm2: the headline fit (default); nmf: NMF imputation estimation and the "
                          "NMF-imputed fit at n=5000, p=2M, nmf-k=8, 10 iterations, K=8, d=2 (FP64); "
