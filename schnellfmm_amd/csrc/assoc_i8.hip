@@ -314,11 +314,12 @@ __device__ __forceinline__ void gt_epilogue(const GtArgs& a, const v4i (&acc)[ST
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int i = g + 4 * q;
+      // X and Y are integers below 2^47, so these FP64 Horner steps are exact
       const int ng = rc->ng[i];
-      int64_t X = (int64_t)red[rc->rg[i][0] * 16 + l16] * 254 + red[rc->rg[i][1] * 16 + l16];
-      if (ng == 3) X = X * 254 + red[rc->rg[i][2] * 16 + l16];
-      const int64_t Y = (int64_t)red[rc->rm[i][0] * 16 + l16] * 254 + red[rc->rm[i][1] * 16 + l16];
-      fq[q] = rc->wg[i] * (double)X + mu * (rc->wm[i] * (double)Y - rc->C[i]);
+      double X = fma((double)red[rc->rg[i][0] * 16 + l16], 254.0, (double)red[rc->rg[i][1] * 16 + l16]);
+      if (ng == 3) X = fma(X, 254.0, (double)red[rc->rg[i][2] * 16 + l16]);
+      const double Y = fma((double)red[rc->rm[i][0] * 16 + l16], 254.0, (double)red[rc->rm[i][1] * 16 + l16]);
+      fq[q] = rc->wg[i] * X + mu * (rc->wm[i] * Y - rc->C[i]);
     }
     asm volatile("" ::: "memory");   // this tile's reads precede the next tile's writes
     if constexpr (MODE == 1) {
