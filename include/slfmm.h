@@ -123,6 +123,25 @@ int slfmm_geno_upload(slfmm_ctx* ctx, const uint8_t* packed, uint64_t n_phys, ui
 int slfmm_geno_wrap_device(slfmm_ctx* ctx, void* d_packed, uint64_t n, uint64_t n_snps,
                            uint64_t ld, slfmm_geno** out);
 
+/* Host-resident panels (out-of-core): the rows stay in host memory and are
+ * streamed to the device in chunks of `chunk_snps` rows (0: about 1 GiB of
+ * rows per chunk) through two pinned staging buffers, the copies overlapping
+ * the association of the previous chunk on a second stream.  Counterpart of
+ * the reference's mmap + IO-thread chunk pipeline (BedFile::open,
+ * src/bed.rs:78-135; parallel_stream, src/parallel.rs:156-329) for panels
+ * larger than device memory.  Every entry point that takes a panel accepts
+ * one, except that the test panel of slfmm_fit_lfmm2_nmf (NMF imputation in
+ * the association pass) must be device-resident.  Results are identical to
+ * those of the same rows held on the device.
+ *
+ * open_bed_host maps <bed_path> read-only (same validation as
+ * slfmm_bed_open); the panel owns the mapping.  wrap_host borrows `packed`
+ * (n_snps rows of ceil(n/4) bytes, the .bed payload after the magic bytes),
+ * which must stay valid and unchanged until slfmm_geno_free. */
+int slfmm_geno_open_bed_host(slfmm_ctx* ctx, const char* bed_path, uint64_t chunk_snps, slfmm_geno** out);
+int slfmm_geno_wrap_host(slfmm_ctx* ctx, const uint8_t* packed, uint64_t n, uint64_t n_snps,
+                         uint64_t chunk_snps, slfmm_geno** out);
+
 void slfmm_geno_free(slfmm_geno* g);
 int slfmm_geno_info(const slfmm_geno* g, uint64_t* n_samples, uint64_t* n_snps, uint64_t* ld);
 

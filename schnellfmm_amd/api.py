@@ -105,12 +105,14 @@ def subset_indices(spec, n_snps: int) -> np.ndarray:
 class BedFile:
     """A device-resident PLINK genotype panel (BedFile, src/bed.rs:60-172)."""
 
-    def __init__(self, handle, n_samples: int, n_snps: int, ctx: Context, path: Optional[str] = None):
+    def __init__(self, handle, n_samples: int, n_snps: int, ctx: Context, path: Optional[str] = None,
+                 host_buffer=None):
         self.h = handle
         self.n_samples = n_samples
         self.n_snps = n_snps
         self.ctx = ctx
         self.path = path
+        self._host_buffer = host_buffer   # rows borrowed by a host-resident panel
 
     @classmethod
     def open(cls, bed_path: str, sample_keep: Optional[Sequence[int]] = None,
@@ -143,6 +145,33 @@ class BedFile:
                                           0 if keep is None else len(keep), C.byref(h)))
         n = len(keep) if keep is not None else n_phys
         return cls(h, n, n_snps, ctx)
+
+    @classmethod
+    def open_host(cls, bed_path: str, chunk_snps: int = 0, ctx: Optional[Context] = None) -> "BedFile":
+        """Host-resident panel: the .bed stays memory-mapped and its rows are
+        streamed to the device chunk by chunk (src/bed.rs:78-135 mmap,
+        src/parallel.rs:156-329 chunk pipeline).  chunk_snps=0 picks about
+        1 GiB of rows per chunk."""
+        ctx = ctx or default_context()
+        h = C.c_void_p()
+        ctx.check(lib().slfmm_geno_open_bed_host(ctx.h, bed_path.encode(), chunk_snps, C.byref(h)))
+        n, p = C.c_uint64(), C.c_uint64()
+        lib().slfmm_geno_info(h, C.byref(n), C.byref(p), None)
+        return cls(h, n.value, p.value, ctx, bed_path)
+
+    @classmethod
+    def wrap_host(cls, packed, n: int, n_snps: int, chunk_snps: int = 0,
+                  ctx: Optional[Context] = None) -> "BedFile":
+        """Host-resident panel over packed .bed rows in host memory (borrowed;
+        the panel keeps a reference to the buffer)."""
+        ctx = ctx or default_context()
+        buf = np.frombuffer(packed, dtype=np.uint8) if not isinstance(packed, np.ndarray) else packed
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        if buf.size < n_snps * ((n + 3) // 4):
+            raise ValueError("packed buffer shorter than n_snps rows of ceil(n/4) bytes")
+        h = C.c_void_p()
+        ctx.check(lib().slfmm_geno_wrap_host(ctx.h, _ptr(buf), n, n_snps, chunk_snps, C.byref(h)))
+        return cls(h, n, n_snps, ctx, host_buffer=buf)
 
     @classmethod
     def synthetic(cls, n: int, p: int, n_pops: int = 20, fst=(0.02, 0.15), maf=(0.05, 0.5),

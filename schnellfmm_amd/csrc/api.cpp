@@ -5,10 +5,12 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <thread>
 
 #include "internal.h"
 
@@ -30,6 +32,32 @@ void* slfmm_ctx::scratch(const std::string& name, size_t bytes) {
   }
   pool[name] = {p, bytes};
   return p;
+}
+
+void* slfmm_ctx::pinned_buf(int i, size_t bytes) {
+  if (pinned[i] && pinned_bytes[i] >= bytes) return pinned[i];
+  if (pinned[i]) {
+    SLFMM_HIP(hipDeviceSynchronize());
+    (void)hipHostFree(pinned[i]);
+    pinned[i] = nullptr;
+    pinned_bytes[i] = 0;
+  }
+  if (hipHostMalloc(&pinned[i], bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    pinned[i] = nullptr;
+    throw Error(SLFMM_E_NOMEM, "pinned host allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  pinned_bytes[i] = bytes;
+  return pinned[i];
+}
+
+void slfmm_ctx::ensure_copy_stream() {
+  if (copy_stream) return;
+  SLFMM_HIP(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+  for (int b = 0; b < 2; ++b) {
+    SLFMM_HIP(hipEventCreateWithFlags(&ev_copied[b], hipEventDisableTiming));
+    SLFMM_HIP(hipEventCreateWithFlags(&ev_consumed[b], hipEventDisableTiming));
+  }
 }
 
 std::pair<hipEvent_t, hipEvent_t> slfmm_ctx::tic(const std::string&) {
@@ -84,10 +112,18 @@ std::vector<int> slfmm_ctx::finish_checks() {
 
 slfmm_ctx::~slfmm_ctx() {
   for (slfmm_ctx* w : workers) delete w;
+  if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+  for (int b = 0; b < 2; ++b) {
+    if (ev_copied[b]) (void)hipEventDestroy(ev_copied[b]);
+    if (ev_consumed[b]) (void)hipEventDestroy(ev_consumed[b]);
+  }
   for (auto& kv : timers)
     for (auto& ev : kv.second.pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
   if (stream) (void)hipStreamSynchronize(stream);
   for (auto& kv : pool) (void)hipFree(kv.second.first);
+  for (int b = 0; b < 2; ++b)
+    if (pinned[b]) (void)hipHostFree(pinned[b]);
+  if (copy_stream) (void)hipStreamDestroy(copy_stream);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -226,6 +262,69 @@ void center_host(const double* x, uint64_t n, uint64_t d, int scale, double* out
 
 }  // namespace
 
+namespace slfmm {
+
+// memcpy over a few host threads (large copies from a mapped file into
+// pinned staging memory run at host-memory speed, not one core's)
+static void parallel_memcpy(uint8_t* dst, const uint8_t* src, size_t bytes) {
+  const size_t piece = (size_t)32 << 20;
+  const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, bytes / piece));
+  if (nt == 1) {
+    if (bytes) std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t per = (bytes + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const size_t a = (size_t)t * per, b = std::min(bytes, a + per);
+    if (a < b) th.emplace_back([=] { std::memcpy(dst + a, src + a, b - a); });
+  }
+  for (auto& x : th) x.join();
+}
+
+uint64_t host_chunk_rows(const slfmm_geno* g) {
+  if (g->chunk_snps) return g->chunk_snps;
+  const uint64_t target = (uint64_t)1 << 30;   // ~1 GiB of rows per chunk
+  return std::max<uint64_t>(512, (target / g->ld) / 512 * 512);
+}
+
+void host_chunk_upload(slfmm_ctx* ctx, const slfmm_geno* g, uint64_t j0, uint64_t j1, int b, uint8_t* dst) {
+  const uint64_t rows = j1 - j0, bytes = rows * g->bps;
+  auto* pin = static_cast<uint8_t*>(ctx->pinned_buf(b, std::max<uint64_t>(bytes, 16)));
+  parallel_memcpy(pin, g->host + j0 * g->bps, bytes);   // the rows of a .bed are contiguous
+  if (rows)
+    SLFMM_HIP(hipMemcpy2DAsync(dst, g->ld, pin, g->bps, g->bps, rows, hipMemcpyHostToDevice, ctx->copy_stream));
+  SLFMM_HIP(hipEventRecord(ctx->ev_copied[b], ctx->copy_stream));
+}
+
+void host_rows_to_device(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* rows, uint64_t m, uint8_t* dst) {
+  const uint64_t piece_rows = std::max<uint64_t>(1, ((uint64_t)256 << 20) / g->bps);
+  for (uint64_t r0 = 0; r0 < m; r0 += piece_rows) {
+    const uint64_t nr = std::min(piece_rows, m - r0);
+    auto* pin = static_cast<uint8_t*>(ctx->pinned_buf(0, nr * g->bps));
+    if (!rows) {
+      parallel_memcpy(pin, g->host + r0 * g->bps, nr * g->bps);
+    } else {
+      for (uint64_t r = 0; r < nr; ++r)
+        std::memcpy(pin + r * g->bps, g->host + (uint64_t)rows[r0 + r] * g->bps, g->bps);
+    }
+    SLFMM_HIP(hipMemcpy2DAsync(dst + r0 * g->ld, g->ld, pin, g->bps, g->bps, nr, hipMemcpyHostToDevice, ctx->stream));
+    geno_clear_tail(ctx->stream, dst + r0 * g->ld, g->ld, nr, g->n);
+    SLFMM_HIP(hipStreamSynchronize(ctx->stream));   // the staging buffer is reused by the next piece
+  }
+}
+
+}  // namespace slfmm
+
+namespace {
+
+void require_device(const slfmm_geno* g) {
+  SLFMM_CHECK(!g || !g->host, SLFMM_E_ARG,
+              "this entry point needs a device-resident panel (this panel is streamed from host memory)");
+}
+
+}  // namespace
+
 extern "C" {
 
 void slfmm_config_default(slfmm_config* c) {
@@ -327,9 +426,43 @@ int slfmm_geno_wrap_device(slfmm_ctx* ctx, void* d_packed, uint64_t n, uint64_t 
   });
 }
 
+int slfmm_geno_open_bed_host(slfmm_ctx* ctx, const char* bed_path, uint64_t chunk_snps, slfmm_geno** out) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && bed_path && out, SLFMM_E_ARG, "NULL argument");
+    Mapped m;
+    uint64_t n = 0, p = 0;
+    bed_open(bed_path, n, p, &m);
+    auto* g = new slfmm_geno();
+    g->ctx = ctx; g->n = n; g->p = p; g->ld = geno_ld(n); g->bps = cdiv(n, 4);
+    g->host = m.p + 3; g->chunk_snps = chunk_snps;
+    g->map_base = (void*)m.p; g->map_len = m.len;
+    m.p = nullptr; m.len = 0;   // the panel owns the mapping now (the descriptor may close)
+    ctx->live_genos += 1;
+    *out = g;
+  });
+}
+
+int slfmm_geno_wrap_host(slfmm_ctx* ctx, const uint8_t* packed, uint64_t n, uint64_t n_snps, uint64_t chunk_snps,
+                         slfmm_geno** out) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && out && (packed || n_snps == 0) && n >= 1, SLFMM_E_ARG, "NULL argument");
+    auto* g = new slfmm_geno();
+    g->ctx = ctx; g->n = n; g->p = n_snps; g->ld = geno_ld(n); g->bps = cdiv(n, 4);
+    g->host = packed; g->chunk_snps = chunk_snps;
+    ctx->live_genos += 1;
+    *out = g;
+  });
+}
+
 void slfmm_geno_free(slfmm_geno* g) {
   if (!g) return;
   slfmm_ctx* c = g->ctx;
+  if (g->host && c) {   // queued copies may still read the rows
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+  }
+  if (g->map_base) munmap(g->map_base, g->map_len);
   if (g->owned && g->d) {
     if (c) {
       (void)hipSetDevice(c->device);
@@ -356,10 +489,15 @@ int slfmm_decode_stats(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* snp_i
     for (uint64_t i = 0; i < n_idx; ++i)
       SLFMM_CHECK(snp_idx[i] >= 0 && (uint64_t)snp_idx[i] < g->p, SLFMM_E_ARG, "SNP index out of range");
     hipStream_t s = ctx->stream;
-    int64_t* didx = to_dev(ctx, "api_idx", snp_idx, n_idx);
     uint8_t* Y = ctx->buf<uint8_t>("api_Y", n_idx * g->ld + kSlack);
-    SLFMM_HIP(hipMemsetAsync(Y + n_idx * g->ld, 0, kSlack, s));
-    geno_gather(ctx, g, didx, n_idx, Y);
+    if (g->host) {   // rows straight from host memory
+      SLFMM_HIP(hipMemsetAsync(Y, 0, n_idx * g->ld + kSlack, s));
+      host_rows_to_device(ctx, g, snp_idx, n_idx, Y);
+    } else {
+      int64_t* didx = to_dev(ctx, "api_idx", snp_idx, n_idx);
+      SLFMM_HIP(hipMemsetAsync(Y + n_idx * g->ld, 0, kSlack, s));
+      geno_gather(ctx, g, didx, n_idx, Y);
+    }
     int32_t* a = ctx->buf<int32_t>("api_s0", n_idx);
     int32_t* b = ctx->buf<int32_t>("api_s1", n_idx);
     int32_t* c = ctx->buf<int32_t>("api_s2", n_idx);
@@ -378,10 +516,22 @@ static int decode_common(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* snp
     for (uint64_t i = 0; i < c; ++i)
       SLFMM_CHECK(snp_idx[i] >= 0 && (uint64_t)snp_idx[i] < g->p, SLFMM_E_ARG, "SNP index out of range");
     hipStream_t s = ctx->stream;
-    int64_t* didx = to_dev(ctx, "api_idx", snp_idx, c);
+    const uint8_t* src = g->d;
+    std::vector<int64_t> iota;
+    const int64_t* hidx = snp_idx;
+    if (g->host) {   // the requested rows from host memory, then decode them in order
+      uint8_t* Y = ctx->buf<uint8_t>("api_hostrows", c * g->ld + kSlack);
+      SLFMM_HIP(hipMemsetAsync(Y, 0, c * g->ld + kSlack, s));
+      host_rows_to_device(ctx, g, snp_idx, c, Y);
+      iota.resize(c);
+      for (uint64_t i = 0; i < c; ++i) iota[i] = (int64_t)i;
+      hidx = iota.data();
+      src = Y;
+    }
+    int64_t* didx = to_dev(ctx, "api_idx", hidx, c);
     const double* dfill = fill ? to_dev(ctx, "api_fill", fill, g->n * c) : nullptr;
     double* dout = ctx->buf<double>("api_out", g->n * c);
-    launch_decode_f64(s, g->d, g->ld, g->n, didx, c, norm, dfill, dout, raw);
+    launch_decode_f64(s, src, g->ld, g->n, didx, c, norm, dfill, dout, raw);
     to_host(ctx, out, dout, g->n * c);
     SLFMM_HIP(hipStreamSynchronize(s));
   });
@@ -625,6 +775,7 @@ int slfmm_fit_lfmm2_nmf(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* su
                         slfmm_assoc_out* out, double* nmf_cv_mae, double* gwas_cv, uint32_t flags) {
   return guarded(ctx, [&] {
     SLFMM_CHECK(ctx && est && g && x && out && cfg && nmf, SLFMM_E_ARG, "NULL argument");
+    require_device(g);
     SLFMM_CHECK(est->n == g->n, SLFMM_E_ARG, "estimation and test panels must have the same samples");
     slfmm_config c = *cfg;
     c.impute = SLFMM_IMPUTE_MEAN;   // the fills are passed explicitly below

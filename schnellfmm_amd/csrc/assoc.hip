@@ -657,52 +657,118 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
   ea.n_miss = nmiss; ea.sum_g = sumg; ea.n_hom = nhom;
   ea.beta = d_beta; ea.tstat = d_t; ea.r2 = d_r2;
   const bool with_nmf = nmf && nmf->W;
-  if (cfg.precision == SLFMM_PREC_I8) {
-    SLFMM_CHECK(!with_nmf, SLFMM_E_ARG, "NMF imputation runs on the FP64 path (use SLFMM_PREC_FP64)");
-    launch_func_i8(ctx, g, pl.Q, m, ea, true);
+  const bool i8 = cfg.precision == SLFMM_PREC_I8;
+  SLFMM_CHECK(!(i8 && with_nmf), SLFMM_E_ARG, "NMF imputation runs on the FP64 path (use SLFMM_PREC_FP64)");
+  SLFMM_CHECK(!(with_nmf && g->host), SLFMM_E_ARG, "NMF imputation needs a device-resident genotype panel");
+  // FP64 operand: the basis padded to whole row blocks (shared by every row range)
+  const uint64_t npad = rup(n, 64);
+  const int nrb = (int)cdiv(m, kRowBlk);
+  const uint64_t lda = (uint64_t)nrb * kRowBlk;
+  double* A = nullptr;
+  double* mu = nullptr;
+  double* sc = nullptr;
+  if (!i8) {
+    A = ctx->buf<double>("as_A64", npad * lda);
+    k_pad_operand<<<(unsigned)cdiv(npad * lda, 256), 256, 0, s>>>(pl.Q, m, n, (int)m, A, lda, npad);
+    mu = ctx->buf<double>("as_mu", p);
+    sc = ctx->buf<double>("as_sc", p);
+  }
+  const size_t shm = plan_len((int)d, (int)m) * sizeof(double);
+  auto epilogue = [&](const EpiArgs& e, const double* F) {
+    if (m <= 32) k_epilogue<32><<<(unsigned)cdiv(e.p, 128), 128, shm, s>>>(e, F, lda);
+    else k_epilogue<64><<<(unsigned)cdiv(e.p, 128), 128, shm, s>>>(e, F, lda);
+    SLFMM_HIP(hipGetLastError());
+  };
+  // Association of the SNP rows [j0, j0 + v.p) held in v (per-SNP outputs offset by j0).
+  auto rows_pass = [&](const slfmm_geno& v, uint64_t j0, double* F) {
+    if (v.p == 0) return;
+    EpiArgs e = ea;
+    e.p = v.p;
+    e.n_miss = nmiss + j0; e.sum_g = sumg + j0; e.n_hom = nhom + j0;
+    e.beta = d_beta + j0 * d; e.tstat = d_t + j0 * d; e.r2 = d_r2 + j0 * 3;
+    if (i8) {
+      launch_func_i8(ctx, &v, pl.Q, m, e, true);
+      return;
+    }
+    launch_stats(s, v.d, v.ld, n, v.p, nmiss + j0, sumg + j0, nhom + j0);
+    launch_mean_scale(s, nmiss + j0, sumg + j0, n, v.p, cfg.norm, mu + j0, sc + j0);
+    {
+      slfmm_timed tm(ctx, "assoc_stream");
+      launch_func_f64(s, v.d, v.ld, v.p, npad, A, lda, nrb, mu + j0, F, lda, nullptr, nullptr);
+    }
+    epilogue(e, F);
+  };
+  if (g->host) {
+    // Host-resident panel: chunks of rows cross PCIe through two pinned
+    // staging buffers into two device buffers on the copy stream while the
+    // compute stream works on the previous chunk.  Chunk c uses buffer c & 1:
+    //   copy stream:    [wait consumed(b)] copy(c) -> copied(b)
+    //   compute stream: [wait copied(b)] clear tails, association -> consumed(b)
+    //   host:           before refilling staging buffer b, wait copied(b).
+    ctx->ensure_copy_stream();
+    const uint64_t cap = std::max<uint64_t>(1, std::min(host_chunk_rows(g), p));
+    const uint64_t cbytes = cap * g->ld + kSlack;
+    uint8_t* cb[2] = {ctx->buf<uint8_t>("as_chunk0", cbytes), ctx->buf<uint8_t>("as_chunk1", cbytes)};
+    double* F = i8 ? nullptr : ctx->buf<double>("as_F", cap * lda);
+    for (int b = 0; b < 2; ++b) {
+      SLFMM_HIP(hipMemsetAsync(cb[b], 0, cbytes, s));   // row bytes past ceil(n/4) stay zero
+      SLFMM_HIP(hipEventRecord(ctx->ev_consumed[b], s));
+    }
+    const uint64_t nchunks = cdiv(p, cap);
+    auto upload = [&](uint64_t c) {
+      const int b = (int)(c & 1);
+      const uint64_t j0 = c * cap, j1 = std::min(p, j0 + cap);
+      SLFMM_HIP(hipEventSynchronize(ctx->ev_copied[b]));   // staging buffer b is free
+      SLFMM_HIP(hipStreamWaitEvent(ctx->copy_stream, ctx->ev_consumed[b], 0));
+      host_chunk_upload(ctx, g, j0, j1, b, cb[b]);
+    };
+    if (nchunks) upload(0);
+    for (uint64_t c = 0; c < nchunks; ++c) {
+      if (c + 1 < nchunks) upload(c + 1);
+      const int b = (int)(c & 1);
+      const uint64_t j0 = c * cap, pc = std::min(p, j0 + cap) - j0;
+      SLFMM_HIP(hipStreamWaitEvent(s, ctx->ev_copied[b], 0));
+      geno_clear_tail(s, cb[b], g->ld, pc, n);
+      if (pc < cap) SLFMM_HIP(hipMemsetAsync(cb[b] + pc * g->ld, 0, kSlack, s));
+      slfmm_geno v = *g;
+      v.host = nullptr; v.map_base = nullptr; v.d = cb[b]; v.p = pc; v.owned = false;
+      rows_pass(v, j0, F);
+      SLFMM_HIP(hipEventRecord(ctx->ev_consumed[b], s));
+    }
+    for (int b = 0; b < 2; ++b) SLFMM_HIP(hipEventSynchronize(ctx->ev_copied[b]));
+  } else if (!with_nmf) {
+    rows_pass(*g, 0, i8 ? nullptr : ctx->buf<double>("as_F", p * lda));
   } else {
     launch_stats(s, g->d, g->ld, n, p, nmiss, sumg, nhom);
-    double* mu = ctx->buf<double>("as_mu", p);
-    double* sc = ctx->buf<double>("as_sc", p);
     launch_mean_scale(s, nmiss, sumg, n, p, cfg.norm, mu, sc);
+    // NMF on-the-fly imputation (src/parallel.rs:252-282): H_j = max(0, W^+ y_j)
+    // of the mean-filled raw column, missing calls filled with W_k . H_j
     FillSpec fs;
-    if (with_nmf) {
-      // NMF on-the-fly imputation (src/parallel.rs:252-282): H_j = max(0, W^+ y_j)
-      // of the mean-filled raw column, missing calls filled with W_k . H_j
-      const int K = nmf->K;
-      const uint64_t ldh = rup(K, kRowBlk);
-      double* Hraw = ctx->buf<double>("as_nmf_hraw", p * ldh);
-      double* Ht = ctx->buf<double>("as_nmf_ht", p * K);
-      raw_functionals(ctx, g->d, g->ld, n, p, mu, nmf->Wp, K, K, Hraw, ldh);
-      relu_rows(s, Hraw, ldh, p, K, Ht);
-      fs.W = nmf->W; fs.Ht = Ht; fs.K = K; fs.n = n;
-      double* fsc = ctx->buf<double>("as_nmf_sc", p);
-      double* ftss = ctx->buf<double>("as_nmf_tss", p);
-      double* fmu = ctx->buf<double>("as_nmf_mu", p);
-      const bool cv = nmf->cv_rate > 0.0 && nmf->cv_out;
-      double* SWp = cv ? ctx->buf<double>("as_nmf_swp", p * K) : nullptr;
-      fill_moments(ctx, g->d, g->ld, p, fs, cfg.norm, cv ? nmf->Wp : nullptr, fmu, fsc, ftss, SWp);
-      if (cv)
-        gwas_cv(ctx, g->d, g->ld, n, p, nmf->W, nmf->Wp, K, Hraw, ldh, SWp, nmiss, sumg, nmf->cv_seed, nmf->chunk,
-                nmf->cv_rate, nmf->cv_out);
-      mu = fmu;
-      ea.fill_sc = fsc;
-      ea.fill_tss = ftss;
-    }
-    const uint64_t npad = rup(n, 64);
-    const int nrb = (int)cdiv(m, kRowBlk);
-    const uint64_t lda = (uint64_t)nrb * kRowBlk;
-    double* A = ctx->buf<double>("as_A64", npad * lda);
-    k_pad_operand<<<(unsigned)cdiv(npad * lda, 256), 256, 0, s>>>(pl.Q, m, n, (int)m, A, lda, npad);
+    const int K = nmf->K;
+    const uint64_t ldh = rup(K, kRowBlk);
+    double* Hraw = ctx->buf<double>("as_nmf_hraw", p * ldh);
+    double* Ht = ctx->buf<double>("as_nmf_ht", p * K);
+    raw_functionals(ctx, g->d, g->ld, n, p, mu, nmf->Wp, K, K, Hraw, ldh);
+    relu_rows(s, Hraw, ldh, p, K, Ht);
+    fs.W = nmf->W; fs.Ht = Ht; fs.K = K; fs.n = n;
+    double* fsc = ctx->buf<double>("as_nmf_sc", p);
+    double* ftss = ctx->buf<double>("as_nmf_tss", p);
+    double* fmu = ctx->buf<double>("as_nmf_mu", p);
+    const bool cv = nmf->cv_rate > 0.0 && nmf->cv_out;
+    double* SWp = cv ? ctx->buf<double>("as_nmf_swp", p * K) : nullptr;
+    fill_moments(ctx, g->d, g->ld, p, fs, cfg.norm, cv ? nmf->Wp : nullptr, fmu, fsc, ftss, SWp);
+    if (cv)
+      gwas_cv(ctx, g->d, g->ld, n, p, nmf->W, nmf->Wp, K, Hraw, ldh, SWp, nmiss, sumg, nmf->cv_seed, nmf->chunk,
+              nmf->cv_rate, nmf->cv_out);
+    EpiArgs e = ea;
+    e.fill_sc = fsc;
+    e.fill_tss = ftss;
     double* F = ctx->buf<double>("as_F", p * lda);
     {
       slfmm_timed tm(ctx, "assoc_stream");
-      launch_func_f64(s, g->d, g->ld, p, npad, A, lda, nrb, mu, F, lda, nullptr, with_nmf ? &fs : nullptr);
+      launch_func_f64(s, g->d, g->ld, p, npad, A, lda, nrb, fmu, F, lda, nullptr, &fs);
     }
-    const size_t shm = plan_len((int)d, (int)m) * sizeof(double);
-    if (m <= 32) k_epilogue<32><<<(unsigned)cdiv(p, 128), 128, shm, s>>>(ea, F, lda);
-    else k_epilogue<64><<<(unsigned)cdiv(p, 128), 128, shm, s>>>(ea, F, lda);
-    SLFMM_HIP(hipGetLastError());
+    epilogue(e, F);
   }
   // histogram, GIF, calibration
   unsigned long long* hist = d_hist ? (unsigned long long*)d_hist

@@ -60,6 +60,12 @@ __global__ void k_clear_tail(uint8_t* d, uint64_t ld, uint64_t p, uint64_t n) {
   d[j * ld + (n >> 2)] &= mask;
 }
 
+void geno_clear_tail(hipStream_t s, uint8_t* d, uint64_t ld, uint64_t p, uint64_t n) {
+  if (p == 0 || (n & 3) == 0) return;
+  k_clear_tail<<<(unsigned)cdiv(p, 256), 256, 0, s>>>(d, ld, p, n);
+  SLFMM_HIP(hipGetLastError());
+}
+
 static dim3 grid_rows(uint64_t x_blocks, uint64_t p) {
   uint64_t y = p < 65535 ? p : 65535;
   uint64_t z = cdiv(p, 65535);
@@ -81,7 +87,7 @@ void geno_upload(slfmm_ctx* ctx, const uint8_t* packed, uint64_t n_phys, uint64_
   hipStream_t s = ctx->stream;
   if (!keep) {
     SLFMM_HIP(hipMemcpy2DAsync(g->d, g->ld, packed, bps, bps, n_snps, hipMemcpyHostToDevice, s));
-    k_clear_tail<<<(unsigned)cdiv(n_snps, 256), 256, 0, s>>>(g->d, g->ld, n_snps, g->n);
+    geno_clear_tail(s, g->d, g->ld, n_snps, g->n);
   } else {
     // Stream the raw rows through a bounded staging buffer, repacking kept samples.
     const uint64_t rows_per = std::max<uint64_t>(1, (256ull << 20) / std::max<uint64_t>(bps, 1));

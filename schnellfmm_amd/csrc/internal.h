@@ -87,6 +87,14 @@ struct slfmm_ctx {
   // Error(code, err) unless code is SLFMM_OK.
   int* check_slot(const std::string& warn, int code, const std::string& err);
   std::string check_prefix;   // prepended to the error text of new slots (batched fits)
+  // Streaming of host-resident panels: a copy stream (default priority),
+  // two pinned staging buffers and their events, created on first use.
+  hipStream_t copy_stream = nullptr;
+  void* pinned[2] = {nullptr, nullptr};
+  size_t pinned_bytes[2] = {0, 0};
+  hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_consumed[2] = {nullptr, nullptr};
+  void* pinned_buf(int i, size_t bytes);   // grow-only, hipHostMalloc
+  void ensure_copy_stream();
   // Contexts of the extra streams of batched fits (same device, own scratch).
   std::vector<slfmm_ctx*> workers;
   // Reads the slots (synchronises the stream), prints warnings in order and
@@ -109,12 +117,34 @@ struct slfmm_geno {
   uint64_t p = 0;         // SNPs
   uint64_t ld = 0;        // row stride in bytes (multiple of 16)
   bool owned = false;
+  // Host-resident panel (streamed to the device chunk by chunk): rows of
+  // bps = ceil(n/4) bytes at `host` (a mapped .bed or caller memory); d is null.
+  const uint8_t* host = nullptr;
+  uint64_t bps = 0;
+  uint64_t chunk_snps = 0;      // rows per streamed chunk (0: automatic)
+  void* map_base = nullptr;     // owned mapping of the .bed file
+  size_t map_len = 0;
 };
 
 namespace slfmm {
 
+// ---- host-resident panels (api.cpp) ---------------------------------------
+// Copies rows rows[0..m) (or 0..m when rows is null) of a host panel to dst
+// (device, row stride g->ld, padding bytes zero) through the pinned buffers,
+// on ctx->stream; returns after the copies are queued and the pinned buffers
+// are free again.
+void host_rows_to_device(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* rows, uint64_t m, uint8_t* dst);
+// Rows per streamed chunk of a host panel.
+uint64_t host_chunk_rows(const slfmm_geno* g);
+// Fills pinned staging buffer b with rows [j0, j1) of host panel g (several
+// host threads) and queues its copy to dst (row stride g->ld) on the copy
+// stream; records ev_copied[b].
+void host_chunk_upload(slfmm_ctx* ctx, const slfmm_geno* g, uint64_t j0, uint64_t j1, int b, uint8_t* dst);
+
 // ---- geno.hip -------------------------------------------------------------
 uint64_t geno_ld(uint64_t n);
+// Zeroes the codes of the padding samples in the last byte of rows [0, p).
+void geno_clear_tail(hipStream_t s, uint8_t* d, uint64_t ld, uint64_t p, uint64_t n);
 void geno_alloc(slfmm_ctx* ctx, slfmm_geno* g);
 void geno_upload(slfmm_ctx* ctx, const uint8_t* packed, uint64_t n_phys, uint64_t n_snps,
                  const int64_t* keep, uint64_t n_keep, slfmm_geno* g);

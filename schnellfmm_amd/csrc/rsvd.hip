@@ -220,12 +220,21 @@ const uint8_t* gather_subset(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t*
     identity = (n_idx == g->p);
     for (uint64_t i = 0; identity && i < n_idx; ++i) identity = (h_idx[i] == (int64_t)i);
   }
-  if (identity) {
+  if (identity && !g->host) {
     *pe = g->p;
     return g->d;
   }
-  for (uint64_t i = 0; i < n_idx; ++i)
-    SLFMM_CHECK(h_idx[i] >= 0 && (uint64_t)h_idx[i] < g->p, SLFMM_E_ARG, "estimation SNP index out of range");
+  if (!identity)
+    for (uint64_t i = 0; i < n_idx; ++i)
+      SLFMM_CHECK(h_idx[i] >= 0 && (uint64_t)h_idx[i] < g->p, SLFMM_E_ARG, "estimation SNP index out of range");
+  if (g->host) {   // rows of a host-resident panel: copied through pinned staging
+    const uint64_t m = identity ? g->p : n_idx;
+    uint8_t* Yg = ctx->buf<uint8_t>(tag, m * g->ld + kSlack);
+    SLFMM_HIP(hipMemsetAsync(Yg, 0, m * g->ld + kSlack, s));
+    host_rows_to_device(ctx, g, identity ? nullptr : h_idx, m, Yg);
+    *pe = m;
+    return Yg;
+  }
   int64_t* didx = ctx->buf<int64_t>(std::string(tag) + "_idx", n_idx);
   SLFMM_HIP(hipMemcpyAsync(didx, h_idx, n_idx * 8, hipMemcpyHostToDevice, s));
   uint8_t* Yg = ctx->buf<uint8_t>(tag, n_idx * g->ld + kSlack);
