@@ -50,10 +50,15 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-snps", type=int, default=40000)
-    ap.add_argument("--workload", choices=["lfmm2", "nmf", "batch"], default="lfmm2",
+    ap.add_argument("--workload", choices=["lfmm2", "nmf", "batch", "host"], default="lfmm2",
                     help="lfmm2: the headline fit (default); nmf: NMF imputation estimation and the "
                          "NMF-imputed fit at n=5000, p=2M, nmf-k=8, 10 iterations, K=8, d=2 (FP64); "
-                         "batch: 64 independent fits of seeded uneven shapes, batched vs looped")
+                         "batch: 64 independent fits of seeded uneven shapes, batched vs looped; "
+                         "host: the headline fit with the genotypes in host memory, streamed over "
+                         "PCIe chunk by chunk (PCIe-inclusive rate)")
+    ap.add_argument("--host-memory", choices=["pageable", "pinned"], default="pageable",
+                    help="host buffer of the genotypes (--workload host)")
+    ap.add_argument("--chunk-snps", type=int, default=0, help="rows per streamed chunk (--workload host)")
     ap.add_argument("--streams", type=int, default=4, help="streams of the batched fit (--workload batch)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB on the device three times (reference kernel for the "
@@ -83,6 +88,9 @@ def main():
         return
     if args.workload == "batch":
         batch_workload(args)
+        return
+    if args.workload == "host":
+        host_workload(args)
         return
 
     import torch
@@ -356,6 +364,78 @@ def batch_workload(args):
         "speedup_vs_loop": dt_loop / dt, "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
+
+
+def host_workload(args):
+    """The headline fit (same panel, covariates and configuration) with the
+    genotypes in host memory: every step streams the 1 % estimation rows and
+    then all p rows over PCIe in double-buffered chunks, the copies
+    overlapping the association of the previous chunk."""
+    import torch
+    import schnellfmm_amd as S
+    from schnellfmm_amd._lib import AssocOut, lib
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    n, p, K, d = args.n, args.p, args.k, args.d
+    ctx = S.Context(0)
+    t0 = time.time()
+    gdev = S.BedFile.synthetic(n, p, n_pops=20, fst=(0.02, 0.15), maf=(0.05, 0.5), miss_rate=0.02,
+                               seed=3, ctx=ctx)
+    bps = (n + 3) // 4
+    if args.host_memory == "pinned":
+        hbuf_t = torch.empty(p * bps, dtype=torch.uint8, pin_memory=True)
+        hbuf = hbuf_t.numpy()
+    else:
+        hbuf_t = None
+        hbuf = np.empty(p * bps, np.uint8)
+    ctx.check(lib().slfmm_geno_read_packed(ctx.h, gdev.h, 0, p, hbuf.ctypes.data_as(C.c_void_p)))
+    gdev.free()
+    geno = S.BedFile.wrap_host(hbuf, n, p, chunk_snps=args.chunk_snps, ctx=ctx)
+    log(f"[bench] host panel ({args.host_memory}, {p * bps / 1e9:.1f} GB) ready in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(3)
+    cov = np.full((d, d), 0.3) + 0.7 * np.eye(d)
+    x = np.ascontiguousarray(rng.multivariate_normal(np.zeros(d), cov, size=n))
+    est_idx = S.subset_indices(args.est_rate, p)
+    t = {k: torch.empty(shape, dtype=torch.float64, device=dev) for k, shape in
+         (("beta", (p, d)), ("t", (p, d)), ("r2", (p, 3)), ("p", (p, d)), ("gpt", (d,)), ("gif", (1,)),
+          ("u", (n, K)))}
+    o = AssocOut()
+    o.beta, o.tstat, o.r2, o.pvalue = (t[k].data_ptr() for k in ("beta", "t", "r2", "p"))
+    o.hist, o.gif_per_trait, o.gif = None, t["gpt"].data_ptr(), t["gif"].data_ptr()
+    cfg = S.Lfmm2Config(k=K, oversampling=10, n_power_iter=args.power_iter, norm=S.EIGENSTRAT,
+                        precision=S.PREC_I8, seed=42).to_c()
+
+    def run():
+        ctx.check(lib().slfmm_fit_lfmm2(ctx.h, geno.h, est_idx.ctypes.data_as(C.c_void_p), len(est_idx), geno.h,
+                                        x.ctypes.data_as(C.c_void_p), d, C.byref(cfg), None,
+                                        C.c_void_p(t["u"].data_ptr()), C.byref(o), 1))
+
+    for i in range(args.warmup):
+        run()
+        log(f"[bench] host warmup {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t1) / args.steps
+    log(f"[bench] host panel: {dt * 1e3:.1f} ms/step")
+    line = {
+        "metric": "SNPs/sec end-to-end (n=10k, K=20), genotypes in host memory (PCIe-inclusive)",
+        "value": p / dt, "unit": "SNPs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "i8-mfma/i32-acc+f64", "data": "synthetic (Balding-Nichols, generated on the device, "
+                                                 "copied to host memory before timing)",
+        "config": {"workload": "lfmm2_fit n=10000 p=10000000 K=20 d=4 est_rate=0.01 q=3 eigenstrat, "
+                               "host-resident genotypes", "n": n, "p": p, "K": K, "d": d,
+                   "host_memory": args.host_memory, "chunk_snps": args.chunk_snps,
+                   "genotype_bytes": p * bps, "pcie_gbs": p * bps / dt / 1e9},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    geno.free()
+    del hbuf_t
 
 
 def measured_traffic(n, p, K, d):

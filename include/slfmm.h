@@ -137,10 +137,17 @@ int slfmm_geno_wrap_device(slfmm_ctx* ctx, void* d_packed, uint64_t n, uint64_t 
  * open_bed_host maps <bed_path> read-only (same validation as
  * slfmm_bed_open); the panel owns the mapping.  wrap_host borrows `packed`
  * (n_snps rows of ceil(n/4) bytes, the .bed payload after the magic bytes),
- * which must stay valid and unchanged until slfmm_geno_free. */
+ * which must stay valid and unchanged until slfmm_geno_free; page-locked
+ * memory (hipHostMalloc) is read by DMA directly, other memory is staged. */
 int slfmm_geno_open_bed_host(slfmm_ctx* ctx, const char* bed_path, uint64_t chunk_snps, slfmm_geno** out);
 int slfmm_geno_wrap_host(slfmm_ctx* ctx, const uint8_t* packed, uint64_t n, uint64_t n_snps,
                          uint64_t chunk_snps, slfmm_geno** out);
+
+/* Copies rows [j0, j0 + rows) of a panel to host memory `out` in .bed payload
+ * layout (ceil(n/4) bytes per SNP, padding codes zero): the payload that
+ * write_bed_file (src/bed.rs:398-429) writes after the magic bytes, e.g. to
+ * save a panel generated on the device. */
+int slfmm_geno_read_packed(slfmm_ctx* ctx, const slfmm_geno* g, uint64_t j0, uint64_t rows, uint8_t* out);
 
 void slfmm_geno_free(slfmm_geno* g);
 int slfmm_geno_info(const slfmm_geno* g, uint64_t* n_samples, uint64_t* n_snps, uint64_t* ld);

@@ -54,6 +54,7 @@ SIGNATURES = {
     "slfmm_geno_wrap_device": (C.c_int, [vp, vp, u64, u64, u64, vp]),
     "slfmm_geno_open_bed_host": (C.c_int, [vp, C.c_char_p, u64, vp]),
     "slfmm_geno_wrap_host": (C.c_int, [vp, vp, u64, u64, u64, vp]),
+    "slfmm_geno_read_packed": (C.c_int, [vp, vp, u64, u64, vp]),
     "slfmm_geno_free": (None, [vp]),
     "slfmm_geno_info": (C.c_int, [vp, vp, vp, vp]),
     "slfmm_decode_stats": (C.c_int, [vp, vp, vp, u64, vp, vp, vp]),

@@ -183,6 +183,13 @@ class BedFile:
                                        miss_rate, seed, C.byref(h)))
         return cls(h, n, p, ctx)
 
+    def read_packed(self, j0: int = 0, rows: Optional[int] = None) -> np.ndarray:
+        """Rows [j0, j0 + rows) as .bed payload bytes (rows x ceil(n/4), uint8)."""
+        rows = self.n_snps - j0 if rows is None else rows
+        out = np.empty((rows, (self.n_samples + 3) // 4), np.uint8)
+        self.ctx.check(lib().slfmm_geno_read_packed(self.ctx.h, self.h, j0, rows, _ptr(out)))
+        return out
+
     def subset_snp_count(self, spec) -> int:
         return len(subset_indices(spec, self.n_snps))
 

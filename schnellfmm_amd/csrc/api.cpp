@@ -290,8 +290,12 @@ uint64_t host_chunk_rows(const slfmm_geno* g) {
 
 void host_chunk_upload(slfmm_ctx* ctx, const slfmm_geno* g, uint64_t j0, uint64_t j1, int b, uint8_t* dst) {
   const uint64_t rows = j1 - j0, bytes = rows * g->bps;
-  auto* pin = static_cast<uint8_t*>(ctx->pinned_buf(b, std::max<uint64_t>(bytes, 16)));
-  parallel_memcpy(pin, g->host + j0 * g->bps, bytes);   // the rows of a .bed are contiguous
+  const uint8_t* pin = g->host + j0 * g->bps;   // the rows of a .bed are contiguous
+  if (!g->host_pinned) {   // pageable rows: through the staging buffer (several host threads)
+    auto* stage = static_cast<uint8_t*>(ctx->pinned_buf(b, std::max<uint64_t>(bytes, 16)));
+    parallel_memcpy(stage, pin, bytes);
+    pin = stage;
+  }
   if (rows)
     SLFMM_HIP(hipMemcpy2DAsync(dst, g->ld, pin, g->bps, g->bps, rows, hipMemcpyHostToDevice, ctx->copy_stream));
   SLFMM_HIP(hipEventRecord(ctx->ev_copied[b], ctx->copy_stream));
@@ -317,6 +321,16 @@ void host_rows_to_device(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* row
 }  // namespace slfmm
 
 namespace {
+
+// Page-locked host memory (hipHostMalloc, torch pin_memory): DMA reads it directly.
+bool is_pinned(const void* ptr) {
+  hipPointerAttribute_t at{};
+  if (hipPointerGetAttributes(&at, ptr) != hipSuccess) {
+    (void)hipGetLastError();   // unregistered memory may report an error: clear it
+    return false;
+  }
+  return at.type == hipMemoryTypeHost;
+}
 
 void require_device(const slfmm_geno* g) {
   SLFMM_CHECK(!g || !g->host, SLFMM_E_ARG,
@@ -449,8 +463,28 @@ int slfmm_geno_wrap_host(slfmm_ctx* ctx, const uint8_t* packed, uint64_t n, uint
     auto* g = new slfmm_geno();
     g->ctx = ctx; g->n = n; g->p = n_snps; g->ld = geno_ld(n); g->bps = cdiv(n, 4);
     g->host = packed; g->chunk_snps = chunk_snps;
+    g->host_pinned = n_snps > 0 && is_pinned(packed) && is_pinned(packed + n_snps * g->bps - 1);
     ctx->live_genos += 1;
     *out = g;
+  });
+}
+
+int slfmm_geno_read_packed(slfmm_ctx* ctx, const slfmm_geno* g, uint64_t j0, uint64_t rows, uint8_t* out) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && g && (out || rows == 0), SLFMM_E_ARG, "NULL argument");
+    SLFMM_CHECK(j0 <= g->p && rows <= g->p - j0, SLFMM_E_ARG, "SNP row range out of range");
+    const uint64_t bps = cdiv(g->n, 4);
+    if (rows == 0) return;
+    if (g->host) {
+      parallel_memcpy(out, g->host + j0 * bps, rows * bps);
+      if (g->n & 3) {   // padding codes of the last byte read as zero, as on the device
+        const uint8_t keep = (uint8_t)((1u << (2 * (g->n & 3))) - 1u);
+        for (uint64_t r = 0; r < rows; ++r) out[r * bps + bps - 1] &= keep;
+      }
+      return;
+    }
+    SLFMM_HIP(hipMemcpy2DAsync(out, bps, g->d + j0 * g->ld, g->ld, bps, rows, hipMemcpyDeviceToHost, ctx->stream));
+    SLFMM_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
 

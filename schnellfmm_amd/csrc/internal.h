@@ -121,6 +121,7 @@ struct slfmm_geno {
   // bps = ceil(n/4) bytes at `host` (a mapped .bed or caller memory); d is null.
   const uint8_t* host = nullptr;
   uint64_t bps = 0;
+  bool host_pinned = false;     // host rows are page-locked: copied without staging
   uint64_t chunk_snps = 0;      // rows per streamed chunk (0: automatic)
   void* map_base = nullptr;     // owned mapping of the .bed file
   size_t map_len = 0;

@@ -107,3 +107,39 @@ def test_host_panel_batch(S):
     probs = [S.Lfmm2Problem(host, 0.5, host, pan.x, cfg), S.Lfmm2Problem(dev, 0.5, dev, pan.x, cfg)]
     got = S.fit_lfmm2_batch(probs, n_streams=2)
     _same(got[0], got[1])
+
+
+def test_read_packed_round_trip(S):
+    import oracle as O
+    n, p = 517, 3001
+    pan, packed = _panel(n, p, seed=51)
+    bps = (n + 3) // 4
+    clean = np.frombuffer(O.encode_bed(pan.g), np.uint8).reshape(p, bps)
+    dev = S.BedFile.from_packed(packed.tobytes(), n, p)
+    host = S.BedFile.wrap_host(packed, n, p)
+    assert np.array_equal(dev.read_packed(), clean)
+    assert np.array_equal(host.read_packed(), clean)
+    assert np.array_equal(dev.read_packed(1000, 17), clean[1000:1017])
+    assert dev.read_packed(p, 0).shape == (0, bps)
+    with pytest.raises(S.api.SlfmmError, match="out of range"):
+        dev.read_packed(p - 5, 6)
+
+
+def test_pinned_host_panel(S):
+    # page-locked rows (hipHostMalloc from the HIP runtime the library loaded)
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7", mode=C.RTLD_GLOBAL)
+    n, p = 333, 5000
+    pan, packed = _panel(n, p, seed=61)
+    ptr = C.c_void_p()
+    assert hip.hipHostMalloc(C.byref(ptr), C.c_size_t(packed.size), C.c_uint(0)) == 0
+    try:
+        pinned = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_uint8)), shape=(packed.size,))
+        pinned[:] = packed
+        dev = S.BedFile.from_packed(packed.tobytes(), n, p)
+        host = S.BedFile.wrap_host(pinned, n, p, chunk_snps=1200)
+        cfg = S.Lfmm2Config(k=5, norm=S.EIGENSTRAT, n_power_iter=2, seed=8, precision=S.PREC_I8)
+        _same(S.fit_lfmm2(host, 0.5, host, pan.x, cfg), S.fit_lfmm2(dev, 0.5, dev, pan.x, cfg))
+        host.free()
+    finally:
+        hip.hipHostFree(ptr)
