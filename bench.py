@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--host-memory", choices=["pageable", "pinned"], default="pageable",
                     help="host buffer of the genotypes (--workload host)")
     ap.add_argument("--chunk-snps", type=int, default=0, help="rows per streamed chunk (--workload host)")
+    ap.add_argument("--write-tsv", default="",
+                    help="directory: after timing, write the results TSV of the main fit there (synthetic .bim "
+                         "metadata), report the writer's time and delete the files")
     ap.add_argument("--streams", type=int, default=4, help="streams of the batched fit (--workload batch)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB on the device three times (reference kernel for the "
@@ -205,6 +208,7 @@ def main():
     cpu = None
     if args.cpu_baseline:
         cpu = cpu_baseline(geno, x, result_main["u"].cpu().numpy(), args, n, K, d)
+    tsv = write_tsv_measure(args.write_tsv, result_main, ctx, p, d) if args.write_tsv else None
 
     line = {
         "metric": "SNPs/sec end-to-end (n=10k, K=20): mixed-prec vs FP64 GPU @ max|Δlog10p|≤1e-3",
@@ -219,6 +223,7 @@ def main():
         "kernels_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kern.items()},
         "fp64_gpu": fp64,
         "cpu_baseline": cpu,
+        "output_tsv": tsv,
     }
     print(json.dumps(line), flush=True)
 
@@ -364,6 +369,32 @@ def batch_workload(args):
         "speedup_vs_loop": dt_loop / dt, "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
+
+
+def write_tsv_measure(dirpath, res, ctx, p, d):
+    """The output stage after the fit: slfmm_write_tsv straight from the HBM
+    outputs (p-values, betas, t, r2) with a synthetic .bim of p records."""
+    import shutil
+    import tempfile
+    from schnellfmm_amd._lib import lib
+    tmp = tempfile.mkdtemp(dir=dirpath)
+    try:
+        bim = os.path.join(tmp, "panel.bim")
+        with open(bim, "w") as f:
+            for j0 in range(0, p, 1_000_000):
+                f.write("".join(f"1\tsnp{j}\t0\t{j + 1}\tA\tG\n" for j in range(j0, min(p, j0 + 1_000_000))))
+        out = os.path.join(tmp, "results.tsv")
+        t0 = time.perf_counter()
+        ctx.check(lib().slfmm_write_tsv(ctx.h, out.encode(), bim.encode(), None, None, None, p, d, None,
+                                        res["p"].data_ptr(), res["beta"].data_ptr(), res["t"].data_ptr(),
+                                        res["r2"].data_ptr(), 1, 0))
+        dt = time.perf_counter() - t0
+        size = os.path.getsize(out)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    log(f"[bench] results TSV: {size / 1e9:.2f} GB in {dt:.2f}s")
+    return {"seconds": dt, "bytes": size, "snps_per_s": p / dt, "threads": min(16, os.cpu_count() or 1),
+            "includes": "D2H copy of the outputs, .bim parse, formatting, file write"}
 
 
 def host_workload(args):

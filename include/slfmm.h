@@ -280,6 +280,29 @@ int slfmm_synth_bn(slfmm_ctx* ctx, uint64_t n, uint64_t n_snps, uint32_t n_pops,
                    double fst_hi, double maf_lo, double maf_hi, double miss_rate, uint64_t seed,
                    slfmm_geno** out);
 
+/* ---- Output stage (src/testing.rs:740-801) ------------------------------ */
+
+/* Writes the calibrated results table of coalesce_output_tsv: header
+ * "chr pos snp_id {p_<c> beta_<c> t_<c>} r2_cov r2_latent r2_resid", tab
+ * separated, one row per SNP in SNP order, every number in Rust's `{:.6e}`
+ * form.  SNP metadata comes from the .bim at `bim_path` (validated as
+ * parse_bim, src/bed.rs:480-503; it must hold p records) or, when bim_path is
+ * NULL, from the arrays chrom / pos / snp_id.  cov_names may be NULL
+ * ("cov_0", "cov_1", ...).  pvalue, beta, tstat are p x d and r2 p x 3
+ * (row-major, as slfmm_fit_lfmm2 returns them); with SLFMM_OUT_DEVICE they are
+ * device pointers (ctx required).  ctx may be NULL for host arrays.
+ * n_threads <= 0: up to 16 formatting threads. */
+int slfmm_write_tsv(slfmm_ctx* ctx, const char* path, const char* bim_path, const char* const* chrom,
+                    const uint64_t* pos, const char* const* snp_id, uint64_t p, uint64_t d,
+                    const char* const* cov_names, const double* pvalue, const double* beta,
+                    const double* tstat, const double* r2, uint32_t flags, int n_threads);
+
+/* The number formatter of slfmm_write_tsv on its own: x[i] in Rust's `{:.6e}`
+ * form, NUL-terminated, at out + i * stride (stride >= 16).  No counterpart in
+ * the reference (Rust's core::fmt does this there); exported for callers that
+ * write their own tables. */
+int slfmm_format_e6(const double* x, uint64_t count, char* out, uint64_t stride);
+
 #ifdef __cplusplus
 }
 #endif

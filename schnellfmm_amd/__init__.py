@@ -8,3 +8,4 @@ from .api import (CENTER_ONLY, EIGENSTRAT, PREC_FP64, PREC_I8, BedFile, Context,
                   estimate_factors_streaming, fit_lfmm2, fit_lfmm2_batch, fit_lfmm2_nmf, nmf_impute_estimation,
                   precompute, safe_inv,
                   subset_indices, test_associations, test_associations_fused)
+from .output import BimRecords, parse_output_format, write_results  # noqa: F401

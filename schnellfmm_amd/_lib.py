@@ -55,6 +55,9 @@ SIGNATURES = {
     "slfmm_geno_open_bed_host": (C.c_int, [vp, C.c_char_p, u64, vp]),
     "slfmm_geno_wrap_host": (C.c_int, [vp, vp, u64, u64, u64, vp]),
     "slfmm_geno_read_packed": (C.c_int, [vp, vp, u64, u64, vp]),
+    "slfmm_write_tsv": (C.c_int, [vp, C.c_char_p, C.c_char_p, vp, vp, vp, u64, u64, vp, vp, vp, vp, vp,
+                                  C.c_uint32, C.c_int]),
+    "slfmm_format_e6": (C.c_int, [vp, u64, vp, u64]),
     "slfmm_geno_free": (None, [vp]),
     "slfmm_geno_info": (C.c_int, [vp, vp, vp, vp]),
     "slfmm_decode_stats": (C.c_int, [vp, vp, vp, u64, vp, vp, vp]),

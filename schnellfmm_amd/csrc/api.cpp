@@ -129,29 +129,6 @@ slfmm_ctx::~slfmm_ctx() {
 
 namespace {
 
-template <class F>
-int guarded(slfmm_ctx* ctx, F&& f) {
-  try {
-    if (ctx) {
-      SLFMM_HIP(hipSetDevice(ctx->device));
-      ctx->checks.clear();
-    }
-    f();
-    if (ctx) ctx->finish_checks();
-    if (ctx) ctx->err.clear();
-    return SLFMM_OK;
-  } catch (const Error& e) {
-    if (ctx) ctx->err = e.what();
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    if (ctx) ctx->err = "host allocation failed";
-    return SLFMM_E_NOMEM;
-  } catch (const std::exception& e) {
-    if (ctx) ctx->err = e.what();
-    return SLFMM_E_ARG;
-  }
-}
-
 // Device copy of a host array through a named scratch buffer.
 template <class T>
 T* to_dev(slfmm_ctx* ctx, const char* name, const T* h, size_t count) {
@@ -261,6 +238,13 @@ void center_host(const double* x, uint64_t n, uint64_t d, int scale, double* out
 }
 
 }  // namespace
+
+namespace slfmm {
+std::string& no_ctx_error() {
+  static thread_local std::string e;
+  return e;
+}
+}  // namespace slfmm
 
 namespace slfmm {
 
@@ -379,7 +363,7 @@ void slfmm_destroy(slfmm_ctx* ctx) {
   delete ctx;
 }
 
-const char* slfmm_last_error(const slfmm_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+const char* slfmm_last_error(const slfmm_ctx* ctx) { return ctx ? ctx->err.c_str() : no_ctx_error().c_str(); }
 
 int slfmm_synchronize(slfmm_ctx* ctx) {
   if (!ctx) return SLFMM_E_ARG;

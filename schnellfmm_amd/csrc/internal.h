@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -141,6 +142,37 @@ uint64_t host_chunk_rows(const slfmm_geno* g);
 // host threads) and queues its copy to dst (row stride g->ld) on the copy
 // stream; records ev_copied[b].
 void host_chunk_upload(slfmm_ctx* ctx, const slfmm_geno* g, uint64_t j0, uint64_t j1, int b, uint8_t* dst);
+
+// Body of every C entry point: runs f with the context's device current,
+// turns exceptions into status codes and the context's error text, and reads
+// the device status words the call queued.
+// Host-only entry points may run without a context; their error text is kept
+// per thread (slfmm_last_error(NULL)).
+std::string& no_ctx_error();
+
+template <class F>
+int guarded(slfmm_ctx* ctx, F&& f) {
+  std::string& err = ctx ? ctx->err : no_ctx_error();
+  try {
+    if (ctx) {
+      SLFMM_HIP(hipSetDevice(ctx->device));
+      ctx->checks.clear();
+    }
+    f();
+    if (ctx) ctx->finish_checks();
+    err.clear();
+    return SLFMM_OK;
+  } catch (const Error& e) {
+    err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    err = "host allocation failed";
+    return SLFMM_E_NOMEM;
+  } catch (const std::exception& e) {
+    err = e.what();
+    return SLFMM_E_ARG;
+  }
+}
 
 // ---- geno.hip -------------------------------------------------------------
 uint64_t geno_ld(uint64_t n);
