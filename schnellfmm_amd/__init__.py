@@ -9,3 +9,4 @@ from .api import (CENTER_ONLY, EIGENSTRAT, PREC_FP64, PREC_I8, BedFile, Context,
                   precompute, safe_inv,
                   subset_indices, test_associations, test_associations_fused)
 from .output import BimRecords, parse_output_format, write_results  # noqa: F401
+from .samples import FamRecord, align_est_bed_samples, load_covariates, read_fam  # noqa: F401
