@@ -69,13 +69,12 @@ __global__ void __launch_bounds__(256) k_gram_partial(const double* __restrict__
   for (uint64_t rb = r0; rb < r1; rb += 32) {
     const int nr = (int)std::min<uint64_t>(32, r1 - rb);
     __syncthreads();
-    for (int e = t; e < 32 * a; e += 256) {
-      const int rr = e / a, c = e % a;
-      sa[rr][c] = rr < nr ? A[(rb + rr) * lda + c] : 0.0;
-    }
-    for (int e = t; e < 32 * b; e += 256) {
-      const int rr = e / b, c = e % b;
-      sb[rr][c] = rr < nr ? B[(rb + rr) * ldb + c] : 0.0;
+    {   // 4 rows x 64 columns per pass (a, b <= 64)
+      const int c = t & 63;
+      for (int rr = t >> 6; rr < 32; rr += 4) {
+        if (c < a) sa[rr][c] = rr < nr ? A[(rb + rr) * lda + c] : 0.0;
+        if (c < b) sb[rr][c] = rr < nr ? B[(rb + rr) * ldb + c] : 0.0;
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -132,10 +131,12 @@ __global__ void __launch_bounds__(256) k_ts_gemm(const double* __restrict__ A, u
   __shared__ double sm[kSm * kSm];
   for (int e = threadIdx.x; e < m * q; e += 256) sm[e] = M[(uint64_t)(e / q) * ldm + (e % q)];
   __syncthreads();
-  const uint64_t total = rows * (uint64_t)q;
-  for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256) {
-    uint64_t r = e / q;
-    int c = (int)(e % q);
+  // thread = (row, column) on a 256 / cw x cw grid, cw = 32 or 64 >= q
+  const int cw = q <= 32 ? 32 : 64;
+  const int c = threadIdx.x & (cw - 1);
+  const int rpb = 256 / cw;
+  if (c >= q) return;
+  for (uint64_t r = (uint64_t)blockIdx.x * rpb + (threadIdx.x / cw); r < rows; r += (uint64_t)gridDim.x * rpb) {
     const double* ar = A + r * lda;
     double s = 0.0;
     for (int t = 0; t < m; ++t) s += ar[t] * sm[t * q + c];
@@ -150,7 +151,8 @@ void ts_gemm(hipStream_t s, const double* A, uint64_t lda, uint64_t m, const dou
              uint64_t ldo, const int* gate) {
   if (rows == 0 || q == 0) return;
   SLFMM_CHECK(m <= (uint64_t)kSm && q <= (uint64_t)kSm, SLFMM_E_ARG, "ts_gemm: too many columns");
-  uint64_t blocks = std::min<uint64_t>(cdiv(rows * q, 256), 1024);   // M is staged once per workgroup
+  const uint64_t rpb = q <= 32 ? 8 : 4;
+  uint64_t blocks = std::min<uint64_t>(cdiv(rows, rpb), 1024);   // M is staged once per workgroup
   k_ts_gemm<<<(unsigned)blocks, 256, 0, s>>>(A, lda, (int)m, M, ldm, (int)q, rows, alpha, beta, out, ldo, gate);
   SLFMM_HIP(hipGetLastError());
 }
@@ -234,11 +236,10 @@ __global__ void __launch_bounds__(256) k_chol(const double* __restrict__ G, int 
       const double rjj = A[j * m + j];
       for (int i = j + 1 + t; i < m; i += 256) A[j * m + i] /= rjj;
       __syncthreads();
-      const int w = m - j - 1;
-      for (int e = t; e < w * w; e += 256) {
-        int i = j + 1 + e / w, l = j + 1 + e % w;
-        if (l >= i) A[i * m + l] -= A[j * m + i] * A[j * m + l];
-      }
+      // trailing update on a 16 x 16 thread grid (no division by the runtime width)
+      for (int i = j + 1 + (t >> 4); i < m; i += 16)
+        for (int l = j + 1 + (t & 15); l < m; l += 16)
+          if (l >= i) A[i * m + l] -= A[j * m + i] * A[j * m + l];
       __syncthreads();
     }
     if (!fail || attempt == 1 || !allow_shift) break;
@@ -345,11 +346,8 @@ __global__ void __launch_bounds__(256) k_safe_inv(const double* __restrict__ In,
       const double inv = 1.0 / A[j * m + j];
       for (int i = j + 1 + t; i < m; i += 256) A[i * m + j] *= inv;
       __syncthreads();
-      const int w = m - j - 1;
-      for (int e = t; e < w * w; e += 256) {
-        int i = j + 1 + e / w, l = j + 1 + e % w;
-        A[i * m + l] -= A[i * m + j] * A[j * m + l];
-      }
+      for (int i = j + 1 + (t >> 4); i < m; i += 16)   // 16 x 16 thread grid
+        for (int l = j + 1 + (t & 15); l < m; l += 16) A[i * m + l] -= A[i * m + j] * A[j * m + l];
       __syncthreads();
     }
     if (!sing) {
@@ -403,8 +401,11 @@ __global__ void __launch_bounds__(256) k_jacobi(const double* __restrict__ In, i
     for (int r = 0; r < mp - 1; ++r) {
       for (int pi = grp; pi < mp / 2; pi += 32) {
         int pos_a = pi, pos_b = mp - 1 - pi;
-        int a = pos_a == 0 ? 0 : ((pos_a - 1 + r) % (mp - 1)) + 1;
-        int b = pos_b == 0 ? 0 : ((pos_b - 1 + r) % (mp - 1)) + 1;
+        int va = pos_a - 1 + r, vb = pos_b - 1 + r;   // (pos - 1 + r) mod (mp - 1), both terms below mp - 1
+        if (va >= mp - 1) va -= mp - 1;
+        if (vb >= mp - 1) vb -= mp - 1;
+        int a = pos_a == 0 ? 0 : va + 1;
+        int b = pos_b == 0 ? 0 : vb + 1;
         bool live = a < m && b < m;
         double al = 0.0, be = 0.0, ga = 0.0;
         if (live)
