@@ -308,6 +308,22 @@ void cholqr2(slfmm_ctx* ctx, double* A, uint64_t lda, uint64_t m, uint64_t rows,
   copy2d(s, Rtmp, m, Rout, m, m, m, st);
 }
 
+// One CholeskyQR pass (shifted if plain Cholesky breaks down): out = A R^{-1}.
+// For the intermediate bases of the power iterations, where only the column
+// span is carried forward: the loss of orthogonality, about cond(A)^2 u, does
+// not change the span.
+void cholqr1(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t m, uint64_t rows, double* out, uint64_t ldo,
+             int* st) {
+  SLFMM_CHECK(m >= 1 && m <= (uint64_t)kSm, SLFMM_E_ARG, "cholqr1: 1..64 columns");
+  hipStream_t s = ctx->stream;
+  double* G = ctx->buf<double>("cqr_G", m * m);
+  double* R1 = ctx->buf<double>("cqr_R1", m * m);
+  double* R1i = ctx->buf<double>("cqr_R1i", m * m);
+  ts_gram(ctx, A, lda, m, A, lda, m, rows, G);
+  launch_chol(s, G, m, rows, true, R1, R1i, st, 0, nullptr);
+  ts_gemm(s, A, lda, m, R1i, m, m, rows, 1.0, 0.0, out, ldo);
+}
+
 // ---- one-workgroup LU inverse with the safe_inv fallback -------------------
 // src/testing.rs:996-1024: invert; if LU meets an exactly zero pivot, add
 // eps = 1e-8 * max(max|diag|, 1e-300) to the diagonal and invert again.

@@ -222,6 +222,9 @@ void colabsmax_scale(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t row
 // pass, then two clean ones, if plain Cholesky breaks down); R (m x m upper)
 // to d_R if non-null.  *d_st <- 0 ok, 1 ok after the shift, 2 not full rank.
 void cholqr2(slfmm_ctx* ctx, double* A, uint64_t lda, uint64_t m, uint64_t rows, double* d_R, int* d_st);
+// One shifted-CholeskyQR pass, out = A R^{-1} (column span only; see dense.hip).
+void cholqr1(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t m, uint64_t rows, double* out, uint64_t ldo,
+             int* st);
 // safe_inv on device (m <= 64): *d_st <- 0 ok, 1 regularised, 2 failed.
 void safe_inv_dev(slfmm_ctx* ctx, const double* d_a, uint64_t m, double* d_out, int* d_st);
 // out (m x m) <- In + lambda I

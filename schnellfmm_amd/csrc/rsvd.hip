@@ -308,15 +308,19 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
       launch_func_f64(s, Y, g->ld, pe, npad, A, lda, nrb, mu, Z, lda, sc, fill);
     }
   };
-  auto orth_z = [&]() {
-    cholqr2(ctx, Z, lda, l, pe, nullptr,
+  // The bases between power iterations carry only their column span forward,
+  // so one CholeskyQR pass orthonormalises them; the basis of the final
+  // projection, whose singular vectors are the result, gets CholeskyQR2.
+  double* Zq = ctx->buf<double>("rs_Zq", pe * lda);
+  auto orth_z = [&]() {   // Zq = orthonormal basis of span(Z)
+    cholqr1(ctx, Z, lda, l, pe, Zq, lda,
             ctx->check_slot("", SLFMM_E_NUMERIC, "QR of the RSVD sketch failed (rank deficient)"));
   };
   auto forward = [&]() {   // W = N Y Q_z
     {
       slfmm_timed tm(ctx, "rsvd_forward");
-      if (mixed) rsvd_accumulate_i8(ctx, YT, ldt, n, pe, Z, lda, l, mu, sc, W, l);
-      else accumulate_f64(ctx, Y, g->ld, n, pe, Z, lda, l, mu, sc, W, l, fill);
+      if (mixed) rsvd_accumulate_i8(ctx, YT, ldt, n, pe, Zq, lda, l, mu, sc, W, l);
+      else accumulate_f64(ctx, Y, g->ld, n, pe, Zq, lda, l, mu, sc, W, l, fill);
     }
     apply_N(ctx, Qd, d, dlam, W, l, l, n, false);
   };
@@ -324,8 +328,7 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   orth_z();
   for (uint64_t it = 0; it < cfg.n_power_iter; ++it) {  // src/rsvd.rs:79-133
     forward();
-    cholqr2(ctx, W, l, l, n, nullptr, ctx->check_slot("", SLFMM_E_NUMERIC, "QR of the power-iteration block failed"));
-    SLFMM_HIP(hipMemcpyAsync(B, W, n * l * 8, hipMemcpyDeviceToDevice, s));
+    cholqr1(ctx, W, l, l, n, B, l, ctx->check_slot("", SLFMM_E_NUMERIC, "QR of the power-iteration block failed"));
     apply_N(ctx, Qd, d, dlam, B, l, l, n, false);
     project();
     orth_z();
