@@ -516,14 +516,17 @@ __global__ void __launch_bounds__(256) k_gif(const unsigned long long* __restric
 }
 
 // p = 2 Phi(-|t / sqrt(GIF)|) = erfc(|z| / sqrt 2)  (src/testing.rs:705-709)
+// One thread per SNP, looping over its d traits (no 64-bit modulo).
 __global__ void k_calibrate(const double* __restrict__ tstat, uint64_t p, int d, const double* __restrict__ gif_pt,
                             double* pval) {
-  uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= p * (uint64_t)d) return;
-  int tr = (int)(e % d);
-  double gs = sqrt(gif_pt[tr]);
-  double z = tstat[e] / gs;
-  pval[e] = 2.0 * (0.5 * erfc(fabs(z) / 1.4142135623730951));
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= p) return;
+  for (int tr = 0; tr < d; ++tr) {
+    const double gs = sqrt(gif_pt[tr]);
+    const uint64_t e = j * d + tr;
+    const double z = tstat[e] / gs;
+    pval[e] = 2.0 * (0.5 * erfc(fabs(z) / 1.4142135623730951));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -787,7 +790,7 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
       part, (int)nblk, (int)d, LB, rows_z, hist);
   k_hist_chunks<<<dim3(kHistChunks, (unsigned)d), 256, 0, s>>>(hist, ctot);
   k_gif<<<1, 256, 0, s>>>(hist, ctot, (int)d, d_gif_pt, d_gif);
-  if (p) k_calibrate<<<(unsigned)cdiv(p * d, 256), 256, 0, s>>>(d_t, p, (int)d, d_gif_pt, d_p);
+  if (p) k_calibrate<<<(unsigned)cdiv(p, 256), 256, 0, s>>>(d_t, p, (int)d, d_gif_pt, d_p);
   SLFMM_HIP(hipGetLastError());
 }
 
