@@ -430,6 +430,7 @@ int slfmm_geno_open_bed_host(slfmm_ctx* ctx, const char* bed_path, uint64_t chun
     Mapped m;
     uint64_t n = 0, p = 0;
     bed_open(bed_path, n, p, &m);
+    SLFMM_CHECK(n >= 1, SLFMM_E_ARG, "a host-resident panel needs at least one sample");
     auto* g = new slfmm_geno();
     g->ctx = ctx; g->n = n; g->p = p; g->ld = geno_ld(n); g->bps = cdiv(n, 4);
     g->host = m.p + 3; g->chunk_snps = chunk_snps;

@@ -305,7 +305,7 @@ int slfmm_write_tsv(slfmm_ctx* ctx, const char* path, const char* bim_path, cons
     };
     std::deque<std::future<std::string>> q;
     uint64_t next = 0;
-    const size_t window = (size_t)nt * 2;
+    const size_t window = (size_t)nt;   // blocks formatted at once
     auto launch = [&](uint64_t b) { return std::async(std::launch::async, format_block, b); };
     for (; next < nblk && q.size() < window; ++next) q.push_back(launch(next));
     while (!q.empty()) {
