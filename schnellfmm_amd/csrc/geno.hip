@@ -240,40 +240,71 @@ void launch_decode_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64
 // 16-byte stores of the sample rows.  Every byte [0, ldt) of rows i < n is
 // written (codes past p are 0); ldt must be a multiple of 16.
 // ---------------------------------------------------------------------------
+// 16 x 16 two-bit codes held as 16 words (word r = row r, code c at bits
+// 2c): four rounds of masked block swaps (8, 4, 2, 1 rows apart) leave word c
+// holding column c, code r at bits 2r.
+__device__ __forceinline__ void transpose16_2bit(uint32_t (&w)[16]) {
+#pragma unroll
+  for (int round = 0; round < 4; ++round) {
+    const int s = 8 >> round;
+    const int sh = 2 * s;
+    const uint32_t mask = round == 0 ? 0x0000FFFFu : round == 1 ? 0x00FF00FFu : round == 2 ? 0x0F0F0F0Fu : 0x33333333u;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      if (r & s) continue;
+      const uint32_t t = ((w[r] >> sh) ^ w[r + s]) & mask;
+      w[r] ^= t << sh;
+      w[r + s] ^= t;
+    }
+  }
+}
+
+// Tile of 256 SNPs x 256 samples: 256 SNP rows of 64 bytes are staged in LDS
+// (four threads per row, so a wave moves 16 whole 64-byte row pieces), each
+// thread transposes one 16 x 16 block of codes in registers, and the 256
+// sample rows go back out the same way.  Transposed words are stored with
+// their word index rotated by the sample block (LDS banks), and read back
+// with the same rotation.
 __global__ void __launch_bounds__(256) k_transpose(const uint8_t* __restrict__ in, uint64_t ld, uint64_t n,
                                                    uint64_t p, uint8_t* __restrict__ out, uint64_t ldt) {
-  __shared__ uint32_t tile[128][9];   // 128 SNPs x 8 words (128 samples)
-  const uint64_t j0 = (uint64_t)blockIdx.x * 128;   // SNP tile
-  const uint64_t i0 = (uint64_t)blockIdx.y * 128;   // sample tile
+  __shared__ uint32_t tile[256][17];
+  const uint64_t j0 = (uint64_t)blockIdx.x * 256;   // SNP tile
+  const uint64_t i0 = (uint64_t)blockIdx.y * 256;   // sample tile
   const int t = threadIdx.x;
-  {
-    const int r = t >> 1, h = t & 1;
-    const uint64_t j = j0 + r, off = i0 / 4 + 16 * h;
-    uint4 q = make_uint4(0, 0, 0, 0);
-    if (j < p && off + 16 <= ld) q = *reinterpret_cast<const uint4*>(in + j * ld + off);
-    tile[r][4 * h + 0] = q.x; tile[r][4 * h + 1] = q.y; tile[r][4 * h + 2] = q.z; tile[r][4 * h + 3] = q.w;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {   // piece e: SNP row e / 4, 16 bytes e % 4 (rows past p, bytes past ld: zero)
+    const int e = t + 256 * k, row = e >> 2, q = e & 3;
+    const uint64_t j = j0 + row, off = i0 / 4 + 16 * q;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (j < p && off + 16 <= ld) v = *reinterpret_cast<const uint4*>(in + j * ld + off);
+    tile[row][4 * q + 0] = v.x; tile[row][4 * q + 1] = v.y; tile[row][4 * q + 2] = v.z; tile[row][4 * q + 3] = v.w;
   }
   __syncthreads();
-  const int ii = t >> 1, h = t & 1;
-  const uint64_t i = i0 + ii;
-  const uint64_t ob = j0 / 4 + 16 * h;
-  if (i >= n || ob >= ldt) return;
-  uint32_t w[4];
+  const int vb = t >> 4, wb = t & 15;   // SNP block vb, sample block wb
+  uint32_t w[16];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {   // word 4h + u: SNPs 16 (4h + u) .. + 15 of the tile
-    uint32_t v = 0;
+  for (int r = 0; r < 16; ++r) w[r] = tile[16 * vb + r][wb];
+  transpose16_2bit(w);
+  __syncthreads();
 #pragma unroll
-    for (int s2 = 0; s2 < 16; ++s2) v |= ((tile[16 * (4 * h + u) + s2][ii >> 4] >> (2 * (ii & 15))) & 3u) << (2 * s2);
-    w[u] = v;
+  for (int c = 0; c < 16; ++c) tile[16 * wb + c][(vb + wb) & 15] = w[c];   // sample row 16 wb + c, word vb
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {   // piece e: sample row e / 4, words 4 (e % 4) .. + 3
+    const int e = t + 256 * k, row = e >> 2, q = e & 3, rot = row >> 4;
+    const uint64_t i = i0 + row, ob = j0 / 4 + 16 * q;
+    if (i < n && ob + 16 <= ldt)
+      *reinterpret_cast<uint4*>(out + i * ldt + ob) =
+          make_uint4(tile[row][(4 * q + 0 + rot) & 15], tile[row][(4 * q + 1 + rot) & 15],
+                     tile[row][(4 * q + 2 + rot) & 15], tile[row][(4 * q + 3 + rot) & 15]);
   }
-  *reinterpret_cast<uint4*>(out + i * ldt + ob) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 void launch_transpose(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
                       uint8_t* out, uint64_t ldt) {
   if (p == 0 || n == 0) return;
   SLFMM_CHECK(ldt % 16 == 0 && ld % 16 == 0, SLFMM_E_ARG, "transpose strides must be multiples of 16 bytes");
-  k_transpose<<<dim3((unsigned)cdiv(p, 128), (unsigned)cdiv(n, 128)), 256, 0, s>>>(packed, ld, n, p, out, ldt);
+  k_transpose<<<dim3((unsigned)cdiv(p, 256), (unsigned)cdiv(n, 256)), 256, 0, s>>>(packed, ld, n, p, out, ldt);
   SLFMM_HIP(hipGetLastError());
 }
 
