@@ -95,6 +95,59 @@ __global__ void __launch_bounds__(256) k_gram_partial(const double* __restrict__
   }
 }
 
+// Same partial Gram for a, b <= 32 with register blocking: each wave takes
+// every fourth row of a 32-row tile, each lane a 4 x 4 block of the 32 x 32
+// output (8 LDS reads per 16 FMAs), and the four waves' sums are added in a
+// fixed order at the end.
+__global__ void __launch_bounds__(256) k_gram_partial32(const double* __restrict__ A, uint64_t lda, int a,
+                                                        const double* __restrict__ B, uint64_t ldb, int b,
+                                                        uint64_t rows, uint64_t rpb, double* __restrict__ part,
+                                                        const int* gate) {
+  if (gate && *gate != 1) return;
+  __shared__ __align__(16) double sa[32][32];
+  __shared__ __align__(16) double sb[32][32];
+  __shared__ double red[4][32][33];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, ti = lane >> 3, tj = lane & 7;
+  const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
+  const uint64_t r1 = std::min(rows, r0 + rpb);
+  double acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = 0.0;
+  for (uint64_t rb = r0; rb < r1; rb += 32) {
+    const int nr = (int)std::min<uint64_t>(32, r1 - rb);
+    __syncthreads();
+    {   // 8 rows x 32 columns per pass; columns past a (b) and rows past the range are zero
+      const int c = t & 31;
+      for (int rr = t >> 5; rr < 32; rr += 8) {
+        sa[rr][c] = (rr < nr && c < a) ? A[(rb + rr) * lda + c] : 0.0;
+        sb[rr][c] = (rr < nr && c < b) ? B[(rb + rr) * ldb + c] : 0.0;
+      }
+    }
+    __syncthreads();
+    for (int rr = wave; rr < nr; rr += 4) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) { av[x] = sa[rr][4 * ti + x]; bv[x] = sb[rr][4 * tj + x]; }
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] += av[x] * bv[y];
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) red[wave][4 * ti + x][4 * tj + y] = acc[x][y];
+  __syncthreads();
+  const int nout = a * b;
+  for (int e = t; e < nout; e += 256) {
+    const int i = e / b, j = e % b;
+    part[(uint64_t)blockIdx.x * nout + e] = ((red[0][i][j] + red[1][i][j]) + red[2][i][j]) + red[3][i][j];
+  }
+}
+
 __global__ void __launch_bounds__(256) k_gram_reduce(const double* __restrict__ part, int nblk, int nout,
                                                      double* __restrict__ out, const int* gate) {
   if (gate && *gate != 1) return;
@@ -117,7 +170,10 @@ void ts_gram(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t a, const do
   uint64_t rpb = cdiv(rows, nblk);
   nblk = cdiv(rows, rpb);
   double* part = ctx->buf<double>("gram_part", nblk * a * b);
-  k_gram_partial<<<(unsigned)nblk, 256, 0, s>>>(A, lda, (int)a, B, ldb, (int)b, rows, rpb, part, gate);
+  if (a <= 32 && b <= 32)
+    k_gram_partial32<<<(unsigned)nblk, 256, 0, s>>>(A, lda, (int)a, B, ldb, (int)b, rows, rpb, part, gate);
+  else
+    k_gram_partial<<<(unsigned)nblk, 256, 0, s>>>(A, lda, (int)a, B, ldb, (int)b, rows, rpb, part, gate);
   k_gram_reduce<<<(unsigned)cdiv(a * b, 4), 256, 0, s>>>(part, (int)nblk, (int)(a * b), out, gate);
   SLFMM_HIP(hipGetLastError());
 }
