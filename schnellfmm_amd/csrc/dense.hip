@@ -384,7 +384,8 @@ void cholqr1(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t m, uint64_t
 // src/testing.rs:996-1024: invert; if LU meets an exactly zero pivot, add
 // eps = 1e-8 * max(max|diag|, 1e-300) to the diagonal and invert again.
 __global__ void __launch_bounds__(256) k_safe_inv(const double* __restrict__ In, int m, double* Out, int* info) {
-  extern __shared__ double A[];   // m*m
+  extern __shared__ double A[];   // m*m LU factors, then Xs (m*m): column t of the inverse at Xs[i * m + t]
+  double* Xs = A + m * m;
   __shared__ int piv[kSm];
   __shared__ int sing;
   __shared__ double eps;
@@ -423,13 +424,21 @@ __global__ void __launch_bounds__(256) k_safe_inv(const double* __restrict__ In,
       __syncthreads();
     }
     if (!sing) {
-      if (t < m) {   // column t of A^{-1}: P b, forward (unit L), backward (U)
-        double x[kSm];
-        for (int i = 0; i < m; ++i) x[i] = (i == t) ? 1.0 : 0.0;
-        for (int j = 0; j < m; ++j) { int p = piv[j]; double tmp = x[j]; x[j] = x[p]; x[p] = tmp; }
-        for (int i = 0; i < m; ++i) { double s = x[i]; for (int k = 0; k < i; ++k) s -= A[i * m + k] * x[k]; x[i] = s; }
-        for (int i = m - 1; i >= 0; --i) { double s = x[i]; for (int k = i + 1; k < m; ++k) s -= A[i * m + k] * x[k]; x[i] = s / A[i * m + i]; }
-        for (int i = 0; i < m; ++i) Out[i * m + t] = x[i];
+      if (t < m) {   // column t of A^{-1}: P b, forward (unit L), backward (U); the column lives in LDS
+        double* x = Xs + t;   // x[i] is Xs[i * m + t]
+        for (int i = 0; i < m; ++i) x[i * m] = (i == t) ? 1.0 : 0.0;
+        for (int j = 0; j < m; ++j) { int p = piv[j]; double tmp = x[j * m]; x[j * m] = x[p * m]; x[p * m] = tmp; }
+        for (int i = 0; i < m; ++i) {
+          double s = x[i * m];
+          for (int k = 0; k < i; ++k) s -= A[i * m + k] * x[k * m];
+          x[i * m] = s;
+        }
+        for (int i = m - 1; i >= 0; --i) {
+          double s = x[i * m];
+          for (int k = i + 1; k < m; ++k) s -= A[i * m + k] * x[k * m];
+          x[i * m] = s / A[i * m + i];
+        }
+        for (int i = 0; i < m; ++i) Out[i * m + t] = x[i * m];
       }
       if (t == 0) *info = attempt;   // 0 = plain, 1 = regularised
       return;
@@ -445,7 +454,11 @@ __global__ void __launch_bounds__(256) k_safe_inv(const double* __restrict__ In,
 
 void safe_inv_dev(slfmm_ctx* ctx, const double* d_a, uint64_t m, double* d_out, int* d_st) {
   SLFMM_CHECK(m >= 1 && m <= (uint64_t)kSm, SLFMM_E_ARG, "safe_inv: order must be 1..64");
-  k_safe_inv<<<1, 256, m * m * sizeof(double), ctx->stream>>>(d_a, (int)m, d_out, d_st);
+  const size_t lds = 2 * m * m * sizeof(double);
+  if (lds > 48 * 1024)
+    SLFMM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_safe_inv),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  k_safe_inv<<<1, 256, lds, ctx->stream>>>(d_a, (int)m, d_out, d_st);
   SLFMM_HIP(hipGetLastError());
 }
 
