@@ -530,11 +530,15 @@ __global__ void __launch_bounds__(256) k_jacobi(const double* __restrict__ In, i
     sv[t] = sqrt(s);
   }
   __syncthreads();
-  if (t == 0) {
-    for (int i = 0; i < m; ++i) order[i] = i;
-    for (int i = 0; i < m; ++i)
-      for (int j = i + 1; j < m; ++j)
-        if (sv[order[j]] > sv[order[i]]) { int tmp = order[i]; order[i] = order[j]; order[j] = tmp; }
+  if (t < m) {   // descending order: rank of column t (ties keep the column order; NaN sorts last)
+    auto key = [&](int j) { const double x = sv[j]; return x != x ? -__builtin_huge_val() : x; };
+    const double v = key(t);
+    int rank = 0;
+    for (int j = 0; j < m; ++j) {
+      const double u = key(j);
+      rank += (u > v || (u == v && j < t)) ? 1 : 0;
+    }
+    order[rank] = t;   // the ranks are a permutation of 0 .. m-1
   }
   __syncthreads();
   for (int e = t; e < m * m; e += 256) {
