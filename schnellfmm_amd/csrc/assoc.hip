@@ -448,22 +448,45 @@ __global__ void __launch_bounds__(256) k_hist_chunks(const unsigned long long* _
 // exceeds N/2, i.e. the bin of sorted position N/2; for odd N the bin of
 // position (N-1)/2.  The chunk holding a position is found from the chunk
 // totals, then the bin inside it by a workgroup scan.
+// Exclusive prefix sum of one value per thread over a 256-thread block (exact
+// integer sums, so the order of the additions does not matter); total gets
+// the block sum.  Every thread of the block must call it.
+__device__ unsigned long long block_exclusive_scan(unsigned long long v, unsigned long long* wsum,
+                                                   unsigned long long& total) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  unsigned long long x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned long long y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  unsigned long long base = 0;
+  for (int i = 0; i < w; ++i) base += wsum[i];
+  total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  __syncthreads();   // wsum may be reused by the next call
+  return base + x - v;
+}
+
 __global__ void __launch_bounds__(256) k_gif(const unsigned long long* __restrict__ hist,
                                              const unsigned long long* __restrict__ chunk_tot, int d,
                                              double* gif_pt, double* gif_avg) {
   __shared__ unsigned long long cum[kHistChunks + 1];
-  __shared__ unsigned long long scan[256];
   __shared__ int fbin[2];
   __shared__ double raw[kMaxD];
+  __shared__ unsigned long long wsum[4];
+  static_assert(kHistChunks < 256, "one chunk total per thread");
   const int t = threadIdx.x;
   for (int tr = 0; tr < d; ++tr) {
     const unsigned long long* h = hist + (uint64_t)tr * kBins;
-    if (t == 0) {
-      unsigned long long c = 0;
-      for (int i = 0; i < kHistChunks; ++i) { cum[i] = c; c += chunk_tot[tr * kHistChunks + i]; }
-      cum[kHistChunks] = c;
-      fbin[0] = fbin[1] = -1;
+    {   // cum[i] = sum of the chunk totals before chunk i; cum[kHistChunks] = N
+      unsigned long long tot = 0;
+      const unsigned long long e =
+          block_exclusive_scan(t < kHistChunks ? chunk_tot[tr * kHistChunks + t] : 0ull, wsum, tot);
+      if (t <= kHistChunks) cum[t] = e;
     }
+    if (t == 0) fbin[0] = fbin[1] = -1;
     __syncthreads();
     const unsigned long long N = cum[kHistChunks];
     const bool even = (N % 2) == 0;
@@ -479,14 +502,8 @@ __global__ void __launch_bounds__(256) k_gif(const unsigned long long* __restric
         v[i] = (b0 + i < kBins) ? h[b0 + i] : 0ull;
         s += v[i];
       }
-      scan[t] = s;
-      __syncthreads();
-      if (t == 0) {
-        unsigned long long c = cum[ch];
-        for (int i = 0; i < 256; ++i) { unsigned long long x = scan[i]; scan[i] = c; c += x; }
-      }
-      __syncthreads();
-      unsigned long long c = scan[t];
+      unsigned long long tot = 0;
+      unsigned long long c = cum[ch] + block_exclusive_scan(s, wsum, tot);   // count below bin b0
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const unsigned long long c2 = c + v[i];
