@@ -46,6 +46,8 @@ def _panel(n, p, seed, d=2, miss=0.02, n_causal=10, effect=0.8, **kw):
     (257, 1037, 4, 1, 0, 0.10),     # ragged sample and SNP tails, heavy missingness
     (1100, 2000, 12, 1, 1, 0.05),
     (640, 1500, 20, 4, 1, 0.02),    # K + d = 24 (headline shape)
+    (400, 1200, 22, 6, 1, 0.03),    # K + d = 28: 8 row tiles, 32-functional epilogue
+    (400, 1100, 24, 8, 0, 0.03),    # K + d = 32: 9 row tiles, the unpipelined kernel
 ])
 def test_i8_association_vs_oracle(S, n, p, K, d, norm, miss):
     pan, packed, bps = _panel(n, p, seed=n + p, d=d, miss=miss)
