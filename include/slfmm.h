@@ -49,7 +49,10 @@ enum { SLFMM_IMPUTE_MEAN = 0, SLFMM_IMPUTE_NMF = 1 };              /* src/lib.rs
 enum {
   SLFMM_PREC_FP64 = 0,    /* FP64 throughout                                   */
   SLFMM_PREC_I8 = 1       /* int8 MFMA on exact genotypes, multi-term int8
-                             operands, exact int32 accumulation, FP64 epilogue */
+                             operands, exact int32 accumulation, FP64 epilogue.
+                             The int8 kernels carry at most 32 columns: a sketch
+                             with K + oversampling > 32, or an association with
+                             K + d > 32, runs that stage's FP64 kernels. */
 };
 
 enum { SLFMM_OUT_DEVICE = 1u };

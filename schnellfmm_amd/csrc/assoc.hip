@@ -677,8 +677,10 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
   ea.n_miss = nmiss; ea.sum_g = sumg; ea.n_hom = nhom;
   ea.beta = d_beta; ea.tstat = d_t; ea.r2 = d_r2;
   const bool with_nmf = nmf && nmf->W;
-  const bool i8 = cfg.precision == SLFMM_PREC_I8;
-  SLFMM_CHECK(!(i8 && with_nmf), SLFMM_E_ARG, "NMF imputation runs on the FP64 path (use SLFMM_PREC_FP64)");
+  SLFMM_CHECK(!(cfg.precision == SLFMM_PREC_I8 && with_nmf), SLFMM_E_ARG,
+              "NMF imputation runs on the FP64 path (use SLFMM_PREC_FP64)");
+  // the int8 kernels carry at most 32 functionals (K + d); beyond that the FP64 kernels run
+  const bool i8 = cfg.precision == SLFMM_PREC_I8 && m <= 32;
   SLFMM_CHECK(!(with_nmf && g->host), SLFMM_E_ARG, "NMF imputation needs a device-resident genotype panel");
   // FP64 operand: the basis padded to whole row blocks (shared by every row range)
   const uint64_t npad = rup(n, 64);

@@ -276,7 +276,8 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
     launch_mean_scale(s, nmiss, sumg, n, pe, cfg.norm, mu, sc);
   }
 
-  const bool mixed = cfg.precision == SLFMM_PREC_I8;
+  // the int8 sketch carries at most 32 columns; wider sketches run the FP64 kernels
+  const bool mixed = cfg.precision == SLFMM_PREC_I8 && l <= 32;
   uint8_t* YT = nullptr;
   uint64_t ldt = 0;
   if (mixed) {   // sample-major copy for the forward passes

@@ -164,3 +164,20 @@ def test_i8_fit_end_to_end_log10p_bound(S):
     dlp = np.abs(O.log10_pvalue(got.tstat, got.gif_per_trait) - O.log10_pvalue(ref.t, ref.gif_per_trait))
     assert np.max(dlp) <= 1e-3
     assert np.max(np.abs(ref.t)) > 5       # the panel has real signal
+
+
+def test_i8_wide_shapes_fall_back_to_fp64_kernels(S):
+    """Past 32 int8 columns the mixed-precision fit runs the FP64 kernels for
+    that stage: K + oversampling > 32 gives the FP64 sketch (so U_hat equals
+    the FP64 fit's bit for bit), and K + d > 32 the FP64 association too."""
+    n, p = 500, 2500
+    pan, packed, bps = _panel(n, p, seed=77, d=1, miss=0.03)
+    g = S.BedFile.from_packed(packed, n, p)
+    for K in (30, 40):
+        f64 = S.fit_lfmm2(g, 0.5, g, pan.x, S.Lfmm2Config(k=K, norm=S.EIGENSTRAT, precision=S.PREC_FP64))
+        mix = S.fit_lfmm2(g, 0.5, g, pan.x, S.Lfmm2Config(k=K, norm=S.EIGENSTRAT, precision=S.PREC_I8))
+        assert np.array_equal(mix.u_hat, f64.u_hat)
+        if K + 1 > 32:
+            assert np.array_equal(mix.tstat, f64.tstat)
+        dlp = np.abs(O.log10_pvalue(mix.tstat, mix.gif_per_trait) - O.log10_pvalue(f64.tstat, f64.gif_per_trait))
+        assert np.max(dlp) <= 1e-3
