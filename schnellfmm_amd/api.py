@@ -103,7 +103,9 @@ def subset_indices(spec, n_snps: int) -> np.ndarray:
 
 
 class BedFile:
-    """A device-resident PLINK genotype panel (BedFile, src/bed.rs:60-172)."""
+    """A PLINK genotype panel (BedFile, src/bed.rs:60-172): device-resident
+    (open, from_packed, synthetic) or host-resident and streamed to the device
+    in chunks (open_host, wrap_host)."""
 
     def __init__(self, handle, n_samples: int, n_snps: int, ctx: Context, path: Optional[str] = None,
                  host_buffer=None):
