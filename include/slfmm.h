@@ -249,8 +249,9 @@ int slfmm_fit_lfmm2_batch(slfmm_ctx* ctx, slfmm_problem* problems, uint64_t coun
  * FP64: NMF estimation on est's subset, W^+ = safe_inv(W^T W) W^T, the RSVD
  * on the NMF-filled estimation rows (NmfInRam) and the association on every
  * SNP of g with NMF on-the-fly fill (NmfOnTheFly).  nmf_omega
- * (n x (nmf k + 10)) may be injected; nmf_cv_mae (n_iter, host) receives the
- * estimation CV error per iteration; gwas_cv (3, host: NMF MAE, mean-fill
+ * (n x (nmf k + 10)) may be injected; nmf_cv_mae (n_iter + 1, host) receives
+ * the estimation CV error per iteration, then the mean-imputation error on the
+ * final iteration's mask (results.nmf_cv_mean, src/lib.rs:259); gwas_cv (3, host: NMF MAE, mean-fill
  * MAE, held-out count) enables the association-phase CV with rate
  * nmf->cv_rate and seed cfg->seed (NULL disables it). */
 int slfmm_fit_lfmm2_nmf(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset_idx, uint64_t n_idx,

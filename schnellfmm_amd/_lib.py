@@ -101,9 +101,10 @@ class SlfmmError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[{code}] {msg}")
         self.code = code
+        self.msg = msg
 
 
 def check(ctx, rc: int) -> None:
     if rc != 0:
-        msg = lib().slfmm_last_error(ctx).decode(errors="replace") if ctx else ""
+        msg = lib().slfmm_last_error(ctx).decode(errors="replace")   # NULL ctx: the thread's last error
         raise SlfmmError(rc, msg)

@@ -277,6 +277,7 @@ class TestResults:
     pvalue: np.ndarray
     hist: Optional[np.ndarray] = None
     nmf_cv: Optional[np.ndarray] = None          # estimation CV MAE per NMF iteration
+    nmf_cv_mean: Optional[float] = None          # mean-fill MAE on the final iteration's mask
     gwas_cv: Optional[tuple] = None              # (NMF MAE, mean-fill MAE, held-out count)
 
     __test__ = False   # not a pytest class
@@ -456,7 +457,7 @@ def fit_lfmm2_nmf(y_est: BedFile, est_subset, y_full: BedFile, x, config: Lfmm2C
     ncfg = nmf.to_c()
     b = None if b0 is None else _f64(b0)
     om = None if nmf_omega is None else _f64(nmf_omega)
-    cv_iter = np.empty(max(nmf.n_iter, 1))
+    cv_iter = np.empty(nmf.n_iter + 1)
     gcv = np.zeros(3) if gwas_cv else None
     ctx.check(lib().slfmm_fit_lfmm2_nmf(ctx.h, y_est.h, _ptr(idx), 0 if idx is None else len(idx), y_full.h,
                                         _ptr(x), d, C.byref(cfg), C.byref(ncfg), _ptr(b), _ptr(om), _ptr(u),
@@ -464,6 +465,7 @@ def fit_lfmm2_nmf(y_est: BedFile, est_subset, y_full: BedFile, x, config: Lfmm2C
     beta, t, r2, p, hist, gpt, gif = arrs
     res = TestResults(u, float(gif[0]), gpt, beta, t, r2, p, hist)
     res.nmf_cv = cv_iter[:nmf.n_iter].copy()
+    res.nmf_cv_mean = float(cv_iter[nmf.n_iter])
     if gcv is not None:
         res.gwas_cv = (float(gcv[0]), float(gcv[1]), int(gcv[2]))
     return res

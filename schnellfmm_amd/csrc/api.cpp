@@ -863,8 +863,11 @@ int slfmm_fit_lfmm2_nmf(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* su
     if (nmf->n_iter) to_host(ctx, hcv.data(), dcv, 3 * nmf->n_iter);
     if (gwas_cv) to_host(ctx, gwas_cv, dgcv, 3);
     SLFMM_HIP(hipStreamSynchronize(s));
-    if (nmf_cv_mae)
+    if (nmf_cv_mae) {
       for (uint64_t i = 0; i < nmf->n_iter; ++i) nmf_cv_mae[i] = hcv[3 * i];
+      // mean-imputation error on the final iteration's mask (results.nmf_cv_mean, src/lib.rs:259)
+      nmf_cv_mae[nmf->n_iter] = nmf->n_iter ? hcv[3 * (nmf->n_iter - 1) + 1] : 0.0;
+    }
   });
 }
 

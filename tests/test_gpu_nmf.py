@@ -94,6 +94,7 @@ def test_fit_lfmm2_nmf_vs_oracle(S):
     assert _close(got.r2, ref.r2, 1e-8)
     assert np.allclose(got.gif_per_trait, ref.gif_per_trait, rtol=1e-12)
     assert np.allclose(got.nmf_cv, nres.cv_mae_per_iter, rtol=1e-9, atol=0)
+    assert np.isclose(got.nmf_cv_mean, nres.cv_mae_mean_impute, rtol=1e-12, atol=0)
     assert got.gwas_cv[2] == ref.cv[2] > 0
     assert np.allclose(got.gwas_cv[:2], ref.cv[:2], rtol=1e-9, atol=0)
     assert np.max(np.abs(ref.t)) > 4   # the panel has real signal

@@ -77,3 +77,76 @@ def test_cli_flow_matches_hand_aligned_fit(tmp_path, prec):
     want = OO.results_tsv([r[0] for r in recs], [r[3] for r in recs], [r[1] for r in recs], names,
                           res.pvalue, res.beta, res.tstat, res.r2)
     assert open(out).read() == want
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_cli_end_to_end(tmp_path, prec, capsys):
+    """python -m schnellfmm_amd (src/main.rs:114-289): the results file and the
+    summary equal those of the library steps run by hand on the same inputs."""
+    import schnellfmm_amd as S
+    from schnellfmm_amd import cli
+    n, p = 300, 5000
+    pan = synth.balding_nichols(n, p, n_pops=3, miss_rate=0.02, d=2, n_causal=8, effect=1.0, seed=21)
+    iids = [f"s{i}" for i in range(n)]
+    main = str(tmp_path / "main")
+    _write_panel(main, pan.g, iids)
+    keep = [i for i in range(n) if i % 7 != 3]
+    with open(tmp_path / "cov.csv", "w") as f:
+        f.write("id,env_a,env_b\n")
+        for i in reversed(keep):
+            f.write(f"{iids[i]},{float(pan.x[i, 0])!r},{float(pan.x[i, 1])!r}\n")
+    out = str(tmp_path / "res")
+    rc = cli.main(["-b", main + ".bed", "-c", str(tmp_path / "cov.csv"), "-k", "4", "-o", out, "-t", "3",
+                   "--est-rate", "0.25", "--intersect-samples", "--power-iter", "3", "--seed", "11",
+                   "--precision", prec])
+    err = capsys.readouterr().err
+    assert rc == 0, err
+    assert "Running LFMM2: K=4, lambda=1.0e-5, chunk_size=10000, threads=3, oversampling=10, power_iter=3" in err
+
+    cfg = S.Lfmm2Config(k=4, n_power_iter=3, seed=11, norm=S.EIGENSTRAT,
+                        precision=S.PREC_I8 if prec == "mixed" else S.PREC_FP64)
+    g = S.BedFile.open(main + ".bed", sample_keep=keep)
+    ref = S.fit_lfmm2(g, 0.25, g, pan.x[keep], cfg)
+    want = str(tmp_path / "want.tsv")
+    S.write_results(want, ref, main + ".bed", cov_names=["env_a", "env_b"])
+    assert open(out + ".tsv").read() == open(want).read()
+    assert open(out + ".summary.txt").read() == (
+        f"GIF: {ref.gif:.6f}\nn_samples: {len(keep)}\nn_snps: {p}\nK: 4\nd: 2\ncovariates: env_a, env_b\n"
+        "lambda: 0.00001\nthreads: 3\nimputation: mean\n")
+    assert f"GIF: {ref.gif:.4f}" in err and err.rstrip().endswith(f"Done. Output: {out}.tsv, {out}.summary.txt")
+
+
+def test_cli_nmf_parquet(tmp_path, capsys):
+    """--nmf-impute with Parquet output: the NMF CV lines of the summary
+    (src/main.rs:262-282) and the Parquet table equal the library's."""
+    import pyarrow.parquet as pq
+    import schnellfmm_amd as S
+    from schnellfmm_amd import cli
+    n, p = 240, 3000
+    pan = synth.balding_nichols(n, p, n_pops=3, miss_rate=0.05, d=1, n_causal=5, effect=1.0, seed=8)
+    iids = [f"s{i}" for i in range(n)]
+    main = str(tmp_path / "main")
+    _write_panel(main, pan.g, iids)
+    with open(tmp_path / "cov.tsv", "w") as f:
+        f.write("id\ttrait\n")
+        for i in range(n):
+            f.write(f"{iids[i]}\t{float(pan.x[i, 0])!r}\n")
+    out = str(tmp_path / "res")
+    rc = cli.main(["-b", main + ".bed", "-c", str(tmp_path / "cov.tsv"), "-k", "3", "-o", out, "-t", "2",
+                   "-f", "parquet", "--chunk-size", "700", "--nmf-impute", "--nmf-k", "4", "--nmf-iter", "3",
+                   "--nmf-cv-rate", "0.01", "--est-rate", "0.5"])
+    err = capsys.readouterr().err
+    assert rc == 0, err
+    cfg = S.Lfmm2Config(k=3, chunk_size=700, norm=S.EIGENSTRAT, impute="nmf")
+    nmf = S.NmfParams(k=4, n_iter=3, cv_rate=0.01, chunk_size=700, seed=42)
+    g = S.BedFile.open(main + ".bed")
+    ref = S.fit_lfmm2_nmf(g, 0.5, g, pan.x, cfg, nmf)
+    want = str(tmp_path / "want.parquet")
+    S.write_results(want, ref, main + ".bed", cov_names=["trait"], fmt="parquet", n_workers=2, chunk_size=700)
+    assert pq.read_table(out + ".parquet").equals(pq.read_table(want))
+    lines = open(out + ".summary.txt").read().splitlines()
+    assert lines[8:] == (["imputation: nmf", "nmf_iterations: 3"]
+                         + [f"nmf_cv_mae_iter_{i + 1}: {v:.6f}" for i, v in enumerate(ref.nmf_cv)]
+                         + [f"mean_impute_cv_mae: {ref.nmf_cv_mean:.6f}", f"nmf_gwas_cv_mae: {ref.gwas_cv[0]:.6f}",
+                            f"nmf_gwas_cv_mean_mae: {ref.gwas_cv[1]:.6f}", f"nmf_gwas_cv_count: {ref.gwas_cv[2]}"])
+    assert ref.gwas_cv[2] > 0 and 0 < ref.nmf_cv_mean
