@@ -62,7 +62,7 @@ def main():
     ap.add_argument("--write-tsv", default="",
                     help="directory: after timing, write the results TSV of the main fit there (synthetic .bim "
                          "metadata), report the writer's time and delete the files")
-    ap.add_argument("--streams", type=int, default=4, help="streams of the batched fit (--workload batch)")
+    ap.add_argument("--streams", type=int, default=8, help="streams of the batched fit (--workload batch)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB on the device three times (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter calibration) and exit")

@@ -406,7 +406,7 @@ class Lfmm2Problem:
     b0: Optional[np.ndarray] = None
 
 
-def fit_lfmm2_batch(problems, n_streams: int = 4, with_hist: bool = False):
+def fit_lfmm2_batch(problems, n_streams: int = 8, with_hist: bool = False):
     """Many independent fit_lfmm2 problems in one call (slfmm_fit_lfmm2_batch):
     problem i runs on stream i % n_streams of the context, so small problems
     overlap on the GPU.  Returns one TestResults per problem, equal to what

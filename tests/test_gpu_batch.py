@@ -41,7 +41,7 @@ def _problems(S):
     return probs
 
 
-@pytest.mark.parametrize("n_streams", [1, 3])
+@pytest.mark.parametrize("n_streams", [1, 3, 8])
 def test_batch_equals_single_fits(S, n_streams):
     probs = _problems(S)
     got = S.fit_lfmm2_batch(probs, n_streams=n_streams)
