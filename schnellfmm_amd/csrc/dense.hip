@@ -266,31 +266,32 @@ __global__ void __launch_bounds__(256) k_chol(const double* __restrict__ G, int 
   if (gate && *gate != 1) return;
   extern __shared__ double A[];   // m*m
   __shared__ double diag0[kSm];
+  __shared__ double rdiag[kSm];
   __shared__ double dinv[kSm];
-  __shared__ int fail;
   __shared__ double shift;
   const int t = threadIdx.x;
   if (t == 0) shift = 0.0;
-  int attempt = 0;
+  int attempt = 0, fail = 0;
   for (;; ++attempt) {
     __syncthreads();
     const double sh = shift;
     for (int e = t; e < m * m; e += 256) A[e] = G[e] + ((e / m == e % m) ? sh : 0.0);
-    if (t == 0) fail = 0;
     __syncthreads();
     if (t < m) diag0[t] = A[t * m + t];
     __syncthreads();
     const double tol = attempt == 0 ? 1e-13 : 0.0;
+    fail = 0;
     for (int j = 0; j < m; ++j) {
-      if (t == 0) {
-        double dj = A[j * m + j];
-        if (!(dj > tol * diag0[j]) || !(dj < 1e308)) fail = j + 1;
-        else A[j * m + j] = sqrt(dj);
+      // every thread reads the pivot and takes the same decision; the
+      // diagonal goes to rdiag so that A[j][j] is not written while read
+      const double dj = A[j * m + j];
+      if (!(dj > tol * diag0[j]) || !(dj < 1e308)) {
+        fail = j + 1;
+        break;
       }
-      __syncthreads();
-      if (fail) break;
-      const double rjj = A[j * m + j];
+      const double rjj = sqrt(dj);
       for (int i = j + 1 + t; i < m; i += 256) A[j * m + i] /= rjj;
+      if (t == 0) rdiag[j] = rjj;
       __syncthreads();
       // trailing update on a 16 x 16 thread grid (no division by the runtime width)
       for (int i = j + 1 + (t >> 4); i < m; i += 16)
@@ -309,14 +310,29 @@ __global__ void __launch_bounds__(256) k_chol(const double* __restrict__ G, int 
     if (t == 0) *st = 2;
     return;
   }
-  if (t < m) dinv[t] = 1.0 / A[t * m + t];
+  if (t < m) {
+    A[t * m + t] = rdiag[t];
+    dinv[t] = 1.0 / rdiag[t];
+  }
   __syncthreads();
-  if (t < m) {   // column c of R^{-1}: X[i][c] = -(sum_{i<k<=c} R[i][k] X[k][c]) / R[i][i]
-    const int c = t;
-    for (int i = c - 1; i >= 0; --i) {
-      double s = A[i * m + c] * dinv[c];
-      for (int k = i + 1; k < c; ++k) s += A[i * m + k] * A[c * m + k];
-      A[c * m + i] = -s * dinv[i];
+  // Column c of R^{-1} by back substitution, 8 lanes per column:
+  //   X[i][c] = -(sum_{i<k<=c} R[i][k] X[k][c]) / R[i][i],  X[c][c] = 1 / R[c][c],
+  // lane `sub` summing the terms k = i+1+sub, i+9+sub, ...  X[k][c] is kept at
+  // A[c][k], the strictly lower triangle, which the factor does not use.  The
+  // 8 lanes of a column are in one wave and all hold the reduced sum.
+  {
+    const int grp = t >> 3, sub = t & 7;
+    for (int c = grp; c < m; c += 32) {
+      const double xcc = dinv[c];
+      for (int i = c - 1; i >= 0; --i) {
+        double s = 0.0;
+        for (int k = i + 1 + sub; k <= c; k += 8) s += A[i * m + k] * (k == c ? xcc : A[c * m + k]);
+        s += __shfl_xor(s, 4, 8);
+        s += __shfl_xor(s, 2, 8);
+        s += __shfl_xor(s, 1, 8);
+        if (sub == 0) A[c * m + i] = -s * dinv[i];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // the group's next reads see X[i][c]
+      }
     }
   }
   __syncthreads();
