@@ -396,11 +396,13 @@ __global__ void __launch_bounds__(1024) k_hist_part(const double* __restrict__ t
   __syncthreads();
   const uint64_t j0 = (uint64_t)blockIdx.x * snps_per_blk;
   const uint64_t j1 = std::min(p, j0 + snps_per_blk);
-  for (uint64_t e = j0 * d + t; e < j1 * d; e += 1024) {
+  // trait of element e = e mod d, advanced by 1024 mod d per step (no 64-bit modulo in the loop)
+  const int step = 1024 % d;
+  int tr = (int)((j0 * d + t) % d);
+  for (uint64_t e = j0 * d + t; e < j1 * d; e += 1024, tr = tr + step >= d ? tr + step - d : tr + step) {
     const double tv = tstat[e];
     const double tsq = tv * tv;
     if (!isfinite(tsq)) continue;
-    const int tr = (int)(e % d);
     const double q = tsq / kBinWidth;
     const int bin = q >= (double)(kBins - 1) ? kBins - 1 : (int)q;
     if (bin < LB) atomicAdd(&h[tr * LB + bin], 1u);
