@@ -538,10 +538,13 @@ __global__ void __launch_bounds__(256) k_gif(const unsigned long long* __restric
 // One thread per SNP, looping over its d traits (no 64-bit modulo).
 __global__ void k_calibrate(const double* __restrict__ tstat, uint64_t p, int d, const double* __restrict__ gif_pt,
                             double* pval) {
+  __shared__ double gsq[kMaxD];   // sqrt(GIF) per trait, once per workgroup
+  if ((int)threadIdx.x < d) gsq[threadIdx.x] = sqrt(gif_pt[threadIdx.x]);
+  __syncthreads();
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= p) return;
   for (int tr = 0; tr < d; ++tr) {
-    const double gs = sqrt(gif_pt[tr]);
+    const double gs = gsq[tr];
     const uint64_t e = j * d + tr;
     const double z = tstat[e] / gs;
     pval[e] = 2.0 * (0.5 * erfc(fabs(z) / 1.4142135623730951));
