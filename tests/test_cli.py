@@ -134,3 +134,17 @@ def test_missing_sample_and_bad_bed(panel, capsys):
     (panel / "p.bed").write_bytes(raw[:-1])
     rc, err = _run(capsys, ["-b", str(panel / "p.bed"), "-c", str(panel / "c.tsv"), "-k", "2"])
     assert rc == 1 and err.splitlines()[-1].startswith("Error: ") and "size" in err.lower()
+
+
+def test_module_entry_point(panel):
+    """python -m schnellfmm_amd: --help exits 0, a usage error exits 2, a failing run exits 1."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    run = lambda *a: subprocess.run([sys.executable, "-m", "schnellfmm_amd", *a], cwd=root,
+                                    capture_output=True, text=True, timeout=120)
+    r = run("--help")
+    assert r.returncode == 0 and "--nmf-impute" in r.stdout and "--est-rate" in r.stdout
+    assert run("-b", "x.bed").returncode == 2
+    r = run("-b", str(panel / "p.bed"), "-c", str(panel / "c.tsv"), "-k", "2", "--est-rate", "2")
+    assert r.returncode == 1 and r.stderr.splitlines()[-1] == "Error: --est-rate must be in (0.0, 1.0], got 2"
