@@ -150,3 +150,43 @@ def test_cli_nmf_parquet(tmp_path, capsys):
                          + [f"mean_impute_cv_mae: {ref.nmf_cv_mean:.6f}", f"nmf_gwas_cv_mae: {ref.gwas_cv[0]:.6f}",
                             f"nmf_gwas_cv_mean_mae: {ref.gwas_cv[1]:.6f}", f"nmf_gwas_cv_count: {ref.gwas_cv[2]}"])
     assert ref.gwas_cv[2] > 0 and 0 < ref.nmf_cv_mean
+
+
+def test_cli_est_bed_center_only_scale_cov(tmp_path, capsys):
+    """--est-bed (samples permuted, extras), --norm center-only, --scale-cov,
+    -l and --oversampling through the command line, against the same steps by hand."""
+    import schnellfmm_amd as S
+    from schnellfmm_amd import cli
+    n, p = 260, 4000
+    pan = synth.balding_nichols(n, p, n_pops=3, miss_rate=0.03, d=2, n_causal=6, effect=1.0, seed=31)
+    iids = [f"s{i}" for i in range(n)]
+    main = str(tmp_path / "main")
+    _write_panel(main, pan.g, iids)
+    with open(tmp_path / "cov.tsv", "w") as f:
+        f.write("id\ta\tb\n")
+        for i in range(n):
+            f.write(f"{iids[i]}\t{float(pan.x[i, 0])!r}\t{float(pan.x[i, 1])!r}\n")
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(n)
+    est_snps = np.ascontiguousarray(pan.g[:, ::3])
+    extra = synth.balding_nichols(4, est_snps.shape[1], n_pops=1, miss_rate=0.0, d=1, n_causal=0, seed=9).g
+    est = str(tmp_path / "est")
+    _write_panel(est, np.concatenate([est_snps[perm], extra], axis=0),
+                 [iids[i] for i in perm] + [f"x{k}" for k in range(4)])
+    out = str(tmp_path / "res")
+    rc = cli.main(["-b", main + ".bed", "-c", str(tmp_path / "cov.tsv"), "-k", "3", "-o", out, "-t", "2",
+                   "--est-bed", est + ".bed", "--norm", "center-only", "--scale-cov", "-l", "0.001",
+                   "--oversampling", "6", "--precision", "fp64"])
+    err = capsys.readouterr().err
+    assert rc == 0, err
+    assert f"  {n} samples x {est_snps.shape[1]} SNPs (for factor estimation)" in err.splitlines()
+
+    idx = S.align_est_bed_samples(S.read_fam(est + ".fam"), S.read_fam(main + ".fam"))
+    cfg = S.Lfmm2Config(k=3, lam=0.001, oversampling=6, norm=S.CENTER_ONLY, scale_cov=True)
+    g = S.BedFile.open(main + ".bed")
+    ge = S.BedFile.open(est + ".bed", sample_keep=idx)
+    ref = S.fit_lfmm2(ge, None, g, pan.x, cfg)
+    want = str(tmp_path / "want.tsv")
+    S.write_results(want, ref, main + ".bed", cov_names=["a", "b"])
+    assert open(out + ".tsv").read() == open(want).read()
+    assert "lambda: 0.001\n" in open(out + ".summary.txt").read()
