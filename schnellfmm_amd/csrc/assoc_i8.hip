@@ -208,10 +208,14 @@ struct Frag {
 constexpr uint32_t kPermG0 = 0x02010000u;     // 00->0, 01->0, 10->1, 11->2
 constexpr uint32_t kPermMiss = 0x00000100u;   // 01->1, else 0
 
-__device__ __forceinline__ Frag decode_word(uint32_t w, uint32_t& c_hom, uint32_t& c_miss) {
-  const uint32_t sh = w >> 1;
-  c_hom += __popc(w & sh & 0x55555555u);
-  c_miss += __popc(w & ~sh & 0x55555555u);
+// Per-word counts for the epilogue, three instructions per 16 codes:
+// popc(w) sums b0 + b1 over the codes (00->0, 01->1, 10->1, 11->2), which
+// exceeds the dosage (0, 0, 1, 2) by one exactly at missing calls; popc of the
+// high bits counts 10 and 11.  With the exact dosage sum sg of the all-ones
+// operand row, n_miss = pop - sg and n_hom = sg - hi.
+__device__ __forceinline__ Frag decode_word(uint32_t w, uint32_t& c_pop, uint32_t& c_hi) {
+  c_pop += __popc(w);
+  c_hi += __popc(w & 0xAAAAAAAAu);
   Frag f;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -287,14 +291,14 @@ __device__ __forceinline__ const Recomb* stage_recomb(const Recomb* src, unsigne
 }
 
 template <int ST, int NT, int MAXM, int MODE>
-__device__ __forceinline__ void gt_epilogue(const GtArgs& a, const v4i (&acc)[ST][NT], uint32_t (&c_hom)[ST],
-                                            uint32_t (&c_miss)[ST], int* red, const double* plan, const Recomb* rc,
+__device__ __forceinline__ void gt_epilogue(const GtArgs& a, const v4i (&acc)[ST][NT], uint32_t (&c_pop)[ST],
+                                            uint32_t (&c_hi)[ST], int* red, const double* plan, const Recomb* rc,
                                             uint64_t base, int g, int l16) {
   constexpr int NQ = epi_nq<MAXM, MODE>();
 #pragma unroll
   for (int st = 0; st < ST; ++st) {
-    c_hom[st] += __shfl_xor(c_hom[st], 16); c_hom[st] += __shfl_xor(c_hom[st], 32);
-    c_miss[st] += __shfl_xor(c_miss[st], 16); c_miss[st] += __shfl_xor(c_miss[st], 32);
+    c_pop[st] += __shfl_xor(c_pop[st], 16); c_pop[st] += __shfl_xor(c_pop[st], 32);
+    c_hi[st] += __shfl_xor(c_hi[st], 16); c_hi[st] += __shfl_xor(c_hi[st], 32);
   }
   const int ones = rc->ones;
 #pragma unroll
@@ -306,9 +310,9 @@ __device__ __forceinline__ void gt_epilogue(const GtArgs& a, const v4i (&acc)[ST
     asm volatile("" ::: "memory");   // the wave's sums are written before any lane reads them
     const uint64_t j = base + 16 * st + l16;
     const bool valid = j < a.p;
-    const int64_t nmiss = c_miss[st], nhom = c_hom[st];
-    const int64_t nobs = (int64_t)a.n - nmiss;
     const int64_t sg = red[ones * 16 + l16];   // dosage sum (all-ones operand row)
+    const int64_t nmiss = (int64_t)c_pop[st] - sg, nhom = sg - (int64_t)c_hi[st];
+    const int64_t nobs = (int64_t)a.n - nmiss;
     const double mu = nobs > 0 ? (double)sg / (double)nobs : 0.0;
     double fq[NQ];   // functionals past m have zero weights in the table: fq = 0
 #pragma unroll
@@ -364,9 +368,9 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
   for (int st = 0; st < ST; ++st)
 #pragma unroll
     for (int rt = 0; rt < NT; ++rt) acc[st][rt] = v4i{0, 0, 0, 0};
-  uint32_t c_hom[ST], c_miss[ST];
+  uint32_t c_pop[ST], c_hi[ST];
 #pragma unroll
-  for (int st = 0; st < ST; ++st) c_hom[st] = c_miss[st] = 0;
+  for (int st = 0; st < ST; ++st) c_pop[st] = c_hi[st] = 0;
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
 
@@ -393,7 +397,7 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
     for (int s = 0; s < KS; ++s) {
       Frag f[ST];
 #pragma unroll
-      for (int st = 0; st < ST; ++st) f[st] = decode_word(gw[st][s], c_hom[st], c_miss[st]);
+      for (int st = 0; st < ST; ++st) f[st] = decode_word(gw[st][s], c_pop[st], c_hi[st]);
 #pragma unroll
       for (int rt = 0; rt < NT; ++rt) {
         const v4i af = Ab[(s * NT + rt) * 64 + lane];
@@ -421,7 +425,7 @@ __global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
   }
   const Recomb* rc = stage_recomb(a.rc, lds + LDS_RED + PLAN_MAX * 8, t, 256);
   __syncthreads();
-  gt_epilogue<ST, NT, MAXM, MODE>(a, acc, c_hom, c_miss, reinterpret_cast<int*>(lds) + wave * (NT * 256), plan, rc,
+  gt_epilogue<ST, NT, MAXM, MODE>(a, acc, c_pop, c_hi, reinterpret_cast<int*>(lds) + wave * (NT * 256), plan, rc,
                                   base, g, l16);
 }
 
@@ -492,7 +496,7 @@ template <int NT0, int NT, int ST>
 __device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, const PipeCtx& pc, int wave, int lane, int g,
                                            const unsigned char* a_rd, unsigned char* a_wr,
                                            const unsigned char* g_rd, unsigned char* g_wr,
-                                           v4i (&acc)[ST][NT], uint32_t (&c_hom)[ST], uint32_t (&c_miss)[ST]) {
+                                           v4i (&acc)[ST][NT], uint32_t (&c_pop)[ST], uint32_t (&c_hi)[ST]) {
   constexpr int KS = 4;
   constexpr int W = kPipeWaves;
   constexpr int CH = KS * NT;
@@ -537,7 +541,7 @@ __device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, const PipeCt
   for (int c = 0; c < D; ++c) af[c] = Ab[c * 64 + lane];
   Frag fc[ST];
 #pragma unroll
-  for (int st = 0; st < ST; ++st) fc[st] = decode_word(gw[st][0], c_hom[st], c_miss[st]);
+  for (int st = 0; st < ST; ++st) fc[st] = decode_word(gw[st][0], c_pop[st], c_hi[st]);
 #pragma unroll
   for (int s = 0; s < KS; ++s) {
 #pragma unroll
@@ -551,7 +555,7 @@ __device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, const PipeCt
     }
     if (s + 1 < KS) {
 #pragma unroll
-      for (int st = 0; st < ST; ++st) fc[st] = decode_word(gw[st][s + 1], c_hom[st], c_miss[st]);
+      for (int st = 0; st < ST; ++st) fc[st] = decode_word(gw[st][s + 1], c_pop[st], c_hi[st]);
     }
   }
 }
@@ -586,9 +590,9 @@ __global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
   for (int st = 0; st < ST; ++st)
 #pragma unroll
     for (int rt = 0; rt < NT; ++rt) acc[st][rt] = v4i{0, 0, 0, 0};
-  uint32_t c_hom[ST], c_miss[ST];
+  uint32_t c_pop[ST], c_hi[ST];
 #pragma unroll
-  for (int st = 0; st < ST; ++st) c_hom[st] = c_miss[st] = 0;
+  for (int st = 0; st < ST; ++st) c_pop[st] = c_hi[st] = 0;
   // Prologue in the order the blocks issue them: operand 0 and genotypes 0,
   // then genotypes 1.  The first block's wait leaves the genotypes of block 1 in flight.
   for (int c = wave; c < CH; c += W) buf_to_lds(pc.opnd, c * 1024 + 16 * lane, A0 + c * 1024);
@@ -601,7 +605,7 @@ __global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
       buf_to_lds(pc.geno, pc.gvoff + st * pc.tstride + src, G1 + wave * (ST * 1024) + st * 1024);
   }
 #define SLFMM_PIPE(PH, AR_, AW_, GR_, GW_) \
-  if (kb + PH < a.nblk) pipe_block<NT0, NT, ST>(kb + PH, a, pc, wave, lane, g, AR_, AW_, GR_, GW_, acc, c_hom, c_miss);
+  if (kb + PH < a.nblk) pipe_block<NT0, NT, ST>(kb + PH, a, pc, wave, lane, g, AR_, AW_, GR_, GW_, acc, c_pop, c_hi);
   for (int kb = 0; kb < a.nblk; kb += 6) {
     SLFMM_PIPE(0, A0, A1, G0, G2)
     SLFMM_PIPE(1, A1, A0, G1, G0)
@@ -627,7 +631,7 @@ __global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
   }
   const Recomb* rc = stage_recomb(a.rc, A1, t, 512);
   __syncthreads();
-  gt_epilogue<ST, NT, MAXM, MODE>(a, acc, c_hom, c_miss, red, plan, rc, base, g, l16);
+  gt_epilogue<ST, NT, MAXM, MODE>(a, acc, c_pop, c_hi, red, plan, rc, base, g, l16);
 }
 
 #define SLFMM_GT_CASES(X) \
