@@ -525,7 +525,11 @@ __device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, const PipeCt
   // the ring slots written below are not the ones read in this block
   if (kb + 1 < a.nblk) {
     const uint32_t src = (uint32_t)(kb + 1) * CH * 1024 + 16 * lane;
-    for (int c = wave; c < CH; c += W) buf_to_lds(pc.opnd, src + c * 1024, a_wr + c * 1024);
+#pragma unroll
+    for (int i = 0; i < (CH + W - 1) / W; ++i) {   // fragments wave, wave + W, ... (compile-time trip count)
+      const int c = wave + i * W;
+      if (c < CH) buf_to_lds(pc.opnd, src + c * 1024, a_wr + c * 1024);
+    }
   }
   {
     const uint32_t src = (uint32_t)(kb + 2 < a.nblk ? kb + 2 : 0) * 64;
