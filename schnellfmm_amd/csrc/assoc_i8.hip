@@ -725,26 +725,32 @@ void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_
 template <int NCB, int KS>
 __global__ void k_pack_cols(const double* __restrict__ Qz, uint64_t ldq, uint64_t pe, int l,
                             const double* __restrict__ ssnp, const double* __restrict__ mu,
-                            const double* __restrict__ csc, uint32_t total, int8_t* __restrict__ out) {
-  constexpr int NTB = 4 * NCB;
-  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
-    const int j = (int)(e & 15), lane = (int)((e >> 4) & 63);
-    const uint32_t frag = e >> 10;
-    const int ct = (int)(frag % NTB);
-    const uint32_t ks = frag / NTB;
-    const int s = (int)(ks % KS);
-    const uint64_t blk = ks / KS;
-    const int v = j >> 2, c = j & 3, g = lane >> 4;
-    const uint64_t snp = blk * 64 * KS + 16 * KS * g + 16 * s + 4 * c + v;
-    const int col = 16 * (ct % NCB) + (lane & 15);
-    const int term = (ct / NCB) & 1, part = ct / (2 * NCB);
-    int val = 0;
+                            const double* __restrict__ csc, uint32_t nsnp, int8_t* __restrict__ out) {
+  // One thread per (SNP, column) of the padded block grid (nsnp = nblk 64 KS rows,
+  // 16 NCB columns): both parts (Qt, mu Qt) and both terms are written from
+  // one load of Q_z.  Byte (part, term) goes to fragment ct = part 2 NCB +
+  // term NCB + col / 16 of k-step (blk, s), lane g 16 + col % 16, byte j.
+  constexpr int NTB = 4 * NCB, NC = 16 * NCB;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < nsnp * NC; e += gridDim.x * blockDim.x) {
+    const int col = (int)(e % NC);
+    const uint32_t snp = e / NC;
+    const uint32_t blk = snp / (64 * KS), r = snp % (64 * KS);
+    const int g = (int)(r / (16 * KS)), s = (int)((r / 16) % KS), r3 = (int)(r % 16);
+    const int j = (r3 & 3) * 4 + (r3 >> 2);
+    int q[2][2] = {{0, 0}, {0, 0}};
     if (col < l && snp < pe) {
-      double x = Qz[snp * ldq + col] * ssnp[snp];
-      if (part) x *= mu[snp];
-      val = quant_term(x / csc[part * l + col], term);
+      const double x0 = Qz[(uint64_t)snp * ldq + col] * ssnp[snp];
+      const double x1 = x0 * mu[snp];
+      const double y0 = x0 / csc[col], y1 = x1 / csc[l + col];
+      q[0][0] = quant_term(y0, 0); q[0][1] = quant_term(y0, 1);
+      q[1][0] = quant_term(y1, 0); q[1][1] = quant_term(y1, 1);
     }
-    out[e] = (int8_t)val;
+    const uint64_t base = (uint64_t)(blk * KS + s) * NTB * 1024 + (uint64_t)(g * 16 + col % 16) * 16 + j;
+#pragma unroll
+    for (int part = 0; part < 2; ++part)
+#pragma unroll
+      for (int term = 0; term < 2; ++term)
+        out[base + (uint64_t)(part * 2 * NCB + term * NCB + col / 16) * 1024] = (int8_t)q[part][term];
   }
 }
 
@@ -897,11 +903,12 @@ void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_
   const uint64_t total = nblk * KS * NTB * 1024;
   int8_t* Bp = ctx->buf<int8_t>("fw_pack", total);
   SLFMM_CHECK(total < 0xFFFFFFFFull, SLFMM_E_ARG, "estimation subset too large for the int8 forward pass");
-  const unsigned pgrid = (unsigned)std::min<uint64_t>(cdiv(total, 256), 16384);
+  const uint64_t nsnp = nblk * 64 * KS;   // every (SNP, column) slot of the padded blocks, 16 NCB columns
+  const unsigned pgrid = (unsigned)std::min<uint64_t>(cdiv(nsnp * 16 * NCB, 256), 16384);
   if (NCB == 1)
-    k_pack_cols<1, 4><<<pgrid, 256, 0, s>>>(Qz, ldq, pe, (int)l, ssnp, mu, csc, (uint32_t)total, Bp);
+    k_pack_cols<1, 4><<<pgrid, 256, 0, s>>>(Qz, ldq, pe, (int)l, ssnp, mu, csc, (uint32_t)nsnp, Bp);
   else
-    k_pack_cols<2, 4><<<pgrid, 256, 0, s>>>(Qz, ldq, pe, (int)l, ssnp, mu, csc, (uint32_t)total, Bp);
+    k_pack_cols<2, 4><<<pgrid, 256, 0, s>>>(Qz, ldq, pe, (int)l, ssnp, mu, csc, (uint32_t)nsnp, Bp);
   double* vhat = ctx->buf<double>("fw_vhat", l);
   colreduce<1, kColSum>(ctx, FnVhat{Qz, ldq, ssnp, mu, csc}, GStore1{vhat}, pe, (int)l);
   const uint64_t sblocks = cdiv(n, 64 * kFwdWaves);
