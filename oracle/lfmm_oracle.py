@@ -19,10 +19,16 @@ Pinning (what checks this oracle, see tests/test_oracle_golden.py):
   * rsvd                             -- exact SVD of M.Y on tiny inputs (HMT
                                         bound) and QR invariants src/rsvd.rs:185-233.
   * association                      -- brute-force per-SNP lstsq on [1|X|U],
-                                        closed-form ridge, r2 partition identity.
-  * NMF                              -- dense multiplicative updates on tiny
-                                        inputs (src/nmf.rs:530-585 thresholds)
-                                        plus a brute-force dense restatement.
+                                        closed-form ridge; r2 against the
+                                        Type-I sums of squares of nested lstsq
+                                        fits (tests/test_oracle_nmf_pin.py).
+  * NMF (nmf_impute_estimation,      -- per-entry brute-force loops in the
+    init_from_random_probe,             reference's order (src/nmf.rs:108-232,
+    compute_cv_error_masked,            :279-362, :379-455; src/testing.rs:
+    _gwas_cv_chunk)                     423-487), each paired with mutations
+                                        that must move the result
+                                        (tests/test_oracle_nmf_pin.py); dense
+                                        NMF thresholds src/nmf.rs:530-585.
 Parity unpinned (no reference vector exists in this environment):
   * the exact ChaCha8/rand_distr random streams (src/rsvd.rs:44-46,
     src/nmf.rs:291-293, :433-434, src/testing.rs:437-438).  The oracle takes
