@@ -3,18 +3,17 @@
 // Genotype codes are exact small integers, so they enter the matrix cores
 // as int8 with no rounding: per 16 samples of one SNP the lane decodes one
 // 32-bit word of packed codes into two int8 operand fragments,
-//   g0   = dosage with missing -> 0   (00->0, 01->0, 10->1, 11->2)
-//   miss = missing indicator          (01->1, else 0)
+//   g'   = scaled dosage, missing -> fill code  (00->0, 01->c, 10->S, 11->2S)
+//   miss = missing indicator                    (01->1, else 0)
 // and v_mfma_i32_16x16x64_i8 accumulates exactly in int32.  The dense
 // operand (an orthonormal basis Q of [U_c|X] for the association, the
 // sketch B for the RSVD) is carried as 2 or 3 int8 terms per entry with a
 // per-column scale, Q ~ sc (t0 + t1/254 + t2/254^2) (about 15 / 22
-// significant bits).  Per SNP and functional i, with G_t, M_t, O_t the int32
-// sums of term t against g0, miss and 1:
-//   f_i = sum_t w_t (G_t - mu O_t) + mu sum_t w_t M_t,   w_t = sc_i 254^-t
-// is the functional of the centred column (g - mu on observed calls, 0 on
-// missing calls; src/bed.rs:339-343), and the FP64 per-SNP epilogue of the
-// association pass runs on it unchanged.
+// significant bits) against g', and as its leading 1 or 2 terms against the
+// missing indicator.  The recombined functional of every SNP is that of the
+// centred column (g - mu on observed calls, 0 on missing calls;
+// src/bed.rs:339-343), and the FP64 per-SNP epilogue of the association pass
+// runs on it unchanged.
 //
 // Operand fragments: for a block of 64*KS samples, lane group g = lane/16
 // of a wave loads KS consecutive words of its SNP row, word s serving k-step
@@ -47,39 +46,78 @@ __device__ __forceinline__ int quant_term(double q, int t) {
   return clamp127(rint((r1 - md) * 254.0));
 }
 
-// Row layout of a packed operand with m functionals: columns i < k3 carry
-// 2 int8 terms, columns i >= k3 carry 3, on the g0 side (rows 0 .. 16 NT0);
-// every column carries 2 terms on the miss side (rows 16 NT0 ..).
-// The g0 block ends with one all-ones row, whose accumulator is the dosage
-// sum of the SNP over its observed calls.
+// Row layout of the packed operand of the SNP-side kernels.  The result of
+// v_mfma_i32_16x16x64_i8 row 16 rt + 4 g + r lands in register r of lane
+// group g, so functional i (a column of the dense operand) is given to lane
+// group g = i % 4 at position q = i / 4, and every int32 sum it needs ends
+// in the registers of the lanes that recombine it (no transpose through
+// LDS).  Within a lane group the rows of tile rt are slots s = 4 rt + r:
+//   g' tiles (rt < NT0):   terms 0 and 1 of position q at slots 2q, 2q + 1;
+//                          term 2 (columns i >= k3) at slot 3 NQ - 1 - q
+//   miss tiles (NT1):      term 0 of position q at slot q; term 1 (columns
+//                          i >= km) at slot 2 NQ - 1 - q.
+// One all-ones row sits at the last slot of lane group g1: in a miss tile
+// (slot 4 NT1 - 1, the missing count) or, when the miss tiles are full, in a
+// g' tile (slot 4 NT0 - 1, the sum of g').  All slot indices are
+// compile-time functions of q, so the accumulators are addressed statically.
 struct RowMap {
-  int m, k3, NT0, NT1;
-  __host__ __device__ static RowMap make(int m, int k3) {
-    RowMap r{m, k3, 0, 0};
-    r.NT0 = (2 * k3 + 3 * (m - k3) + 1 + 15) / 16;
-    r.NT1 = (2 * m + 15) / 16;
+  int m, k3, km, NQ, NT0, NT1, g1, ones_g;
+  static RowMap make(int m, int k3, int km) {
+    RowMap r{m, k3, km, (m + 3) / 4, 0, 0, 0, 0};
+    int topG = 0, topM = 0, tgs[4], tms[4];
+    for (int g = 0; g < 4; ++g) {
+      const int nq = g < m ? (m - g + 3) / 4 : 0;
+      int qx = nq, qm = nq;   // first positions holding a column >= k3, >= km
+      for (int q = nq - 1; q >= 0; --q) {
+        if (g + 4 * q >= k3) qx = q;
+        if (g + 4 * q >= km) qm = q;
+      }
+      int tg = 2 * nq - 1, tm = nq - 1;
+      if (qx < nq) tg = std::max(tg, 3 * r.NQ - 1 - qx);
+      if (qm < nq) tm = std::max(tm, 2 * r.NQ - 1 - qm);
+      tgs[g] = tg; tms[g] = tm;
+      topG = std::max(topG, tg);
+      topM = std::max(topM, tm);
+    }
+    r.NT0 = (topG + 4) / 4;
+    r.NT1 = (topM + 4) / 4;
+    for (int g = 0; g < 4; ++g)
+      if (tms[g] < 4 * r.NT1 - 1) { r.g1 = g; return r; }
+    for (int g = 0; g < 4; ++g)
+      if (tgs[g] < 4 * r.NT0 - 1) { r.g1 = g; r.ones_g = 1; return r; }
+    ++r.NT1;   // room for the ones row
     return r;
   }
-  __host__ __device__ int ones() const { return 2 * k3 + 3 * (m - k3); }
   // (column, term) of packed row `row`: column -1 for padding, -2 for the ones row
   __device__ int2 info(int row) const {
-    if (row < 16 * NT0) {
-      if (row < 2 * k3) return make_int2(row / 2, row % 2);
-      const int r = row - 2 * k3;
-      if (r < 3 * (m - k3)) return make_int2(k3 + r / 3, r % 3);
-      return row == ones() ? make_int2(-2, 0) : make_int2(-1, 0);
+    const int rt = row >> 4, g = (row >> 2) & 3, r = row & 3;
+    if (rt < NT0) {
+      const int s = 4 * rt + r;
+      if (ones_g && g == g1 && s == 4 * NT0 - 1) return make_int2(-2, 0);
+      if (s < 2 * NQ) {
+        const int i = g + 4 * (s >> 1);
+        return i < m ? make_int2(i, s & 1) : make_int2(-1, 0);
+      }
+      const int q = 3 * NQ - 1 - s, i = g + 4 * q;
+      return q >= 0 && i < m && i >= k3 ? make_int2(i, 2) : make_int2(-1, 0);
     }
-    const int r = row - 16 * NT0;
-    return r < 2 * m ? make_int2(r / 2, r % 2) : make_int2(-1, 0);
+    const int s = 4 * (rt - NT0) + r;
+    if (!ones_g && g == g1 && s == 4 * NT1 - 1) return make_int2(-2, 0);
+    if (s < NQ) {
+      const int i = g + 4 * s;
+      return i < m ? make_int2(i, 0) : make_int2(-1, 0);
+    }
+    const int q = 2 * NQ - 1 - s, i = g + 4 * q;
+    return q >= 0 && i < m && i >= km ? make_int2(i, 1) : make_int2(-1, 0);
   }
-  __host__ __device__ int ng(int i) const { return i < k3 ? 2 : 3; }
-  __host__ __device__ int rg(int i, int t) const { return i < k3 ? 2 * i + t : 2 * k3 + 3 * (i - k3) + t; }
-  __host__ __device__ int rm(int i, int t) const { return 16 * NT0 + 2 * i + t; }
 };
 
 // Apack for the SNP-side kernels: rows = functional terms, K = samples.
+// Fragment (blk, s, rt) is 1 KB: lane l holds row 16 rt + l % 16, byte 4v + c
+// = sample 256 blk + 64 (l / 16) + 16 s + 4 c + v.
 __global__ void k_pack_rows(const double* __restrict__ Q, uint64_t ldq, uint64_t n, RowMap rmap,
-                            const double* __restrict__ sc, int NT, int KS, uint64_t total, int8_t* __restrict__ out) {
+                            const double* __restrict__ sc, int NT, uint64_t total, int8_t* __restrict__ out) {
+  constexpr int KS = 4;
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
     const int j = (int)(e & 15), lane = (int)((e >> 4) & 63);
     const uint64_t frag = e >> 10;                 // (blk*KS + s)*NT + rt
@@ -116,79 +154,68 @@ struct GStore3 {
   }
 };
 
-// Row recombination table of one packed operand (device copy).
-struct Recomb {
-  int m;
-  int ones;                // row of the all-ones operand row (dosage sum)
-  int ng[kMaxM];           // terms on the g0 side (2 or 3)
-  int rg[kMaxM][3];        // their rows
-  int rm[kMaxM][2];        // rows of the two terms on the miss side
-  // With X = sum_t G_t 254^(ng-1-t), Y = M_0 254 + M_1 (exact integers):
-  //   f_i = wg_i X + mu (wm_i Y - C_i),   C_i = sum_t sc_i 254^-t O_t
-  double wg[kMaxM];        // sc_i / 254^(ng-1)
-  double wm[kMaxM];        // sc_i / 254
-  double C[kMaxM];
-};
-
-__global__ void k_recomb(RowMap rmap, const double* __restrict__ sc, const double* __restrict__ O, Recomb* rc) {
+// Recombination weights of one packed operand, [3][32] doubles:
+// wg_i = sc_i 254^-(ng-1) / S,  wm_i = sc_i 254^-(nm-1),
+// C_i  = sc_i sum_{t < ng} 254^-t O_t   (ng = 3 for i >= k3, else 2; nm = 2 for
+// i >= km, else 1; zero for i >= m).
+constexpr int kWtab = 3 * 32;
+__global__ void k_weights(int m, int k3, int km, double S, const double* __restrict__ sc, const double* __restrict__ O,
+                          double* __restrict__ w) {
   const int i = threadIdx.x;
-  if (i == 0) {
-    rc->m = rmap.m;
-    rc->ones = rmap.ones();
+  if (i >= 32) return;
+  double wg = 0.0, wm = 0.0, C = 0.0;
+  if (i < m) {
+    const bool x = i >= k3;
+    wg = sc[i] * (x ? 1.0 / 64516.0 : 1.0 / 254.0) / S;
+    wm = sc[i] * (i >= km ? 1.0 / 254.0 : 1.0);
+    double c = O[i * 3 + 0] + O[i * 3 + 1] / 254.0;
+    if (x) c += O[i * 3 + 2] / 64516.0;
+    C = sc[i] * c;
   }
-  if (i >= kMaxM) return;
-  if (i >= rmap.m) {   // padding functionals: zero weights on valid rows, so f_i = 0
-    rc->ng[i] = 2;
-    for (int t = 0; t < 3; ++t) rc->rg[i][t] = 0;
-    rc->rm[i][0] = rc->rm[i][1] = 0;
-    rc->wg[i] = rc->wm[i] = rc->C[i] = 0.0;
-    return;
-  }
-  const int ng = rmap.ng(i);
-  rc->ng[i] = ng;
-  for (int t = 0; t < 3; ++t) rc->rg[i][t] = t < ng ? rmap.rg(i, t) : -1;
-  rc->rm[i][0] = rmap.rm(i, 0);
-  rc->rm[i][1] = rmap.rm(i, 1);
-  rc->wg[i] = sc[i] * (ng == 3 ? 1.0 / 64516.0 : 1.0 / 254.0);
-  rc->wm[i] = sc[i] / 254.0;
-  double c = O[i * 3 + 0] + O[i * 3 + 1] / 254.0;
-  if (ng == 3) c += O[i * 3 + 2] / 64516.0;
-  rc->C[i] = sc[i] * c;
+  w[i] = wg;
+  w[32 + i] = wm;
+  w[64 + i] = C;
 }
 
 struct Operand {
   int8_t* pack = nullptr;
-  int NT0 = 0, NT1 = 0, KS = 4;
+  RowMap rmap{};
   uint64_t nblk = 0;
-  Recomb* rc = nullptr;
+  double* wtab = nullptr;
+  int S = 1;
+  int fill = 0;   // per-SNP fill code in the g' fragment
 };
 
-static int ks_for(int NT) { return NT <= 8 ? 4 : 2; }   // k-steps per staged block of the SNP-side kernels
+// Scale of the dosage in the g' fragment (00 -> 0, 01 -> c, 10 -> S, 11 -> 2S):
+// the int32 sums stay below 2^31 while 127 * 2S * n does.
+static int fill_scale(uint64_t n) { return n <= 2000000 ? 4 : (n <= 4000000 ? 2 : 1); }
 
-// Quantise Q (n x m, ldq) into the packed fragment layout; columns from k3
-// on carry 3 terms on the g0 side.  Device work only (no host round trip).
+// Quantise Q (n x m, ldq) into the packed fragment layout: columns from k3
+// on carry 3 terms on the g' side, columns from km on 2 terms on the miss
+// side.  With fill, the g' fragment carries the per-SNP fill code.  Device
+// work only.
 static Operand prepare_operand(slfmm_ctx* ctx, const char* tag, const double* Q, uint64_t ldq, uint64_t n, int m,
-                               int k3) {
+                               int k3, int km, bool fill) {
   hipStream_t s = ctx->stream;
-  SLFMM_CHECK(m >= 1 && m <= kMaxM && k3 >= 0 && k3 <= m, SLFMM_E_ARG, "operand shape");
+  SLFMM_CHECK(m >= 1 && m <= 32 && k3 >= 0 && k3 <= m, SLFMM_E_ARG, "operand shape");
+  SLFMM_CHECK(n <= 8000000, SLFMM_E_ARG, "too many samples for the int8 kernels (use SLFMM_PREC_FP64)");
   Operand op;
   std::string T(tag);
-  const RowMap rmap = RowMap::make(m, k3);
-  op.NT0 = rmap.NT0;
-  op.NT1 = rmap.NT1;
-  const int NT = op.NT0 + op.NT1;
-  op.KS = ks_for(NT);
-  op.nblk = cdiv(n, 64 * op.KS);
+  op.rmap = RowMap::make(m, k3, km);
+  op.fill = fill ? 1 : 0;
+  op.S = fill ? fill_scale(n) : 1;
+  const int NT = op.rmap.NT0 + op.rmap.NT1;
+  op.nblk = cdiv(n, 256);
   double* sc = ctx->buf<double>(T + "_sc", m);
   colabsmax_scale(ctx, Q, ldq, n, m, nullptr, nullptr, sc);
   double* O = ctx->buf<double>(T + "_O", m * 3);
   colreduce<3, kColSum>(ctx, FnTerms{Q, ldq, sc}, GStore3{O}, n, m);
-  op.rc = ctx->buf<Recomb>(T + "_rc", 1);
-  k_recomb<<<1, kMaxM, 0, s>>>(rmap, sc, O, op.rc);
-  const uint64_t total = op.nblk * op.KS * NT * 1024;
+  op.wtab = ctx->buf<double>(T + "_w", kWtab);
+  k_weights<<<1, 32, 0, s>>>(m, k3, km, (double)op.S, sc, O, op.wtab);
+  const uint64_t total = op.nblk * 4 * NT * 1024;
   op.pack = ctx->buf<int8_t>(T + "_pack", total);
-  k_pack_rows<<<(unsigned)std::min<uint64_t>(cdiv(total, 256), 16384), 256, 0, s>>>(Q, ldq, n, rmap, sc, NT, op.KS,
-                                                                                  total, op.pack);
+  k_pack_rows<<<(unsigned)std::min<uint64_t>(cdiv(total, 256), 16384), 256, 0, s>>>(Q, ldq, n, op.rmap, sc, NT, total,
+                                                                                  op.pack);
   SLFMM_HIP(hipGetLastError());
   return op;
 }
@@ -203,16 +230,13 @@ struct Frag {
 // One word = 16 two-bit codes of one SNP.  Register v of a fragment takes
 // the codes at bit 8c + 2v (c = 0..3): one shift and mask leave one code per
 // byte, and v_perm_b32 maps each byte through a 4-entry table (selector
-// bytes 0..3 pick bytes of the second operand).  Counts: code 11 (hom-alt)
-// and code 01 (missing); the dosage sum comes from the all-ones operand row.
+// bytes 0..3 pick bytes of the second operand).
 constexpr uint32_t kPermG0 = 0x02010000u;     // 00->0, 01->0, 10->1, 11->2
 constexpr uint32_t kPermMiss = 0x00000100u;   // 01->1, else 0
 
-// Per-word counts for the epilogue, three instructions per 16 codes:
-// popc(w) sums b0 + b1 over the codes (00->0, 01->1, 10->1, 11->2), which
-// exceeds the dosage (0, 0, 1, 2) by one exactly at missing calls; popc of the
-// high bits counts 10 and 11.  With the exact dosage sum sg of the all-ones
-// operand row, n_miss = pop - sg and n_hom = sg - hi.
+// Plain dosage / missing fragments (RSVD forward pass) with the per-word
+// counts popc(w) (dosage + 1 per missing call) and popc of the high bits
+// (het + hom).
 __device__ __forceinline__ Frag decode_word(uint32_t w, uint32_t& c_pop, uint32_t& c_hi) {
   c_pop += __popc(w);
   c_hi += __popc(w & 0xAAAAAAAAu);
@@ -239,22 +263,6 @@ __device__ __forceinline__ void load_words(const uint8_t* p, uint32_t (&w)[KS]) 
   }
 }
 
-// ---------------------------------------------------------------------------
-// SNP-side kernel: functionals of every SNP row against a packed operand.
-// MODE 0: fused association epilogue;  MODE 1: store s_c * f (RSVD sketch).
-// Workgroup: 4 waves x 4 SNP tiles of 16 SNPs (256 SNPs).
-// ---------------------------------------------------------------------------
-struct GtArgs {
-  const uint8_t* packed;
-  uint64_t ld, p, n;
-  int nblk, m, norm;
-  const v4i* A;
-  const Recomb* rc;
-  EpiArgs ea;
-  double* Z;
-  uint64_t ldZ;
-};
-
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
 // Asynchronous copy of one 1 KB fragment chunk (64 lanes x 16 B) from global
@@ -264,196 +272,17 @@ __device__ __forceinline__ void chunk_to_lds(const unsigned char* src_chunk, uns
   __builtin_amdgcn_global_load_lds((const void*)(src_chunk + 16 * lane), (lds_ptr_t)(dst_chunk), 16, 0, 0);
 }
 
-// ---------------------------------------------------------------------------
-// Epilogue of the SNP-side kernels.  After the block loop, lane (g, l16) of
-// a wave holds rows 16 rt + 4 g + r of SNP column l16 of every tile st.  Per
-// tile the wave stages its int32 row sums in its own `red` area ([16 NT
-// rows][16 SNPs]); then every lane recombines the functionals i = g + 4 q of
-// SNP j = base + 16 st + l16 (all 64 lanes busy), and either stores them
-// scaled (MODE 1, RSVD sketch) or feeds the split per-SNP epilogue (MODE 0).
-// `red` is private to the wave and a wave's LDS accesses complete in order,
-// so the tiles need no workgroup barrier.  The recombination table is read
-// from its LDS copy.
-// ---------------------------------------------------------------------------
-// functionals per lane group: MAXM / 4 for the association (MAXM = 24 or 32),
-// 8 for the RSVD sketch (l <= 32)
-template <int MAXM, int MODE>
-constexpr int epi_nq() { return MODE == 1 ? 8 : MAXM / 4; }
-
-// Copies the recombination table to LDS (all threads of the workgroup take
-// part; the caller synchronises before reading it).
-__device__ __forceinline__ const Recomb* stage_recomb(const Recomb* src, unsigned char* dst, int t, int nthreads) {
-  static_assert(sizeof(Recomb) % 4 == 0, "table is copied as words");
-  const int* s = reinterpret_cast<const int*>(src);
-  int* d = reinterpret_cast<int*>(dst);
-  for (int e = t; e < (int)(sizeof(Recomb) / 4); e += nthreads) d[e] = s[e];
-  return reinterpret_cast<const Recomb*>(dst);
-}
-
-template <int ST, int NT, int MAXM, int MODE>
-__device__ __forceinline__ void gt_epilogue(const GtArgs& a, const v4i (&acc)[ST][NT], uint32_t (&c_pop)[ST],
-                                            uint32_t (&c_hi)[ST], int* red, const double* plan, const Recomb* rc,
-                                            uint64_t base, int g, int l16) {
-  constexpr int NQ = epi_nq<MAXM, MODE>();
-#pragma unroll
-  for (int st = 0; st < ST; ++st) {
-    c_pop[st] += __shfl_xor(c_pop[st], 16); c_pop[st] += __shfl_xor(c_pop[st], 32);
-    c_hi[st] += __shfl_xor(c_hi[st], 16); c_hi[st] += __shfl_xor(c_hi[st], 32);
-  }
-  const int ones = rc->ones;
-#pragma unroll
-  for (int st = 0; st < ST; ++st) {
-#pragma unroll
-    for (int rt = 0; rt < NT; ++rt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) red[(16 * rt + 4 * g + r) * 16 + l16] = acc[st][rt][r];
-    asm volatile("" ::: "memory");   // the wave's sums are written before any lane reads them
-    const uint64_t j = base + 16 * st + l16;
-    const bool valid = j < a.p;
-    const int64_t sg = red[ones * 16 + l16];   // dosage sum (all-ones operand row)
-    const int64_t nmiss = (int64_t)c_pop[st] - sg, nhom = sg - (int64_t)c_hi[st];
-    const int64_t nobs = (int64_t)a.n - nmiss;
-    const double mu = nobs > 0 ? (double)sg / (double)nobs : 0.0;
-    double fq[NQ];   // functionals past m have zero weights in the table: fq = 0
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int i = g + 4 * q;
-      // X and Y are integers below 2^47, so these FP64 Horner steps are exact
-      const int ng = rc->ng[i];
-      double X = fma((double)red[rc->rg[i][0] * 16 + l16], 254.0, (double)red[rc->rg[i][1] * 16 + l16]);
-      if (ng == 3) X = fma(X, 254.0, (double)red[rc->rg[i][2] * 16 + l16]);
-      const double Y = fma((double)red[rc->rm[i][0] * 16 + l16], 254.0, (double)red[rc->rm[i][1] * 16 + l16]);
-      fq[q] = rc->wg[i] * X + mu * (rc->wm[i] * Y - rc->C[i]);
-    }
-    asm volatile("" ::: "memory");   // this tile's reads precede the next tile's writes
-    if constexpr (MODE == 1) {
-      const double scale = epi_scale(mu, a.norm);
-#pragma unroll
-      for (int q = 0; q < NQ; ++q)
-        if (valid && g + 4 * q < a.m) a.Z[j * a.ldZ + g + 4 * q] = scale * fq[q];
-    } else {
-      snp_epilogue_split<NQ>(a.ea, plan, j, valid, g, fq, nmiss, sg, nhom);
-    }
-  }
-}
-
-template <int NT0, int NT1, int MAXM, int MODE>
-__global__ void __launch_bounds__(256, 2) k_gt_i8(GtArgs a) {
-  constexpr int NT = NT0 + NT1;
-  constexpr int KS = 2;                                    // k-steps (64 samples) per staged block
-  constexpr int ST = 4;                                    // SNP tiles of 16 per wave
-  constexpr int CH = KS * NT;                              // 1 KB operand chunks per block
-  constexpr int LDS_A = 2 * CH * 1024;
-  constexpr int LDS_RED = 4 * NT * 1024;                   // one SNP tile of int32 sums per wave
-  constexpr int PLAN_MAX = MODE == 0 ? 3 * kMaxD * 32 + kMaxD + MAXM * MAXM + 1 : 0;   // split_plan_len bound
-  constexpr int LDS_E = LDS_RED + PLAN_MAX * 8 + (int)sizeof(Recomb);
-  constexpr int LDS = LDS_A > LDS_E ? LDS_A : LDS_E;
-  __shared__ __align__(16) unsigned char lds[LDS];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, l16 = lane & 15;
-  const uint64_t base = ((uint64_t)blockIdx.x * 4 + wave) * (ST * 16);
-  const unsigned char* Ag = reinterpret_cast<const unsigned char*>(a.A);
-  const uint8_t* rowp[ST];
-#pragma unroll
-  for (int st = 0; st < ST; ++st) {
-    uint64_t j = base + 16 * st + l16;
-    if (j >= a.p) j = a.p - 1;
-    rowp[st] = a.packed + j * a.ld + 4 * KS * g;
-  }
-  for (int c = wave; c < CH; c += 4) chunk_to_lds(Ag + (uint64_t)c * 1024, lds + c * 1024, lane);
-  uint32_t gw[ST][KS], gn[ST][KS];
-#pragma unroll
-  for (int st = 0; st < ST; ++st) load_words<KS>(rowp[st], gw[st]);
-  v4i acc[ST][NT];
-#pragma unroll
-  for (int st = 0; st < ST; ++st)
-#pragma unroll
-    for (int rt = 0; rt < NT; ++rt) acc[st][rt] = v4i{0, 0, 0, 0};
-  uint32_t c_pop[ST], c_hi[ST];
-#pragma unroll
-  for (int st = 0; st < ST; ++st) c_pop[st] = c_hi[st] = 0;
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-
-  for (int kb = 0; kb < a.nblk; ++kb) {
-    const bool more = kb + 1 < a.nblk;
-    if (more) {
-      unsigned char* dst = lds + ((kb + 1) & 1) * CH * 1024;
-      const unsigned char* src = Ag + (uint64_t)(kb + 1) * CH * 1024;
-      for (int c = wave; c < CH; c += 4) chunk_to_lds(src + (uint64_t)c * 1024, dst + c * 1024, lane);
-#pragma unroll
-      for (int st = 0; st < ST; ++st) load_words<KS>(rowp[st] + (uint64_t)(kb + 1) * 16 * KS, gn[st]);
-    }
-    if (kb == a.nblk - 1) {   // bytes at or past the row stride belong to the next row
-#pragma unroll
-      for (int s = 0; s < KS; ++s) {
-        const int64_t lim = (int64_t)a.ld - ((int64_t)kb * 16 * KS + 4 * KS * g + 4 * s);
-        const uint32_t mask = lim >= 4 ? 0xFFFFFFFFu : (lim <= 0 ? 0u : ((1u << (8 * lim)) - 1u));
-#pragma unroll
-        for (int st = 0; st < ST; ++st) gw[st][s] &= mask;
-      }
-    }
-    const v4i* Ab = reinterpret_cast<const v4i*>(lds + (kb & 1) * CH * 1024);
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      Frag f[ST];
-#pragma unroll
-      for (int st = 0; st < ST; ++st) f[st] = decode_word(gw[st][s], c_pop[st], c_hi[st]);
-#pragma unroll
-      for (int rt = 0; rt < NT; ++rt) {
-        const v4i af = Ab[(s * NT + rt) * 64 + lane];
-#pragma unroll
-        for (int st = 0; st < ST; ++st)
-          acc[st][rt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, rt < NT0 ? f[st].g0 : f[st].miss, acc[st][rt], 0, 0, 0);
-      }
-    }
-    if (more) {
-#pragma unroll
-      for (int st = 0; st < ST; ++st)
-#pragma unroll
-        for (int s = 0; s < KS; ++s) gw[st][s] = gn[st][s];
-    }
-    __builtin_amdgcn_s_waitcnt(0);   // the LDS copy of block kb+1 has landed
-    __syncthreads();
-  }
-
-  // ---- epilogue --------------------------------------------------------------
-  const double* plan = nullptr;
-  if constexpr (MODE == 0) {
-    double* sp = reinterpret_cast<double*>(lds + LDS_RED);
-    stage_split_plan(a.ea.plan, a.ea.d, a.ea.m, sp, t, 256);
-    plan = sp;
-  }
-  const Recomb* rc = stage_recomb(a.rc, lds + LDS_RED + PLAN_MAX * 8, t, 256);
-  __syncthreads();
-  gt_epilogue<ST, NT, MAXM, MODE>(a, acc, c_pop, c_hi, reinterpret_cast<int*>(lds) + wave * (NT * 256), plan, rc,
-                                  base, g, l16);
-}
-
-// ---------------------------------------------------------------------------
-// Pipelined kernel for NT <= 8: 8 waves x 4 tiles of 16 SNPs per workgroup,
-// blocks of 4 k-steps (256 samples, 64 bytes per SNP row).  Both streams go
-// to LDS by buffer-descriptor DMA (buffer_load_dwordx4 ... lds): the
-// genotype segments into a 3-slot ring (blocks kb+1 and kb+2 in flight while
-// kb is computed), the operand blocks into a 2-slot ring.  The block loop is
-// unrolled over the ring period (6) so every slot is a distinct LDS object.
-// ---------------------------------------------------------------------------
-constexpr int kPipeWaves = 8;
-constexpr int kPipeTiles = 4;   // SNP tiles of 16 per wave
 constexpr int kOperandPrefetch = 6;   // operand fragments read ahead of their MFMAs
 
+// The wave's ST 1 KB genotype pieces at p, p + 1024, ... into registers;
+// the wait is inside the asm block, so no load is in flight when it ends.
 template <int ST>
 __device__ __forceinline__ void lds_read_tiles(const unsigned char* p, v4i (&q)[ST]) {
   const uint32_t addr = (uint32_t)(uintptr_t)(lds_ptr_t)p;
-  static_assert(ST >= 1 && ST <= 4, "tile count");
-  if constexpr (ST == 1) {
-    asm volatile("ds_read_b128 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=&v"(q[0]) : "v"(addr) : "memory");
-  } else if constexpr (ST == 2) {
+  static_assert(ST == 2 || ST == 4, "tile count");
+  if constexpr (ST == 2) {
     asm volatile("ds_read_b128 %0, %2\n\tds_read_b128 %1, %2 offset:1024\n\ts_waitcnt lgkmcnt(0)"
                  : "=&v"(q[0]), "=&v"(q[1]) : "v"(addr) : "memory");
-  } else if constexpr (ST == 3) {
-    asm volatile("ds_read_b128 %0, %3\n\tds_read_b128 %1, %3 offset:1024\n\t"
-                 "ds_read_b128 %2, %3 offset:2048\n\ts_waitcnt lgkmcnt(0)"
-                 : "=&v"(q[0]), "=&v"(q[1]), "=&v"(q[2]) : "v"(addr) : "memory");
   } else {
     asm volatile("ds_read_b128 %0, %4\n\tds_read_b128 %1, %4 offset:1024\n\t"
                  "ds_read_b128 %2, %4 offset:2048\n\tds_read_b128 %3, %4 offset:3072\n\ts_waitcnt lgkmcnt(0)"
@@ -475,77 +304,143 @@ typedef __amdgpu_buffer_rsrc_t rsrc_t;
 __device__ __forceinline__ rsrc_t make_rsrc(const void* base, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
 }
-// LDS-DMA through a buffer descriptor: lane l's 16 bytes at byte offset voff
-// land at dst + 16 l.  (The buffer form keeps the compiler's LDS-read waits
-// partial: after a global_load_lds it waits for every outstanding LDS read.)
-__device__ __forceinline__ void buf_to_lds(rsrc_t rs, uint32_t voff, unsigned char* dst) {
-  __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_ptr_t)dst, 16, voff, 0, 0, 0);
+// LDS-DMA through a buffer descriptor: lane l's 16 bytes at byte offset
+// voff + soff land at dst + 16 l (soff wave-uniform, so the per-chunk offset
+// needs no vector register).  (The buffer form keeps the compiler's LDS-read
+// waits partial: after a global_load_lds it waits for every outstanding LDS read.)
+__device__ __forceinline__ void buf_to_lds(rsrc_t rs, uint32_t voff, uint32_t soff, unsigned char* dst) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_ptr_t)dst, 16, voff, soff, 0, 0);
 }
 
-struct PipeCtx {
-  rsrc_t geno;        // this workgroup's SNP rows; rows past the last SNP read zeros
+// ---------------------------------------------------------------------------
+// SNP-side kernel: functionals of every SNP row against a packed operand.
+// MODE 0: fused association epilogue;  MODE 1: store s_j * f (RSVD sketch).
+//
+// Fill code.  Before the stream, each SNP gets an integer fill code c from
+// its first 256 samples (c = round(S mu_256)), and the g' fragment decodes
+// 00 -> 0, 01 -> c, 10 -> S, 11 -> 2S (one byte-permute table per lane).
+// With q^f the g'-side quantisation (2 or 3 terms) and q^c the miss-side one
+// (1 or 2 terms, the leading terms of the same expansion):
+//   f_i = (1/S) sum_k q^f g'_k - mu sum_k q^f + (mu - c/S) sum_{miss} q^c
+//       = sum_{obs} q^f (g - mu) + (mu - c/S) sum_{miss} (q^c - q^f),
+// so the coarse miss side only meets the small weight |mu - c/S|.
+//
+// Workgroup: 4 waves x ST tiles of 16 SNPs, two workgroups per CU, so one
+// workgroup's barriers and epilogue overlap the other's stream.  Blocks of
+// 256 samples (4 k-steps of 64):
+//   * the operand block (4 NT fragments) goes to a 2-slot LDS ring by
+//     buffer-descriptor DMA, shared by the four waves;
+//   * each lane's 16 bytes of its SNP row per tile go to a one-block LDS slot
+//     of its wave by the same DMA (the wave reads back only its own pieces).
+// Rows past the last SNP, bytes past the row stride and blocks past the last
+// read zeros through the buffer descriptor.
+// ---------------------------------------------------------------------------
+struct SnpArgs {
+  const uint8_t* packed;
+  uint64_t ld, p, n;
+  int nblk, m, k3, km, g1, ones_g, norm, S, fill;
+  const int8_t* A;
+  const double* wtab;
+  EpiArgs ea;
+  double* Z;
+  uint64_t ldZ;
+};
+
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+constexpr int kSnpWaves = 4;
+
+// Byte-permute table of the g' fragment from the SNP's first 256 samples
+// (lane group g holds samples 64 g .. 64 g + 63 of block 0 in w).
+// With the all-ones row in a g' tile the missing count is recovered from
+// sum g' = S sg + c n_miss and popc = sg + n_miss, which needs c != S; the
+// code then moves to the neighbour on the side of the estimate.
+__device__ __forceinline__ uint32_t fill_table(const v4u& w, int S, int fill, int ones_g) {
+  int sg = 0, miss = 0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const uint32_t x = w[v], lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
+    sg += __popc(hi) + __popc(lo & hi);   // het + 2 hom
+    miss += __popc(lo & ~hi);
+  }
+  int obs = 64 - miss;
+  sg += __shfl_xor(sg, 16); sg += __shfl_xor(sg, 32);
+  obs += __shfl_xor(obs, 16); obs += __shfl_xor(obs, 32);
+  int c = obs > 0 ? (2 * S * sg + obs) / (2 * obs) : 0;   // round(S sg / obs)
+  c = c < 2 * S ? c : 2 * S;
+  if (ones_g && c == S) c = sg >= obs ? S + 1 : S - 1;
+  if (!fill) c = 0;
+  return ((uint32_t)(2 * S) << 24) | ((uint32_t)S << 16) | ((uint32_t)c << 8);
+}
+
+// One word = 16 codes; register v of a fragment takes the codes at bit
+// 8c + 2v (one shift and mask), mapped through the lane's g' table and the
+// missing-indicator table.
+__device__ __forceinline__ Frag decode_fill(uint32_t w, uint32_t tbl, uint32_t& c_pop, uint32_t& c_hi) {
+  c_pop += __popc(w);                  // dosage + 1 per missing call
+  c_hi += __popc(w & 0xAAAAAAAAu);     // het + hom
+  Frag f;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const uint32_t cv = (w >> (2 * v)) & 0x03030303u;
+    f.g0[v] = (int)__builtin_amdgcn_perm(0u, tbl, cv);
+    f.miss[v] = (int)__builtin_amdgcn_perm(0u, kPermMiss, cv);
+  }
+  return f;
+}
+
+struct SnpPipe {
+  rsrc_t geno;        // this workgroup's SNP rows (exactly); rows past the last SNP read zeros
   rsrc_t opnd;        // the operand blocks
-  uint32_t gvoff;     // byte offset of this lane's 16-byte piece of its tile-0 row
+  uint32_t rowoff;    // byte offset of this lane's tile-0 row in the workgroup's rows
   uint32_t tstride;   // 16 rows
 };
 
-// One block of the pipelined kernel.  Operand block kb+1 and genotype block
-// kb+2 are issued while kb is computed, in that order, so the wait at the top
-// of block kb leaves only the genotypes of kb+1 in flight.
+// LDS-DMA of the 16-byte pieces of block kb of the lane's ST rows into the
+// wave's part of a genotype slot (tile st at dst + 1024 st).  A piece at or
+// past the row stride (the next row's bytes, or a block past the last) is
+// addressed past the descriptor and lands as zeros without a memory access.
+template <int ST>
+__device__ __forceinline__ void geno_to_lds(const SnpArgs& a, const SnpPipe& pc, int kb, int g, unsigned char* dst) {
+  const uint32_t byte = (uint32_t)kb * 64 + 16 * g;
+  const uint32_t o = byte < (uint32_t)a.ld ? pc.rowoff + byte : 0xF0000000u;
+#pragma unroll
+  for (int st = 0; st < ST; ++st) buf_to_lds(pc.geno, o + st * pc.tstride, 0, dst + st * 1024);
+}
+
+// One block.  Operand block kb + 1 goes to the other slot of the 2-slot
+// operand ring.  The genotype slot holds only the wave's own pieces, so the
+// wave reads block kb into registers and then reuses the slot for block
+// kb + 1 (the reads have completed before the DMA is issued).
 template <int NT0, int NT, int ST>
-__device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, const PipeCtx& pc, int wave, int lane, int g,
-                                           const unsigned char* a_rd, unsigned char* a_wr,
-                                           const unsigned char* g_rd, unsigned char* g_wr,
-                                           v4i (&acc)[ST][NT], uint32_t (&c_pop)[ST], uint32_t (&c_hi)[ST]) {
-  constexpr int KS = 4;
-  constexpr int W = kPipeWaves;
-  constexpr int CH = KS * NT;
-  wait_vmcnt<ST>();
-  __builtin_amdgcn_s_barrier();
-  const v4i* Ab = reinterpret_cast<const v4i*>(a_rd);
-  uint32_t gw[ST][KS];
-  {
-    v4i q[ST];
-    lds_read_tiles<ST>(g_rd + wave * (ST * 1024) + 16 * lane, q);
-#pragma unroll
-    for (int st = 0; st < ST; ++st)
-#pragma unroll
-      for (int s = 0; s < KS; ++s) gw[st][s] = (uint32_t)q[st][s];
-  }
-  if (kb == a.nblk - 1) {   // bytes at or past the row stride belong to the next row
-    asm volatile("" ::: "memory");   // keep this a branch: only the last block pays for the mask
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      const int64_t lim = (int64_t)a.ld - ((int64_t)kb * 64 + 16 * g + 4 * s);
-      const uint32_t mask = lim >= 4 ? 0xFFFFFFFFu : (lim <= 0 ? 0u : ((1u << (8 * lim)) - 1u));
-#pragma unroll
-      for (int st = 0; st < ST; ++st) gw[st][s] &= mask;
-    }
-  }
-  // the ring slots written below are not the ones read in this block
+__device__ __forceinline__ void snp_block(int kb, const SnpArgs& a, const SnpPipe& pc, int wave, int lane, int g,
+                                          const unsigned char* a_rd, unsigned char* a_wr, unsigned char* g_slot,
+                                          const uint32_t (&tbl)[ST], v4i (&acc)[ST][NT], uint32_t (&c_pop)[ST],
+                                          uint32_t (&c_hi)[ST]) {
+  constexpr int KS = 4, W = kSnpWaves, CH = KS * NT;
+  wait_vmcnt<0>();
+  __builtin_amdgcn_s_barrier();   // block kb is in LDS; every wave is done with block kb - 1's operand slot
+  v4i gc[ST];
+  lds_read_tiles<ST>(g_slot + 16 * lane, gc);   // waits for its reads
   if (kb + 1 < a.nblk) {
-    const uint32_t src = (uint32_t)(kb + 1) * CH * 1024 + 16 * lane;
+    const uint32_t src = (uint32_t)(kb + 1) * CH * 1024;
 #pragma unroll
-    for (int i = 0; i < (CH + W - 1) / W; ++i) {   // fragments wave, wave + W, ... (compile-time trip count)
+    for (int i = 0; i < (CH + W - 1) / W; ++i) {
       const int c = wave + i * W;
-      if (c < CH) buf_to_lds(pc.opnd, src + c * 1024, a_wr + c * 1024);
+      if (c < CH) buf_to_lds(pc.opnd, 16 * lane, src + c * 1024, a_wr + c * 1024);
     }
+    geno_to_lds<ST>(a, pc, kb + 1, g, g_slot);
   }
-  {
-    const uint32_t src = (uint32_t)(kb + 2 < a.nblk ? kb + 2 : 0) * 64;
-    unsigned char* dst = g_wr + wave * (ST * 1024);
-#pragma unroll
-    for (int st = 0; st < ST; ++st) buf_to_lds(pc.geno, pc.gvoff + st * pc.tstride + src, dst + st * 1024);
-  }
+  __builtin_amdgcn_sched_barrier(0);   // keep the prefetches at the top of the block
   // operand fragment c = s NT + rt is read kOperandPrefetch fragments ahead;
   // the genotype fragments of k-step s + 1 are decoded after the MFMAs of s
   constexpr int D = kOperandPrefetch < CH ? kOperandPrefetch : CH;
+  const v4i* Ab = reinterpret_cast<const v4i*>(a_rd);
   v4i af[CH];
 #pragma unroll
   for (int c = 0; c < D; ++c) af[c] = Ab[c * 64 + lane];
   Frag fc[ST];
 #pragma unroll
-  for (int st = 0; st < ST; ++st) fc[st] = decode_word(gw[st][0], c_pop[st], c_hi[st]);
+  for (int st = 0; st < ST; ++st) fc[st] = decode_fill((uint32_t)gc[st][0], tbl[st], c_pop[st], c_hi[st]);
 #pragma unroll
   for (int s = 0; s < KS; ++s) {
 #pragma unroll
@@ -559,35 +454,62 @@ __device__ __forceinline__ void pipe_block(int kb, const GtArgs& a, const PipeCt
     }
     if (s + 1 < KS) {
 #pragma unroll
-      for (int st = 0; st < ST; ++st) fc[st] = decode_word(gw[st][s + 1], c_pop[st], c_hi[st]);
+      for (int st = 0; st < ST; ++st) fc[st] = decode_fill((uint32_t)gc[st][s + 1], tbl[st], c_pop[st], c_hi[st]);
     }
   }
 }
 
-template <int NT0, int NT1, int MAXM, int MODE>
-__global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
-  constexpr int NT = NT0 + NT1;
-  constexpr int KS = 4;
-  constexpr int ST = kPipeTiles;
-  constexpr int W = kPipeWaves;
-  constexpr int CH = KS * NT;
+// Functionals of lane group g at positions q = 0 .. NQ-1 (i = g + 4 q) from
+// the lane's own accumulators (slot layout of RowMap); wt = the weight table.
+template <int NQ, int NT0, int NT1>
+__device__ __forceinline__ void recombine(const v4i (&acc)[NT0 + NT1], int g, int k3, int km, double mu, double dmu,
+                                          const double* wt, double (&fq)[NQ]) {
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int i = g + 4 * q;
+    const bool x = i >= k3, xm = i >= km;
+    // X and Y are integers below 2^53, so these FP64 Horner steps are exact
+    double X = fma((double)acc[(2 * q) / 4][(2 * q) % 4], 254.0, (double)acc[(2 * q + 1) / 4][(2 * q + 1) % 4]);
+    // slots of term 2 (g' side) and term 1 (miss side) of columns >= k3; the
+    // loop is unrolled, so the conditions and indices fold to constants
+    const int s2 = 3 * NQ - 1 - q, s1 = 2 * NQ - 1 - q;
+    if (s2 < 4 * NT0 && x) {
+      const int s = s2 < 4 * NT0 ? s2 : 0;
+      X = fma(X, 254.0, (double)acc[s / 4][s % 4]);
+    }
+    double Y = (double)acc[NT0 + q / 4][q % 4];
+    if (s1 < 4 * NT1 && xm) {
+      const int s = s1 < 4 * NT1 ? s1 : 0;
+      Y = fma(Y, 254.0, (double)acc[NT0 + s / 4][s % 4]);
+    }
+    fq[q] = wt[i] * X - mu * wt[64 + i] + dmu * (wt[32 + i] * Y);   // zero weights past m
+  }
+}
+
+// Bytes of the split plan for m <= 4 NQ functionals and d <= min(kMaxD, 4 NQ) traits.
+__host__ __device__ constexpr int snp_plan_bytes(int NQ) {
+  return 8 * (3 * (4 * NQ < kMaxD ? 4 * NQ : kMaxD) * 32 + (4 * NQ < kMaxD ? 4 * NQ : kMaxD) + 16 * NQ * NQ + 1);
+}
+
+template <int NQ, int NT0, int NT1, int ST, int MODE>
+__global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
+  constexpr int NT = NT0 + NT1, KS = 4, W = kSnpWaves, CH = KS * NT;
   constexpr int A_SLOT = CH * 1024;
+  constexpr int A0_BYTES = MODE == 0 && snp_plan_bytes(NQ) > A_SLOT ? snp_plan_bytes(NQ) : A_SLOT;
+  static_assert(kWtab * 8 <= A_SLOT, "weight table must fit an operand slot");
   constexpr int G_SLOT = W * ST * 1024;
-  static_assert(NT * 1024 * W <= 3 * G_SLOT, "epilogue sums of all waves must fit the genotype ring");
-  static_assert(sizeof(Recomb) <= A_SLOT, "recombination table must fit an operand slot");
-  __shared__ __align__(16) unsigned char A0[A_SLOT];
+  __shared__ __align__(16) unsigned char A0[A0_BYTES];
   __shared__ __align__(16) unsigned char A1[A_SLOT];
   __shared__ __align__(16) unsigned char G0[G_SLOT];
-  __shared__ __align__(16) unsigned char G1[G_SLOT];
-  __shared__ __align__(16) unsigned char G2[G_SLOT];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, l16 = lane & 15;
-  const uint64_t wgbase = (uint64_t)blockIdx.x * W * ST * 16;
+  const int t = threadIdx.x, lane = t & 63, g = lane >> 4, l16 = lane & 15;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);   // wave-uniform: scalar DMA offsets
+  const uint64_t wgbase = (uint64_t)blockIdx.x * (W * ST * 16);
   const uint64_t base = wgbase + (uint64_t)wave * (ST * 16);
-  PipeCtx pc;
-  const uint64_t gbytes = (a.p - wgbase) * a.ld;
-  pc.geno = make_rsrc(a.packed + wgbase * a.ld, (uint32_t)(gbytes < 0xFFFFFFFFull ? gbytes : 0xFFFFFFFFull));
+  SnpPipe pc;
+  const uint64_t wrows = a.p - wgbase < (uint64_t)(W * ST * 16) ? a.p - wgbase : (uint64_t)(W * ST * 16);
+  pc.geno = make_rsrc(a.packed + wgbase * a.ld, (uint32_t)(wrows * a.ld));
   pc.opnd = make_rsrc(a.A, (uint32_t)a.nblk * CH * 1024);
-  pc.gvoff = (uint32_t)((base - wgbase + l16) * a.ld + 16 * g);
+  pc.rowoff = (uint32_t)((base - wgbase + l16) * a.ld);
   pc.tstride = (uint32_t)(16 * a.ld);
   v4i acc[ST][NT];
 #pragma unroll
@@ -597,120 +519,140 @@ __global__ void __launch_bounds__(512, 1) k_gt_i8p(GtArgs a) {
   uint32_t c_pop[ST], c_hi[ST];
 #pragma unroll
   for (int st = 0; st < ST; ++st) c_pop[st] = c_hi[st] = 0;
-  // Prologue in the order the blocks issue them: operand 0 and genotypes 0,
-  // then genotypes 1.  The first block's wait leaves the genotypes of block 1 in flight.
-  for (int c = wave; c < CH; c += W) buf_to_lds(pc.opnd, c * 1024 + 16 * lane, A0 + c * 1024);
-#pragma unroll
-  for (int st = 0; st < ST; ++st) buf_to_lds(pc.geno, pc.gvoff + st * pc.tstride, G0 + wave * (ST * 1024) + st * 1024);
+  // prologue in the order the blocks issue them: operand 0, genotypes 0, genotypes 1
+  for (int c = wave; c < CH; c += W) buf_to_lds(pc.opnd, 16 * lane, c * 1024, A0 + c * 1024);
+  geno_to_lds<ST>(a, pc, 0, g, G0 + wave * (ST * 1024));
+  wait_vmcnt<0>();
+  uint32_t tbl[ST];
   {
-    const uint32_t src = a.nblk > 1 ? 64 : 0;
+    const v4u* gp = reinterpret_cast<const v4u*>(G0 + wave * (ST * 1024));
 #pragma unroll
-    for (int st = 0; st < ST; ++st)
-      buf_to_lds(pc.geno, pc.gvoff + st * pc.tstride + src, G1 + wave * (ST * 1024) + st * 1024);
+    for (int st = 0; st < ST; ++st) tbl[st] = fill_table(gp[st * 64 + lane], a.S, a.fill, a.ones_g);
   }
-#define SLFMM_PIPE(PH, AR_, AW_, GR_, GW_) \
-  if (kb + PH < a.nblk) pipe_block<NT0, NT, ST>(kb + PH, a, pc, wave, lane, g, AR_, AW_, GR_, GW_, acc, c_pop, c_hi);
-  for (int kb = 0; kb < a.nblk; kb += 6) {
-    SLFMM_PIPE(0, A0, A1, G0, G2)
-    SLFMM_PIPE(1, A1, A0, G1, G0)
-    SLFMM_PIPE(2, A0, A1, G2, G1)
-    SLFMM_PIPE(3, A1, A0, G0, G2)
-    SLFMM_PIPE(4, A0, A1, G1, G0)
-    SLFMM_PIPE(5, A1, A0, G2, G1)
+  unsigned char* gslot = G0 + wave * (ST * 1024);
+#define SLFMM_SNP(PH, AR_, AW_) \
+  if (kb + PH < a.nblk) snp_block<NT0, NT, ST>(kb + PH, a, pc, wave, lane, g, AR_, AW_, gslot, tbl, acc, c_pop, c_hi);
+  for (int kb = 0; kb < a.nblk; kb += 2) {
+    SLFMM_SNP(0, A0, A1)
+    SLFMM_SNP(1, A1, A0)
   }
-#undef SLFMM_PIPE
+#undef SLFMM_SNP
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
 
-  // ---- epilogue: per-wave int32 sums of one SNP tile; waves spread over the
-  // three genotype slots, the plan goes to A0, the recombination table to A1
-  constexpr int WPS = G_SLOT / (NT * 1024);   // waves per slot
-  unsigned char* gslot = wave < WPS ? G0 : (wave < 2 * WPS ? G1 : G2);
-  int* red = reinterpret_cast<int*>(gslot) + (wave % WPS) * (NT * 256);
+  // ---- epilogue: the plan goes to A0, the weight table to A1 ----------------
   const double* plan = nullptr;
   if constexpr (MODE == 0) {
     double* sp = reinterpret_cast<double*>(A0);
-    stage_split_plan(a.ea.plan, a.ea.d, a.ea.m, sp, t, 512);
+    stage_split_plan(a.ea.plan, a.ea.d, a.ea.m, sp, t, 64 * W);
     plan = sp;
   }
-  const Recomb* rc = stage_recomb(a.rc, A1, t, 512);
+  double* wt = reinterpret_cast<double*>(A1);
+  for (int e = t; e < kWtab; e += 64 * W) wt[e] = a.wtab[e];
   __syncthreads();
-  gt_epilogue<ST, NT, MAXM, MODE>(a, acc, c_pop, c_hi, red, plan, rc, base, g, l16);
-}
-
-#define SLFMM_GT_CASES(X) \
-  X(1, 1) X(2, 1) X(3, 1) X(2, 2) X(3, 2) X(4, 2) X(3, 3) X(4, 3) X(5, 3) X(4, 4) X(5, 4) X(6, 4)
-
-template <int A0, int A1>
-static bool launch_pipe(hipStream_t s, unsigned grid, GtArgs& a, int mode, int maxm) {
-  if constexpr (A0 + A1 <= 8) {
-    if (mode == 1) k_gt_i8p<A0, A1, 1, 1><<<grid, 512, 0, s>>>(a);
-    else if (maxm <= 24) k_gt_i8p<A0, A1, 24, 0><<<grid, 512, 0, s>>>(a);
-    else if (maxm <= 32) k_gt_i8p<A0, A1, 32, 0><<<grid, 512, 0, s>>>(a);
-    else return false;
-    return true;
-  }
-  return false;
-}
-
-static void launch_gt(hipStream_t s, const Operand& op, GtArgs& a, int mode, int maxm) {
-  if (a.p == 0) return;
-  const int key = op.NT0 * 100 + op.NT1;
-  const int NT = op.NT0 + op.NT1;
-  bool done = false;
-  if (NT <= 8) {
-    // buffer descriptors address a workgroup's rows and the operand with 32-bit byte offsets
-    const uint64_t rows = 16 * kPipeTiles * kPipeWaves;
-    SLFMM_CHECK(rows * a.ld + 64 * (uint64_t)a.nblk < 0xFFFFFFFFull &&
-                    (uint64_t)a.nblk * 4 * NT * 1024 < 0xFFFFFFFFull,
-                SLFMM_E_ARG, "panel rows too long for the int8 association kernel (use SLFMM_PREC_FP64)");
-    const unsigned grid = (unsigned)cdiv(a.p, rows);
-#define X(A0, A1)                                                                      \
-    if (!done && key == A0 * 100 + A1) done = launch_pipe<A0, A1>(s, grid, a, mode, maxm);
-    SLFMM_GT_CASES(X)
-#undef X
-  } else {
-    const unsigned grid = (unsigned)cdiv(a.p, 256);
-#define X(A0, A1)                                                                      \
-    if (!done && key == A0 * 100 + A1 && A0 + A1 > 8) {                                \
-      if (mode == 1) k_gt_i8<A0, A1, 1, 1><<<grid, 256, 0, s>>>(a);                    \
-      else if (maxm <= 24) k_gt_i8<A0, A1, 24, 0><<<grid, 256, 0, s>>>(a);             \
-      else if (maxm <= 32) k_gt_i8<A0, A1, 32, 0><<<grid, 256, 0, s>>>(a);             \
-      done = true;                                                                     \
+#pragma unroll
+  for (int st = 0; st < ST; ++st) {
+    uint32_t cp = c_pop[st], ch = c_hi[st];
+    cp += __shfl_xor(cp, 16); cp += __shfl_xor(cp, 32);
+    ch += __shfl_xor(ch, 16); ch += __shfl_xor(ch, 32);
+    // the all-ones row: the missing count (miss tile) or sum g' = S sg + c n_miss (g' tile)
+    const int ones_m = __shfl(acc[st][NT - 1][3], 16 * a.g1 + l16);
+    const int ones_g = __shfl(acc[st][NT0 - 1][3], 16 * a.g1 + l16);
+    const int cfill = (int)((tbl[st] >> 8) & 0xFFu);
+    const int64_t nmiss = a.ones_g ? ((int64_t)a.S * cp - ones_g) / (a.S - cfill) : (int64_t)ones_m;
+    const int64_t sg = (int64_t)cp - nmiss, nhom = sg - (int64_t)ch;
+    const int64_t nobs = (int64_t)a.n - nmiss;
+    const double mu = nobs > 0 ? (double)sg / (double)nobs : 0.0;
+    const double dmu = mu - (double)cfill / (double)a.S;
+    double fq[NQ];
+    recombine<NQ, NT0, NT1>(acc[st], g, a.k3, a.km, mu, dmu, wt, fq);
+    const uint64_t j = base + 16 * st + l16;
+    const bool valid = j < a.p;
+    if constexpr (MODE == 1) {
+      const double scale = epi_scale(mu, a.norm);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (valid && g + 4 * q < a.m) a.Z[j * a.ldZ + g + 4 * q] = scale * fq[q];
+    } else {
+      snp_epilogue_split<NQ>(a.ea, plan, j, valid, g, fq, nmiss, sg, nhom);
     }
-    SLFMM_GT_CASES(X)
-#undef X
   }
-  SLFMM_CHECK(done, SLFMM_E_ARG, "no int8 kernel for this tile shape (K + d too large)");
+}
+
+// (NQ, NT0, NT1) of every RowMap with m <= 32 (association: d <= 16, K >= 1;
+// RSVD: k3 = m)
+#define SLFMM_SNP_ASSOC(X)                                                                                     \
+  X(1, 1, 1) X(2, 2, 1) X(3, 2, 1) X(3, 2, 2) X(3, 3, 2) X(4, 3, 2) X(5, 3, 2) X(5, 3, 3) X(5, 4, 2) X(5, 4, 3) \
+  X(6, 4, 2) X(6, 4, 3) X(6, 4, 4) X(6, 5, 3) X(7, 4, 2) X(7, 4, 3) X(7, 4, 4) X(7, 5, 3) X(7, 5, 4) X(8, 5, 3) \
+  X(8, 5, 4) X(8, 5, 5) X(8, 6, 3) X(8, 6, 4)
+#define SLFMM_SNP_RSVD(X) \
+  X(1, 1, 1) X(2, 1, 1) X(3, 2, 1) X(4, 2, 1) X(4, 2, 2) X(5, 3, 2) X(6, 3, 2) X(7, 4, 2) X(8, 4, 2) X(8, 4, 3)
+
+template <int NT>
+constexpr int snp_tiles() { return NT <= 8 ? 4 : 2; }   // SNP tiles per wave (accumulator registers)
+
+static void launch_snp(hipStream_t s, const Operand& op, SnpArgs& a, int mode) {
+  if (a.p == 0) return;
+  const RowMap& r = op.rmap;
+  const int NT = r.NT0 + r.NT1;
+  const int st = NT <= 8 ? 4 : 2;
+  const uint64_t rows = 16 * (uint64_t)st * kSnpWaves;
+  // buffer descriptors address a workgroup's rows and the operand with 32-bit byte offsets
+  SLFMM_CHECK((rows + 3 * 16) * a.ld < 0xF0000000ull && (uint64_t)a.nblk * 4 * NT * 1024 < 0xFFFFFFFFull,
+              SLFMM_E_ARG, "panel rows too long for the int8 kernels (use SLFMM_PREC_FP64)");
+  a.m = r.m; a.k3 = r.k3; a.km = r.km; a.g1 = r.g1; a.ones_g = r.ones_g; a.S = op.S; a.fill = op.fill;
+  const unsigned grid = (unsigned)cdiv(a.p, rows);
+  bool done = false;
+#define X(Q, A0_, A1_)                                                                                        \
+  if (!done && r.NQ == Q && r.NT0 == A0_ && r.NT1 == A1_) {                                                   \
+    k_snp_i8<Q, A0_, A1_, snp_tiles<A0_ + A1_>(), 0><<<grid, 64 * kSnpWaves, 0, s>>>(a);                      \
+    done = true;                                                                                              \
+  }
+  if (mode == 0) { SLFMM_SNP_ASSOC(X) }
+#undef X
+#define X(Q, A0_, A1_)                                                                                        \
+  if (!done && r.NQ == Q && r.NT0 == A0_ && r.NT1 == A1_) {                                                   \
+    k_snp_i8<Q, A0_, A1_, snp_tiles<A0_ + A1_>(), 1><<<grid, 64 * kSnpWaves, 0, s>>>(a);                      \
+    done = true;                                                                                              \
+  }
+  if (mode == 1) { SLFMM_SNP_RSVD(X) }
+#undef X
+  SLFMM_CHECK(done, SLFMM_E_ARG, "no int8 kernel for this operand layout");
   SLFMM_HIP(hipGetLastError());
 }
+
+// Miss-side terms of the latent columns in the association pass.
+#ifndef SLFMM_ASSOC_MISS_TERMS
+#define SLFMM_ASSOC_MISS_TERMS 2
+#endif
+constexpr int kAssocMissTerms = SLFMM_ASSOC_MISS_TERMS;
 
 // Association pass over every SNP of g (fused epilogue).
 void launch_func_i8(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_Q, uint64_t m, const EpiArgs& ea,
                     bool /*fused_epilogue*/) {
-  // Latent directions: 2 int8 terms; covariate directions (the t-statistic
-  // numerators, last d columns of the basis): 3 terms while K + d <= 32.
+  // Latent directions: 2 int8 terms (g' side) and 1 (miss side); covariate
+  // directions (the t-statistic numerators, last d columns): 3 and 2.
   SLFMM_CHECK(m <= 32, SLFMM_E_ARG, "the int8 association path supports K + d <= 32 (use SLFMM_PREC_FP64)");
-  Operand op = prepare_operand(ctx, "as_i8", d_Q, m, g->n, (int)m, ea.K);
-  const uint64_t plan_bytes = 8 * (uint64_t)split_plan_len(ea.d, (int)m);
-  SLFMM_CHECK(op.NT0 + op.NT1 > 8 || plan_bytes <= (uint64_t)4 * (op.NT0 + op.NT1) * 1024, SLFMM_E_ARG,
+  Operand op = kAssocMissTerms == 2 ? prepare_operand(ctx, "as_i8", d_Q, m, g->n, (int)m, ea.K, 0, true)
+                                    : prepare_operand(ctx, "as_i8", d_Q, m, g->n, (int)m, ea.K, ea.K, true);
+  SLFMM_CHECK(8 * split_plan_len(ea.d, (int)m) <= snp_plan_bytes(op.rmap.NQ), SLFMM_E_ARG,
               "association plan does not fit the int8 kernel's LDS (use SLFMM_PREC_FP64)");
-  GtArgs a{};
-  a.packed = g->d; a.ld = g->ld; a.p = g->p; a.n = g->n; a.nblk = (int)op.nblk; a.m = (int)m; a.norm = ea.norm;
-  a.A = reinterpret_cast<const v4i*>(op.pack); a.rc = op.rc; a.ea = ea;
+  SnpArgs a{};
+  a.packed = g->d; a.ld = g->ld; a.p = g->p; a.n = g->n; a.nblk = (int)op.nblk; a.norm = ea.norm;
+  a.A = op.pack; a.wtab = op.wtab; a.ea = ea;
   slfmm_timed tm(ctx, "assoc_stream");
-  launch_gt(ctx->stream, op, a, 0, (int)m);
+  launch_snp(ctx->stream, op, a, 0);
 }
 
 // RSVD sketch / backward pass: Z[j][c] = s_j * (B^T y_j)_c for every SNP row.
 void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p, const double* d_B,
                      uint64_t l, int norm, double* Z, uint64_t ldZ) {
   SLFMM_CHECK(l >= 1 && l <= 32, SLFMM_E_ARG, "the int8 RSVD path supports K + oversampling <= 32");
-  Operand op = prepare_operand(ctx, "rs_i8", d_B, l, n, (int)l, (int)l);
-  GtArgs a{};
-  a.packed = packed; a.ld = ld; a.p = p; a.n = n; a.nblk = (int)op.nblk; a.m = (int)l; a.norm = norm;
-  a.A = reinterpret_cast<const v4i*>(op.pack); a.rc = op.rc; a.Z = Z; a.ldZ = ldZ;
-  launch_gt(ctx->stream, op, a, 1, 1);
+  Operand op = prepare_operand(ctx, "rs_i8", d_B, l, n, (int)l, (int)l, (int)l, true);
+  SnpArgs a{};
+  a.packed = packed; a.ld = ld; a.p = p; a.n = n; a.nblk = (int)op.nblk; a.norm = norm;
+  a.A = op.pack; a.wtab = op.wtab; a.Z = Z; a.ldZ = ldZ;
+  launch_snp(ctx->stream, op, a, 1);
 }
 
 // ---------------------------------------------------------------------------
