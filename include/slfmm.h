@@ -274,6 +274,14 @@ int slfmm_profile_read(slfmm_ctx* ctx, const char* name, double* total_ms, uint6
  * all row-major.  A GPU test pins the lane layout with it. */
 int slfmm_debug_mfma_i8(slfmm_ctx* ctx, const int8_t* a, const int8_t* b, int32_t* d);
 
+/* Diagnostic (no reference counterpart, host only): the packed-operand row
+ * layout of the int8 SNP-side kernels for m functionals, columns >= k3 with 3
+ * dosage-side terms and columns >= km with 2 miss-side terms.
+ * dims[5] = {NQ, NT0, NT1, g1, ones_g}; rows (cap >= 32 (NT0 + NT1) ints,
+ * may be NULL) = (column, term) per packed row, column -1 padding, -2 the
+ * all-ones row.  A CPU test pins the layout with it. */
+int slfmm_debug_i8_layout(int32_t m, int32_t k3, int32_t km, int32_t* dims, int32_t* rows, uint64_t cap);
+
 /* ---- Synthetic data (test / benchmark inputs, no reference counterpart) --- */
 
 /* Balding-Nichols panel generated on the device into a packed SNP-major

@@ -73,6 +73,7 @@ SIGNATURES = {
     "slfmm_nmf_estimate": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp, u32]),
     "slfmm_fit_lfmm2_nmf": (C.c_int, [vp, vp, vp, u64, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp, vp, u32]),
     "slfmm_debug_mfma_i8": (C.c_int, [vp, vp, vp, vp]),
+    "slfmm_debug_i8_layout": (C.c_int, [i32, i32, i32, vp, vp, u64]),
     "slfmm_profile_enable": (C.c_int, [vp, i32]),
     "slfmm_profile_read": (C.c_int, [vp, C.c_char_p, vp, vp, i32]),
     "slfmm_synth_bn": (C.c_int, [vp, u64, u64, u32, dbl, dbl, dbl, dbl, dbl, u64, vp]),

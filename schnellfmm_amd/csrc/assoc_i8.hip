@@ -89,7 +89,7 @@ struct RowMap {
     return r;
   }
   // (column, term) of packed row `row`: column -1 for padding, -2 for the ones row
-  __device__ int2 info(int row) const {
+  __host__ __device__ int2 info(int row) const {
     const int rt = row >> 4, g = (row >> 2) & 3, r = row & 3;
     if (rt < NT0) {
       const int s = 4 * rt + r;
@@ -579,12 +579,11 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
   }
 }
 
-// (NQ, NT0, NT1) of every RowMap with m <= 32 (association: d <= 16, K >= 1;
-// RSVD: k3 = m)
+// (NQ, NT0, NT1) of every RowMap the passes use with m <= 32: association
+// RowMap::make(K + d, K, 0) for d <= 16, K >= 1; RSVD RowMap::make(l, l, l).
 #define SLFMM_SNP_ASSOC(X)                                                                                     \
-  X(1, 1, 1) X(2, 2, 1) X(3, 2, 1) X(3, 2, 2) X(3, 3, 2) X(4, 3, 2) X(5, 3, 2) X(5, 3, 3) X(5, 4, 2) X(5, 4, 3) \
-  X(6, 4, 2) X(6, 4, 3) X(6, 4, 4) X(6, 5, 3) X(7, 4, 2) X(7, 4, 3) X(7, 4, 4) X(7, 5, 3) X(7, 5, 4) X(8, 5, 3) \
-  X(8, 5, 4) X(8, 5, 5) X(8, 6, 3) X(8, 6, 4)
+  X(1, 1, 1) X(2, 2, 1) X(3, 2, 2) X(3, 3, 2) X(4, 3, 2) X(5, 3, 3) X(5, 4, 3) X(6, 4, 3) X(6, 4, 4) X(6, 5, 3) \
+  X(7, 4, 4) X(7, 5, 4) X(8, 5, 4) X(8, 5, 5) X(8, 6, 4)
 #define SLFMM_SNP_RSVD(X) \
   X(1, 1, 1) X(2, 1, 1) X(3, 2, 1) X(4, 2, 1) X(4, 2, 2) X(5, 3, 2) X(6, 3, 2) X(7, 4, 2) X(8, 4, 2) X(8, 4, 3)
 
@@ -621,20 +620,32 @@ static void launch_snp(hipStream_t s, const Operand& op, SnpArgs& a, int mode) {
   SLFMM_HIP(hipGetLastError());
 }
 
-// Miss-side terms of the latent columns in the association pass.
-#ifndef SLFMM_ASSOC_MISS_TERMS
-#define SLFMM_ASSOC_MISS_TERMS 2
-#endif
-constexpr int kAssocMissTerms = SLFMM_ASSOC_MISS_TERMS;
+// Layout diagnostic (host only): dims = {NQ, NT0, NT1, g1, ones_g}; rows
+// (2 x 16 (NT0 + NT1) ints, if non-null) = (column, term) of every packed row.
+void i8_layout(int m, int k3, int km, int32_t* dims, int32_t* rows, uint64_t cap) {
+  SLFMM_CHECK(m >= 1 && m <= 32 && k3 >= 0 && k3 <= m && km >= 0 && km <= m, SLFMM_E_ARG, "operand shape");
+  const RowMap r = RowMap::make(m, k3, km);
+  dims[0] = r.NQ; dims[1] = r.NT0; dims[2] = r.NT1; dims[3] = r.g1; dims[4] = r.ones_g;
+  if (!rows) return;
+  const int nrow = 16 * (r.NT0 + r.NT1);
+  SLFMM_CHECK(cap >= (uint64_t)(2 * nrow), SLFMM_E_ARG, "rows buffer too small");
+  for (int row = 0; row < nrow; ++row) {
+    const int2 v = r.info(row);
+    rows[2 * row] = v.x;
+    rows[2 * row + 1] = v.y;
+  }
+}
 
 // Association pass over every SNP of g (fused epilogue).
 void launch_func_i8(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_Q, uint64_t m, const EpiArgs& ea,
                     bool /*fused_epilogue*/) {
-  // Latent directions: 2 int8 terms (g' side) and 1 (miss side); covariate
-  // directions (the t-statistic numerators, last d columns): 3 and 2.
+  // Latent directions: 2 int8 terms on both sides; covariate directions (the
+  // t-statistic numerators, last d columns): 3 on the g' side, 2 on the miss
+  // side.  (One miss-side term for the latent directions would save a row
+  // tile, but its error is amplified by the RSS difference: |delta log10 p|
+  // reached 8e-4 on test panels.)
   SLFMM_CHECK(m <= 32, SLFMM_E_ARG, "the int8 association path supports K + d <= 32 (use SLFMM_PREC_FP64)");
-  Operand op = kAssocMissTerms == 2 ? prepare_operand(ctx, "as_i8", d_Q, m, g->n, (int)m, ea.K, 0, true)
-                                    : prepare_operand(ctx, "as_i8", d_Q, m, g->n, (int)m, ea.K, ea.K, true);
+  Operand op = prepare_operand(ctx, "as_i8", d_Q, m, g->n, (int)m, ea.K, 0, true);
   SLFMM_CHECK(8 * split_plan_len(ea.d, (int)m) <= snp_plan_bytes(op.rmap.NQ), SLFMM_E_ARG,
               "association plan does not fit the int8 kernel's LDS (use SLFMM_PREC_FP64)");
   SnpArgs a{};

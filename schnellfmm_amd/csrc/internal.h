@@ -277,6 +277,7 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
 
 // ---- assoc_i8.hip: single-MFMA lane-layout probe (test-only entry point) ----
 void probe_mfma_i8(slfmm_ctx* ctx, const int8_t* hA, const int8_t* hB, int32_t* hD);
+void i8_layout(int m, int k3, int km, int32_t* dims, int32_t* rows, uint64_t cap);
 
 // ---- rsvd.hip ---------------------------------------------------------------
 // fill (optional): NMF in-RAM imputation of the estimation rows, with
