@@ -296,7 +296,7 @@ def batch_workload(args):
 
     torch.cuda.init()
     dev = torch.device("cuda:0")
-    rng = np.random.default_rng(4)
+    from schnellfmm_amd import synth
     ctx = S.Context(0)
     count = 64
     arr = (Problem * count)()
@@ -304,14 +304,9 @@ def batch_workload(args):
     shapes = []
     total_snps = 0
     total_bytes = 0
-    for i in range(count):
-        n = int(round(math.exp(rng.uniform(math.log(200), math.log(5000)))))
-        p = int(round(math.exp(rng.uniform(math.log(2e4), math.log(2e6)))))
-        K, d = int(rng.integers(3, 21)), int(rng.integers(1, 9))
-        pops, miss = int(rng.integers(2, 11)), float(rng.uniform(0.0, 0.05))
+    for i, n, p, K, d, pops, miss, x in synth.batch_problems(seed=4, count=count):
         g = S.BedFile.synthetic(n, p, n_pops=pops, fst=(0.02, 0.15), maf=(0.05, 0.5), miss_rate=miss,
                                 seed=4000 + i, ctx=ctx)
-        x = np.ascontiguousarray(rng.standard_normal((n, d)))
         idx = np.ascontiguousarray(S.subset_indices(0.1, p), dtype=np.int64)
         t = {k: torch.empty(sh, dtype=torch.float64, device=dev) for k, sh in
              (("beta", (p, d)), ("t", (p, d)), ("r2", (p, 3)), ("p", (p, d)), ("gpt", (d,)), ("gif", (1,)),

@@ -292,6 +292,33 @@ int slfmm_synth_bn(slfmm_ctx* ctx, uint64_t n, uint64_t n_snps, uint32_t n_pops,
                    double fst_hi, double maf_lo, double maf_hi, double miss_rate, uint64_t seed,
                    slfmm_geno** out);
 
+/* Generalised synthetic panel (no reference counterpart): admixed samples,
+ * inbred lines, Beta-distributed ancestral frequencies and structured
+ * missingness.  Per SNP: p_anc ~ U(maf_a, maf_b) or Beta(maf_a, maf_b)
+ * clamped to [maf_min, 1 - maf_min]; per population k: F_k ~ U(fst_lo, fst_hi),
+ * p_k ~ Beta(p_anc (1-F)/F, (1-p_anc)(1-F)/F).  Sample i's frequency is
+ * sum_k ancestry[i][k] p_k (or p of its group when ancestry is NULL: n_pops
+ * equal contiguous groups).  Genotype: Bin(2, p) (het_rate < 0) or an inbred
+ * line, a het call with probability het_rate and else 2 Bernoulli(p).
+ * Missing with probability min(1, miss_scale (1 - (1 - miss_rate)(1 - a_i)
+ * (1 - b_j))), a_i = sample_miss[i] (or 0), b_j ~ Beta(snp_miss_a,
+ * snp_miss_b) (0 when snp_miss_a <= 0); miss_scale <= 0 means 1.  Host
+ * arrays; n_pops <= 64. */
+typedef struct {
+  uint32_t n_pops;
+  double fst_lo, fst_hi;
+  int32_t maf_beta;
+  double maf_a, maf_b, maf_min;
+  const float* ancestry;
+  double het_rate;
+  double miss_rate;
+  const float* sample_miss;
+  double snp_miss_a, snp_miss_b;
+  double miss_scale;
+  uint64_t seed;
+} slfmm_synth_spec;
+int slfmm_synth_bn_ex(slfmm_ctx* ctx, uint64_t n, uint64_t n_snps, const slfmm_synth_spec* spec, slfmm_geno** out);
+
 /* ---- Output stage (src/testing.rs:740-801) ------------------------------ */
 
 /* Writes the calibrated results table of coalesce_output_tsv: header

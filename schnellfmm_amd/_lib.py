@@ -33,6 +33,14 @@ class AssocOut(C.Structure):
                 ("gif_per_trait", vp), ("gif", vp)]
 
 
+class SynthSpec(C.Structure):
+    """slfmm_synth_spec: options of the generalised synthetic panel."""
+    _fields_ = [("n_pops", u32), ("fst_lo", dbl), ("fst_hi", dbl), ("maf_beta", i32), ("maf_a", dbl),
+                ("maf_b", dbl), ("maf_min", dbl), ("ancestry", vp), ("het_rate", dbl), ("miss_rate", dbl),
+                ("sample_miss", vp), ("snp_miss_a", dbl), ("snp_miss_b", dbl), ("miss_scale", dbl),
+                ("seed", u64)]
+
+
 class Problem(C.Structure):
     """slfmm_problem: the arguments of slfmm_fit_lfmm2 for one problem of a batch."""
     _fields_ = [("est", vp), ("subset_idx", vp), ("n_idx", u64), ("g", vp), ("x", vp), ("d", u64),
@@ -77,6 +85,7 @@ SIGNATURES = {
     "slfmm_profile_enable": (C.c_int, [vp, i32]),
     "slfmm_profile_read": (C.c_int, [vp, C.c_char_p, vp, vp, i32]),
     "slfmm_synth_bn": (C.c_int, [vp, u64, u64, u32, dbl, dbl, dbl, dbl, dbl, u64, vp]),
+    "slfmm_synth_bn_ex": (C.c_int, [vp, u64, u64, vp, vp]),
 }
 
 _lib = None

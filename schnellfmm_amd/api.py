@@ -185,6 +185,40 @@ class BedFile:
                                        miss_rate, seed, C.byref(h)))
         return cls(h, n, p, ctx)
 
+    @classmethod
+    def synthetic_panel(cls, n: int, p: int, n_pops: int = 20, fst=(0.02, 0.15), maf=(0.05, 0.5),
+                        maf_beta: bool = False, maf_min: float = 0.0, ancestry=None, het_rate: float = -1.0,
+                        miss_rate: float = 0.0, sample_miss=None, snp_miss_beta=None, miss_scale: float = 0.0,
+                        seed: int = 1, ctx: Optional[Context] = None):
+        """Generalised Balding-Nichols panel in HBM (slfmm_synth_bn_ex): admixture
+        proportions (n x n_pops), inbred lines (het_rate >= 0), Beta(maf) ancestral
+        frequencies, per-sample (sample_miss) and per-SNP (Beta snp_miss_beta)
+        missingness."""
+        from ._lib import SynthSpec
+        ctx = ctx or default_context()
+        sp = SynthSpec()
+        sp.n_pops, sp.fst_lo, sp.fst_hi = n_pops, fst[0], fst[1]
+        sp.maf_beta, sp.maf_a, sp.maf_b, sp.maf_min = int(bool(maf_beta)), maf[0], maf[1], maf_min
+        keep = []
+        if ancestry is not None:
+            a = np.ascontiguousarray(ancestry, dtype=np.float32)
+            if a.shape != (n, n_pops):
+                raise ValueError("ancestry must be n x n_pops")
+            keep.append(a)
+            sp.ancestry = a.ctypes.data
+        sp.het_rate, sp.miss_rate = het_rate, miss_rate
+        if sample_miss is not None:
+            m = np.ascontiguousarray(sample_miss, dtype=np.float32)
+            if m.shape != (n,):
+                raise ValueError("sample_miss must have n entries")
+            keep.append(m)
+            sp.sample_miss = m.ctypes.data
+        sp.snp_miss_a, sp.snp_miss_b = (snp_miss_beta if snp_miss_beta is not None else (0.0, 0.0))
+        sp.miss_scale, sp.seed = miss_scale, seed
+        h = C.c_void_p()
+        ctx.check(lib().slfmm_synth_bn_ex(ctx.h, n, p, C.byref(sp), C.byref(h)))
+        return cls(h, n, p, ctx)
+
     def read_packed(self, j0: int = 0, rows: Optional[int] = None) -> np.ndarray:
         """Rows [j0, j0 + rows) as .bed payload bytes (rows x ceil(n/4), uint8)."""
         rows = self.n_snps - j0 if rows is None else rows

@@ -922,4 +922,26 @@ int slfmm_synth_bn(slfmm_ctx* ctx, uint64_t n, uint64_t n_snps, uint32_t n_pops,
   });
 }
 
+int slfmm_synth_bn_ex(slfmm_ctx* ctx, uint64_t n, uint64_t n_snps, const slfmm_synth_spec* spec, slfmm_geno** out) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && out && spec && n > 0 && spec->n_pops > 0 && spec->n_pops <= 64, SLFMM_E_ARG, "bad argument");
+    SLFMM_CHECK(spec->fst_lo > 0.0 && spec->fst_hi < 1.0 && spec->fst_lo <= spec->fst_hi, SLFMM_E_ARG,
+                "Fst range must lie in (0,1)");
+    SLFMM_CHECK(!spec->maf_beta || (spec->maf_a > 0.0 && spec->maf_b > 0.0 && spec->maf_min >= 0.0 &&
+                                    spec->maf_min < 0.5),
+                SLFMM_E_ARG, "Beta MAF parameters must be positive");
+    SLFMM_CHECK(spec->snp_miss_a <= 0.0 || spec->snp_miss_b > 0.0, SLFMM_E_ARG, "per-SNP missingness Beta(a, b) needs b > 0");
+    auto* g = new slfmm_geno();
+    g->n = n; g->p = n_snps;
+    try {
+      geno_alloc(ctx, g);
+      launch_synth_ex(ctx->stream, g->d, g->ld, n, n_snps, *spec);
+    } catch (...) {
+      slfmm_geno_free(g);
+      throw;
+    }
+    *out = g;
+  });
+}
+
 }  // extern "C"

@@ -371,6 +371,123 @@ __global__ void k_bn_geno(const float* __restrict__ freq, uint8_t* packed, uint6
   reinterpret_cast<uint32_t*>(packed + j * ld)[w] = word;
 }
 
+// ---- generalised panel: admixture, inbred lines, Beta MAF, structured missingness
+struct SynthDev {
+  uint32_t n_pops;
+  double fst_lo, fst_hi;
+  int maf_beta;
+  double maf_a, maf_b, maf_min;
+  const float* ancestry;      // device n x n_pops, or null
+  double het_rate;            // < 0: Hardy-Weinberg
+  double miss_rate;
+  const float* sample_miss;   // device n, or null
+  double snp_miss_a, snp_miss_b;
+  double miss_scale;
+  uint64_t seed;
+};
+
+__global__ void k_bn_freqs_ex(float* freq, float* snp_miss, uint64_t p, SynthDev sp, uint64_t seed) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= p) return;
+  double panc;
+  if (sp.maf_beta) {
+    const double x = gamma_draw(sp.maf_a, seed + 101, j * 1024), y = gamma_draw(sp.maf_b, seed + 103, j * 1024);
+    panc = x / (x + y);
+    panc = panc < sp.maf_min ? sp.maf_min : (panc > 1.0 - sp.maf_min ? 1.0 - sp.maf_min : panc);
+  } else {
+    panc = sp.maf_a + (sp.maf_b - sp.maf_a) * u01(seed ^ 0xA5A5A5A5ull, j);
+  }
+  for (uint32_t k = 0; k < sp.n_pops; ++k) {
+    double F = sp.fst_lo + (sp.fst_hi - sp.fst_lo) * u01(seed ^ 0x5EEDull, k);
+    double a = panc * (1.0 - F) / F, b = (1.0 - panc) * (1.0 - F) / F;
+    uint64_t base = (j * sp.n_pops + k) * 1024;
+    double x = gamma_draw(a, seed + 17, base), y = gamma_draw(b, seed + 29, base);
+    double f = x / (x + y);
+    f = f < 1e-6 ? 1e-6 : (f > 1.0 - 1e-6 ? 1.0 - 1e-6 : f);
+    freq[j * sp.n_pops + k] = (float)f;
+  }
+  double bm = 0.0;
+  if (sp.snp_miss_a > 0.0) {
+    const double x = gamma_draw(sp.snp_miss_a, seed + 211, j * 1024), y = gamma_draw(sp.snp_miss_b, seed + 223, j * 1024);
+    bm = x / (x + y);
+  }
+  snp_miss[j] = (float)bm;
+}
+
+__global__ void k_bn_geno_ex(const float* __restrict__ freq, const float* __restrict__ snp_miss, uint8_t* packed,
+                             uint64_t ld, uint64_t n, uint64_t p, SynthDev sp, uint64_t seed) {
+  uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t j = blockIdx.y + (uint64_t)blockIdx.z * 65535u;
+  if (j >= p || w * 16 >= n) return;
+  uint32_t word = 0;
+  const double bj = snp_miss[j];
+  const uint32_t thr_het = sp.het_rate >= 0.0 ? (uint32_t)(sp.het_rate * 2097152.0) : 0u;
+  for (int s = 0; s < 16; ++s) {
+    uint64_t i = w * 16 + s;
+    if (i >= n) break;
+    float f;
+    if (sp.ancestry) {
+      float acc = 0.0f;
+      for (uint32_t k = 0; k < sp.n_pops; ++k) acc += sp.ancestry[i * sp.n_pops + k] * freq[j * sp.n_pops + k];
+      f = acc;
+    } else {
+      f = freq[j * sp.n_pops + (uint32_t)((i * sp.n_pops) / n)];
+    }
+    const double ai = sp.sample_miss ? (double)sp.sample_miss[i] : 0.0;
+    double pm = sp.miss_scale * (1.0 - (1.0 - sp.miss_rate) * (1.0 - ai) * (1.0 - bj));
+    pm = pm < 1.0 ? pm : 1.0;
+    const uint32_t tf = (uint32_t)(f * 2097152.0f), tm = (uint32_t)(pm * 2097152.0);
+    uint64_t h = mix64(seed, j * n + i);
+    uint32_t u1 = (uint32_t)(h & 0x1FFFFF), u2 = (uint32_t)((h >> 21) & 0x1FFFFF), u3 = (uint32_t)(h >> 43);
+    uint32_t g;
+    if (sp.het_rate >= 0.0) g = u2 < thr_het ? 1u : 2u * (u1 < tf);
+    else g = (u1 < tf) + (u2 < tf);
+    uint32_t code = (u3 < tm) ? 1u : (g == 0 ? 0u : (g == 1 ? 2u : 3u));
+    word |= code << (2 * s);
+  }
+  reinterpret_cast<uint32_t*>(packed + j * ld)[w] = word;
+}
+
+void launch_synth_ex(hipStream_t s, uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p, const slfmm_synth_spec& spec) {
+  if (p == 0 || n == 0) return;
+  SynthDev sp{spec.n_pops, spec.fst_lo, spec.fst_hi, spec.maf_beta, spec.maf_a, spec.maf_b, spec.maf_min, nullptr,
+              spec.het_rate, spec.miss_rate, nullptr, spec.snp_miss_a, spec.snp_miss_b,
+              spec.miss_scale > 0.0 ? spec.miss_scale : 1.0, spec.seed};
+  const uint64_t chunk = 1ull << 22;   // SNPs per frequency batch (bounded scratch)
+  float *freq = nullptr, *bmiss = nullptr, *anc = nullptr, *smiss = nullptr;
+  try {
+    SLFMM_HIP(hipMalloc(&freq, std::min(chunk, p) * spec.n_pops * sizeof(float)));
+    SLFMM_HIP(hipMalloc(&bmiss, std::min(chunk, p) * sizeof(float)));
+    if (spec.ancestry) {
+      SLFMM_HIP(hipMalloc(&anc, n * spec.n_pops * sizeof(float)));
+      SLFMM_HIP(hipMemcpyAsync(anc, spec.ancestry, n * spec.n_pops * sizeof(float), hipMemcpyHostToDevice, s));
+      sp.ancestry = anc;
+    }
+    if (spec.sample_miss) {
+      SLFMM_HIP(hipMalloc(&smiss, n * sizeof(float)));
+      SLFMM_HIP(hipMemcpyAsync(smiss, spec.sample_miss, n * sizeof(float), hipMemcpyHostToDevice, s));
+      sp.sample_miss = smiss;
+    }
+    for (uint64_t j0 = 0; j0 < p; j0 += chunk) {
+      uint64_t nj = std::min(chunk, p - j0);
+      const uint64_t sd = spec.seed + 7919 * (j0 / chunk);
+      k_bn_freqs_ex<<<(unsigned)cdiv(nj, 256), 256, 0, s>>>(freq, bmiss, nj, sp, sd);
+      k_bn_geno_ex<<<grid_rows(cdiv(cdiv(n, 16), 256), nj), 256, 0, s>>>(freq, bmiss, packed + j0 * ld, ld, n, nj, sp,
+                                                                          spec.seed * 31 + j0);
+      SLFMM_HIP(hipGetLastError());
+    }
+    SLFMM_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    hipFree(freq); hipFree(bmiss); hipFree(anc); hipFree(smiss);
+    throw;
+  }
+  SLFMM_HIP(hipFree(freq));
+  SLFMM_HIP(hipFree(bmiss));
+  if (anc) SLFMM_HIP(hipFree(anc));
+  if (smiss) SLFMM_HIP(hipFree(smiss));
+}
+
 void launch_synth_bn(hipStream_t s, uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
                      uint32_t n_pops, double fst_lo, double fst_hi, double maf_lo, double maf_hi,
                      double miss_rate, uint64_t seed) {

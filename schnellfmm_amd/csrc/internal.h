@@ -195,6 +195,7 @@ void launch_decode_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64
 void launch_synth_bn(hipStream_t s, uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
                      uint32_t n_pops, double fst_lo, double fst_hi, double maf_lo, double maf_hi,
                      double miss_rate, uint64_t seed);
+void launch_synth_ex(hipStream_t s, uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p, const slfmm_synth_spec& spec);
 // Transposed copy (sample-major packed: row = sample, 4 SNPs per byte).
 void launch_transpose(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
                       uint8_t* out, uint64_t ldt);

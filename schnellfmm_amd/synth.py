@@ -98,3 +98,46 @@ def normal_matrix(seed: int, n: int, l: int) -> np.ndarray:
     u1 = 1.0 - u[:, 0]                                   # (0,1]
     r = np.sqrt(-2.0 * np.log(u1))
     return (r * np.cos(2.0 * np.pi * u[:, 1])).reshape(n, l)
+
+
+# ---------------------------------------------------------------------------
+# Workload shapes shared by the tests and the benchmark
+# ---------------------------------------------------------------------------
+
+def batch_problems(seed: int = 4, count: int = 64):
+    """The seeded batch of uneven LFMM2 problems: per problem (in draw order)
+    n ~ logU[200, 5000], p ~ logU[2e4, 2e6], K ~ U{3..20}, d ~ U{1..8},
+    2-10 populations, 0-5 % missing, then the n x d covariates.  Yields
+    (i, n, p, K, d, n_pops, miss, x); panel i uses generator seed 1000 seed + i."""
+    import math
+    rng = np.random.default_rng(seed)
+    for i in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(200), math.log(5000)))))
+        p = int(round(math.exp(rng.uniform(math.log(2e4), math.log(2e6)))))
+        K, d = int(rng.integers(3, 21)), int(rng.integers(1, 9))
+        pops, miss = int(rng.integers(2, 11)), float(rng.uniform(0.0, 0.05))
+        x = np.ascontiguousarray(rng.standard_normal((n, d)))
+        yield i, n, p, K, d, pops, miss, x
+
+
+def inbred_spec(n: int, seed: int = 2):
+    """Inbred-panel options (8 admixed groups, Dirichlet(0.3) ancestry, 5 % het
+    calls, MAF ~ Beta(0.4, 1.6) truncated at 0.01, per-sample missingness
+    ~ Beta(1, 19)) for BedFile.synthetic_panel, plus the ancestry matrix."""
+    rng = np.random.default_rng(seed)
+    anc = rng.dirichlet(np.full(8, 0.3), size=n)
+    smiss = rng.beta(1.0, 19.0, size=n)
+    return dict(n_pops=8, fst=(0.05, 0.15), maf=(0.4, 1.6), maf_beta=True, maf_min=0.01, ancestry=anc,
+                het_rate=0.05, sample_miss=smiss, seed=seed), anc
+
+
+def nmf_spec(n: int, seed: int = 5):
+    """NMF-workload options (8 components, Dirichlet(0.2) ancestry, missing
+    P_ij = 1 - (1 - a_i)(1 - b_j), a_i ~ Beta(2, 18), b_j ~ Beta(1, 30),
+    rescaled to 10 % overall) for BedFile.synthetic_panel."""
+    rng = np.random.default_rng(seed)
+    anc = rng.dirichlet(np.full(8, 0.2), size=n)
+    a = rng.beta(2.0, 18.0, size=n)
+    raw = 1.0 - (1.0 - a.mean()) * (1.0 - 1.0 / 31.0)      # E[P_ij] before rescaling
+    return dict(n_pops=8, fst=(0.02, 0.15), maf=(0.05, 0.5), ancestry=anc, sample_miss=a,
+                snp_miss_beta=(1.0, 30.0), miss_scale=0.10 / raw, seed=seed), anc

@@ -143,3 +143,25 @@ def test_pinned_host_panel(S):
         host.free()
     finally:
         hip.hipHostFree(ptr)
+
+
+def test_host_panel_fit_vs_oracle(S):
+    """A host-streamed FP64 fit against the oracle directly (same injected
+    sketch): U_hat angles < 1e-8, t / beta / r2 within 1e-8 relative."""
+    import oracle as O
+    n, p, K = 517, 7013, 6
+    pan, packed = _panel(n, p, seed=23)
+    clean = O.encode_bed(pan.g)          # the oracle reads the rows without padding garbage
+    xs = O.center_covariates(pan.x)
+    pre = O.precompute(xs, 1e-5)
+    b0 = pre.m.T @ np.random.default_rng(9).standard_normal((n, K + 10))
+    est = O.subset_indices(0.3, p)
+    cfg_o = O.Config(k=K, norm=O.EIGENSTRAT, chunk_size=1000, n_power_iter=2, seed=5)
+    u_ref, ref, _, _ = O.fit_lfmm2(clean, O.bytes_per_snp(n), n, p, pan.x, cfg_o, est, b0=b0)
+    host = S.BedFile.wrap_host(packed, n, p, chunk_snps=777)
+    cfg = S.Lfmm2Config(k=K, norm=S.EIGENSTRAT, n_power_iter=2, seed=5, precision=S.PREC_FP64)
+    got = S.fit_lfmm2(host, 0.3, host, pan.x, cfg, b0=b0)
+    assert np.max(O.principal_angles(got.u_hat, u_ref)) < 1e-8
+    assert np.allclose(got.tstat, ref.t, rtol=1e-8, atol=1e-10)
+    assert np.allclose(got.beta, ref.beta, rtol=1e-8, atol=1e-12)
+    assert np.allclose(got.r2, ref.r2, rtol=1e-8, atol=1e-12)
