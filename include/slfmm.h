@@ -197,6 +197,20 @@ int slfmm_test_associations(slfmm_ctx* ctx, const slfmm_geno* g, const double* x
                             const double* u_hat, const slfmm_config* cfg, slfmm_assoc_out* out,
                             uint32_t flags);
 
+/* Allocation-free variant of slfmm_test_associations (no counterpart in the
+ * reference): for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and outputs; every scratch buffer comes from a static device
+ * array of 96 MiB per GPU (held until the process ends), so the call makes no
+ * device allocation.  Requires SLFMM_OUT_DEVICE, a device-resident panel and
+ * SLFMM_PREC_I8 with K + d <= 32; shapes whose scratch would exceed the array
+ * (about n (4 (K + d) + ...) doubles plus the histogram) are rejected with
+ * SLFMM_E_ARG before any work is queued.  Calls that overlap on one GPU share
+ * the array and race. */
+int slfmm_test_associations_noalloc(slfmm_ctx* ctx, const slfmm_geno* g, const double* xs, uint64_t d,
+                                    const double* u_hat, const slfmm_config* cfg, slfmm_assoc_out* out,
+                                    uint32_t flags);
+
 /* fit_lfmm2 (src/lib.rs:162-263), mean imputation: centres x, estimates
  * U_hat on est (subset_idx of est, NULL = all), tests every SNP of g. */
 int slfmm_fit_lfmm2(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset_idx,

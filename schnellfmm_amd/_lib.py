@@ -76,6 +76,7 @@ SIGNATURES = {
     "slfmm_precompute": (C.c_int, [vp, vp, u64, u64, dbl, vp, vp, vp, vp]),
     "slfmm_estimate_factors": (C.c_int, [vp, vp, vp, u64, vp, u64, vp, vp, vp, u32]),
     "slfmm_test_associations": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, u32]),
+    "slfmm_test_associations_noalloc": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, u32]),
     "slfmm_fit_lfmm2": (C.c_int, [vp, vp, vp, u64, vp, vp, u64, vp, vp, vp, vp, u32]),
     "slfmm_fit_lfmm2_batch": (C.c_int, [vp, vp, u64, C.c_int, u32]),
     "slfmm_nmf_estimate": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp, u32]),

@@ -18,6 +18,18 @@ using namespace slfmm;
 
 void* slfmm_ctx::scratch(const std::string& name, size_t bytes) {
   if (bytes == 0) bytes = 16;
+  if (arena) {
+    auto a = arena_map.find(name);
+    if (a != arena_map.end() && a->second.second >= bytes) return arena + a->second.first;
+    const size_t off = (arena_top + 255) & ~(size_t)255;
+    if (off > arena_cap || bytes > arena_cap - off)
+      throw Error(SLFMM_E_NOMEM, "allocation-free call: scratch (" + name + ", " + std::to_string(bytes) +
+                                     " bytes) exceeds the " + std::to_string(arena_cap >> 20) +
+                                     " MiB static buffer");
+    arena_map[name] = {off, bytes};
+    arena_top = off + bytes;
+    return arena + off;
+  }
   auto it = pool.find(name);
   if (it != pool.end() && it->second.second >= bytes) return it->second.first;
   if (it != pool.end()) {
@@ -344,7 +356,10 @@ int slfmm_create(int device, slfmm_ctx** out) {
   *out = nullptr;
   slfmm_ctx* c = new slfmm_ctx();
   c->device = device;
-  int rc = guarded(c, [&] { SLFMM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)); });
+  int rc = guarded(c, [&] {
+    SLFMM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    noalloc_register(device);
+  });
   if (rc) {
     delete c;
     return rc;

@@ -673,19 +673,20 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
 
   AssocPlan pl = make_plan(ctx, n, d_xs, d, d_u, k, ridge);
   const uint64_t m = pl.m;
-  int32_t* nmiss = ctx->buf<int32_t>("as_nmiss", p);
-  int32_t* sumg = ctx->buf<int32_t>("as_sumg", p);
-  int32_t* nhom = ctx->buf<int32_t>("as_nhom", p);
+  const bool with_nmf = nmf && nmf->W;
+  // the int8 kernels carry at most 32 functionals (K + d); beyond that the FP64 kernels run
+  const bool i8 = cfg.precision == SLFMM_PREC_I8 && m <= 32;
+  // per-SNP integer counts of the FP64 kernels (the int8 kernels count in registers)
+  int32_t* nmiss = i8 ? nullptr : ctx->buf<int32_t>("as_nmiss", p);
+  int32_t* sumg = i8 ? nullptr : ctx->buf<int32_t>("as_sumg", p);
+  int32_t* nhom = i8 ? nullptr : ctx->buf<int32_t>("as_nhom", p);
   EpiArgs ea;
   ea.n = n; ea.p = p; ea.d = (int)d; ea.K = (int)k; ea.m = (int)m;
   ea.norm = cfg.norm; ea.df = (double)(n - 1 - d - k); ea.plan = pl.plan;
   ea.n_miss = nmiss; ea.sum_g = sumg; ea.n_hom = nhom;
   ea.beta = d_beta; ea.tstat = d_t; ea.r2 = d_r2;
-  const bool with_nmf = nmf && nmf->W;
   SLFMM_CHECK(!(cfg.precision == SLFMM_PREC_I8 && with_nmf), SLFMM_E_ARG,
               "NMF imputation runs on the FP64 path (use SLFMM_PREC_FP64)");
-  // the int8 kernels carry at most 32 functionals (K + d); beyond that the FP64 kernels run
-  const bool i8 = cfg.precision == SLFMM_PREC_I8 && m <= 32;
   SLFMM_CHECK(!(with_nmf && g->host), SLFMM_E_ARG, "NMF imputation needs a device-resident genotype panel");
   // FP64 operand: the basis padded to whole row blocks (shared by every row range)
   const uint64_t npad = rup(n, 64);
@@ -711,12 +712,12 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
     if (v.p == 0) return;
     EpiArgs e = ea;
     e.p = v.p;
-    e.n_miss = nmiss + j0; e.sum_g = sumg + j0; e.n_hom = nhom + j0;
     e.beta = d_beta + j0 * d; e.tstat = d_t + j0 * d; e.r2 = d_r2 + j0 * 3;
     if (i8) {
       launch_func_i8(ctx, &v, pl.Q, m, e, true);
       return;
     }
+    e.n_miss = nmiss + j0; e.sum_g = sumg + j0; e.n_hom = nhom + j0;
     launch_stats(s, v.d, v.ld, n, v.p, nmiss + j0, sumg + j0, nhom + j0);
     launch_mean_scale(s, nmiss + j0, sumg + j0, n, v.p, cfg.norm, mu + j0, sc + j0);
     {

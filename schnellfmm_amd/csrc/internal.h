@@ -64,6 +64,12 @@ struct slfmm_ctx {
   std::string err;
   // Named, grow-only device scratch buffers reused across calls.
   std::map<std::string, std::pair<void*, size_t>> pool;
+  // Allocation-free calls: while `arena` is set, scratch() hands out 256-byte
+  // aligned slices of that device buffer (bump allocation, name -> slice)
+  // instead of allocating; a request past arena_cap throws SLFMM_E_NOMEM.
+  unsigned char* arena = nullptr;
+  size_t arena_cap = 0, arena_top = 0;
+  std::map<std::string, std::pair<size_t, size_t>> arena_map;
   void* scratch(const std::string& name, size_t bytes);
   template <class T>
   T* buf(const std::string& name, size_t count) {
@@ -279,6 +285,10 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
 // ---- assoc_i8.hip: single-MFMA lane-layout probe (test-only entry point) ----
 void probe_mfma_i8(slfmm_ctx* ctx, const int8_t* hA, const int8_t* hB, int32_t* hD);
 void i8_layout(int m, int k3, int km, int32_t* dims, int32_t* rows, uint64_t cap);
+// ---- noalloc.hip ----------------------------------------------------------
+// Resolves the device address of the allocation-free entry point's static
+// scratch array on the current device (once per device).
+void noalloc_register(int device);
 
 // ---- rsvd.hip ---------------------------------------------------------------
 // fill (optional): NMF in-RAM imputation of the estimation rows, with
