@@ -351,7 +351,11 @@ static void launch_chol(hipStream_t s, const double* G, uint64_t m, uint64_t row
   SLFMM_HIP(hipGetLastError());
 }
 
-void cholqr2(slfmm_ctx* ctx, double* A, uint64_t lda, uint64_t m, uint64_t rows, double* d_R, int* st) {
+// With shift_all every pass may take the shift: a rank-deficient A then
+// yields A = Q R with Q orthonormal to O(shift / s_i^2) in the directions of
+// its singular values s_i (exact where s_i^2 >> shift), instead of failing.
+void cholqr2(slfmm_ctx* ctx, double* A, uint64_t lda, uint64_t m, uint64_t rows, double* d_R, int* st,
+             bool shift_all) {
   SLFMM_CHECK(m >= 1 && m <= (uint64_t)kSm, SLFMM_E_ARG, "cholqr2: 1..64 columns");
   hipStream_t s = ctx->stream;
   double* G = ctx->buf<double>("cqr_G", m * m);
@@ -368,12 +372,12 @@ void cholqr2(slfmm_ctx* ctx, double* A, uint64_t lda, uint64_t m, uint64_t rows,
   ts_gemm(s, A, lda, m, R1i, m, m, rows, 1.0, 0.0, T, m);
   // pass 2: T -> A
   ts_gram(ctx, T, m, m, T, m, m, rows, G);
-  launch_chol(s, G, m, rows, false, R2, R2i, st, 1, nullptr);
+  launch_chol(s, G, m, rows, shift_all, R2, R2i, st, 1, nullptr);
   ts_gemm(s, T, m, m, R2i, m, m, rows, 1.0, 0.0, A, lda);
   ts_gemm(s, R2, m, m, R1, m, m, m, 1.0, 0.0, Rout, m);
   // pass 3, only after a shifted first pass (*st == 1)
   ts_gram(ctx, A, lda, m, A, lda, m, rows, G, st);
-  launch_chol(s, G, m, rows, false, R2, R2i, st, 1, st);
+  launch_chol(s, G, m, rows, shift_all, R2, R2i, st, 1, st);
   ts_gemm(s, A, lda, m, R2i, m, m, rows, 1.0, 0.0, T, m, st);
   copy2d(s, T, m, A, lda, rows, m, st);
   ts_gemm(s, R2, m, m, Rout, m, m, m, 1.0, 0.0, Rtmp, m, st);

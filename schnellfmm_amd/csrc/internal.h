@@ -228,7 +228,8 @@ void colabsmax_scale(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t row
 // In-place thin QR of A (rows x m, lda) by CholeskyQR2 (a shifted first
 // pass, then two clean ones, if plain Cholesky breaks down); R (m x m upper)
 // to d_R if non-null.  *d_st <- 0 ok, 1 ok after the shift, 2 not full rank.
-void cholqr2(slfmm_ctx* ctx, double* A, uint64_t lda, uint64_t m, uint64_t rows, double* d_R, int* d_st);
+void cholqr2(slfmm_ctx* ctx, double* A, uint64_t lda, uint64_t m, uint64_t rows, double* d_R, int* d_st,
+             bool shift_all = false);
 // One shifted-CholeskyQR pass, out = A R^{-1} (column span only; see dense.hip).
 void cholqr1(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t m, uint64_t rows, double* out, uint64_t ldo,
              int* st);

@@ -339,7 +339,10 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   double* Rt = ctx->buf<double>("rs_Rt", l * l);
   double* Vs = ctx->buf<double>("rs_V", l * l);
   double* sv = ctx->buf<double>("rs_sv", l);
-  cholqr2(ctx, W, l, l, n, R, ctx->check_slot("", SLFMM_E_NUMERIC, "QR of the final projection failed"));
+  // every pass may shift: W is rank deficient when l = n (its columns are
+  // centred) or when the estimation panel has fewer than l polymorphic SNPs,
+  // and only its top K directions are kept (src/rsvd.rs:160-173)
+  cholqr2(ctx, W, l, l, n, R, ctx->check_slot("", SLFMM_E_NUMERIC, "QR of the final projection failed"), true);
   k_transpose_small<<<(unsigned)cdiv(l * l, 256), 256, 0, s>>>(R, (int)l, Rt);
   jacobi_svd_small(ctx, Rt, l, Vs, sv);                // left singular vectors of R
   ts_gemm(s, W, l, l, Vs, l, K, n, 1.0, 0.0, d_uhat, K);

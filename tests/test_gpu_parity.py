@@ -213,6 +213,36 @@ def test_rsvd_fp64_vs_oracle_injected_sketch(S):
         assert np.allclose(got * sgn, ref, rtol=1e-6, atol=1e-8)
 
 
+@pytest.mark.parametrize("case", ["l_equals_n", "few_polymorphic"])
+def test_rsvd_rank_deficient_final_projection(S, case):
+    """The final projection W = N Y Q_z is rank deficient when l = n (its
+    columns are centred) or when fewer than l estimation SNPs are polymorphic;
+    the reference's thin SVD still returns U_hat (src/rsvd.rs:160-173), and
+    so must the GPU: the K leading directions against the oracle."""
+    if case == "l_equals_n":
+        n, p, K, os_ = 30, 400, 3, 27
+        pan, packed, bps = _panel(n=n, p=p, seed=13, d=1)
+        idx = np.arange(p)
+    else:
+        n, p, K, os_ = 120, 60, 3, 12
+        pan = synth.balding_nichols(n, p, n_pops=3, fst=0.3, miss_rate=0.0, d=1, seed=14)
+        g = pan.g.copy()
+        g[:, 12:] = 0                       # 12 polymorphic SNPs < l = 15
+        pan.g[:] = g
+        packed, bps = O.encode_bed(pan.g), O.bytes_per_snp(n)
+        idx = np.arange(p)
+    xs = O.center_covariates(pan.x)
+    pre = O.precompute(xs, 1e-5)
+    omega = np.random.default_rng(6).standard_normal((n, K + os_))
+    b0 = pre.m.T @ omega
+    ref = O.estimate_factors_streaming(packed, bps, n, idx, pre, K, os_, 1, 100, 1, O.Impute(), b0=b0)
+    gb = S.BedFile.from_packed(packed, n, p)
+    got = S.estimate_factors_streaming(gb, None, xs, S.Lfmm2Config(k=K, oversampling=os_, n_power_iter=1, norm=1),
+                                       b0=b0)
+    assert np.all(np.isfinite(got))
+    assert np.max(O.principal_angles(got, ref)) < 1e-6
+
+
 def test_rsvd_generated_sketch_matches_oracle_with_same_stream(S):
     # The device draws Omega' (n x l, seed) and sketches with B0 = N Omega'.
     # The reference form with Omega = Q^T Omega' gives the same B0 = M^T Omega.
