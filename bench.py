@@ -47,9 +47,12 @@ def main():
     ap.add_argument("--est-rate", type=float, default=0.01)
     ap.add_argument("--power-iter", type=int, default=3)
     ap.add_argument("--precision", choices=["auto", "i8", "fp64"], default="auto")
-    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--cpu-snps", type=int, default=40000)
+    ap.add_argument("--cpu-snps", type=int, default=10000,
+                    help="association prefix of the CPU baseline (SNPs)")
+    ap.add_argument("--cpu-est-snps", type=int, default=5000,
+                    help="estimation SNPs of the CPU baseline's RSVD (scaled to the workload's subset)")
     ap.add_argument("--workload", choices=["lfmm2", "nmf", "batch", "host"], default="lfmm2",
                     help="lfmm2: the headline fit (default); nmf: NMF imputation estimation and the "
                          "NMF-imputed fit at n=5000, p=2M, nmf-k=8, 10 iterations, K=8, d=2 (FP64); "
@@ -138,25 +141,30 @@ def main():
                                         C.byref(out), 1))
 
     def timed(prec, steps, warmup, label):
+        """Wall time per fit (no kernel timing events), then a separate pass
+        with HIP-event timing of the named kernels on the library's stream."""
         tensors, out = alloc_out()
         for i in range(warmup):
             run(prec, tensors, out)
             log(f"[bench] {label} warmup {i + 1}/{warmup} done")
-        torch.cuda.synchronize()
-        lib().slfmm_profile_enable(ctx.h, 1)
-        for name in (b"assoc_stream", b"rsvd_project", b"rsvd_forward"):
-            lib().slfmm_profile_read(ctx.h, name, None, None, 1)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         for i in range(steps):
             run(prec, tensors, out)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t1) / steps
+        lib().slfmm_profile_enable(ctx.h, 1)
+        for name in (b"assoc_stream", b"rsvd_project", b"rsvd_forward"):
+            lib().slfmm_profile_read(ctx.h, name, None, None, 1)
+        prof_steps = max(1, min(steps, 3))
+        for i in range(prof_steps):
+            run(prec, tensors, out)
+        torch.cuda.synchronize()
         kern = {}
         for name in ("assoc_stream", "rsvd_project", "rsvd_forward"):
             ms, cnt = C.c_double(), C.c_uint64()
             lib().slfmm_profile_read(ctx.h, name.encode(), C.byref(ms), C.byref(cnt), 1)
-            kern[name] = (ms.value, cnt.value)
+            kern[name] = (ms.value, cnt.value, prof_steps)
         lib().slfmm_profile_enable(ctx.h, 0)
         log(f"[bench] {label}: {dt * 1e3:.2f} ms/step, kernels {kern}")
         return dt, kern, tensors
@@ -178,7 +186,7 @@ def main():
     fp64 = None
     max_dlp = None
     if prec_main != S.PREC_FP64 and args.fp64_steps > 0:
-        dt64, kern64, res64 = timed(S.PREC_FP64, args.fp64_steps, 0, "fp64")
+        dt64, kern64, res64 = timed(S.PREC_FP64, args.fp64_steps, 1, "fp64")
         fp64 = {"ms_per_step": dt64 * 1e3, "snps_per_s": p / dt64}
         import oracle as O
         t_m = result_main["t"].cpu().numpy(); g_m = result_main["gpt"].cpu().numpy()
@@ -190,7 +198,7 @@ def main():
         del res64
 
     # roofline of the dominant kernel (association genotype stream)
-    ams, acnt = kern["assoc_stream"]
+    ams, acnt, _ = kern["assoc_stream"]
     avg_ms = ams / max(acnt, 1)
     bytes_per_launch = p * math.ceil(n / 4)
     flops_per_launch = p * 2.0 * n * (2 + 2 * d + K)
@@ -203,7 +211,7 @@ def main():
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n, p, K, d),
-                "kernel": "k_gt_i8p<4,3,24,0>", "avg_ms": avg_ms, "bytes_per_launch": bytes_per_launch}
+                "kernel": i8_kernel_name(K, d, 0), "avg_ms": avg_ms, "bytes_per_launch": bytes_per_launch}
 
     cpu = None
     if args.cpu_baseline:
@@ -220,7 +228,7 @@ def main():
                    "n": n, "p": p, "K": K, "d": d, "est_snps": int(len(est_idx)),
                    "power_iter": args.power_iter, "parallelism": "single-gpu"},
         "roofline": roof,
-        "kernels_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kern.items()},
+        "kernels_ms_per_step": {k: v[0] / max(v[2], 1) for k, v in kern.items()},
         "fp64_gpu": fp64,
         "cpu_baseline": cpu,
         "output_tsv": tsv,
@@ -464,11 +472,24 @@ def host_workload(args):
     del hbuf_t
 
 
+def i8_kernel_name(K, d, mode):
+    """Template instance of the int8 SNP-side kernel that runs the pass
+    (layout from the library: k_snp_i8<NQ, NT0, NT1, ST, MODE>)."""
+    from schnellfmm_amd._lib import lib
+    m = K + d
+    dims = np.zeros(5, np.int32)
+    lib().slfmm_debug_i8_layout(m, K if mode == 0 else m, 0 if mode == 0 else m,
+                                dims.ctypes.data_as(C.c_void_p), None, 0)
+    nq, nt0, nt1 = (int(v) for v in dims[:3])
+    st = 4 if nt0 + nt1 <= 8 else 2
+    return f"k_snp_i8<{nq}, {nt0}, {nt1}, {st}, {mode}>"
+
+
 def measured_traffic(n, p, K, d):
     """HBM bytes per launch of the association kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (counters
     scaled by the device-copy calibration); None if no summary matches."""
-    path = os.path.join(ROOT, "profiles", "k_gt_i8p_traffic.json")
+    path = os.path.join(ROOT, "profiles", "k_snp_i8_traffic_n10000_p10M.json")
     try:
         with open(path) as f:
             rec = json.load(f)
@@ -480,32 +501,68 @@ def measured_traffic(n, p, K, d):
     return rec.get("hbm_bytes_per_launch")
 
 
+def _host_cpu():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(geno, x, u_hat, args, n, K, d):
-    """CPU restatement of the reference algorithm on a bounded SNP prefix."""
+    """CPU restatement of the reference algorithm (oracle/, numpy + OpenBLAS)
+    on a bounded sample of the same workload: the randomized SVD on a
+    --cpu-est-snps subset of the estimation SNPs (its cost is linear in them,
+    so the time is scaled to the workload's subset) and the dense
+    reference-form association on a --cpu-snps prefix (scaled to p).  The
+    value is p / (scaled RSVD + scaled association) in SNPs/s."""
     import oracle as O
     try:
         from threadpoolctl import threadpool_info
-        threads = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+        info = threadpool_info()
+        threads = max((i.get("num_threads", 1) for i in info), default=1)
+        blas = sorted({f"{i.get('internal_api')} {i.get('version')}" for i in info})
     except Exception:
-        threads = os.cpu_count() or 1
-    m = int(args.cpu_snps)
-    idx = np.arange(m)
-    raw = geno.decode_raw(idx)                 # n x m genotypes (exact), re-packed for the oracle
-    g = np.where(np.isnan(raw), 255, raw).astype(np.uint8)
-    packed = O.encode_bed(g)
-    bps = O.bytes_per_snp(n)
+        threads, blas = os.cpu_count() or 1, []
+    p = geno.n_snps
+    p_est = len(np.arange(0, p, int(math.ceil(1.0 / args.est_rate))))
     xs = O.center_covariates(x)
+    # RSVD on a sample of the estimation SNPs (every 100th SNP, the first ones)
+    me = min(int(args.cpu_est_snps), p_est)
+    step = int(math.ceil(1.0 / args.est_rate))
+    est = np.arange(0, me * step, step)
+    raw = geno.decode_raw(est)
+    packed_e = O.encode_bed(np.where(np.isnan(raw), 255, raw).astype(np.uint8))
+    del raw
+    bps = O.bytes_per_snp(n)
+    omega = np.random.default_rng(1).standard_normal((n, K + 10))
     t0 = time.perf_counter()
     pre = O.precompute(xs, 1e-5)
     t1 = time.perf_counter()
-    O.test_associations_fused(packed, bps, n, m, xs, u_hat, pre, K, 500, O.EIGENSTRAT, O.Impute())
+    O.estimate_factors_streaming(packed_e, bps, n, np.arange(me), pre, K, 10, args.power_iter, 10_000,
+                                 O.EIGENSTRAT, O.Impute(), omega=omega)
     t2 = time.perf_counter()
-    rate = m / (t2 - t1)
-    return {"value": rate, "unit": "SNPs/s", "cores": int(threads), "kind": "port",
-            "sample": f"oracle dense reference-form association on the first {m} SNPs at n={n}, "
-                      f"K={K}, d={d} (chunks of 500; precompute {t1 - t0:.1f}s not counted; "
-                      f"RSVD excluded, U_hat from the GPU run)",
-            "seconds": t2 - t1}
+    # association on a prefix, with the GPU's U_hat
+    m = min(int(args.cpu_snps), p)
+    raw = geno.decode_raw(np.arange(m))
+    packed = O.encode_bed(np.where(np.isnan(raw), 255, raw).astype(np.uint8))
+    del raw
+    t3 = time.perf_counter()
+    O.test_associations_fused(packed, bps, n, m, xs, u_hat, pre, K, 10_000, O.EIGENSTRAT, O.Impute())
+    t4 = time.perf_counter()
+    rsvd_full = (t1 - t0) + (t2 - t1) * p_est / me
+    assoc_full = (t4 - t3) * p / m
+    return {"value": p / (rsvd_full + assoc_full), "unit": "SNPs/s", "cores": int(threads), "kind": "port",
+            "sample": f"oracle/ float64 restatement of the reference algorithm at n={n}, K={K}, d={d}: "
+                      f"precompute + RSVD (q={args.power_iter}) on {me} of the {p_est} estimation SNPs, "
+                      f"scaled x{p_est / me:.0f}, and the dense association on the first {m} SNPs, "
+                      f"scaled x{p / m:.0f}; chunks of 10000",
+            "seconds": {"precompute": t1 - t0, "rsvd_sample": t2 - t1, "assoc_sample": t4 - t3,
+                        "rsvd_scaled": rsvd_full, "assoc_scaled": assoc_full},
+            "host": {"cpu": _host_cpu(), "nproc": os.cpu_count(), "blas": blas, "threads": int(threads)}}
 
 
 if __name__ == "__main__":

@@ -102,8 +102,18 @@ def _codes(packed: np.ndarray, bps: int, n_phys: int, snp_idx: Sequence[int],
     return code.T                                      # n_out x c
 
 
+# Byte -> the four decoded values of its samples (bits 2j..2j+1 -> sample j).
+_BYTE_TABLE = DECODE_TABLE[(np.arange(256)[:, None] >> (2 * np.arange(4))[None, :]) & 0x03]
+
+
 def decode_raw(packed, bps: int, n_phys: int, snp_idx, sample_keep=None) -> np.ndarray:
     """{0,1,2,NaN} decode; src/bed.rs:246-281."""
+    if sample_keep is None:   # every physical sample in order: one table lookup per byte
+        pk = np.frombuffer(packed, dtype=np.uint8) if not isinstance(packed, np.ndarray) else packed
+        idx = np.asarray(snp_idx, dtype=np.int64)
+        rows = pk.reshape(-1, bps)[idx] if pk.size >= bps else np.zeros((0, bps), np.uint8)
+        vals = _BYTE_TABLE[rows].reshape(len(idx), 4 * bps)[:, :n_phys]
+        return np.ascontiguousarray(vals.T)
     return DECODE_TABLE[_codes(packed, bps, n_phys, snp_idx, sample_keep)]
 
 
@@ -134,7 +144,8 @@ def decode_chunk(packed, bps: int, n_phys: int, snp_idx, norm: int,
         nan = np.isnan(col)
         col[nan] = np.asarray(fill)[nan]
     valid = ~np.isnan(col)
-    s = _seq_colsum(np.where(valid, col, 0.0))
+    # integer dosages sum exactly in any order; filled values need the reference's
+    s = _seq_colsum(np.where(valid, col, 0.0)) if fill is not None else np.where(valid, col, 0.0).sum(axis=0)
     nv = valid.sum(axis=0)
     mean = np.where(nv > 0, s / np.maximum(nv, 1), 0.0)
     scale = compute_scale(mean, norm)
