@@ -55,7 +55,12 @@ enum {
                              K + d > 32, runs that stage's FP64 kernels. */
 };
 
-enum { SLFMM_OUT_DEVICE = 1u };
+/* SLFMM_OUT_DEVICE: outputs stay in device memory (the pointers in the
+ * output structs are device pointers).  SLFMM_EAGER: queue every kernel of
+ * a fit directly; without it, fits with device-resident panels and device
+ * outputs replay a HIP graph captured by an earlier call with the same
+ * shapes, configuration and pointers (no counterpart in the reference). */
+enum { SLFMM_OUT_DEVICE = 1u, SLFMM_EAGER = 2u };
 
 /* Lfmm2Config (src/lib.rs:47-98) plus the GPU precision switch. */
 typedef struct {
@@ -282,6 +287,10 @@ int slfmm_profile_enable(slfmm_ctx* ctx, int32_t on);
  * clears it after reading. */
 int slfmm_profile_read(slfmm_ctx* ctx, const char* name, double* total_ms, uint64_t* launches,
                        int32_t reset);
+/* Fits queued as HIP graphs on the context and its batch streams since it
+ * was created: graphs captured, and replays of a cached graph (no reference
+ * counterpart). */
+int slfmm_graph_stats(slfmm_ctx* ctx, uint64_t* captured, uint64_t* replayed);
 
 /* Diagnostic (no reference counterpart): one int8 MFMA 16x16x64 with the
  * fragment mapping of the mixed-precision kernels; D = A (16x64) B (64x16),

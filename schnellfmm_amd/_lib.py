@@ -85,6 +85,7 @@ SIGNATURES = {
     "slfmm_debug_i8_layout": (C.c_int, [i32, i32, i32, vp, vp, u64]),
     "slfmm_profile_enable": (C.c_int, [vp, i32]),
     "slfmm_profile_read": (C.c_int, [vp, C.c_char_p, vp, vp, i32]),
+    "slfmm_graph_stats": (C.c_int, [vp, vp, vp]),
     "slfmm_synth_bn": (C.c_int, [vp, u64, u64, u32, dbl, dbl, dbl, dbl, dbl, u64, vp]),
     "slfmm_synth_bn_ex": (C.c_int, [vp, u64, u64, vp, vp]),
 }

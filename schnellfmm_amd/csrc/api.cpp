@@ -32,6 +32,8 @@ void* slfmm_ctx::scratch(const std::string& name, size_t bytes) {
   }
   auto it = pool.find(name);
   if (it != pool.end() && it->second.second >= bytes) return it->second.first;
+  if (capturing) throw CaptureAbort();
+  ++pool_gen;
   if (it != pool.end()) {
     SLFMM_HIP(hipStreamSynchronize(stream));
     SLFMM_HIP(hipFree(it->second.first));
@@ -61,6 +63,39 @@ void* slfmm_ctx::pinned_buf(int i, size_t bytes) {
   }
   pinned_bytes[i] = bytes;
   return pinned[i];
+}
+
+void slfmm_ctx::upload(void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  if (arena) {
+    SLFMM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    return;
+  }
+  size_t off = (up_top + 255) & ~(size_t)255;
+  if (off > up_cap || bytes > up_cap - off) {   // grow once the queued uploads are done
+    SLFMM_HIP(hipStreamSynchronize(stream));
+    if (up_buf) (void)hipHostFree(up_buf);
+    up_buf = nullptr;
+    up_cap = 0;
+    const size_t cap = std::max<size_t>(std::max<size_t>(bytes, 2 * off), 1 << 20);
+    if (hipHostMalloc(reinterpret_cast<void**>(&up_buf), cap, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      up_buf = nullptr;
+      throw Error(SLFMM_E_NOMEM, "pinned host allocation of " + std::to_string(cap) + " bytes failed");
+    }
+    up_cap = cap;
+    off = 0;
+  }
+  std::memcpy(up_buf + off, src, bytes);
+  SLFMM_HIP(hipMemcpyAsync(dst, up_buf + off, bytes, hipMemcpyHostToDevice, stream));
+  if (!ev_up) SLFMM_HIP(hipEventCreateWithFlags(&ev_up, hipEventDisableTiming));
+  SLFMM_HIP(hipEventRecord(ev_up, stream));
+  up_top = off + bytes;
+}
+
+void slfmm_ctx::upload_reset() {
+  if (up_top && ev_up) SLFMM_HIP(hipEventSynchronize(ev_up));
+  up_top = 0;
 }
 
 void slfmm_ctx::ensure_copy_stream() {
@@ -103,7 +138,7 @@ void slfmm_ctx::collect() {
 int* slfmm_ctx::check_slot(const std::string& warn, int code, const std::string& e) {
   SLFMM_CHECK(checks.size() < (size_t)kMaxChecks, SLFMM_E_ARG, "too many solver status slots in one call");
   if (!status) status = static_cast<int*>(scratch("__status", kMaxChecks * sizeof(int)));
-  if (checks.empty()) SLFMM_HIP(hipMemsetAsync(status, 0, kMaxChecks * sizeof(int), stream));
+  if (checks.empty()) launch_fill(stream, reinterpret_cast<double*>(status), 0.0, kMaxChecks / 2);   // a kernel: graph-safe
   checks.push_back({warn, code, check_prefix + e});
   return status + (checks.size() - 1);
 }
@@ -132,9 +167,13 @@ slfmm_ctx::~slfmm_ctx() {
   for (auto& kv : timers)
     for (auto& ev : kv.second.pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
   if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& kv : graphs)
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
   for (auto& kv : pool) (void)hipFree(kv.second.first);
   for (int b = 0; b < 2; ++b)
     if (pinned[b]) (void)hipHostFree(pinned[b]);
+  if (up_buf) (void)hipHostFree(up_buf);
+  if (ev_up) (void)hipEventDestroy(ev_up);
   if (copy_stream) (void)hipStreamDestroy(copy_stream);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -145,7 +184,7 @@ namespace {
 template <class T>
 T* to_dev(slfmm_ctx* ctx, const char* name, const T* h, size_t count) {
   T* d = ctx->buf<T>(name, count);
-  if (count) SLFMM_HIP(hipMemcpyAsync(d, h, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  if (count) ctx->upload(d, h, count * sizeof(T));
   return d;
 }
 
@@ -660,6 +699,108 @@ static void assoc_host(slfmm_ctx* ctx, const slfmm_geno* g, const double* dx, ui
   }
 }
 
+// Queues the device work of one fit (randomized SVD, association, GIF) on
+// ctx's stream.  With device-resident panels and device outputs the work is
+// captured into a HIP graph once per (shapes, configuration, pointers, pool
+// generation, first status slot) and replayed by later calls: one graph
+// launch instead of ~200 kernel launches.  The host inputs are uploaded
+// before (outside) the graph, so a replay reads this call's values.  A
+// capture that would need a new device buffer is dropped and the work is
+// queued directly; the next call captures again.
+extern "C++" {
+namespace {
+constexpr size_t kMaxGraphs = 96;   // cached fit graphs per context (per stream of a batch)
+
+template <class T>
+void key_add(std::string& k, const T& v) {
+  k.append(reinterpret_cast<const char*>(&v), sizeof(T));
+}
+}  // namespace
+}
+
+static void queue_fit(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* idx, uint64_t n_idx,
+                      const slfmm_geno* g, const double* dx, uint64_t d, const slfmm_config& cfg,
+                      const double* db0, double* du, slfmm_assoc_out* out, uint32_t flags) {
+  auto direct = [&] {
+    rsvd_run(ctx, est, idx, n_idx, dx, d, cfg, db0, du);   // src/lib.rs:231
+    assoc_host(ctx, g, dx, d, du, &cfg, out, flags);       // src/lib.rs:254
+  };
+  const bool graphable = (flags & SLFMM_OUT_DEVICE) && !(flags & SLFMM_EAGER) && !est->host && !g->host &&
+                         !ctx->profiling && !ctx->arena;
+  if (!graphable) {
+    direct();
+    return;
+  }
+  // the estimation indices go to their device buffer now; an identity subset needs none
+  bool identity = idx == nullptr || n_idx == est->p;
+  for (uint64_t i = 0; identity && idx && i < n_idx; ++i) identity = idx[i] == (int64_t)i;
+  if (!identity) {
+    int64_t* didx = ctx->buf<int64_t>("rs_Y_idx", n_idx);
+    ctx->upload(didx, idx, n_idx * 8);
+  }
+  std::string key;
+  key_add(key, est); key_add(key, est->d); key_add(key, est->n); key_add(key, est->p); key_add(key, est->ld);
+  key_add(key, g); key_add(key, g->d); key_add(key, g->n); key_add(key, g->p); key_add(key, g->ld);
+  key_add(key, n_idx); key_add(key, identity); key_add(key, d); key_add(key, cfg);
+  key_add(key, dx); key_add(key, db0); key_add(key, du); key_add(key, *out); key_add(key, flags);
+  const size_t slot0 = ctx->checks.size();
+  key_add(key, slot0);
+  auto it = ctx->graphs.find(key);
+  if (it != ctx->graphs.end() && it->second.pool_gen == ctx->pool_gen) {
+    SLFMM_HIP(hipGraphLaunch(it->second.exec, ctx->stream));
+    ctx->checks.insert(ctx->checks.end(), it->second.checks.begin(), it->second.checks.end());
+    ++ctx->graphs_replayed;
+    return;
+  }
+  if (it != ctx->graphs.end() || ctx->graphs.size() >= kMaxGraphs) {
+    SLFMM_HIP(hipStreamSynchronize(ctx->stream));   // an earlier replay may still run
+    if (it != ctx->graphs.end()) {
+      (void)hipGraphExecDestroy(it->second.exec);
+      ctx->graphs.erase(it);
+    } else {
+      for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second.exec);
+      ctx->graphs.clear();
+    }
+  }
+  const uint64_t gen0 = ctx->pool_gen;
+  SLFMM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = true;
+  ctx->staged_idx = identity ? nullptr : idx;
+  bool aborted = false;
+  try {
+    direct();
+  } catch (const std::exception&) {   // CaptureAbort, or an error that the direct run reports again
+    aborted = true;
+  }
+  ctx->capturing = false;
+  ctx->staged_idx = nullptr;
+  hipGraph_t gr = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &gr);
+  if (aborted || ec != hipSuccess || !gr) {
+    if (gr) (void)hipGraphDestroy(gr);
+    (void)hipGetLastError();
+    ctx->checks.resize(slot0);
+    ctx->staged_idx = identity ? nullptr : idx;   // the indices are already in their buffer
+    try {
+      direct();
+    } catch (...) {
+      ctx->staged_idx = nullptr;
+      throw;
+    }
+    ctx->staged_idx = nullptr;
+    return;
+  }
+  slfmm_ctx::FitGraph fg;
+  fg.pool_gen = gen0;
+  fg.checks.assign(ctx->checks.begin() + slot0, ctx->checks.end());
+  const hipError_t ei = hipGraphInstantiate(&fg.exec, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  SLFMM_HIP(ei);
+  SLFMM_HIP(hipGraphLaunch(fg.exec, ctx->stream));
+  ctx->graphs.emplace(key, std::move(fg));
+  ++ctx->graphs_captured;
+}
+
 int slfmm_test_associations(slfmm_ctx* ctx, const slfmm_geno* g, const double* xs, uint64_t d,
                             const double* u_hat, const slfmm_config* cfg, slfmm_assoc_out* out, uint32_t flags) {
   return guarded(ctx, [&] {
@@ -686,8 +827,7 @@ int slfmm_fit_lfmm2(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset
     const double* db0 = b0 ? to_dev(ctx, "api_b0", b0, n * l) : nullptr;
     const bool dev = flags & SLFMM_OUT_DEVICE;
     double* du = (dev && u_hat) ? u_hat : ctx->buf<double>("api_uhat", n * K);
-    rsvd_run(ctx, est, subset_idx, subset_idx ? n_idx : est->p, dx, d, *cfg, db0, du);   // src/lib.rs:231
-    assoc_host(ctx, g, dx, d, du, cfg, out, flags);                    // src/lib.rs:254
+    queue_fit(ctx, est, subset_idx, subset_idx ? n_idx : est->p, g, dx, d, *cfg, db0, du, out, flags);
     if (!dev && u_hat) to_host(ctx, u_hat, du, n * K);
     SLFMM_HIP(hipStreamSynchronize(ctx->stream));
   });
@@ -720,7 +860,10 @@ int slfmm_fit_lfmm2_batch(slfmm_ctx* ctx, slfmm_problem* probs, uint64_t count, 
     // panels and inputs queued on the context's own stream come first
     SLFMM_HIP(hipStreamSynchronize(ctx->stream));
     std::vector<slfmm_ctx*> ws(ctx->workers.begin(), ctx->workers.begin() + n_streams);
-    for (slfmm_ctx* w : ws) w->checks.clear();
+    for (slfmm_ctx* w : ws) {
+      w->checks.clear();
+      w->upload_reset();
+    }
     std::vector<std::vector<double>> xs(count);   // host sources of the queued copies
     const bool dev = flags & SLFMM_OUT_DEVICE;
     auto drain = [&] {
@@ -739,8 +882,8 @@ int slfmm_fit_lfmm2_batch(slfmm_ctx* ctx, slfmm_problem* probs, uint64_t count, 
         double* dx = to_dev(w, "api_xs", xs[i].data(), n * d);
         const double* db0 = P.b0 ? to_dev(w, "api_b0", P.b0, n * l) : nullptr;
         double* du = (dev && P.u_hat) ? P.u_hat : w->buf<double>("api_uhat", n * K);
-        rsvd_run(w, P.est, P.subset_idx, P.subset_idx ? P.n_idx : P.est->p, dx, d, P.cfg, db0, du);
-        assoc_host(w, P.g, dx, d, du, &P.cfg, &P.out, flags);
+        queue_fit(w, P.est, P.subset_idx, P.subset_idx ? P.n_idx : P.est->p, P.g, dx, d, P.cfg, db0, du, &P.out,
+                  flags);
         if (!dev && P.u_hat) to_host(w, P.u_hat, du, n * K);
       } catch (const Error& e) {
         const std::string tag = w->check_prefix;
@@ -915,6 +1058,19 @@ int slfmm_profile_read(slfmm_ctx* ctx, const char* name, double* total_ms, uint6
     if (total_ms) *total_ms = t.ms;
     if (launches) *launches = t.launches;
     if (reset) { t.ms = 0.0; t.launches = 0; }
+  });
+}
+
+int slfmm_graph_stats(slfmm_ctx* ctx, uint64_t* captured, uint64_t* replayed) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx, SLFMM_E_ARG, "NULL context");
+    uint64_t c = ctx->graphs_captured, r = ctx->graphs_replayed;
+    for (const slfmm_ctx* w : ctx->workers) {
+      c += w->graphs_captured;
+      r += w->graphs_replayed;
+    }
+    if (captured) *captured = c;
+    if (replayed) *replayed = r;
   });
 }
 

@@ -636,7 +636,7 @@ static AssocPlan make_plan(slfmm_ctx* ctx, uint64_t n, const double* d_xs, uint6
   // to [U_c|X] itself if that fails or if C^T C needed regularisation.
   pl.Q = ctx->buf<double>("as_Q", n * m);
   double* T = ctx->buf<double>("as_T", m * m);
-  SLFMM_HIP(hipMemcpyAsync(pl.Q, cc, n * m * 8, hipMemcpyDeviceToDevice, s));
+  copy2d(s, cc, m, pl.Q, m, n, m);
   int* st_cq = ctx->check_slot("", SLFMM_OK, "");
   cholqr2(ctx, pl.Q, m, m, n, T, st_cq);
   k_select_basis<<<(unsigned)std::min<uint64_t>(cdiv(n * m, 256), 2048), 256, 0, s>>>(st_ginv, st_cq, cc, pl.Q, n * m);
@@ -802,7 +802,7 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
   unsigned long long* hist = d_hist ? (unsigned long long*)d_hist
                                     : ctx->buf<unsigned long long>("as_hist", d * (uint64_t)kBins);
   unsigned long long* ctot = ctx->buf<unsigned long long>("as_hist_chunks", d * (uint64_t)kHistChunks);
-  SLFMM_HIP(hipMemsetAsync(hist, 0, d * (uint64_t)kBins * 8, s));
+  launch_fill(s, reinterpret_cast<double*>(hist), 0.0, d * (uint64_t)kBins);   // zero bits
   const int LB = (int)std::min<uint64_t>(16384, 32768 / d);   // LDS bins per trait (128 KB)
   const uint64_t nblk = std::max<uint64_t>(1, std::min<uint64_t>(256, cdiv(p, 4096)));
   const uint64_t spb = std::max<uint64_t>(1, cdiv(p, nblk));

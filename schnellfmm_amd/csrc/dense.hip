@@ -179,37 +179,55 @@ void ts_gram(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t a, const do
 }
 
 // ---- tall-skinny GEMM: out = alpha A M + beta out -------------------------
+// Each pass of a workgroup stages rpb = 256 / cw rows of A in LDS with
+// coalesced loads; thread (row, column) then forms its dot product from LDS.
 __global__ void __launch_bounds__(256) k_ts_gemm(const double* __restrict__ A, uint64_t lda, int m,
                                                  const double* __restrict__ M, uint64_t ldm, int q,
                                                  uint64_t rows, double alpha, double beta, double* out,
-                                                 uint64_t ldo, const int* gate) {
+                                                 uint64_t ldo, const int* gate, const double* dlam, int inverse) {
   if (gate && *gate != 1) return;
   __shared__ double sm[kSm * kSm];
-  for (int e = threadIdx.x; e < m * q; e += 256) sm[e] = M[(uint64_t)(e / q) * ldm + (e % q)];
-  __syncthreads();
+  __shared__ double sa[8][kSm + 1];
+  for (int e = threadIdx.x; e < m * q; e += 256) {
+    double v = M[(uint64_t)(e / q) * ldm + (e % q)];
+    if (dlam) {   // row t of M scaled by -(1 - d_t), or 1 / d_t - 1 for the inverse
+      const double dl = dlam[e / q];
+      v *= inverse ? (1.0 / dl - 1.0) : -(1.0 - dl);
+    }
+    sm[e] = v;
+  }
   // thread = (row, column) on a 256 / cw x cw grid, cw = 32 or 64 >= q
   const int cw = q <= 32 ? 32 : 64;
   const int c = threadIdx.x & (cw - 1);
   const int rpb = 256 / cw;
-  if (c >= q) return;
-  for (uint64_t r = (uint64_t)blockIdx.x * rpb + (threadIdx.x / cw); r < rows; r += (uint64_t)gridDim.x * rpb) {
-    const double* ar = A + r * lda;
-    double s = 0.0;
-    for (int t = 0; t < m; ++t) s += ar[t] * sm[t * q + c];
-    double v = alpha * s;
-    if (beta != 0.0) v += beta * out[r * ldo + c];
-    out[r * ldo + c] = v;
+  const int rr = threadIdx.x / cw;
+  for (uint64_t r0 = (uint64_t)blockIdx.x * rpb; r0 < rows; r0 += (uint64_t)gridDim.x * rpb) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < rpb * m; e += 256) {
+      const int i = e / m, t = e % m;
+      sa[i][t] = r0 + i < rows ? A[(r0 + i) * lda + t] : 0.0;
+    }
+    __syncthreads();
+    const uint64_t r = r0 + rr;
+    if (c < q && r < rows) {
+      double s = 0.0;
+      for (int t = 0; t < m; ++t) s += sa[rr][t] * sm[t * q + c];
+      double v = alpha * s;
+      if (beta != 0.0) v += beta * out[r * ldo + c];
+      out[r * ldo + c] = v;
+    }
   }
 }
 
 void ts_gemm(hipStream_t s, const double* A, uint64_t lda, uint64_t m, const double* M,
              uint64_t ldm, uint64_t q, uint64_t rows, double alpha, double beta, double* out,
-             uint64_t ldo, const int* gate) {
+             uint64_t ldo, const int* gate, const double* dlam, bool inverse) {
   if (rows == 0 || q == 0) return;
   SLFMM_CHECK(m <= (uint64_t)kSm && q <= (uint64_t)kSm, SLFMM_E_ARG, "ts_gemm: too many columns");
   const uint64_t rpb = q <= 32 ? 8 : 4;
   uint64_t blocks = std::min<uint64_t>(cdiv(rows, rpb), 1024);   // M is staged once per workgroup
-  k_ts_gemm<<<(unsigned)blocks, 256, 0, s>>>(A, lda, (int)m, M, ldm, (int)q, rows, alpha, beta, out, ldo, gate);
+  k_ts_gemm<<<(unsigned)blocks, 256, 0, s>>>(A, lda, (int)m, M, ldm, (int)q, rows, alpha, beta, out, ldo, gate, dlam,
+                                             inverse ? 1 : 0);
   SLFMM_HIP(hipGetLastError());
 }
 
@@ -344,10 +362,93 @@ __global__ void __launch_bounds__(256) k_chol(const double* __restrict__ G, int 
   if (t == 0 && mode == 0) *st = attempt;
 }
 
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, lane), hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), lane);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// The same factorisation, shift rule, outputs and status words as k_chol
+// for m <= 32, on one wave with no LDS: lane l keeps column l of the
+// working matrix in registers, and the values other lanes need are read
+// with v_readlane (wave-uniform).  The arithmetic of every element is
+// k_chol's: R[j][l] = A[j][l] / r_jj, then A[i][l] -= R[j][i] R[j][l] for
+// j < i <= l.  R^{-1}: lane c forms column c by back substitution,
+// X[i] = -(sum_{k > i} R[i][k] X[k]) / R[i][i].
+// Returns 0, or j + 1 for a failed pivot j; column l of R is left in col.
+__device__ __forceinline__ int chol32_factor(const double* __restrict__ G, int m, double shift, double tol,
+                                             double (&col)[32]) {
+  const int l = threadIdx.x;
+  const bool lv = l < m;
+#pragma unroll
+  for (int i = 0; i < 32; ++i) col[i] = (lv && i < m) ? G[i * m + l] + (i == l ? shift : 0.0) : 0.0;
+  const double diag0 = lv ? G[l * m + l] + shift : 0.0;   // lane j's pivot reference, read by step j
+  int fail = 0;
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    if (j < m && !fail) {   // wave-uniform
+      const double dj = lane_value(col[j], j);
+      const double d0 = lane_value(diag0, j);
+      if (!(dj > tol * d0) || !(dj < 1e308)) {
+        fail = j + 1;
+      } else {
+        const double rjj = sqrt(dj);
+        const double r = l > j ? col[j] / rjj : (l == j ? rjj : 0.0);   // R[j][l]
+        col[j] = r;
+#pragma unroll
+        for (int i = j + 1; i < 32; ++i) col[i] -= lane_value(r, i) * r;   // lanes and rows >= m stay zero
+      }
+    }
+  }
+  return fail;
+}
+
+__global__ void __launch_bounds__(64) k_chol32(const double* __restrict__ G, int m, double rows, int allow_shift,
+                                               double* R, double* Rinv, int* st, int mode, const int* gate) {
+  if (gate && *gate != 1) return;
+  const int l = threadIdx.x;
+  double col[32];
+  int attempt = 0;
+  int fail = chol32_factor(G, m, 0.0, 1e-13, col);
+  if (fail && allow_shift) {
+    double tr = 0.0;
+    for (int i = 0; i < m; ++i) tr += G[i * m + i];
+    const double shift = 11.0 * (rows * m + (double)m * (m + 1)) * 1.1102230246251565e-16 * tr;
+    attempt = 1;
+    fail = chol32_factor(G, m, shift, 0.0, col);
+  }
+  if (fail) {
+    if (l == 0) *st = 2;
+    return;
+  }
+  double X[32];
+#pragma unroll
+  for (int i = 31; i >= 0; --i) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = i + 1; k < 32; ++k) s += lane_value(col[i], k) * X[k];   // R[i][k]; X[k] = 0 past the column
+    const double rii = lane_value(col[i], i);
+    const double dinv = i < m ? 1.0 / rii : 0.0;
+    X[i] = i < l ? -s * dinv : (i == l ? dinv : 0.0);
+  }
+  if (l < m) {
+#pragma unroll
+    for (int i = 0; i < 32; ++i)
+      if (i < m) {
+        R[i * m + l] = i <= l ? col[i] : 0.0;
+        Rinv[i * m + l] = X[i];
+      }
+    if (l == 0 && mode == 0) *st = attempt;
+  }
+}
+
 static void launch_chol(hipStream_t s, const double* G, uint64_t m, uint64_t rows, bool allow_shift, double* R,
                         double* Rinv, int* st, int mode, const int* gate) {
-  k_chol<<<1, 256, m * m * sizeof(double), s>>>(G, (int)m, (double)rows, allow_shift ? 1 : 0, R, Rinv, st, mode,
-                                                gate);
+  if (m <= 32)
+    k_chol32<<<1, 64, 0, s>>>(G, (int)m, (double)rows, allow_shift ? 1 : 0, R, Rinv, st, mode, gate);
+  else
+    k_chol<<<1, 256, m * m * sizeof(double), s>>>(G, (int)m, (double)rows, allow_shift ? 1 : 0, R, Rinv, st, mode,
+                                                  gate);
   SLFMM_HIP(hipGetLastError());
 }
 
@@ -482,10 +583,32 @@ void safe_inv_dev(slfmm_ctx* ctx, const double* d_a, uint64_t m, double* d_out, 
   SLFMM_HIP(hipGetLastError());
 }
 
+// A double moved across lanes by a DPP pattern (two 32-bit moves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double x) {
+  const uint64_t u = (uint64_t)__double_as_longlong(x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Sum over an aligned group of 8 lanes without LDS: quad_perm swaps of
+// lanes l ^ 1 and l ^ 2, then row_half_mirror (lane 7 - l of the group).
+// Each step adds two equal-bit partial sums in commuted order, so every
+// lane of the group ends with the same bits.  All 8 lanes must be active.
+__device__ __forceinline__ double sum8(double x) {
+  x += dpp_mov<0xB1>(x);
+  x += dpp_mov<0x4E>(x);
+  x += dpp_mov<0x141>(x);
+  return x;
+}
+
 // ---- one-workgroup one-sided Jacobi -----------------------------------------
 // Rotates column pairs of A (m x m) until they are mutually orthogonal;
 // V accumulates the rotations, so A_in V = U diag(sv).  Pairs of a sweep are
 // scheduled by the circle method: m/2 disjoint pairs per step, 8 lanes each.
+// A and V are kept column by column in LDS (element (i, c) at c m + i), so
+// the 8 lanes of a pair touch consecutive words.
 __global__ void __launch_bounds__(256) k_jacobi(const double* __restrict__ In, int m, double* Vout, double* svout) {
   extern __shared__ double A[];   // m*m, then V (m*m)
   double* V = A + m * m;
@@ -495,7 +618,7 @@ __global__ void __launch_bounds__(256) k_jacobi(const double* __restrict__ In, i
   const int t = threadIdx.x;
   const int mp = (m + 1) & ~1;
   for (int e = t; e < m * m; e += 256) {
-    A[e] = In[e];
+    A[(e % m) * m + e / m] = In[e];
     V[e] = (e / m == e % m) ? 1.0 : 0.0;
   }
   __syncthreads();
@@ -515,25 +638,23 @@ __global__ void __launch_bounds__(256) k_jacobi(const double* __restrict__ In, i
         double al = 0.0, be = 0.0, ga = 0.0;
         if (live)
           for (int i = sub; i < m; i += 8) {
-            double x = A[i * m + a], y = A[i * m + b];
+            double x = A[a * m + i], y = A[b * m + i];
             al += x * x; be += y * y; ga += x * y;
           }
-        for (int off = 4; off > 0; off >>= 1) {
-          al += __shfl_xor(al, off, 8);
-          be += __shfl_xor(be, off, 8);
-          ga += __shfl_xor(ga, off, 8);
-        }
+        al = sum8(al);
+        be = sum8(be);
+        ga = sum8(ga);
         if (live && ga != 0.0 && fabs(ga) > 4e-16 * (double)m * sqrt(al * be)) {
           double zeta = (be - al) / (2.0 * ga);
           double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
           double c = 1.0 / sqrt(1.0 + tt * tt), s = c * tt;
           for (int i = sub; i < m; i += 8) {
-            double x = A[i * m + a], y = A[i * m + b];
-            A[i * m + a] = c * x - s * y;
-            A[i * m + b] = s * x + c * y;
-            x = V[i * m + a]; y = V[i * m + b];
-            V[i * m + a] = c * x - s * y;
-            V[i * m + b] = s * x + c * y;
+            double x = A[a * m + i], y = A[b * m + i];
+            A[a * m + i] = c * x - s * y;
+            A[b * m + i] = s * x + c * y;
+            x = V[a * m + i]; y = V[b * m + i];
+            V[a * m + i] = c * x - s * y;
+            V[b * m + i] = s * x + c * y;
           }
           if (sub == 0) rotated = 1;
         }
@@ -546,7 +667,7 @@ __global__ void __launch_bounds__(256) k_jacobi(const double* __restrict__ In, i
   }
   if (t < m) {
     double s = 0.0;
-    for (int i = 0; i < m; ++i) s += A[i * m + t] * A[i * m + t];
+    for (int i = 0; i < m; ++i) s += A[t * m + i] * A[t * m + i];
     sv[t] = sqrt(s);
   }
   __syncthreads();
@@ -563,7 +684,7 @@ __global__ void __launch_bounds__(256) k_jacobi(const double* __restrict__ In, i
   __syncthreads();
   for (int e = t; e < m * m; e += 256) {
     int i = e / m, c = e % m;
-    Vout[e] = V[i * m + order[c]];
+    Vout[e] = V[order[c] * m + i];
   }
   if (t < m) svout[t] = sv[order[t]];
 }

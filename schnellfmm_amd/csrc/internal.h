@@ -21,6 +21,13 @@ struct Error : std::runtime_error {
   Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
+// Thrown by scratch() when a buffer would have to be allocated while the
+// context's stream is being captured into a graph; the capture is dropped
+// and the work is queued directly.
+struct CaptureAbort : std::runtime_error {
+  CaptureAbort() : std::runtime_error("graph capture needs a new device buffer") {}
+};
+
 #define SLFMM_HIP(expr)                                                              \
   do {                                                                               \
     hipError_t e_ = (expr);                                                          \
@@ -102,6 +109,32 @@ struct slfmm_ctx {
   hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_consumed[2] = {nullptr, nullptr};
   void* pinned_buf(int i, size_t bytes);   // grow-only, hipHostMalloc
   void ensure_copy_stream();
+  // Uploads of host arrays: copied into a grow-only pinned buffer and queued
+  // with hipMemcpyAsync from there, so the call returns without waiting for
+  // the copy (a pageable source would be staged synchronously).  Slices are
+  // bump-allocated within one API call; the next call reuses the buffer
+  // after the event of the last queued upload.  In allocation-free calls the
+  // copy is queued straight from the caller's memory.
+  unsigned char* up_buf = nullptr;
+  size_t up_cap = 0, up_top = 0;
+  hipEvent_t ev_up = nullptr;
+  void upload(void* dst, const void* src, size_t bytes);
+  void upload_reset();
+  // Graph replay of whole fits.  pool_gen counts pool (re)allocations: a
+  // graph captured at one generation is replayed only at that generation.
+  // While `capturing`, scratch() may not allocate (CaptureAbort), and
+  // gather_subset takes the estimation indices that were uploaded before the
+  // capture (staged_idx, the caller's host pointer) from its buffer.
+  uint64_t pool_gen = 0;
+  bool capturing = false;
+  const int64_t* staged_idx = nullptr;
+  struct FitGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<Check> checks;   // the status slots the graph writes, from slot `slot0` on
+    uint64_t pool_gen = 0;
+  };
+  std::map<std::string, FitGraph> graphs;
+  uint64_t graphs_captured = 0, graphs_replayed = 0;
   // Contexts of the extra streams of batched fits (same device, own scratch).
   std::vector<slfmm_ctx*> workers;
   // Reads the slots (synchronises the stream), prints warnings in order and
@@ -163,6 +196,7 @@ int guarded(slfmm_ctx* ctx, F&& f) {
     if (ctx) {
       SLFMM_HIP(hipSetDevice(ctx->device));
       ctx->checks.clear();
+      ctx->upload_reset();
     }
     f();
     if (ctx) ctx->finish_checks();
@@ -211,10 +245,11 @@ void launch_transpose(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_
 // out (a x b) = A^T B, A rows x a (lda), B rows x b (ldb); deterministic.
 void ts_gram(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t a, const double* B,
              uint64_t ldb, uint64_t b, uint64_t rows, double* out, const int* gate = nullptr);
-// out (rows x q, ldo) = alpha * A (rows x m, lda) * M (m x q, ldm) + beta * out
+// out (rows x q, ldo) = alpha * A (rows x m, lda) * M' (m x q) + beta * out, with
+// M' = M, or (dlam set) row t of M times -(1 - dlam[t]) (inverse: 1 / dlam[t] - 1)
 void ts_gemm(hipStream_t s, const double* A, uint64_t lda, uint64_t m, const double* M,
              uint64_t ldm, uint64_t q, uint64_t rows, double alpha, double beta, double* out,
-             uint64_t ldo, const int* gate = nullptr);
+             uint64_t ldo, const int* gate = nullptr, const double* dlam = nullptr, bool inverse = false);
 void launch_fill(hipStream_t s, double* p, double v, uint64_t count);
 // dst (rows x cols, ldd) <- src (rows x cols, lds)
 void copy2d(hipStream_t s, const double* src, uint64_t lds, double* dst, uint64_t ldd, uint64_t rows,

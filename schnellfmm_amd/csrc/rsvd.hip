@@ -128,20 +128,14 @@ static void accumulate_f64(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, u
 }
 
 // ---- low-rank operator N (and N^{-1}) ----------------------------------------
-__global__ void k_scale_rows(double* C, int d, int c, const double* dlam, int inverse) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= d * c) return;
-  double dl = dlam[e / c];
-  C[e] *= inverse ? (1.0 / dl - 1.0) : -(1.0 - dl);
-}
-
-// V (rows x c, ldv) <- N V  or  N^{-1} V
+// V (rows x c, ldv) <- N V  or  N^{-1} V:  V + Q_d S (Q_d^T V) with
+// S = diag(-(1 - d_lambda)) (N) or diag(1 / d_lambda - 1) (N^{-1}),
+// the scaling applied while the GEMM stages Q_d^T V.
 static void apply_N(slfmm_ctx* ctx, const double* Qd, uint64_t d, const double* dlam, double* V, uint64_t ldv,
                     uint64_t c, uint64_t rows, bool inverse) {
   double* C = ctx->buf<double>("rs_NC", d * c);
   ts_gram(ctx, Qd, d, d, V, ldv, c, rows, C);
-  k_scale_rows<<<(unsigned)cdiv(d * c, 256), 256, 0, ctx->stream>>>(C, (int)d, (int)c, dlam, inverse ? 1 : 0);
-  ts_gemm(ctx->stream, Qd, d, d, C, c, c, rows, 1.0, 1.0, V, ldv);
+  ts_gemm(ctx->stream, Qd, d, d, C, c, c, rows, 1.0, 1.0, V, ldv, nullptr, dlam, inverse);
 }
 
 __global__ void k_normal(double* out, uint64_t n, int l, uint64_t ld, uint64_t seed) {
@@ -236,9 +230,9 @@ const uint8_t* gather_subset(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t*
     return Yg;
   }
   int64_t* didx = ctx->buf<int64_t>(std::string(tag) + "_idx", n_idx);
-  SLFMM_HIP(hipMemcpyAsync(didx, h_idx, n_idx * 8, hipMemcpyHostToDevice, s));
+  if (ctx->staged_idx != h_idx) ctx->upload(didx, h_idx, n_idx * 8);
   uint8_t* Yg = ctx->buf<uint8_t>(tag, n_idx * g->ld + kSlack);
-  SLFMM_HIP(hipMemsetAsync(Yg + n_idx * g->ld, 0, kSlack, s));
+  launch_fill(s, reinterpret_cast<double*>(Yg + n_idx * g->ld), 0.0, kSlack / 8);
   geno_gather(ctx, g, didx, n_idx, Yg);
   *pe = n_idx;
   return Yg;
@@ -283,7 +277,7 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   if (mixed) {   // sample-major copy for the forward passes
     ldt = rup(cdiv(pe, 4), 16);
     YT = ctx->buf<uint8_t>("rs_YT", n * ldt + kSlack);
-    SLFMM_HIP(hipMemsetAsync(YT + n * ldt, 0, kSlack, s));   // the transpose writes every row byte
+    launch_fill(s, reinterpret_cast<double*>(YT + n * ldt), 0.0, kSlack / 8);   // the transpose writes every row byte
     launch_transpose(s, Y, g->ld, n, pe, YT, ldt);
   }
 
@@ -295,7 +289,7 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   double* Z = ctx->buf<double>("rs_Z", pe * lda);
   double* W = ctx->buf<double>("rs_W", n * l);
   if (d_b0) {
-    SLFMM_HIP(hipMemcpyAsync(B, d_b0, n * l * 8, hipMemcpyDeviceToDevice, s));
+    copy2d(s, d_b0, l, B, l, n, l);
   } else {
     launch_normal(s, B, n, l, l, cfg.seed);
     apply_N(ctx, Qd, d, dlam, B, l, l, n, false);
