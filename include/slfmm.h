@@ -256,9 +256,11 @@ typedef struct slfmm_problem {
 /* Many independent fit_lfmm2 problems (each src/lib.rs:162-263) in one call.
  * No counterpart in the reference, which fits one problem per process; this
  * is the batching pattern of the hot path (many small LFMM2 problems, uneven
- * sizes).  Problem i runs on internal stream i % n_streams (1..8) of the
- * context, so small problems overlap on the device, and the host synchronises
- * once per stream at the end.  Each problem's results are those of
+ * sizes).  The problems are spread over n_streams (1..16) internal streams
+ * of the context, longest estimated first, each to the stream with the least
+ * queued work, so small problems overlap on the device; the host synchronises
+ * once per stream at the end.  With SLFMM_OUT_DEVICE each problem's work is
+ * replayed from a captured graph (see SLFMM_EAGER).  Each problem's results are those of
  * slfmm_fit_lfmm2 on the same inputs; outputs follow `flags` as there.  An
  * error names the problem it came from ("problem i: ..."). */
 int slfmm_fit_lfmm2_batch(slfmm_ctx* ctx, slfmm_problem* problems, uint64_t count, int n_streams,

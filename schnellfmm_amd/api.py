@@ -442,9 +442,10 @@ class Lfmm2Problem:
 
 def fit_lfmm2_batch(problems, n_streams: int = 8, with_hist: bool = False):
     """Many independent fit_lfmm2 problems in one call (slfmm_fit_lfmm2_batch):
-    problem i runs on stream i % n_streams of the context, so small problems
-    overlap on the GPU.  Returns one TestResults per problem, equal to what
-    fit_lfmm2 returns for the same inputs."""
+    the problems are spread over n_streams (1..16) streams of the context,
+    longest first, so small problems overlap on the GPU.  Returns one
+    TestResults per problem, equal to what fit_lfmm2 returns for the same
+    inputs."""
     from ._lib import Problem
     problems = list(problems)
     if not problems:

@@ -41,7 +41,7 @@ def _problems(S):
     return probs
 
 
-@pytest.mark.parametrize("n_streams", [1, 3, 8])
+@pytest.mark.parametrize("n_streams", [1, 3, 8, 16])
 def test_batch_equals_single_fits(S, n_streams):
     probs = _problems(S)
     got = S.fit_lfmm2_batch(probs, n_streams=n_streams)
@@ -68,5 +68,6 @@ def test_batch_reports_failing_problem(S):
 
 def test_batch_rejects_bad_stream_count(S):
     probs = _problems(S)[:1]
-    with pytest.raises(S.api.SlfmmError, match="n_streams"):
-        S.fit_lfmm2_batch(probs, n_streams=0)
+    for bad in (0, 17):
+        with pytest.raises(S.api.SlfmmError, match="n_streams"):
+            S.fit_lfmm2_batch(probs, n_streams=bad)
