@@ -49,9 +49,9 @@ def main():
     ap.add_argument("--precision", choices=["auto", "i8", "fp64"], default="auto")
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--cpu-snps", type=int, default=10000,
+    ap.add_argument("--cpu-snps", type=int, default=20000,
                     help="association prefix of the CPU baseline (SNPs)")
-    ap.add_argument("--cpu-est-snps", type=int, default=5000,
+    ap.add_argument("--cpu-est-snps", type=int, default=20000,
                     help="estimation SNPs of the CPU baseline's RSVD (scaled to the workload's subset)")
     ap.add_argument("--workload", choices=["lfmm2", "nmf", "batch", "host"], default="lfmm2",
                     help="lfmm2: the headline fit (default); nmf: NMF imputation estimation and the "
