@@ -478,7 +478,7 @@ def i8_kernel_name(K, d, mode):
     from schnellfmm_amd._lib import lib
     m = K + d
     dims = np.zeros(5, np.int32)
-    lib().slfmm_debug_i8_layout(m, K if mode == 0 else m, 0 if mode == 0 else m,
+    lib().slfmm_debug_i8_layout(m, K if mode == 0 else m, K if mode == 0 else m,
                                 dims.ctypes.data_as(C.c_void_p), None, 0)
     nq, nt0, nt1 = (int(v) for v in dims[:3])
     st = 4 if nt0 + nt1 <= 8 else 2

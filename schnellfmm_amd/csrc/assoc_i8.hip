@@ -188,7 +188,10 @@ struct Operand {
 
 // Scale of the dosage in the g' fragment (00 -> 0, 01 -> c, 10 -> S, 11 -> 2S):
 // the int32 sums stay below 2^31 while 127 * 2S * n does.
-static int fill_scale(uint64_t n) { return n <= 2000000 ? 4 : (n <= 4000000 ? 2 : 1); }
+// Scale S of the g' codes {0, c, S, 2S}: 127 (2S) n must stay below 2^31.
+static int fill_scale(uint64_t n) {
+  return n <= 262144 ? 32 : (n <= 524288 ? 16 : (n <= 1048576 ? 8 : (n <= 2000000 ? 4 : (n <= 4000000 ? 2 : 1))));
+}
 
 // Quantise Q (n x m, ldq) into the packed fragment layout: columns from k3
 // on carry 3 terms on the g' side, columns from km on 2 terms on the miss
@@ -349,20 +352,30 @@ struct SnpArgs {
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 constexpr int kSnpWaves = 4;
 
-// Byte-permute table of the g' fragment from the SNP's first 256 samples
-// (lane group g holds samples 64 g .. 64 g + 63 of block 0 in w).
-// With the all-ones row in a g' tile the missing count is recovered from
-// sum g' = S sg + c n_miss and popc = sg + n_miss, which needs c != S; the
-// code then moves to the neighbour on the side of the estimate.
-__device__ __forceinline__ uint32_t fill_table(const v4u& w, int S, int fill, int ones_g) {
-  int sg = 0, miss = 0;
+// Byte-permute table of the g' fragment from the SNP's first
+// 256 kFillBlocks samples (lane group g holds samples 256 b + 64 g .. + 63
+// of block b in w[b]; samples past n are not counted).  With the all-ones
+// row in a g' tile the missing count is recovered from sum g' = S sg +
+// c n_miss and popc = sg + n_miss, which needs c != S; the code then moves
+// to the neighbour on the side of the estimate.  The code only sets how
+// large the miss-side residual weight |mu - c/S| is: any c gives exact
+// functionals.
+constexpr int kFillBlocks = 4;
+__device__ __forceinline__ uint32_t fill_table(const v4u (&w)[kFillBlocks], uint64_t n, int g, int S, int fill,
+                                               int ones_g) {
+  int sg = 0, miss = 0, valid = 0;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const uint32_t x = w[v], lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
-    sg += __popc(hi) + __popc(lo & hi);   // het + 2 hom
-    miss += __popc(lo & ~hi);
+  for (int b = 0; b < kFillBlocks; ++b) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const uint32_t x = w[b][v], lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
+      sg += __popc(hi) + __popc(lo & hi);   // het + 2 hom
+      miss += __popc(lo & ~hi);
+    }
+    const int64_t left = (int64_t)n - (256 * b + 64 * g);
+    valid += left <= 0 ? 0 : (left >= 64 ? 64 : (int)left);
   }
-  int obs = 64 - miss;
+  int obs = valid - miss;
   sg += __shfl_xor(sg, 16); sg += __shfl_xor(sg, 32);
   obs += __shfl_xor(obs, 16); obs += __shfl_xor(obs, 32);
   int c = obs > 0 ? (2 * S * sg + obs) / (2 * obs) : 0;   // round(S sg / obs)
@@ -522,12 +535,26 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
   // prologue in the order the blocks issue them: operand 0, genotypes 0, genotypes 1
   for (int c = wave; c < CH; c += W) buf_to_lds(pc.opnd, 16 * lane, c * 1024, A0 + c * 1024);
   geno_to_lds<ST>(a, pc, 0, g, G0 + wave * (ST * 1024));
+  // blocks 1 .. kFillBlocks - 1 of the lane's rows for the fill code (plain
+  // loads; bytes past the row stride read zeros through the descriptor)
+  v4u fw[ST][kFillBlocks];
+#pragma unroll
+  for (int st = 0; st < ST; ++st)
+#pragma unroll
+    for (int b = 1; b < kFillBlocks; ++b) {
+      const uint32_t byte = (uint32_t)b * 64 + 16 * g;
+      const uint32_t o = byte < (uint32_t)a.ld ? pc.rowoff + byte + st * pc.tstride : 0xF0000000u;
+      fw[st][b] = __builtin_bit_cast(v4u, __builtin_amdgcn_raw_buffer_load_b128(pc.geno, o, 0, 0));
+    }
   wait_vmcnt<0>();
   uint32_t tbl[ST];
   {
     const v4u* gp = reinterpret_cast<const v4u*>(G0 + wave * (ST * 1024));
 #pragma unroll
-    for (int st = 0; st < ST; ++st) tbl[st] = fill_table(gp[st * 64 + lane], a.S, a.fill, a.ones_g);
+    for (int st = 0; st < ST; ++st) {
+      fw[st][0] = gp[st * 64 + lane];
+      tbl[st] = fill_table(fw[st], a.n, g, a.S, a.fill, a.ones_g);
+    }
   }
   unsigned char* gslot = G0 + wave * (ST * 1024);
 #define SLFMM_SNP(PH, AR_, AW_) \
@@ -580,10 +607,10 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
 }
 
 // (NQ, NT0, NT1) of every RowMap the passes use with m <= 32: association
-// RowMap::make(K + d, K, 0) for d <= 16, K >= 1; RSVD RowMap::make(l, l, l).
+// RowMap::make(K + d, K, K) for d <= 16, K >= 1; RSVD RowMap::make(l, l, l).
 #define SLFMM_SNP_ASSOC(X)                                                                                     \
-  X(1, 1, 1) X(2, 2, 1) X(3, 2, 2) X(3, 3, 2) X(4, 3, 2) X(5, 3, 3) X(5, 4, 3) X(6, 4, 3) X(6, 4, 4) X(6, 5, 3) \
-  X(7, 4, 4) X(7, 5, 4) X(8, 5, 4) X(8, 5, 5) X(8, 6, 4)
+  X(1, 1, 1) X(2, 2, 1) X(3, 2, 1) X(3, 2, 2) X(3, 3, 2) X(4, 3, 2) X(5, 3, 2) X(5, 4, 2) X(5, 4, 3) X(6, 4, 2) \
+  X(6, 4, 3) X(6, 5, 3) X(7, 4, 2) X(7, 4, 3) X(7, 5, 3) X(8, 5, 3) X(8, 5, 4) X(8, 6, 4)
 #define SLFMM_SNP_RSVD(X) \
   X(1, 1, 1) X(2, 1, 1) X(3, 2, 1) X(4, 2, 1) X(4, 2, 2) X(5, 3, 2) X(6, 3, 2) X(7, 4, 2) X(8, 4, 2) X(8, 4, 3)
 
@@ -639,13 +666,15 @@ void i8_layout(int m, int k3, int km, int32_t* dims, int32_t* rows, uint64_t cap
 // Association pass over every SNP of g (fused epilogue).
 void launch_func_i8(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_Q, uint64_t m, const EpiArgs& ea,
                     bool /*fused_epilogue*/) {
-  // Latent directions: 2 int8 terms on both sides; covariate directions (the
-  // t-statistic numerators, last d columns): 3 on the g' side, 2 on the miss
-  // side.  (One miss-side term for the latent directions would save a row
-  // tile, but its error is amplified by the RSS difference: |delta log10 p|
-  // reached 8e-4 on test panels.)
+  // Latent directions: 2 int8 terms on the g' side, 1 on the miss side;
+  // covariate directions (the t-statistic numerators, last d columns): 3 on
+  // the g' side, 2 on the miss side.  The miss side meets only the weight
+  // |mu - c/S|, and the fill code c comes from the SNP's first 1024 samples
+  // on a scale S = 32 (n <= 262144), which keeps that weight small enough
+  // for one latent miss term (a second one costs a row tile: 7 instead of
+  // 6 at K + d = 24).
   SLFMM_CHECK(m <= 32, SLFMM_E_ARG, "the int8 association path supports K + d <= 32 (use SLFMM_PREC_FP64)");
-  Operand op = prepare_operand(ctx, "as_i8", d_Q, m, g->n, (int)m, ea.K, 0, true);
+  Operand op = prepare_operand(ctx, "as_i8", d_Q, m, g->n, (int)m, ea.K, ea.K, true);
   SLFMM_CHECK(8 * split_plan_len(ea.d, (int)m) <= snp_plan_bytes(op.rmap.NQ), SLFMM_E_ARG,
               "association plan does not fit the int8 kernel's LDS (use SLFMM_PREC_FP64)");
   SnpArgs a{};

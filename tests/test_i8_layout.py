@@ -24,7 +24,8 @@ def _cases():
     for m in range(1, 33):
         for d in range(1, 17):
             if m - d >= 1:
-                yield m, m - d, 0          # association: 2 miss terms for every column
+                yield m, m - d, m - d      # association: 1 miss term for the latent columns
+                yield m, m - d, 0          # 2 miss terms for every column
         yield m, m, m                      # RSVD sketch: 2 + 1 terms
         yield m, max(m - 4, 0), max(m - 4, 0)
 
@@ -86,7 +87,7 @@ def test_every_layout_has_a_kernel_instantiation():
         return {tuple(int(v) for v in t) for t in re.findall(r"X\((\d+), (\d+), (\d+)\)", body)}
 
     assoc, rsvd = shapes("SLFMM_SNP_ASSOC"), shapes("SLFMM_SNP_RSVD")
-    need_a = {layout(m, m - d, 0)[:3] for m in range(2, 33) for d in range(1, 17) if m - d >= 1}
+    need_a = {layout(m, m - d, m - d)[:3] for m in range(2, 33) for d in range(1, 17) if m - d >= 1}
     need_r = {layout(l, l, l)[:3] for l in range(1, 33)}
     assert need_a == assoc
     assert need_r == rsvd
