@@ -481,7 +481,7 @@ def i8_kernel_name(K, d, mode):
     lib().slfmm_debug_i8_layout(m, K if mode == 0 else m, K if mode == 0 else m,
                                 dims.ctypes.data_as(C.c_void_p), None, 0)
     nq, nt0, nt1 = (int(v) for v in dims[:3])
-    st = 4 if nt0 + nt1 <= 8 else 2
+    st = 2 if nt0 + nt1 <= 3 else 1
     return f"k_snp_i8<{nq}, {nt0}, {nt1}, {st}, {mode}>"
 
 

@@ -15,12 +15,16 @@
 // src/bed.rs:339-343), and the FP64 per-SNP epilogue of the association pass
 // runs on it unchanged.
 //
-// Operand fragments: for a block of 64*KS samples, lane group g = lane/16
-// of a wave loads KS consecutive words of its SNP row, word s serving k-step
-// s; byte c of decoded register v holds sample
-//   64*KS*blk + 16*KS*g + 16*s + 4*c + v,
-// and the dense operand is pre-permuted in memory into exactly this order
-// (one 16-byte fragment per lane), so the same K index meets on both sides.
+// SNP-side kernels (association pass, RSVD sketch) run
+// v_mfma_i32_32x32x32_i8: lane l holds SNP column l % 32 of a 32-SNP tile
+// and samples 16 (l / 32) .. + 15 of a k-step.  For a block of 256 samples,
+// lane half h = l / 32 owns samples 128 h .. 128 h + 127 of its SNP row (32
+// contiguous bytes, 8 words), word s serving k-step s; byte c of decoded
+// register v holds sample 256 blk + 128 h + 16 s + 4 c + v.  The dense
+// operand is pre-permuted into exactly this order (one 16-byte fragment per
+// lane), so the same K index meets on both sides.  (The RSVD forward pass,
+// on the sample-major copy, uses v_mfma_i32_16x16x64_i8 with the analogous
+// order.)
 #include "internal.h"
 #include "colreduce.h"
 #include "epilogue.h"
@@ -46,78 +50,80 @@ __device__ __forceinline__ int quant_term(double q, int t) {
   return clamp127(rint((r1 - md) * 254.0));
 }
 
-// Row layout of the packed operand of the SNP-side kernels.  The result of
-// v_mfma_i32_16x16x64_i8 row 16 rt + 4 g + r lands in register r of lane
-// group g, so functional i (a column of the dense operand) is given to lane
-// group g = i % 4 at position q = i / 4, and every int32 sum it needs ends
-// in the registers of the lanes that recombine it (no transpose through
-// LDS).  Within a lane group the rows of tile rt are slots s = 4 rt + r:
+// Row layout of the packed operand of the SNP-side kernels.  Row 32 rt + r
+// of a v_mfma_i32_32x32x32_i8 result, r = 8 b + 4 h + v, lands in register
+// 4 b + v of lane half h, so functional i (a column of the dense operand) is
+// given to half h = i % 2 at position q = i / 2, and every int32 sum it
+// needs ends in the registers of the lanes that recombine it (no transpose
+// through LDS).  Within a half the rows of tile rt are slots s = 16 rt +
+// 4 b + v:
 //   g' tiles (rt < NT0):   terms 0 and 1 of position q at slots 2q, 2q + 1;
 //                          term 2 (columns i >= k3) at slot 3 NQ - 1 - q
 //   miss tiles (NT1):      term 0 of position q at slot q; term 1 (columns
 //                          i >= km) at slot 2 NQ - 1 - q.
-// One all-ones row sits at the last slot of lane group g1: in a miss tile
-// (slot 4 NT1 - 1, the missing count) or, when the miss tiles are full, in a
-// g' tile (slot 4 NT0 - 1, the sum of g').  All slot indices are
-// compile-time functions of q, so the accumulators are addressed statically.
+// One all-ones row sits at the last slot of half g1: in a miss tile (slot
+// 16 NT1 - 1, the missing count) or, when the miss tiles are full, in a g'
+// tile (slot 16 NT0 - 1, the sum of g').  All slot indices are compile-time
+// functions of q, so the accumulators are addressed statically.
 struct RowMap {
   int m, k3, km, NQ, NT0, NT1, g1, ones_g;
   static RowMap make(int m, int k3, int km) {
-    RowMap r{m, k3, km, (m + 3) / 4, 0, 0, 0, 0};
-    int topG = 0, topM = 0, tgs[4], tms[4];
-    for (int g = 0; g < 4; ++g) {
-      const int nq = g < m ? (m - g + 3) / 4 : 0;
+    RowMap r{m, k3, km, (m + 1) / 2, 0, 0, 0, 0};
+    int topG = 0, topM = 0, tgs[2], tms[2];
+    for (int h = 0; h < 2; ++h) {
+      const int nq = h < m ? (m - h + 1) / 2 : 0;
       int qx = nq, qm = nq;   // first positions holding a column >= k3, >= km
       for (int q = nq - 1; q >= 0; --q) {
-        if (g + 4 * q >= k3) qx = q;
-        if (g + 4 * q >= km) qm = q;
+        if (h + 2 * q >= k3) qx = q;
+        if (h + 2 * q >= km) qm = q;
       }
       int tg = 2 * nq - 1, tm = nq - 1;
       if (qx < nq) tg = std::max(tg, 3 * r.NQ - 1 - qx);
       if (qm < nq) tm = std::max(tm, 2 * r.NQ - 1 - qm);
-      tgs[g] = tg; tms[g] = tm;
+      tgs[h] = tg; tms[h] = tm;
       topG = std::max(topG, tg);
       topM = std::max(topM, tm);
     }
-    r.NT0 = (topG + 4) / 4;
-    r.NT1 = (topM + 4) / 4;
-    for (int g = 0; g < 4; ++g)
-      if (tms[g] < 4 * r.NT1 - 1) { r.g1 = g; return r; }
-    for (int g = 0; g < 4; ++g)
-      if (tgs[g] < 4 * r.NT0 - 1) { r.g1 = g; r.ones_g = 1; return r; }
+    r.NT0 = (topG + 16) / 16;
+    r.NT1 = (topM + 16) / 16;
+    for (int h = 0; h < 2; ++h)
+      if (tms[h] < 16 * r.NT1 - 1) { r.g1 = h; return r; }
+    for (int h = 0; h < 2; ++h)
+      if (tgs[h] < 16 * r.NT0 - 1) { r.g1 = h; r.ones_g = 1; return r; }
     ++r.NT1;   // room for the ones row
     return r;
   }
   // (column, term) of packed row `row`: column -1 for padding, -2 for the ones row
   __host__ __device__ int2 info(int row) const {
-    const int rt = row >> 4, g = (row >> 2) & 3, r = row & 3;
+    const int rt = row >> 5, rr = row & 31, h = (rr >> 2) & 1, sl = 4 * (rr >> 3) + (rr & 3);
     if (rt < NT0) {
-      const int s = 4 * rt + r;
-      if (ones_g && g == g1 && s == 4 * NT0 - 1) return make_int2(-2, 0);
+      const int s = 16 * rt + sl;
+      if (ones_g && h == g1 && s == 16 * NT0 - 1) return make_int2(-2, 0);
       if (s < 2 * NQ) {
-        const int i = g + 4 * (s >> 1);
+        const int i = h + 2 * (s >> 1);
         return i < m ? make_int2(i, s & 1) : make_int2(-1, 0);
       }
-      const int q = 3 * NQ - 1 - s, i = g + 4 * q;
+      const int q = 3 * NQ - 1 - s, i = h + 2 * q;
       return q >= 0 && i < m && i >= k3 ? make_int2(i, 2) : make_int2(-1, 0);
     }
-    const int s = 4 * (rt - NT0) + r;
-    if (!ones_g && g == g1 && s == 4 * NT1 - 1) return make_int2(-2, 0);
+    const int s = 16 * (rt - NT0) + sl;
+    if (!ones_g && h == g1 && s == 16 * NT1 - 1) return make_int2(-2, 0);
     if (s < NQ) {
-      const int i = g + 4 * s;
+      const int i = h + 2 * s;
       return i < m ? make_int2(i, 0) : make_int2(-1, 0);
     }
-    const int q = 2 * NQ - 1 - s, i = g + 4 * q;
+    const int q = 2 * NQ - 1 - s, i = h + 2 * q;
     return q >= 0 && i < m && i >= km ? make_int2(i, 1) : make_int2(-1, 0);
   }
 };
 
 // Apack for the SNP-side kernels: rows = functional terms, K = samples.
-// Fragment (blk, s, rt) is 1 KB: lane l holds row 16 rt + l % 16, byte 4v + c
-// = sample 256 blk + 64 (l / 16) + 16 s + 4 c + v.
+// Fragment (blk, s, rt) is 1 KB (8 k-steps of 32 samples per 256-sample
+// block): lane l holds row 32 rt + l % 32, byte 4v + c = sample
+// 256 blk + 128 (l / 32) + 16 s + 4 c + v.
 __global__ void k_pack_rows(const double* __restrict__ Q, uint64_t ldq, uint64_t n, RowMap rmap,
                             const double* __restrict__ sc, int NT, uint64_t total, int8_t* __restrict__ out) {
-  constexpr int KS = 4;
+  constexpr int KS = 8;
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
     const int j = (int)(e & 15), lane = (int)((e >> 4) & 63);
     const uint64_t frag = e >> 10;                 // (blk*KS + s)*NT + rt
@@ -125,9 +131,9 @@ __global__ void k_pack_rows(const double* __restrict__ Q, uint64_t ldq, uint64_t
     const uint64_t ks = frag / NT;
     const int s = (int)(ks % KS);
     const uint64_t blk = ks / KS;
-    const int v = j >> 2, c = j & 3, g = lane >> 4;
-    const uint64_t sample = blk * 64 * KS + 16 * KS * g + 16 * s + 4 * c + v;
-    const int2 ri = rmap.info(16 * rt + (lane & 15));
+    const int v = j >> 2, c = j & 3, h = lane >> 5;
+    const uint64_t sample = blk * 256 + 128 * h + 16 * s + 4 * c + v;
+    const int2 ri = rmap.info(32 * rt + (lane & 31));
     int val = 0;
     if (sample < n) {
       if (ri.x >= 0) val = quant_term(Q[sample * ldq + ri.x] / sc[ri.x], ri.y);
@@ -215,7 +221,7 @@ static Operand prepare_operand(slfmm_ctx* ctx, const char* tag, const double* Q,
   colreduce<3, kColSum>(ctx, FnTerms{Q, ldq, sc}, GStore3{O}, n, m);
   op.wtab = ctx->buf<double>(T + "_w", kWtab);
   k_weights<<<1, 32, 0, s>>>(m, k3, km, (double)op.S, sc, O, op.wtab);
-  const uint64_t total = op.nblk * 4 * NT * 1024;
+  const uint64_t total = op.nblk * 8 * NT * 1024;
   op.pack = ctx->buf<int8_t>(T + "_pack", total);
   k_pack_rows<<<(unsigned)std::min<uint64_t>(cdiv(total, 256), 16384), 256, 0, s>>>(Q, ldq, n, op.rmap, sc, NT, total,
                                                                                   op.pack);
@@ -277,22 +283,6 @@ __device__ __forceinline__ void chunk_to_lds(const unsigned char* src_chunk, uns
 
 constexpr int kOperandPrefetch = 6;   // operand fragments read ahead of their MFMAs
 
-// The wave's ST 1 KB genotype pieces at p, p + 1024, ... into registers;
-// the wait is inside the asm block, so no load is in flight when it ends.
-template <int ST>
-__device__ __forceinline__ void lds_read_tiles(const unsigned char* p, v4i (&q)[ST]) {
-  const uint32_t addr = (uint32_t)(uintptr_t)(lds_ptr_t)p;
-  static_assert(ST == 2 || ST == 4, "tile count");
-  if constexpr (ST == 2) {
-    asm volatile("ds_read_b128 %0, %2\n\tds_read_b128 %1, %2 offset:1024\n\ts_waitcnt lgkmcnt(0)"
-                 : "=&v"(q[0]), "=&v"(q[1]) : "v"(addr) : "memory");
-  } else {
-    asm volatile("ds_read_b128 %0, %4\n\tds_read_b128 %1, %4 offset:1024\n\t"
-                 "ds_read_b128 %2, %4 offset:2048\n\tds_read_b128 %3, %4 offset:3072\n\ts_waitcnt lgkmcnt(0)"
-                 : "=&v"(q[0]), "=&v"(q[1]), "=&v"(q[2]), "=&v"(q[3]) : "v"(addr) : "memory");
-  }
-}
-
 // vmcnt wait with a compile-time count (vmcnt[3:0]; expcnt and lgkmcnt at
 // their maxima, i.e. not waited on).
 template <int N>
@@ -320,7 +310,7 @@ __device__ __forceinline__ void buf_to_lds(rsrc_t rs, uint32_t voff, uint32_t so
 // MODE 0: fused association epilogue;  MODE 1: store s_j * f (RSVD sketch).
 //
 // Fill code.  Before the stream, each SNP gets an integer fill code c from
-// its first 256 samples (c = round(S mu_256)), and the g' fragment decodes
+// its first 1024 samples (c = round(S mu_1024)), and the g' fragment decodes
 // 00 -> 0, 01 -> c, 10 -> S, 11 -> 2S (one byte-permute table per lane).
 // With q^f the g'-side quantisation (2 or 3 terms) and q^c the miss-side one
 // (1 or 2 terms, the leading terms of the same expansion):
@@ -328,15 +318,18 @@ __device__ __forceinline__ void buf_to_lds(rsrc_t rs, uint32_t voff, uint32_t so
 //       = sum_{obs} q^f (g - mu) + (mu - c/S) sum_{miss} (q^c - q^f),
 // so the coarse miss side only meets the small weight |mu - c/S|.
 //
-// Workgroup: 4 waves x ST tiles of 16 SNPs, two workgroups per CU, so one
+// Workgroup: 4 waves x ST tiles of 32 SNPs, two workgroups per CU, so one
 // workgroup's barriers and epilogue overlap the other's stream.  Blocks of
-// 256 samples (4 k-steps of 64):
-//   * the operand block (4 NT fragments) goes to a 2-slot LDS ring by
+// 256 samples (8 k-steps of 32):
+//   * the operand block (8 NT fragments) goes to a 2-slot LDS ring by
 //     buffer-descriptor DMA, shared by the four waves;
-//   * each lane's 16 bytes of its SNP row per tile go to a one-block LDS slot
-//     of its wave by the same DMA (the wave reads back only its own pieces).
+//   * the 32 x 64 bytes of a tile's rows go to a one-block LDS slot of its
+//     wave by the same DMA, 16 whole 64-byte row pieces per instruction;
+//     each lane then reads back the 32 bytes of its own column and half.
 // Rows past the last SNP, bytes past the row stride and blocks past the last
-// read zeros through the buffer descriptor.
+// read zeros through the buffer descriptor.  The 32x32x32 MFMA issues half
+// as many instructions per MAC as 16x16x64, which leaves the VALU decode
+// more issue cycles beside it.
 // ---------------------------------------------------------------------------
 struct SnpArgs {
   const uint8_t* packed;
@@ -350,34 +343,37 @@ struct SnpArgs {
 };
 
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 constexpr int kSnpWaves = 4;
 
 // Byte-permute table of the g' fragment from the SNP's first
-// 256 kFillBlocks samples (lane group g holds samples 256 b + 64 g .. + 63
-// of block b in w[b]; samples past n are not counted).  With the all-ones
-// row in a g' tile the missing count is recovered from sum g' = S sg +
-// c n_miss and popc = sg + n_miss, which needs c != S; the code then moves
-// to the neighbour on the side of the estimate.  The code only sets how
-// large the miss-side residual weight |mu - c/S| is: any c gives exact
+// 256 kFillBlocks samples (lane half h holds samples 256 b + 128 h .. + 127
+// of block b in w[b][0..1]; samples past n are not counted).  With the
+// all-ones row in a g' tile the missing count is recovered from sum g' =
+// S sg + c n_miss and popc = sg + n_miss, which needs c != S; the code then
+// moves to the neighbour on the side of the estimate.  The code only sets
+// how large the miss-side residual weight |mu - c/S| is: any c gives exact
 // functionals.
 constexpr int kFillBlocks = 4;
-__device__ __forceinline__ uint32_t fill_table(const v4u (&w)[kFillBlocks], uint64_t n, int g, int S, int fill,
+__device__ __forceinline__ uint32_t fill_table(const v4u (&w)[kFillBlocks][2], uint64_t n, int h, int S, int fill,
                                                int ones_g) {
   int sg = 0, miss = 0, valid = 0;
 #pragma unroll
   for (int b = 0; b < kFillBlocks; ++b) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const uint32_t x = w[b][v], lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
-      sg += __popc(hi) + __popc(lo & hi);   // het + 2 hom
-      miss += __popc(lo & ~hi);
-    }
-    const int64_t left = (int64_t)n - (256 * b + 64 * g);
-    valid += left <= 0 ? 0 : (left >= 64 ? 64 : (int)left);
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const uint32_t x = w[b][i][v], lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
+        sg += __popc(hi) + __popc(lo & hi);   // het + 2 hom
+        miss += __popc(lo & ~hi);
+      }
+    const int64_t left = (int64_t)n - (256 * b + 128 * h);
+    valid += left <= 0 ? 0 : (left >= 128 ? 128 : (int)left);
   }
   int obs = valid - miss;
-  sg += __shfl_xor(sg, 16); sg += __shfl_xor(sg, 32);
-  obs += __shfl_xor(obs, 16); obs += __shfl_xor(obs, 32);
+  sg += __shfl_xor(sg, 32);
+  obs += __shfl_xor(obs, 32);
   int c = obs > 0 ? (2 * S * sg + obs) / (2 * obs) : 0;   // round(S sg / obs)
   c = c < 2 * S ? c : 2 * S;
   if (ones_g && c == S) c = sg >= obs ? S + 1 : S - 1;
@@ -404,36 +400,78 @@ __device__ __forceinline__ Frag decode_fill(uint32_t w, uint32_t tbl, uint32_t& 
 struct SnpPipe {
   rsrc_t geno;        // this workgroup's SNP rows (exactly); rows past the last SNP read zeros
   rsrc_t opnd;        // the operand blocks
-  uint32_t rowoff;    // byte offset of this lane's tile-0 row in the workgroup's rows
-  uint32_t tstride;   // 16 rows
+  uint32_t rowoff;    // byte offset of row lane % 16 of the wave's tile 0 in the workgroup's rows
+  uint32_t tstride;   // 32 rows
+  uint32_t hstride;   // 16 rows
 };
 
-// LDS-DMA of the 16-byte pieces of block kb of the lane's ST rows into the
-// wave's part of a genotype slot (tile st at dst + 1024 st).  A piece at or
-// past the row stride (the next row's bytes, or a block past the last) is
-// addressed past the descriptor and lands as zeros without a memory access.
+// LDS-DMA of block kb of the wave's ST tiles into its genotype slot.
+// Instruction i of tile st moves rows 16 i .. 16 i + 15 of the tile, lane l
+// bringing bytes 64 kb + 16 (l / 16) .. + 15 of row 16 i + l % 16 to
+// dst + 1024 (2 st + i) + 16 l: whole 64-byte row pieces per instruction.
+// A piece at or past the row stride (the next row's bytes, or a block past
+// the last) is addressed past the descriptor and lands as zeros without a
+// memory access.
 template <int ST>
-__device__ __forceinline__ void geno_to_lds(const SnpArgs& a, const SnpPipe& pc, int kb, int g, unsigned char* dst) {
-  const uint32_t byte = (uint32_t)kb * 64 + 16 * g;
+__device__ __forceinline__ void geno_to_lds(const SnpArgs& a, const SnpPipe& pc, int kb, int lane, unsigned char* dst) {
+  const uint32_t byte = (uint32_t)kb * 64 + 16 * (lane >> 4);
   const uint32_t o = byte < (uint32_t)a.ld ? pc.rowoff + byte : 0xF0000000u;
 #pragma unroll
-  for (int st = 0; st < ST; ++st) buf_to_lds(pc.geno, o + st * pc.tstride, 0, dst + st * 1024);
+  for (int st = 0; st < ST; ++st)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) buf_to_lds(pc.geno, o + st * pc.tstride + i * pc.hstride, 0, dst + (2 * st + i) * 1024);
 }
 
+// LDS offset of the first of a lane's two 16-byte pieces of tile 0 in a
+// genotype slot: column c = lane % 32 is row c % 16 of instruction c / 16,
+// and half h = lane / 32 owns pieces 2h, 2h + 1 (samples 128 h .. + 127);
+// the second piece is 256 bytes on.  Runs of 16 lanes read 256 contiguous
+// bytes.
+__device__ __forceinline__ uint32_t geno_piece_offset(int lane) {
+  return 1024u * ((lane >> 4) & 1) + 512u * (lane >> 5) + 16u * (lane & 15);
+}
+
+// The lane's 2 ST pieces of a genotype slot (tile st: offsets 2048 st and
+// 2048 st + 256 from p) into registers; the wait is inside the asm block, so
+// no load is in flight when it ends.
+template <int ST>
+__device__ __forceinline__ void lds_read_pieces(const unsigned char* p, v4i (&q)[2 * ST]) {
+  const uint32_t addr = (uint32_t)(uintptr_t)(lds_ptr_t)p;
+  static_assert(ST == 1 || ST == 2, "tile count");
+  if constexpr (ST == 1) {
+    asm volatile("ds_read_b128 %0, %2\n\tds_read_b128 %1, %2 offset:256\n\ts_waitcnt lgkmcnt(0)"
+                 : "=&v"(q[0]), "=&v"(q[1]) : "v"(addr) : "memory");
+  } else {
+    asm volatile("ds_read_b128 %0, %4\n\tds_read_b128 %1, %4 offset:256\n\t"
+                 "ds_read_b128 %2, %4 offset:2048\n\tds_read_b128 %3, %4 offset:2304\n\ts_waitcnt lgkmcnt(0)"
+                 : "=&v"(q[0]), "=&v"(q[1]), "=&v"(q[2]), "=&v"(q[3]) : "v"(addr) : "memory");
+  }
+}
+
+// Depth of the per-wave genotype ring: block kb + kGenoDepth is requested
+// while block kb is computed.
+constexpr int kGenoDepth = 2;
+
 // One block.  Operand block kb + 1 goes to the other slot of the 2-slot
-// operand ring.  The genotype slot holds only the wave's own pieces, so the
-// wave reads block kb into registers and then reuses the slot for block
-// kb + 1 (the reads have completed before the DMA is issued).
+// operand ring.  The genotype slot of block kb holds only the wave's own
+// pieces, so the wave reads them into registers and then reuses the slot for
+// block kb + kGenoDepth (the reads have completed before the DMA is issued).
+// Per block a wave issues the operand DMA first and the genotype DMA (2 ST
+// instructions) last, and loads complete in order, so at the top of block kb
+// only the genotype DMA of block kb + kGenoDepth - 1 may stay in flight.
 template <int NT0, int NT, int ST>
-__device__ __forceinline__ void snp_block(int kb, const SnpArgs& a, const SnpPipe& pc, int wave, int lane, int g,
+__device__ __forceinline__ void snp_block(int kb, const SnpArgs& a, const SnpPipe& pc, int wave, int lane, int h,
                                           const unsigned char* a_rd, unsigned char* a_wr, unsigned char* g_slot,
-                                          const uint32_t (&tbl)[ST], v4i (&acc)[ST][NT], uint32_t (&c_pop)[ST],
+                                          const uint32_t (&tbl)[ST], v16i (&acc)[ST][NT], uint32_t (&c_pop)[ST],
                                           uint32_t (&c_hi)[ST]) {
-  constexpr int KS = 4, W = kSnpWaves, CH = KS * NT;
-  wait_vmcnt<0>();
+  constexpr int KS = 8, W = kSnpWaves, CH = KS * NT;
+  if (kGenoDepth > 1 && kb >= 1 && kb + kGenoDepth - 1 < a.nblk)
+    wait_vmcnt<(kGenoDepth > 1 ? 2 * ST : 0)>();
+  else
+    wait_vmcnt<0>();
   __builtin_amdgcn_s_barrier();   // block kb is in LDS; every wave is done with block kb - 1's operand slot
-  v4i gc[ST];
-  lds_read_tiles<ST>(g_slot + 16 * lane, gc);   // waits for its reads
+  v4i gc[2 * ST];
+  lds_read_pieces<ST>(g_slot + geno_piece_offset(lane), gc);   // waits for its reads
   if (kb + 1 < a.nblk) {
     const uint32_t src = (uint32_t)(kb + 1) * CH * 1024;
 #pragma unroll
@@ -441,8 +479,8 @@ __device__ __forceinline__ void snp_block(int kb, const SnpArgs& a, const SnpPip
       const int c = wave + i * W;
       if (c < CH) buf_to_lds(pc.opnd, 16 * lane, src + c * 1024, a_wr + c * 1024);
     }
-    geno_to_lds<ST>(a, pc, kb + 1, g, g_slot);
   }
+  if (kb + kGenoDepth < a.nblk) geno_to_lds<ST>(a, pc, kb + kGenoDepth, lane, g_slot);
   __builtin_amdgcn_sched_barrier(0);   // keep the prefetches at the top of the block
   // operand fragment c = s NT + rt is read kOperandPrefetch fragments ahead;
   // the genotype fragments of k-step s + 1 are decoded after the MFMAs of s
@@ -453,7 +491,7 @@ __device__ __forceinline__ void snp_block(int kb, const SnpArgs& a, const SnpPip
   for (int c = 0; c < D; ++c) af[c] = Ab[c * 64 + lane];
   Frag fc[ST];
 #pragma unroll
-  for (int st = 0; st < ST; ++st) fc[st] = decode_fill((uint32_t)gc[st][0], tbl[st], c_pop[st], c_hi[st]);
+  for (int st = 0; st < ST; ++st) fc[st] = decode_fill((uint32_t)gc[2 * st][0], tbl[st], c_pop[st], c_hi[st]);
 #pragma unroll
   for (int s = 0; s < KS; ++s) {
 #pragma unroll
@@ -462,103 +500,117 @@ __device__ __forceinline__ void snp_block(int kb, const SnpArgs& a, const SnpPip
       if (c + D < CH) af[c + D] = Ab[(c + D) * 64 + lane];
 #pragma unroll
       for (int st = 0; st < ST; ++st)
-        acc[st][rt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[c], rt < NT0 ? fc[st].g0 : fc[st].miss, acc[st][rt], 0,
+        acc[st][rt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[c], rt < NT0 ? fc[st].g0 : fc[st].miss, acc[st][rt], 0,
                                                             0, 0);
     }
     if (s + 1 < KS) {
 #pragma unroll
-      for (int st = 0; st < ST; ++st) fc[st] = decode_fill((uint32_t)gc[st][s + 1], tbl[st], c_pop[st], c_hi[st]);
+      for (int st = 0; st < ST; ++st)
+        fc[st] = decode_fill((uint32_t)gc[2 * st + ((s + 1) >> 2)][(s + 1) & 3], tbl[st], c_pop[st], c_hi[st]);
     }
   }
 }
 
-// Functionals of lane group g at positions q = 0 .. NQ-1 (i = g + 4 q) from
+// Functionals of lane half h at positions q = 0 .. NQ-1 (i = h + 2 q) from
 // the lane's own accumulators (slot layout of RowMap); wt = the weight table.
 template <int NQ, int NT0, int NT1>
-__device__ __forceinline__ void recombine(const v4i (&acc)[NT0 + NT1], int g, int k3, int km, double mu, double dmu,
+__device__ __forceinline__ void recombine(const v16i (&acc)[NT0 + NT1], int h, int k3, int km, double mu, double dmu,
                                           const double* wt, double (&fq)[NQ]) {
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    const int i = g + 4 * q;
+    const int i = h + 2 * q;
     const bool x = i >= k3, xm = i >= km;
     // X and Y are integers below 2^53, so these FP64 Horner steps are exact
-    double X = fma((double)acc[(2 * q) / 4][(2 * q) % 4], 254.0, (double)acc[(2 * q + 1) / 4][(2 * q + 1) % 4]);
+    double X = fma((double)acc[(2 * q) / 16][(2 * q) % 16], 254.0, (double)acc[(2 * q + 1) / 16][(2 * q + 1) % 16]);
     // slots of term 2 (g' side) and term 1 (miss side) of columns >= k3; the
     // loop is unrolled, so the conditions and indices fold to constants
     const int s2 = 3 * NQ - 1 - q, s1 = 2 * NQ - 1 - q;
-    if (s2 < 4 * NT0 && x) {
-      const int s = s2 < 4 * NT0 ? s2 : 0;
-      X = fma(X, 254.0, (double)acc[s / 4][s % 4]);
+    if (s2 < 16 * NT0 && x) {
+      const int s = s2 < 16 * NT0 ? s2 : 0;
+      X = fma(X, 254.0, (double)acc[s / 16][s % 16]);
     }
-    double Y = (double)acc[NT0 + q / 4][q % 4];
-    if (s1 < 4 * NT1 && xm) {
-      const int s = s1 < 4 * NT1 ? s1 : 0;
-      Y = fma(Y, 254.0, (double)acc[NT0 + s / 4][s % 4]);
+    double Y = (double)acc[NT0 + q / 16][q % 16];
+    if (s1 < 16 * NT1 && xm) {
+      const int s = s1 < 16 * NT1 ? s1 : 0;
+      Y = fma(Y, 254.0, (double)acc[NT0 + s / 16][s % 16]);
     }
     fq[q] = wt[i] * X - mu * wt[64 + i] + dmu * (wt[32 + i] * Y);   // zero weights past m
   }
 }
 
-// Bytes of the split plan for m <= 4 NQ functionals and d <= min(kMaxD, 4 NQ) traits.
+// Bytes of the split plan for m <= 2 NQ functionals and d <= min(kMaxD, 2 NQ) traits.
 __host__ __device__ constexpr int snp_plan_bytes(int NQ) {
-  return 8 * (3 * (4 * NQ < kMaxD ? 4 * NQ : kMaxD) * 32 + (4 * NQ < kMaxD ? 4 * NQ : kMaxD) + 16 * NQ * NQ + 1);
+  return 8 * (3 * (((2 * NQ < kMaxD ? 2 * NQ : kMaxD) + 3) & ~3) * 32 + (((2 * NQ < kMaxD ? 2 * NQ : kMaxD) + 3) & ~3) +
+              4 * NQ * NQ + 1);
 }
 
 template <int NQ, int NT0, int NT1, int ST, int MODE>
 __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
-  constexpr int NT = NT0 + NT1, KS = 4, W = kSnpWaves, CH = KS * NT;
+  constexpr int NT = NT0 + NT1, KS = 8, W = kSnpWaves, CH = KS * NT;
   constexpr int A_SLOT = CH * 1024;
   constexpr int A0_BYTES = MODE == 0 && snp_plan_bytes(NQ) > A_SLOT ? snp_plan_bytes(NQ) : A_SLOT;
   static_assert(kWtab * 8 <= A_SLOT, "weight table must fit an operand slot");
-  constexpr int G_SLOT = W * ST * 1024;
+  constexpr int G_SLOT = W * ST * 2048 * kGenoDepth;
   __shared__ __align__(16) unsigned char A0[A0_BYTES];
   __shared__ __align__(16) unsigned char A1[A_SLOT];
   __shared__ __align__(16) unsigned char G0[G_SLOT];
-  const int t = threadIdx.x, lane = t & 63, g = lane >> 4, l16 = lane & 15;
+  const int t = threadIdx.x, lane = t & 63, h = lane >> 5, col = lane & 31;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);   // wave-uniform: scalar DMA offsets
-  const uint64_t wgbase = (uint64_t)blockIdx.x * (W * ST * 16);
-  const uint64_t base = wgbase + (uint64_t)wave * (ST * 16);
+  const uint64_t wgbase = (uint64_t)blockIdx.x * (W * ST * 32);
+  const uint64_t base = wgbase + (uint64_t)wave * (ST * 32);
   SnpPipe pc;
-  const uint64_t wrows = a.p - wgbase < (uint64_t)(W * ST * 16) ? a.p - wgbase : (uint64_t)(W * ST * 16);
+  const uint64_t wrows = a.p - wgbase < (uint64_t)(W * ST * 32) ? a.p - wgbase : (uint64_t)(W * ST * 32);
   pc.geno = make_rsrc(a.packed + wgbase * a.ld, (uint32_t)(wrows * a.ld));
   pc.opnd = make_rsrc(a.A, (uint32_t)a.nblk * CH * 1024);
-  pc.rowoff = (uint32_t)((base - wgbase + l16) * a.ld);
-  pc.tstride = (uint32_t)(16 * a.ld);
-  v4i acc[ST][NT];
+  pc.rowoff = (uint32_t)((base - wgbase + (lane & 15)) * a.ld);
+  pc.tstride = (uint32_t)(32 * a.ld);
+  pc.hstride = (uint32_t)(16 * a.ld);
+  const uint32_t coloff = (uint32_t)((base - wgbase + col) * a.ld);   // this lane's own column
+  v16i acc[ST][NT];
 #pragma unroll
   for (int st = 0; st < ST; ++st)
 #pragma unroll
-    for (int rt = 0; rt < NT; ++rt) acc[st][rt] = v4i{0, 0, 0, 0};
+    for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[st][rt][e] = 0;
   uint32_t c_pop[ST], c_hi[ST];
 #pragma unroll
   for (int st = 0; st < ST; ++st) c_pop[st] = c_hi[st] = 0;
-  // prologue in the order the blocks issue them: operand 0, genotypes 0, genotypes 1
+  // prologue in the order the blocks issue them: operand 0, genotypes 0
+  unsigned char* gslot = G0 + wave * (ST * 2048 * kGenoDepth);   // the wave's ring, block b in slot b % kGenoDepth
   for (int c = wave; c < CH; c += W) buf_to_lds(pc.opnd, 16 * lane, c * 1024, A0 + c * 1024);
-  geno_to_lds<ST>(a, pc, 0, g, G0 + wave * (ST * 1024));
+#pragma unroll
+  for (int b = 0; b < kGenoDepth; ++b)
+    if (b < a.nblk) geno_to_lds<ST>(a, pc, b, lane, gslot + b * (ST * 2048));
   // blocks 1 .. kFillBlocks - 1 of the lane's rows for the fill code (plain
   // loads; bytes past the row stride read zeros through the descriptor)
-  v4u fw[ST][kFillBlocks];
+  v4u fw[ST][kFillBlocks][2];
 #pragma unroll
   for (int st = 0; st < ST; ++st)
 #pragma unroll
-    for (int b = 1; b < kFillBlocks; ++b) {
-      const uint32_t byte = (uint32_t)b * 64 + 16 * g;
-      const uint32_t o = byte < (uint32_t)a.ld ? pc.rowoff + byte + st * pc.tstride : 0xF0000000u;
-      fw[st][b] = __builtin_bit_cast(v4u, __builtin_amdgcn_raw_buffer_load_b128(pc.geno, o, 0, 0));
-    }
+    for (int b = 1; b < kFillBlocks; ++b)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const uint32_t byte = (uint32_t)b * 64 + 32 * h + 16 * i;
+        const uint32_t o = byte < (uint32_t)a.ld ? coloff + byte + st * pc.tstride : 0xF0000000u;
+        fw[st][b][i] = __builtin_bit_cast(v4u, __builtin_amdgcn_raw_buffer_load_b128(pc.geno, o, 0, 0));
+      }
   wait_vmcnt<0>();
   uint32_t tbl[ST];
   {
-    const v4u* gp = reinterpret_cast<const v4u*>(G0 + wave * (ST * 1024));
+    const unsigned char* gp = gslot + geno_piece_offset(lane);
 #pragma unroll
     for (int st = 0; st < ST; ++st) {
-      fw[st][0] = gp[st * 64 + lane];
-      tbl[st] = fill_table(fw[st], a.n, g, a.S, a.fill, a.ones_g);
+      fw[st][0][0] = *reinterpret_cast<const v4u*>(gp + 2048 * st);
+      fw[st][0][1] = *reinterpret_cast<const v4u*>(gp + 2048 * st + 256);
+      tbl[st] = fill_table(fw[st], a.n, h, a.S, a.fill, a.ones_g);
     }
   }
-  unsigned char* gslot = G0 + wave * (ST * 1024);
-#define SLFMM_SNP(PH, AR_, AW_) \
-  if (kb + PH < a.nblk) snp_block<NT0, NT, ST>(kb + PH, a, pc, wave, lane, g, AR_, AW_, gslot, tbl, acc, c_pop, c_hi);
+  static_assert(kGenoDepth == 1 || kGenoDepth == 2, "the loop below alternates two phases");
+#define SLFMM_SNP(PH, AR_, AW_)                                                                          \
+  if (kb + PH < a.nblk)                                                                                  \
+    snp_block<NT0, NT, ST>(kb + PH, a, pc, wave, lane, h, AR_, AW_, gslot + (PH % kGenoDepth) * (ST * 2048), tbl, \
+                           acc, c_pop, c_hi);
   for (int kb = 0; kb < a.nblk; kb += 2) {
     SLFMM_SNP(0, A0, A1)
     SLFMM_SNP(1, A1, A0)
@@ -580,11 +632,11 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
 #pragma unroll
   for (int st = 0; st < ST; ++st) {
     uint32_t cp = c_pop[st], ch = c_hi[st];
-    cp += __shfl_xor(cp, 16); cp += __shfl_xor(cp, 32);
-    ch += __shfl_xor(ch, 16); ch += __shfl_xor(ch, 32);
+    cp += __shfl_xor(cp, 32);
+    ch += __shfl_xor(ch, 32);
     // the all-ones row: the missing count (miss tile) or sum g' = S sg + c n_miss (g' tile)
-    const int ones_m = __shfl(acc[st][NT - 1][3], 16 * a.g1 + l16);
-    const int ones_g = __shfl(acc[st][NT0 - 1][3], 16 * a.g1 + l16);
+    const int ones_m = __shfl(acc[st][NT - 1][15], 32 * a.g1 + col);
+    const int ones_g = __shfl(acc[st][NT0 - 1][15], 32 * a.g1 + col);
     const int cfill = (int)((tbl[st] >> 8) & 0xFFu);
     const int64_t nmiss = a.ones_g ? ((int64_t)a.S * cp - ones_g) / (a.S - cfill) : (int64_t)ones_m;
     const int64_t sg = (int64_t)cp - nmiss, nhom = sg - (int64_t)ch;
@@ -592,16 +644,16 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
     const double mu = nobs > 0 ? (double)sg / (double)nobs : 0.0;
     const double dmu = mu - (double)cfill / (double)a.S;
     double fq[NQ];
-    recombine<NQ, NT0, NT1>(acc[st], g, a.k3, a.km, mu, dmu, wt, fq);
-    const uint64_t j = base + 16 * st + l16;
+    recombine<NQ, NT0, NT1>(acc[st], h, a.k3, a.km, mu, dmu, wt, fq);
+    const uint64_t j = base + 32 * st + col;
     const bool valid = j < a.p;
     if constexpr (MODE == 1) {
       const double scale = epi_scale(mu, a.norm);
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
-        if (valid && g + 4 * q < a.m) a.Z[j * a.ldZ + g + 4 * q] = scale * fq[q];
+        if (valid && h + 2 * q < a.m) a.Z[j * a.ldZ + h + 2 * q] = scale * fq[q];
     } else {
-      snp_epilogue_split<NQ>(a.ea, plan, j, valid, g, fq, nmiss, sg, nhom);
+      snp_epilogue_split2<NQ>(a.ea, plan, j, valid, h, fq, nmiss, sg, nhom);
     }
   }
 }
@@ -609,22 +661,25 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
 // (NQ, NT0, NT1) of every RowMap the passes use with m <= 32: association
 // RowMap::make(K + d, K, K) for d <= 16, K >= 1; RSVD RowMap::make(l, l, l).
 #define SLFMM_SNP_ASSOC(X)                                                                                     \
-  X(1, 1, 1) X(2, 2, 1) X(3, 2, 1) X(3, 2, 2) X(3, 3, 2) X(4, 3, 2) X(5, 3, 2) X(5, 4, 2) X(5, 4, 3) X(6, 4, 2) \
-  X(6, 4, 3) X(6, 5, 3) X(7, 4, 2) X(7, 4, 3) X(7, 5, 3) X(8, 5, 3) X(8, 5, 4) X(8, 6, 4)
-#define SLFMM_SNP_RSVD(X) \
-  X(1, 1, 1) X(2, 1, 1) X(3, 2, 1) X(4, 2, 1) X(4, 2, 2) X(5, 3, 2) X(6, 3, 2) X(7, 4, 2) X(8, 4, 2) X(8, 4, 3)
+  X(1, 1, 1) X(2, 1, 1) X(3, 1, 1) X(4, 1, 1) X(5, 1, 1) X(6, 1, 1) X(6, 2, 1) X(7, 1, 1) X(7, 2, 1) X(8, 2, 1) \
+  X(9, 2, 1) X(9, 2, 2) X(10, 2, 1) X(10, 2, 2) X(11, 2, 1) X(11, 2, 2) X(12, 2, 1) X(12, 2, 2) X(12, 3, 2)     \
+  X(13, 2, 1) X(13, 2, 2) X(13, 3, 2) X(14, 2, 1) X(14, 2, 2) X(14, 3, 2) X(15, 2, 1) X(15, 2, 2) X(15, 3, 2)   \
+  X(16, 3, 2)
+#define SLFMM_SNP_RSVD(X)                                                                                      \
+  X(1, 1, 1) X(2, 1, 1) X(3, 1, 1) X(4, 1, 1) X(5, 1, 1) X(6, 1, 1) X(7, 1, 1) X(8, 1, 1) X(9, 2, 1) X(10, 2, 1) \
+  X(11, 2, 1) X(12, 2, 1) X(13, 2, 1) X(14, 2, 1) X(15, 2, 1) X(16, 2, 1) X(16, 2, 2)
 
 template <int NT>
-constexpr int snp_tiles() { return NT <= 8 ? 4 : 2; }   // SNP tiles per wave (accumulator registers)
+constexpr int snp_tiles() { return NT <= 3 ? 2 : 1; }   // 32-SNP tiles per wave (accumulator registers)
 
 static void launch_snp(hipStream_t s, const Operand& op, SnpArgs& a, int mode) {
   if (a.p == 0) return;
   const RowMap& r = op.rmap;
   const int NT = r.NT0 + r.NT1;
-  const int st = NT <= 8 ? 4 : 2;
-  const uint64_t rows = 16 * (uint64_t)st * kSnpWaves;
+  const int st = NT <= 3 ? 2 : 1;
+  const uint64_t rows = 32 * (uint64_t)st * kSnpWaves;
   // buffer descriptors address a workgroup's rows and the operand with 32-bit byte offsets
-  SLFMM_CHECK((rows + 3 * 16) * a.ld < 0xF0000000ull && (uint64_t)a.nblk * 4 * NT * 1024 < 0xFFFFFFFFull,
+  SLFMM_CHECK((rows + 2 * 32) * a.ld < 0xF0000000ull && (uint64_t)a.nblk * 8 * NT * 1024 < 0xFFFFFFFFull,
               SLFMM_E_ARG, "panel rows too long for the int8 kernels (use SLFMM_PREC_FP64)");
   a.m = r.m; a.k3 = r.k3; a.km = r.km; a.g1 = r.g1; a.ones_g = r.ones_g; a.S = op.S; a.fill = op.fill;
   const unsigned grid = (unsigned)cdiv(a.p, rows);
@@ -648,13 +703,13 @@ static void launch_snp(hipStream_t s, const Operand& op, SnpArgs& a, int mode) {
 }
 
 // Layout diagnostic (host only): dims = {NQ, NT0, NT1, g1, ones_g}; rows
-// (2 x 16 (NT0 + NT1) ints, if non-null) = (column, term) of every packed row.
+// (2 x 32 (NT0 + NT1) ints, if non-null) = (column, term) of every packed row.
 void i8_layout(int m, int k3, int km, int32_t* dims, int32_t* rows, uint64_t cap) {
   SLFMM_CHECK(m >= 1 && m <= 32 && k3 >= 0 && k3 <= m && km >= 0 && km <= m, SLFMM_E_ARG, "operand shape");
   const RowMap r = RowMap::make(m, k3, km);
   dims[0] = r.NQ; dims[1] = r.NT0; dims[2] = r.NT1; dims[3] = r.g1; dims[4] = r.ones_g;
   if (!rows) return;
-  const int nrow = 16 * (r.NT0 + r.NT1);
+  const int nrow = 32 * (r.NT0 + r.NT1);
   SLFMM_CHECK(cap >= (uint64_t)(2 * nrow), SLFMM_E_ARG, "rows buffer too small");
   for (int row = 0; row < nrow; ++row) {
     const int2 v = r.info(row);

@@ -253,4 +253,103 @@ __device__ void snp_epilogue_split(const EpiArgs& a, const double* sPlan, uint64
   }
 }
 
+// snp_epilogue_split with the functionals of SNP j spread over the two
+// lane halves of a wave (lanes l and l ^ 32): half h holds fq[q] = f_i for
+// i = h + 2 q.  Dot products are formed per lane and summed over the two
+// halves (v_permlane32_swap: the same bits in both lanes); trait jj is
+// finished and stored by half jj % 2, r2 fractions 0 and 2 by half 0 and
+// fraction 1 by half 1.  Every lane of the wave must call this; `valid`
+// masks the stores.
+template <int NQ>
+__device__ void snp_epilogue_split2(const EpiArgs& a, const double* sPlan, uint64_t j, bool valid, int h,
+                                    const double (&fq)[NQ], int64_t nmiss, int64_t sg, int64_t nh) {
+  const int d = a.d, m = a.m;
+  const int64_t nobs = (int64_t)a.n - nmiss;
+  const double mean = nobs > 0 ? (double)sg / (double)nobs : 0.0;
+  const double sc = epi_scale(mean, a.norm);
+  const int64_t sg2 = sg + 2 * nh;
+  const double tss = nobs > 0 ? sc * sc * ((double)(nobs * sg2 - sg * sg) / (double)nobs) : 0.0;
+  double fy[NQ];   // zero for i >= m (fq is)
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) fy[q] = sc * fq[q];
+  const int dp = (d + 3) & ~3;
+  const double* P = sPlan;   // split layout (stage_split_plan)
+  const double* Rm = P + dp * 32;
+  const double* Fm = Rm + dp * 32;
+  const double* Vd = Fm + dp * 32;
+  const double* L = Vd + dp;
+  const bool fallback = L[m * m] != 0.0;
+  double proj = 0.0;
+  if (!fallback) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) proj += fy[q] * fy[q];
+    proj = add_swapped<32>(proj);
+  } else {
+    for (int c = 0; c < m; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (h + 2 * q < m) s += L[(h + 2 * q) * m + c] * fy[q];
+      s = add_swapped<32>(s);
+      proj += s * s;
+    }
+  }
+  double rss = tss - proj;
+  if (rss < 0.0) rss = 0.0;
+  const double sigma2 = rss / a.df;
+  double sscov = 0.0;
+  for (int jb = 0; jb < d; jb += 2) {
+    // the sums of traits jb, jb + 1 reach both halves; half h finishes trait jb + h
+    double cf[2], bt[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {   // traits past d have zero rows: they add exact zeros
+      const int jj = jb + u;
+      double coef = 0.0, b = 0.0, fc = 0.0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int i = h + 2 * q;
+        coef += P[jj * 32 + i] * fy[q];
+        b += Rm[jj * 32 + i] * fy[q];
+        fc += Fm[jj * 32 + i] * fy[q];
+      }
+      coef = add_swapped<32>(coef);
+      b = add_swapped<32>(b);
+      fc = add_swapped<32>(fc);
+      sscov += fc * fc;
+      cf[u] = coef;
+      bt[u] = b;
+    }
+    const int jj = jb + h;
+    const double coef = h == 0 ? cf[0] : cf[1];
+    const double b = h == 0 ? bt[0] : bt[1];
+    if (valid && jj < d) {
+      // t_test (src/testing.rs:1036-1048)
+      const double se = sqrt(sigma2 * Vd[jj]);
+      double tv = 0.0;
+      if (se >= 1e-300 && isfinite(se)) {
+        tv = coef / se;
+        if (!isfinite(tv)) tv = 0.0;
+      }
+      a.tstat[j * d + jj] = tv;
+      a.beta[j * d + jj] = b;
+    }
+  }
+  if (valid) {   // r2 fractions (src/testing.rs:394-404): 0 and 2 by half 0, 1 by half 1
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    if (tss > 1e-300) {
+      double sslat = tss - rss - sscov;
+      if (sslat < 0.0) sslat = 0.0;
+      v0 = sscov / tss;
+      v1 = sslat / tss;
+      v2 = rss / tss;
+    }
+    if (h == 0) {
+      a.r2[j * 3 + 0] = v0;
+      a.r2[j * 3 + 2] = v2;
+    } else {
+      a.r2[j * 3 + 1] = v1;
+    }
+  }
+}
+
 }  // namespace slfmm
