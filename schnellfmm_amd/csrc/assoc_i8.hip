@@ -334,6 +334,7 @@ __device__ __forceinline__ void buf_to_lds(rsrc_t rs, uint32_t voff, uint32_t so
 struct SnpArgs {
   const uint8_t* packed;
   uint64_t ld, p, n;
+  uint32_t rb;   // bytes of a row that are read: ceil(n/4) rounded up to 16 (the rest of the stride reads as zeros)
   int nblk, m, k3, km, g1, ones_g, norm, S, fill;
   const int8_t* A;
   const double* wtab;
@@ -415,7 +416,7 @@ struct SnpPipe {
 template <int ST>
 __device__ __forceinline__ void geno_to_lds(const SnpArgs& a, const SnpPipe& pc, int kb, int lane, unsigned char* dst) {
   const uint32_t byte = (uint32_t)kb * 64 + 16 * (lane >> 4);
-  const uint32_t o = byte < (uint32_t)a.ld ? pc.rowoff + byte : 0xF0000000u;
+  const uint32_t o = byte < a.rb ? pc.rowoff + byte : 0xF0000000u;
 #pragma unroll
   for (int st = 0; st < ST; ++st)
 #pragma unroll
@@ -592,7 +593,7 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         const uint32_t byte = (uint32_t)b * 64 + 32 * h + 16 * i;
-        const uint32_t o = byte < (uint32_t)a.ld ? coloff + byte + st * pc.tstride : 0xF0000000u;
+        const uint32_t o = byte < a.rb ? coloff + byte + st * pc.tstride : 0xF0000000u;
         fw[st][b][i] = __builtin_bit_cast(v4u, __builtin_amdgcn_raw_buffer_load_b128(pc.geno, o, 0, 0));
       }
   wait_vmcnt<0>();
@@ -682,6 +683,7 @@ static void launch_snp(hipStream_t s, const Operand& op, SnpArgs& a, int mode) {
   SLFMM_CHECK((rows + 2 * 32) * a.ld < 0xF0000000ull && (uint64_t)a.nblk * 8 * NT * 1024 < 0xFFFFFFFFull,
               SLFMM_E_ARG, "panel rows too long for the int8 kernels (use SLFMM_PREC_FP64)");
   a.m = r.m; a.k3 = r.k3; a.km = r.km; a.g1 = r.g1; a.ones_g = r.ones_g; a.S = op.S; a.fill = op.fill;
+  a.rb = (uint32_t)std::min<uint64_t>(rup(cdiv(a.n, 4), 16), a.ld);
   const unsigned grid = (unsigned)cdiv(a.p, rows);
   bool done = false;
 #define X(Q, A0_, A1_)                                                                                        \
