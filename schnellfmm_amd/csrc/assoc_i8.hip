@@ -401,78 +401,90 @@ __device__ __forceinline__ Frag decode_fill(uint32_t w, uint32_t tbl, uint32_t& 
 struct SnpPipe {
   rsrc_t geno;        // this workgroup's SNP rows (exactly); rows past the last SNP read zeros
   rsrc_t opnd;        // the operand blocks
-  uint32_t rowoff;    // byte offset of row lane % 16 of the wave's tile 0 in the workgroup's rows
+  uint32_t rowoff;    // byte offset of row lane / 8 of the wave's tile 0 in the workgroup's rows
   uint32_t tstride;   // 32 rows
-  uint32_t hstride;   // 16 rows
+  uint32_t istride;   // 8 rows
+  uint32_t ub;        // byte of the lane's 16-byte unit in a 128-byte row piece (even instructions)
+  uint32_t raddr[4];  // LDS addresses of the lane's pieces (b, k) of tile 0 in its genotype slot
 };
 
-// LDS-DMA of block kb of the wave's ST tiles into its genotype slot.
-// Instruction i of tile st moves rows 16 i .. 16 i + 15 of the tile, lane l
-// bringing bytes 64 kb + 16 (l / 16) .. + 15 of row 16 i + l % 16 to
-// dst + 1024 (2 st + i) + 16 l: whole 64-byte row pieces per instruction.
-// A piece at or past the row stride (the next row's bytes, or a block past
-// the last) is addressed past the descriptor and lands as zeros without a
-// memory access.
+// Genotype staging works on block pairs: pair q is the 128-byte piece
+// 128 q .. 128 q + 127 of every row (samples 512 q .. + 511, blocks 2q and
+// 2q + 1), so every DMA instruction moves whole 128-byte cache lines (rows
+// start on 128-byte boundaries when they are 512 bytes or longer).  A stream
+// of such pieces runs at 6.1 TB/s against 5.3 TB/s for 64-byte pieces.
+//
+// LDS-DMA of pair q of the wave's ST tiles into its 4 ST KB genotype slot.
+// Instruction i of tile st moves rows 8 i .. 8 i + 7 of the tile: lane l
+// brings unit u = (l % 8 - l / 16 - 4 i) mod 8 (16 bytes) of row 8 i + l / 8
+// to slot + 1024 (4 st + i) + 16 l.  The unit rotation by row / 2 spreads
+// the 16 lanes that later read one unit of 16 consecutive rows over all LDS
+// banks.  Units at or past the row's bytes (rb) are addressed past the
+// descriptor and land as zeros without a memory access.
 template <int ST>
-__device__ __forceinline__ void geno_to_lds(const SnpArgs& a, const SnpPipe& pc, int kb, int lane, unsigned char* dst) {
-  const uint32_t byte = (uint32_t)kb * 64 + 16 * (lane >> 4);
-  const uint32_t o = byte < a.rb ? pc.rowoff + byte : 0xF0000000u;
+__device__ __forceinline__ void geno_pair_to_lds(const SnpArgs& a, const SnpPipe& pc, int q, unsigned char* dst) {
 #pragma unroll
-  for (int st = 0; st < ST; ++st)
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t byte = (uint32_t)q * 128 + (pc.ub ^ (i & 1 ? 64u : 0u));
+    const uint32_t o = byte < a.rb ? pc.rowoff + i * pc.istride + byte : 0xF0000000u;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) buf_to_lds(pc.geno, o + st * pc.tstride + i * pc.hstride, 0, dst + (2 * st + i) * 1024);
-}
-
-// LDS offset of the first of a lane's two 16-byte pieces of tile 0 in a
-// genotype slot: column c = lane % 32 is row c % 16 of instruction c / 16,
-// and half h = lane / 32 owns pieces 2h, 2h + 1 (samples 128 h .. + 127);
-// the second piece is 256 bytes on.  Runs of 16 lanes read 256 contiguous
-// bytes.
-__device__ __forceinline__ uint32_t geno_piece_offset(int lane) {
-  return 1024u * ((lane >> 4) & 1) + 512u * (lane >> 5) + 16u * (lane & 15);
-}
-
-// The lane's 2 ST pieces of a genotype slot (tile st: offsets 2048 st and
-// 2048 st + 256 from p) into registers; the wait is inside the asm block, so
-// no load is in flight when it ends.
-template <int ST>
-__device__ __forceinline__ void lds_read_pieces(const unsigned char* p, v4i (&q)[2 * ST]) {
-  const uint32_t addr = (uint32_t)(uintptr_t)(lds_ptr_t)p;
-  static_assert(ST == 1 || ST == 2, "tile count");
-  if constexpr (ST == 1) {
-    asm volatile("ds_read_b128 %0, %2\n\tds_read_b128 %1, %2 offset:256\n\ts_waitcnt lgkmcnt(0)"
-                 : "=&v"(q[0]), "=&v"(q[1]) : "v"(addr) : "memory");
-  } else {
-    asm volatile("ds_read_b128 %0, %4\n\tds_read_b128 %1, %4 offset:256\n\t"
-                 "ds_read_b128 %2, %4 offset:2048\n\tds_read_b128 %3, %4 offset:2304\n\ts_waitcnt lgkmcnt(0)"
-                 : "=&v"(q[0]), "=&v"(q[1]), "=&v"(q[2]), "=&v"(q[3]) : "v"(addr) : "memory");
+    for (int st = 0; st < ST; ++st) buf_to_lds(pc.geno, o + (byte < a.rb ? st * pc.tstride : 0u), 0, dst + (4 * st + i) * 1024);
   }
 }
 
-// Depth of the per-wave genotype ring: block kb + kGenoDepth is requested
-// while block kb is computed.
-constexpr int kGenoDepth = 2;
+// Byte offset in a genotype slot of piece (b, k) of tile 0 for lane
+// (column c = lane % 32, half h = lane / 32): unit u = 4 b + 2 h + k of row
+// c, i.e. samples 256 b + 128 h + 64 k .. + 63 of the pair.
+__device__ __forceinline__ uint32_t geno_piece_offset(int lane, int b, int k) {
+  const int c = lane & 31, h = lane >> 5, u = 4 * b + 2 * h + k;
+  return 1024u * (c >> 3) + 16u * (8 * (c & 7) + ((u + (c >> 1)) & 7));
+}
 
-// One block.  Operand block kb + 1 goes to the other slot of the 2-slot
-// operand ring.  The genotype slot of block kb holds only the wave's own
-// pieces, so the wave reads them into registers and then reuses the slot for
-// block kb + kGenoDepth (the reads have completed before the DMA is issued).
-// Per block a wave issues the operand DMA first and the genotype DMA (2 ST
-// instructions) last, and loads complete in order, so at the top of block kb
-// only the genotype DMA of block kb + kGenoDepth - 1 may stay in flight.
-template <int NT0, int NT, int ST>
-__device__ __forceinline__ void snp_block(int kb, const SnpArgs& a, const SnpPipe& pc, int wave, int lane, int h,
+// The lane's 4 ST pieces of the pair in its genotype slot into registers:
+// q[4 st + 2 b + k] = piece (b, k) of tile st (tile st at +4096 st).  The
+// wait is inside the asm block, so no load is in flight when it ends.
+template <int ST>
+__device__ __forceinline__ void lds_read_pair(const SnpPipe& pc, v4i (&q)[4 * ST]) {
+  static_assert(ST == 1 || ST == 2, "tile count");
+  if constexpr (ST == 1) {
+    asm volatile("ds_read_b128 %0, %4\n\tds_read_b128 %1, %5\n\tds_read_b128 %2, %6\n\tds_read_b128 %3, %7\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(q[0]), "=&v"(q[1]), "=&v"(q[2]), "=&v"(q[3])
+                 : "v"(pc.raddr[0]), "v"(pc.raddr[1]), "v"(pc.raddr[2]), "v"(pc.raddr[3])
+                 : "memory");
+  } else {
+    asm volatile("ds_read_b128 %0, %8\n\tds_read_b128 %1, %9\n\tds_read_b128 %2, %10\n\tds_read_b128 %3, %11\n\t"
+                 "ds_read_b128 %4, %8 offset:4096\n\tds_read_b128 %5, %9 offset:4096\n\t"
+                 "ds_read_b128 %6, %10 offset:4096\n\tds_read_b128 %7, %11 offset:4096\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(q[0]), "=&v"(q[1]), "=&v"(q[2]), "=&v"(q[3]), "=&v"(q[4]), "=&v"(q[5]), "=&v"(q[6]),
+                   "=&v"(q[7])
+                 : "v"(pc.raddr[0]), "v"(pc.raddr[1]), "v"(pc.raddr[2]), "v"(pc.raddr[3])
+                 : "memory");
+  }
+}
+
+// One block (b = kb % 2 of pair kb / 2).  Operand block kb + 1 goes to the
+// other slot of the 2-slot operand ring.  On the first block of a pair the
+// wave reads the pair's pieces from its genotype slot into registers (gc)
+// and then reuses the slot for the next pair (the reads have completed
+// before the DMA is issued).  A wave issues the operand DMA first and the
+// genotype DMA (4 ST instructions, first blocks only) last, and loads
+// complete in order, so at the top of the second block of a pair only the
+// next pair's genotype DMA may stay in flight; at the top of a first block
+// nothing may.
+template <int NT0, int NT, int ST, int B>
+__device__ __forceinline__ void snp_block(int kb, const SnpArgs& a, const SnpPipe& pc, int wave, int lane,
                                           const unsigned char* a_rd, unsigned char* a_wr, unsigned char* g_slot,
-                                          const uint32_t (&tbl)[ST], v16i (&acc)[ST][NT], uint32_t (&c_pop)[ST],
-                                          uint32_t (&c_hi)[ST]) {
-  constexpr int KS = 8, W = kSnpWaves, CH = KS * NT;
-  if (kGenoDepth > 1 && kb >= 1 && kb + kGenoDepth - 1 < a.nblk)
-    wait_vmcnt<(kGenoDepth > 1 ? 2 * ST : 0)>();
+                                          v4i (&gc)[4 * ST], const uint32_t (&tbl)[ST], v16i (&acc)[ST][NT],
+                                          uint32_t (&c_pop)[ST], uint32_t (&c_hi)[ST]) {
+  constexpr int KS = 8, W = kSnpWaves, CH = KS * NT, b = B;   // b = kb % 2
+  if (b == 1 && kb + 1 < a.nblk)
+    wait_vmcnt<4 * ST>();
   else
     wait_vmcnt<0>();
   __builtin_amdgcn_s_barrier();   // block kb is in LDS; every wave is done with block kb - 1's operand slot
-  v4i gc[2 * ST];
-  lds_read_pieces<ST>(g_slot + geno_piece_offset(lane), gc);   // waits for its reads
+  if (b == 0) lds_read_pair<ST>(pc, gc);   // waits for its reads
   if (kb + 1 < a.nblk) {
     const uint32_t src = (uint32_t)(kb + 1) * CH * 1024;
 #pragma unroll
@@ -481,7 +493,7 @@ __device__ __forceinline__ void snp_block(int kb, const SnpArgs& a, const SnpPip
       if (c < CH) buf_to_lds(pc.opnd, 16 * lane, src + c * 1024, a_wr + c * 1024);
     }
   }
-  if (kb + kGenoDepth < a.nblk) geno_to_lds<ST>(a, pc, kb + kGenoDepth, lane, g_slot);
+  if (b == 0 && kb + 2 < a.nblk) geno_pair_to_lds<ST>(a, pc, kb / 2 + 1, g_slot);
   __builtin_amdgcn_sched_barrier(0);   // keep the prefetches at the top of the block
   // operand fragment c = s NT + rt is read kOperandPrefetch fragments ahead;
   // the genotype fragments of k-step s + 1 are decoded after the MFMAs of s
@@ -492,7 +504,7 @@ __device__ __forceinline__ void snp_block(int kb, const SnpArgs& a, const SnpPip
   for (int c = 0; c < D; ++c) af[c] = Ab[c * 64 + lane];
   Frag fc[ST];
 #pragma unroll
-  for (int st = 0; st < ST; ++st) fc[st] = decode_fill((uint32_t)gc[2 * st][0], tbl[st], c_pop[st], c_hi[st]);
+  for (int st = 0; st < ST; ++st) fc[st] = decode_fill((uint32_t)gc[4 * st + 2 * b][0], tbl[st], c_pop[st], c_hi[st]);
 #pragma unroll
   for (int s = 0; s < KS; ++s) {
 #pragma unroll
@@ -507,7 +519,8 @@ __device__ __forceinline__ void snp_block(int kb, const SnpArgs& a, const SnpPip
     if (s + 1 < KS) {
 #pragma unroll
       for (int st = 0; st < ST; ++st)
-        fc[st] = decode_fill((uint32_t)gc[2 * st + ((s + 1) >> 2)][(s + 1) & 3], tbl[st], c_pop[st], c_hi[st]);
+        fc[st] = decode_fill((uint32_t)gc[4 * st + 2 * b + ((s + 1) >> 2)][(s + 1) & 3], tbl[st], c_pop[st],
+                             c_hi[st]);
     }
   }
 }
@@ -551,7 +564,7 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
   constexpr int A_SLOT = CH * 1024;
   constexpr int A0_BYTES = MODE == 0 && snp_plan_bytes(NQ) > A_SLOT ? snp_plan_bytes(NQ) : A_SLOT;
   static_assert(kWtab * 8 <= A_SLOT, "weight table must fit an operand slot");
-  constexpr int G_SLOT = W * ST * 2048 * kGenoDepth;
+  constexpr int G_SLOT = W * ST * 4096;
   __shared__ __align__(16) unsigned char A0[A0_BYTES];
   __shared__ __align__(16) unsigned char A1[A_SLOT];
   __shared__ __align__(16) unsigned char G0[G_SLOT];
@@ -559,13 +572,18 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);   // wave-uniform: scalar DMA offsets
   const uint64_t wgbase = (uint64_t)blockIdx.x * (W * ST * 32);
   const uint64_t base = wgbase + (uint64_t)wave * (ST * 32);
+  unsigned char* gslot = G0 + wave * (ST * 4096);   // the wave's genotype slot (one block pair)
   SnpPipe pc;
   const uint64_t wrows = a.p - wgbase < (uint64_t)(W * ST * 32) ? a.p - wgbase : (uint64_t)(W * ST * 32);
   pc.geno = make_rsrc(a.packed + wgbase * a.ld, (uint32_t)(wrows * a.ld));
   pc.opnd = make_rsrc(a.A, (uint32_t)a.nblk * CH * 1024);
-  pc.rowoff = (uint32_t)((base - wgbase + (lane & 15)) * a.ld);
+  pc.rowoff = (uint32_t)((base - wgbase + (lane >> 3)) * a.ld);
   pc.tstride = (uint32_t)(32 * a.ld);
-  pc.hstride = (uint32_t)(16 * a.ld);
+  pc.istride = (uint32_t)(8 * a.ld);
+  pc.ub = 16u * (uint32_t)(((lane & 7) - (lane >> 4)) & 7);
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    pc.raddr[e] = (uint32_t)(uintptr_t)(lds_ptr_t)gslot + geno_piece_offset(lane, e >> 1, e & 1);
   const uint32_t coloff = (uint32_t)((base - wgbase + col) * a.ld);   // this lane's own column
   v16i acc[ST][NT];
 #pragma unroll
@@ -577,19 +595,17 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
   uint32_t c_pop[ST], c_hi[ST];
 #pragma unroll
   for (int st = 0; st < ST; ++st) c_pop[st] = c_hi[st] = 0;
-  // prologue in the order the blocks issue them: operand 0, genotypes 0
-  unsigned char* gslot = G0 + wave * (ST * 2048 * kGenoDepth);   // the wave's ring, block b in slot b % kGenoDepth
+  // prologue in the order the blocks issue them: operand 0, genotype pair 0
   for (int c = wave; c < CH; c += W) buf_to_lds(pc.opnd, 16 * lane, c * 1024, A0 + c * 1024);
-#pragma unroll
-  for (int b = 0; b < kGenoDepth; ++b)
-    if (b < a.nblk) geno_to_lds<ST>(a, pc, b, lane, gslot + b * (ST * 2048));
-  // blocks 1 .. kFillBlocks - 1 of the lane's rows for the fill code (plain
-  // loads; bytes past the row stride read zeros through the descriptor)
+  geno_pair_to_lds<ST>(a, pc, 0, gslot);
+  // blocks 2 .. kFillBlocks - 1 of the lane's rows for the fill code (plain
+  // loads; bytes past the row's bytes read zeros through the descriptor)
+  static_assert(kFillBlocks >= 2, "the fill code reads block pair 0 from the genotype slot");
   v4u fw[ST][kFillBlocks][2];
 #pragma unroll
   for (int st = 0; st < ST; ++st)
 #pragma unroll
-    for (int b = 1; b < kFillBlocks; ++b)
+    for (int b = 2; b < kFillBlocks; ++b)
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         const uint32_t byte = (uint32_t)b * 64 + 32 * h + 16 * i;
@@ -598,25 +614,20 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
       }
   wait_vmcnt<0>();
   uint32_t tbl[ST];
-  {
-    const unsigned char* gp = gslot + geno_piece_offset(lane);
 #pragma unroll
-    for (int st = 0; st < ST; ++st) {
-      fw[st][0][0] = *reinterpret_cast<const v4u*>(gp + 2048 * st);
-      fw[st][0][1] = *reinterpret_cast<const v4u*>(gp + 2048 * st + 256);
-      tbl[st] = fill_table(fw[st], a.n, h, a.S, a.fill, a.ones_g);
-    }
+  for (int st = 0; st < ST; ++st) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        fw[st][b][i] = *reinterpret_cast<const v4u*>(gslot + 4096 * st + geno_piece_offset(lane, b, i));
+    tbl[st] = fill_table(fw[st], a.n, h, a.S, a.fill, a.ones_g);
   }
-  static_assert(kGenoDepth == 1 || kGenoDepth == 2, "the loop below alternates two phases");
-#define SLFMM_SNP(PH, AR_, AW_)                                                                          \
-  if (kb + PH < a.nblk)                                                                                  \
-    snp_block<NT0, NT, ST>(kb + PH, a, pc, wave, lane, h, AR_, AW_, gslot + (PH % kGenoDepth) * (ST * 2048), tbl, \
-                           acc, c_pop, c_hi);
+  v4i gc[4 * ST];   // the current pair's pieces
   for (int kb = 0; kb < a.nblk; kb += 2) {
-    SLFMM_SNP(0, A0, A1)
-    SLFMM_SNP(1, A1, A0)
+    snp_block<NT0, NT, ST, 0>(kb, a, pc, wave, lane, A0, A1, gslot, gc, tbl, acc, c_pop, c_hi);
+    if (kb + 1 < a.nblk) snp_block<NT0, NT, ST, 1>(kb + 1, a, pc, wave, lane, A1, A0, gslot, gc, tbl, acc, c_pop, c_hi);
   }
-#undef SLFMM_SNP
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
 

@@ -368,67 +368,98 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// The same factorisation, shift rule, outputs and status words as k_chol
-// for m <= 32, on one wave with no LDS: lane l keeps column l of the
-// working matrix in registers, and the values other lanes need are read
-// with v_readlane (wave-uniform).  The arithmetic of every element is
-// k_chol's: R[j][l] = A[j][l] / r_jj, then A[i][l] -= R[j][i] R[j][l] for
-// j < i <= l.  R^{-1}: lane c forms column c by back substitution,
-// X[i] = -(sum_{k > i} R[i][k] X[k]) / R[i][i].
-// Returns 0, or j + 1 for a failed pivot j; column l of R is left in col.
-__device__ __forceinline__ int chol32_factor(const double* __restrict__ G, int m, double shift, double tol,
-                                             double (&col)[32]) {
-  const int l = threadIdx.x;
-  const bool lv = l < m;
-#pragma unroll
-  for (int i = 0; i < 32; ++i) col[i] = (lv && i < m) ? G[i * m + l] + (i == l ? shift : 0.0) : 0.0;
-  const double diag0 = lv ? G[l * m + l] + shift : 0.0;   // lane j's pivot reference, read by step j
-  int fail = 0;
-#pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    if (j < m && !fail) {   // wave-uniform
-      const double dj = lane_value(col[j], j);
-      const double d0 = lane_value(diag0, j);
-      if (!(dj > tol * d0) || !(dj < 1e308)) {
-        fail = j + 1;
-      } else {
-        const double rjj = sqrt(dj);
-        const double r = l > j ? col[j] / rjj : (l == j ? rjj : 0.0);   // R[j][l]
-        col[j] = r;
-#pragma unroll
-        for (int i = j + 1; i < 32; ++i) col[i] -= lane_value(r, i) * r;   // lanes and rows >= m stay zero
-      }
-    }
-  }
-  return fail;
+// Orders one wave's LDS writes before its later LDS reads of other lanes'
+// values (and the compiler's accesses around it).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__global__ void __launch_bounds__(64) k_chol32(const double* __restrict__ G, int m, double rows, int allow_shift,
+// The same factorisation, shift rule, outputs and status words as k_chol
+// for m <= 32, on one wave: lane l keeps column l of the working matrix in
+// registers, padded to 32 x 32 with an identity block past m (pivots 1,
+// zero couplings: the steps past m change nothing, and no step needs a
+// branch on m).  Step j broadcasts row j of the working matrix and then
+// row j of R through LDS (one write per lane, every lane reads the same
+// values), so the trailing update is independent FMAs per lane.  The
+// kernel runs 32 threads (one wave, lanes 32..63 inactive).  The
+// arithmetic of every element is k_chol's: R[j][l] = A[j][l] / r_jj, then
+// A[i][l] -= R[j][i] R[j][l] for j < i <= l.  lds: 5 x 32 doubles
+// (two row pairs used by alternate steps, then the pivot references).
+// Returns 0, or j + 1 for a failed pivot j; column l of R is left in col.
+__device__ __forceinline__ int chol32_factor(const double* __restrict__ G, int m, double shift, double tol,
+                                             double (&col)[32], double* lds) {
+  // the lane index through an opaque move: the caller's retry loop then
+  // keeps the per-step lane masks inside the loop instead of hoisting 64 of
+  // them into scalar registers
+  int l;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(l) : "v"((int)threadIdx.x));
+  const bool lv = l < m;
+#pragma unroll
+  for (int i = 0; i < 32; ++i)
+    col[i] = (lv && i < m) ? G[i * m + l] + (i == l ? shift : 0.0) : (i == l ? 1.0 : 0.0);
+  double* d0s = lds + 4 * 32;   // pivot references (the padded diagonal reads 1)
+  d0s[l] = lv ? G[l * m + l] + shift : 1.0;
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    double* arow = lds + 64 * (j & 1);
+    double* rrow = arow + 32;
+    arow[l] = col[j];   // row j of the working matrix
+    wave_lds_sync();
+    const double dj = arow[j], d0 = d0s[j];
+    if (!(dj > tol * d0) || !(dj < 1e308)) return j + 1;   // wave-uniform
+    const double rjj = sqrt(dj);
+    const double r = l > j ? col[j] / rjj : (l == j ? rjj : 0.0);   // R[j][l]
+    col[j] = r;
+    rrow[l] = r;
+    wave_lds_sync();
+#pragma unroll
+    for (int i = j + 1; i < 32; ++i) col[i] -= rrow[i] * r;
+  }
+  return 0;
+}
+
+__global__ void __launch_bounds__(32) k_chol32(const double* __restrict__ G, int m, double rows, int allow_shift,
                                                double* R, double* Rinv, int* st, int mode, const int* gate) {
   if (gate && *gate != 1) return;
+  __shared__ double rb[5 * 32];
+  __shared__ double Rs[32][33];   // R, row-major (padded rows)
+  __shared__ double dinv_s[32];
   const int l = threadIdx.x;
   double col[32];
-  int attempt = 0;
-  int fail = chol32_factor(G, m, 0.0, 1e-13, col);
-  if (fail && allow_shift) {
+  int attempt = 0, fail = 0;
+  double shift = 0.0, tol = 1e-13;
+#pragma unroll 1
+  for (;; ++attempt) {   // one copy of the factorisation code: attempt 1 is the shifted retry
+    fail = chol32_factor(G, m, shift, tol, col, rb);
+    if (!fail || attempt == 1 || !allow_shift) break;
     double tr = 0.0;
     for (int i = 0; i < m; ++i) tr += G[i * m + i];
-    const double shift = 11.0 * (rows * m + (double)m * (m + 1)) * 1.1102230246251565e-16 * tr;
-    attempt = 1;
-    fail = chol32_factor(G, m, shift, 0.0, col);
+    shift = 11.0 * (rows * m + (double)m * (m + 1)) * 1.1102230246251565e-16 * tr;
+    tol = 0.0;
+    wave_lds_sync();   // the first attempt's last reads of rb come before this attempt's writes
   }
   if (fail) {
     if (l == 0) *st = 2;
     return;
   }
+  // R^{-1}: R goes to LDS (lane l writes column l), the reciprocal
+  // diagonal is formed lane-parallel, and lane c forms column c of X by back
+  // substitution, X[i] = -(sum_{k > i} R[i][k] X[k]) / R[i][i], reading the
+  // rows of R as broadcasts.
+#pragma unroll
+  for (int i = 0; i < 32; ++i) Rs[i][l] = col[i];
+  wave_lds_sync();
+  dinv_s[l] = 1.0 / Rs[l][l];   // 1 on the padded diagonal
+  wave_lds_sync();
   double X[32];
 #pragma unroll
   for (int i = 31; i >= 0; --i) {
     double s = 0.0;
 #pragma unroll
-    for (int k = i + 1; k < 32; ++k) s += lane_value(col[i], k) * X[k];   // R[i][k]; X[k] = 0 past the column
-    const double rii = lane_value(col[i], i);
-    const double dinv = i < m ? 1.0 / rii : 0.0;
+    for (int k = i + 1; k < 32; ++k) s += Rs[i][k] * X[k];   // X[k] = 0 past the column
+    const double dinv = dinv_s[i];
     X[i] = i < l ? -s * dinv : (i == l ? dinv : 0.0);
   }
   if (l < m) {
@@ -445,7 +476,7 @@ __global__ void __launch_bounds__(64) k_chol32(const double* __restrict__ G, int
 static void launch_chol(hipStream_t s, const double* G, uint64_t m, uint64_t rows, bool allow_shift, double* R,
                         double* Rinv, int* st, int mode, const int* gate) {
   if (m <= 32)
-    k_chol32<<<1, 64, 0, s>>>(G, (int)m, (double)rows, allow_shift ? 1 : 0, R, Rinv, st, mode, gate);
+    k_chol32<<<1, 32, 0, s>>>(G, (int)m, (double)rows, allow_shift ? 1 : 0, R, Rinv, st, mode, gate);
   else
     k_chol<<<1, 256, m * m * sizeof(double), s>>>(G, (int)m, (double)rows, allow_shift ? 1 : 0, R, Rinv, st, mode,
                                                   gate);
