@@ -65,7 +65,8 @@ def main():
     ap.add_argument("--write-tsv", default="",
                     help="directory: after timing, write the results TSV of the main fit there (synthetic .bim "
                          "metadata), report the writer's time and delete the files")
-    ap.add_argument("--streams", type=int, default=8, help="streams of the batched fit (--workload batch)")
+    ap.add_argument("--streams", type=int, default=None,
+                    help="streams of the batched fit (--workload batch; default: the tuning table's batch_streams)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB on the device three times (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter calibration) and exit")
@@ -232,6 +233,7 @@ def main():
         "fp64_gpu": fp64,
         "cpu_baseline": cpu,
         "output_tsv": tsv,
+        "launch_shapes": __import__("schnellfmm_amd.tuning", fromlist=["params"]).params(),
     }
     print(json.dumps(line), flush=True)
 
@@ -353,6 +355,9 @@ def batch_workload(args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
 
+    from schnellfmm_amd import tuning
+    if args.streams is None:
+        args.streams = tuning.params()["batch_streams"]
     steps, warmup = max(args.steps, 1), max(args.warmup, 1)
     dt_loop = timed(run_loop, steps, warmup)
     log(f"[bench] loop of single fits: {dt_loop * 1e3:.1f} ms")
@@ -370,6 +375,7 @@ def batch_workload(args):
                    "genotype_bytes": total_bytes, "streams": args.streams, "est_rate": 0.1},
         "loop_ms": dt_loop * 1e3, "batch_ms_by_streams": {str(k): v * 1e3 for k, v in dt_b.items()},
         "speedup_vs_loop": dt_loop / dt, "roofline": None, "cpu_baseline": None,
+        "launch_shapes": tuning.params(),
     }
     print(json.dumps(line), flush=True)
 

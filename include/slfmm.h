@@ -280,6 +280,36 @@ int slfmm_fit_lfmm2_nmf(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* su
                         const slfmm_nmf_config* nmf, const double* b0, const double* nmf_omega, double* u_hat,
                         slfmm_assoc_out* out, double* nmf_cv_mae, double* gwas_cv, uint32_t flags);
 
+/* ---- Launch shapes (no reference counterpart) ----
+ * The tile and launch shapes that the kernels pick per problem, as a table
+ * of thresholds and targets.  The defaults are the measured MI355X choices;
+ * an offline tuning pass (python -m schnellfmm_amd.autotune) times the
+ * candidates on the device and writes a table that the host layer applies
+ * to every context it creates.  The tile choice leaves the results bit for
+ * bit unchanged; the workgroup targets change only how FP64 partial sums are
+ * grouped before they are added (rounding level). */
+typedef struct {
+  /* int8 SNP-side kernels (association pass, RSVD sketch) with at most 3 row
+   * tiles: 2 tiles of 32 SNPs per wave (256 SNPs per workgroup) when that
+   * grid has at least this many workgroups, else 1 tile (128 SNPs per
+   * workgroup).  0: always 2. */
+  uint32_t snp_two_tiles_min_wg;
+  /* int8 RSVD forward pass: target number of workgroups (sample blocks x
+   * SNP chunks); fewer chunks mean fewer partial sums to reduce. */
+  uint32_t fwd_target_wg;
+  /* tall-skinny Gram: maximum number of row blocks (partial Grams). */
+  uint32_t gram_max_blocks;
+  /* batched fits: the schedule's estimate of a problem's device time,
+   * batch_fit_ms + (genotype bytes streamed) / (batch_gb_per_ms 1e9). */
+  double batch_fit_ms;
+  double batch_gb_per_ms;
+} slfmm_tuning;
+
+/* Replace the context's launch shapes (its batch streams take them too). */
+int slfmm_set_tuning(slfmm_ctx* ctx, const slfmm_tuning* t);
+/* The context's launch shapes (the defaults until slfmm_set_tuning). */
+int slfmm_get_tuning(slfmm_ctx* ctx, slfmm_tuning* t);
+
 /* ---- Kernel timing (no reference counterpart; used by the benchmark) ---- */
 
 /* Enable HIP-event timing of the genotype-stream kernels on the context's

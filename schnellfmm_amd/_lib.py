@@ -41,6 +41,12 @@ class SynthSpec(C.Structure):
                 ("seed", u64)]
 
 
+class Tuning(C.Structure):
+    """slfmm_tuning: launch shapes (tile and workgroup-count choices)."""
+    _fields_ = [("snp_two_tiles_min_wg", u32), ("fwd_target_wg", u32), ("gram_max_blocks", u32),
+                ("batch_fit_ms", dbl), ("batch_gb_per_ms", dbl)]
+
+
 class Problem(C.Structure):
     """slfmm_problem: the arguments of slfmm_fit_lfmm2 for one problem of a batch."""
     _fields_ = [("est", vp), ("subset_idx", vp), ("n_idx", u64), ("g", vp), ("x", vp), ("d", u64),
@@ -83,6 +89,8 @@ SIGNATURES = {
     "slfmm_fit_lfmm2_nmf": (C.c_int, [vp, vp, vp, u64, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp, vp, u32]),
     "slfmm_debug_mfma_i8": (C.c_int, [vp, vp, vp, vp]),
     "slfmm_debug_i8_layout": (C.c_int, [i32, i32, i32, vp, vp, u64]),
+    "slfmm_set_tuning": (C.c_int, [vp, vp]),
+    "slfmm_get_tuning": (C.c_int, [vp, vp]),
     "slfmm_profile_enable": (C.c_int, [vp, i32]),
     "slfmm_profile_read": (C.c_int, [vp, C.c_char_p, vp, vp, i32]),
     "slfmm_graph_stats": (C.c_int, [vp, vp, vp]),

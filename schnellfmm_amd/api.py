@@ -37,6 +37,8 @@ class Context:
         h = C.c_void_p()
         check(None, lib().slfmm_create(device, C.byref(h)))
         self.h = h
+        from . import tuning
+        tuning.apply(self)   # launch shapes of the tuning table (results do not depend on them)
 
     def close(self):
         if self.h:
@@ -440,13 +442,16 @@ class Lfmm2Problem:
     b0: Optional[np.ndarray] = None
 
 
-def fit_lfmm2_batch(problems, n_streams: int = 8, with_hist: bool = False):
+def fit_lfmm2_batch(problems, n_streams: Optional[int] = None, with_hist: bool = False):
     """Many independent fit_lfmm2 problems in one call (slfmm_fit_lfmm2_batch):
-    the problems are spread over n_streams (1..16) streams of the context,
-    longest first, so small problems overlap on the GPU.  Returns one
-    TestResults per problem, equal to what fit_lfmm2 returns for the same
-    inputs."""
+    the problems are spread over n_streams (1..16; default: the tuning
+    table's batch_streams) streams of the context, longest first, so small
+    problems overlap on the GPU.  Returns one TestResults per problem, equal
+    to what fit_lfmm2 returns for the same inputs."""
     from ._lib import Problem
+    from . import tuning
+    if n_streams is None:
+        n_streams = tuning.params()["batch_streams"]
     problems = list(problems)
     if not problems:
         return []

@@ -743,6 +743,8 @@ static void queue_fit(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* idx,
   key_add(key, g); key_add(key, g->d); key_add(key, g->n); key_add(key, g->p); key_add(key, g->ld);
   key_add(key, n_idx); key_add(key, identity); key_add(key, d); key_add(key, cfg);
   key_add(key, dx); key_add(key, db0); key_add(key, du); key_add(key, *out); key_add(key, flags);
+  const slfmm_tuning& tu = ctx->tuning;   // launch shapes are captured with the graph
+  key_add(key, tu.snp_two_tiles_min_wg); key_add(key, tu.fwd_target_wg); key_add(key, tu.gram_max_blocks);
   const size_t slot0 = ctx->checks.size();
   key_add(key, slot0);
   auto it = ctx->graphs.find(key);
@@ -851,6 +853,7 @@ int slfmm_fit_lfmm2_batch(slfmm_ctx* ctx, slfmm_problem* probs, uint64_t count, 
     while ((int)ctx->workers.size() < n_streams) {
       slfmm_ctx* w = new slfmm_ctx();
       w->device = ctx->device;
+      w->tuning = ctx->tuning;
       if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
         delete w;
         throw Error(SLFMM_E_HIP, "stream creation failed");
@@ -861,6 +864,7 @@ int slfmm_fit_lfmm2_batch(slfmm_ctx* ctx, slfmm_problem* probs, uint64_t count, 
     SLFMM_HIP(hipStreamSynchronize(ctx->stream));
     std::vector<slfmm_ctx*> ws(ctx->workers.begin(), ctx->workers.begin() + n_streams);
     for (slfmm_ctx* w : ws) {
+      w->tuning = ctx->tuning;
       w->checks.clear();
       w->upload_reset();
     }
@@ -871,16 +875,17 @@ int slfmm_fit_lfmm2_batch(slfmm_ctx* ctx, slfmm_problem* probs, uint64_t count, 
     };
     // Longest-processing-time-first schedule: problems in decreasing
     // estimated device time, each to the stream with the least queued time.
-    // Estimate: a fixed cost per fit (its chain of small dense solves, about
-    // 1.4 ms on MI355X) plus the genotype bytes its passes stream at about
-    // 2.3 GB per ms (the association pass once, the estimation rows 2 q + 2
-    // times).
+    // Estimate: a fixed cost per fit (its chain of small dense solves,
+    // tuning.batch_fit_ms) plus the genotype bytes its passes stream at
+    // tuning.batch_gb_per_ms (the association pass once, the estimation rows
+    // 2 q + 2 times).
     std::vector<double> cost(count);
     for (uint64_t i = 0; i < count; ++i) {
       const slfmm_problem& P = probs[i];
       const double bps = (double)((P.g->n + 3) / 4);
       const double est_rows = P.subset_idx ? (double)P.n_idx : (double)P.est->p;
-      cost[i] = 1.4 + ((double)P.g->p * bps + (2.0 * P.cfg.n_power_iter + 2.0) * est_rows * bps) / 2.3e9;
+      cost[i] = ctx->tuning.batch_fit_ms + ((double)P.g->p * bps + (2.0 * P.cfg.n_power_iter + 2.0) * est_rows * bps) /
+                                               (ctx->tuning.batch_gb_per_ms * 1e9);
     }
     std::vector<uint64_t> order(count);
     for (uint64_t i = 0; i < count; ++i) order[i] = i;
@@ -1065,6 +1070,26 @@ int slfmm_debug_i8_layout(int32_t m, int32_t k3, int32_t km, int32_t* dims, int3
   return guarded(nullptr, [&] {
     SLFMM_CHECK(dims, SLFMM_E_ARG, "NULL argument");
     i8_layout(m, k3, km, dims, rows, cap);
+  });
+}
+
+int slfmm_set_tuning(slfmm_ctx* ctx, const slfmm_tuning* t) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && t, SLFMM_E_ARG, "NULL argument");
+    SLFMM_CHECK(t->fwd_target_wg >= 1 && t->fwd_target_wg <= 65536, SLFMM_E_ARG, "fwd_target_wg must be 1..65536");
+    SLFMM_CHECK(t->gram_max_blocks >= 1 && t->gram_max_blocks <= 65536, SLFMM_E_ARG,
+                "gram_max_blocks must be 1..65536");
+    SLFMM_CHECK(t->batch_fit_ms >= 0.0 && t->batch_gb_per_ms > 0.0, SLFMM_E_ARG,
+                "batch_fit_ms must be >= 0 and batch_gb_per_ms > 0");
+    ctx->tuning = *t;
+    for (slfmm_ctx* w : ctx->workers) w->tuning = *t;
+  });
+}
+
+int slfmm_get_tuning(slfmm_ctx* ctx, slfmm_tuning* t) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && t, SLFMM_E_ARG, "NULL argument");
+    *t = ctx->tuning;
   });
 }
 

@@ -681,14 +681,20 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
   X(1, 1, 1) X(2, 1, 1) X(3, 1, 1) X(4, 1, 1) X(5, 1, 1) X(6, 1, 1) X(7, 1, 1) X(8, 1, 1) X(9, 2, 1) X(10, 2, 1) \
   X(11, 2, 1) X(12, 2, 1) X(13, 2, 1) X(14, 2, 1) X(15, 2, 1) X(16, 2, 1) X(16, 2, 2)
 
+// 32-SNP tiles per wave: at most 2, and 2 only with at most 3 row tiles
+// (accumulator registers)
 template <int NT>
-constexpr int snp_tiles() { return NT <= 3 ? 2 : 1; }   // 32-SNP tiles per wave (accumulator registers)
+constexpr int snp_tiles_max() { return NT <= 3 ? 2 : 1; }
 
-static void launch_snp(hipStream_t s, const Operand& op, SnpArgs& a, int mode) {
+// Layouts of up to 3 row tiles run 2 tiles per wave when that grid has at
+// least tuning.snp_two_tiles_min_wg workgroups, else 1 (twice the
+// workgroups, for short panels); the per-SNP arithmetic is the same.
+static void launch_snp(slfmm_ctx* ctx, const Operand& op, SnpArgs& a, int mode) {
   if (a.p == 0) return;
+  hipStream_t s = ctx->stream;
   const RowMap& r = op.rmap;
   const int NT = r.NT0 + r.NT1;
-  const int st = NT <= 3 ? 2 : 1;
+  const int st = NT <= 3 && cdiv(a.p, 2 * 32 * kSnpWaves) >= (uint64_t)ctx->tuning.snp_two_tiles_min_wg ? 2 : 1;
   const uint64_t rows = 32 * (uint64_t)st * kSnpWaves;
   // buffer descriptors address a workgroup's rows and the operand with 32-bit byte offsets
   SLFMM_CHECK((rows + 2 * 32) * a.ld < 0xF0000000ull && (uint64_t)a.nblk * 8 * NT * 1024 < 0xFFFFFFFFull,
@@ -699,17 +705,18 @@ static void launch_snp(hipStream_t s, const Operand& op, SnpArgs& a, int mode) {
   bool done = false;
 #define X(Q, A0_, A1_)                                                                                        \
   if (!done && r.NQ == Q && r.NT0 == A0_ && r.NT1 == A1_) {                                                   \
-    k_snp_i8<Q, A0_, A1_, snp_tiles<A0_ + A1_>(), 0><<<grid, 64 * kSnpWaves, 0, s>>>(a);                      \
+    if (st == 2)                                                                                              \
+      k_snp_i8<Q, A0_, A1_, snp_tiles_max<A0_ + A1_>(), MODE_><<<grid, 64 * kSnpWaves, 0, s>>>(a);            \
+    else                                                                                                      \
+      k_snp_i8<Q, A0_, A1_, 1, MODE_><<<grid, 64 * kSnpWaves, 0, s>>>(a);                                     \
     done = true;                                                                                              \
   }
+#define MODE_ 0
   if (mode == 0) { SLFMM_SNP_ASSOC(X) }
-#undef X
-#define X(Q, A0_, A1_)                                                                                        \
-  if (!done && r.NQ == Q && r.NT0 == A0_ && r.NT1 == A1_) {                                                   \
-    k_snp_i8<Q, A0_, A1_, snp_tiles<A0_ + A1_>(), 1><<<grid, 64 * kSnpWaves, 0, s>>>(a);                      \
-    done = true;                                                                                              \
-  }
+#undef MODE_
+#define MODE_ 1
   if (mode == 1) { SLFMM_SNP_RSVD(X) }
+#undef MODE_
 #undef X
   SLFMM_CHECK(done, SLFMM_E_ARG, "no int8 kernel for this operand layout");
   SLFMM_HIP(hipGetLastError());
@@ -749,7 +756,7 @@ void launch_func_i8(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_Q, uint
   a.packed = g->d; a.ld = g->ld; a.p = g->p; a.n = g->n; a.nblk = (int)op.nblk; a.norm = ea.norm;
   a.A = op.pack; a.wtab = op.wtab; a.ea = ea;
   slfmm_timed tm(ctx, "assoc_stream");
-  launch_snp(ctx->stream, op, a, 0);
+  launch_snp(ctx, op, a, 0);
 }
 
 // RSVD sketch / backward pass: Z[j][c] = s_j * (B^T y_j)_c for every SNP row.
@@ -760,7 +767,7 @@ void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_
   SnpArgs a{};
   a.packed = packed; a.ld = ld; a.p = p; a.n = n; a.nblk = (int)op.nblk; a.norm = norm;
   a.A = op.pack; a.wtab = op.wtab; a.Z = Z; a.ldZ = ldZ;
-  launch_snp(ctx->stream, op, a, 1);
+  launch_snp(ctx, op, a, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -962,9 +969,10 @@ void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_
   double* vhat = ctx->buf<double>("fw_vhat", l);
   colreduce<1, kColSum>(ctx, FnVhat{Qz, ldq, ssnp, mu, csc}, GStore1{vhat}, pe, (int)l);
   const uint64_t sblocks = cdiv(n, 64 * kFwdWaves);
-  // about 512 workgroups: fewer SNP chunks mean fewer partial sums to reduce
-  // (measured at n = 10 000, 100 000 SNPs: 512 beats 256, 1024 and 2048)
-  uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(nblk, cdiv(512, sblocks)));
+  // about tuning.fwd_target_wg workgroups: fewer SNP chunks mean fewer
+  // partial sums to reduce
+  const uint64_t target = std::max<uint32_t>(1u, ctx->tuning.fwd_target_wg);
+  uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(nblk, cdiv(target, sblocks)));
   const uint64_t per = cdiv(nblk, nchunk);
   nchunk = cdiv(nblk, per);
   const uint64_t ldp = l;

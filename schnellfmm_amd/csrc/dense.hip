@@ -166,7 +166,8 @@ void ts_gram(slfmm_ctx* ctx, const double* A, uint64_t lda, uint64_t a, const do
   SLFMM_CHECK(a <= (uint64_t)kSm && b <= (uint64_t)kSm, SLFMM_E_ARG, "ts_gram: too many columns");
   hipStream_t s = ctx->stream;
   if (rows == 0) { launch_fill(s, out, 0.0, a * b); return; }
-  uint64_t nblk = std::min<uint64_t>(cdiv(rows, 32), 1024);   // one 32-row tile per block up to 32k rows
+  // one 32-row tile per block up to tuning.gram_max_blocks blocks
+  uint64_t nblk = std::min<uint64_t>(cdiv(rows, 32), std::max<uint32_t>(1u, ctx->tuning.gram_max_blocks));
   uint64_t rpb = cdiv(rows, nblk);
   nblk = cdiv(rows, rpb);
   double* part = ctx->buf<double>("gram_part", nblk * a * b);

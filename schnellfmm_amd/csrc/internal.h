@@ -63,6 +63,8 @@ __host__ __device__ inline uint64_t rup(uint64_t a, uint64_t b) { return cdiv(a,
 struct slfmm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Launch shapes (slfmm_set_tuning); the defaults are the measured MI355X choices.
+  slfmm_tuning tuning{0u, 512u, 1024u, 1.4, 2.3};
   // Panels bound to this context that are still alive.  slfmm_destroy on a
   // context with live panels only marks it closed; the last slfmm_geno_free
   // deletes it, so the two may be called in either order.
