@@ -641,6 +641,8 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
   double* wt = reinterpret_cast<double*>(A1);
   for (int e = t; e < kWtab; e += 64 * W) wt[e] = a.wtab[e];
   __syncthreads();
+  double fq[ST][NQ];
+  int64_t nmiss[ST], sg[ST], nhom[ST];
 #pragma unroll
   for (int st = 0; st < ST; ++st) {
     uint32_t cp = c_pop[st], ch = c_hi[st];
@@ -650,23 +652,42 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
     const int ones_m = __shfl(acc[st][NT - 1][15], 32 * a.g1 + col);
     const int ones_g = __shfl(acc[st][NT0 - 1][15], 32 * a.g1 + col);
     const int cfill = (int)((tbl[st] >> 8) & 0xFFu);
-    const int64_t nmiss = a.ones_g ? ((int64_t)a.S * cp - ones_g) / (a.S - cfill) : (int64_t)ones_m;
-    const int64_t sg = (int64_t)cp - nmiss, nhom = sg - (int64_t)ch;
-    const int64_t nobs = (int64_t)a.n - nmiss;
-    const double mu = nobs > 0 ? (double)sg / (double)nobs : 0.0;
+    nmiss[st] = a.ones_g ? ((int64_t)a.S * cp - ones_g) / (a.S - cfill) : (int64_t)ones_m;
+    sg[st] = (int64_t)cp - nmiss[st];
+    nhom[st] = sg[st] - (int64_t)ch;
+    const int64_t nobs = (int64_t)a.n - nmiss[st];
+    const double mu = nobs > 0 ? (double)sg[st] / (double)nobs : 0.0;
     const double dmu = mu - (double)cfill / (double)a.S;
-    double fq[NQ];
-    recombine<NQ, NT0, NT1>(acc[st], h, a.k3, a.km, mu, dmu, wt, fq);
-    const uint64_t j = base + 32 * st + col;
-    const bool valid = j < a.p;
+    recombine<NQ, NT0, NT1>(acc[st], h, a.k3, a.km, mu, dmu, wt, fq[st]);
     if constexpr (MODE == 1) {
+      const uint64_t j = base + 32 * st + col;
       const double scale = epi_scale(mu, a.norm);
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
-        if (valid && h + 2 * q < a.m) a.Z[j * a.ldZ + h + 2 * q] = scale * fq[q];
-    } else {
-      snp_epilogue_split2<NQ>(a.ea, plan, j, valid, h, fq, nmiss, sg, nhom);
+        if (j < a.p && h + 2 * q < a.m) a.Z[j * a.ldZ + h + 2 * q] = scale * fq[st][q];
     }
+  }
+  if constexpr (MODE == 0) {
+    // One SNP per lane: functional i = h + 2 q of tile st sits in lane half h,
+    // so one exchange between the halves per position gives lane half h every
+    // functional of its column of tile h (two tiles), or of tile 0 in both
+    // halves (one tile; half 1 then stores nothing).  The per-SNP arithmetic
+    // is the same either way.
+    double f[2 * NQ];
+    int tile = 0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      double x = fq[0][q], y = fq[ST - 1][q];
+      swap_halves(x, y);
+      f[2 * q] = x;       // even functionals: lower half's values
+      f[2 * q + 1] = y;   // odd functionals: upper half's values
+    }
+    if constexpr (ST == 2) tile = h;
+    const uint64_t j = base + 32 * tile + col;
+    const bool valid = j < a.p && (ST == 2 || h == 0);
+    const int64_t nm = tile ? nmiss[ST - 1] : nmiss[0], s = tile ? sg[ST - 1] : sg[0],
+                  nh = tile ? nhom[ST - 1] : nhom[0];
+    snp_epilogue_lane<2 * NQ>(a.ea, plan, j, valid, f, nm, s, nh);
   }
 }
 

@@ -134,6 +134,95 @@ __device__ void snp_epilogue(const EpiArgs& a, const double* sPlan, uint64_t j, 
 }
 
 
+// The same outputs as snp_epilogue for one SNP per lane: all functionals of
+// the SNP in the lane's registers (f[i] = f_i, zero past m), the plan in the
+// split layout (stage_split_plan).  No cross-lane traffic; `valid` masks the
+// stores.
+template <int M>
+__device__ void snp_epilogue_lane(const EpiArgs& a, const double* sPlan, uint64_t j, bool valid, const double (&f)[M],
+                                  int64_t nmiss, int64_t sg, int64_t nh) {
+  static_assert(M <= 32, "split plan rows hold 32 functionals");
+  const int d = a.d, m = a.m;
+  const int64_t nobs = (int64_t)a.n - nmiss;
+  const double mean = nobs > 0 ? (double)sg / (double)nobs : 0.0;
+  const double sc = epi_scale(mean, a.norm);
+  const int64_t sg2 = sg + 2 * nh;
+  const double tss = nobs > 0 ? sc * sc * ((double)(nobs * sg2 - sg * sg) / (double)nobs) : 0.0;
+  double fy[M];   // zero for i >= m (f is)
+#pragma unroll
+  for (int i = 0; i < M; ++i) fy[i] = sc * f[i];
+  const int dp = (d + 3) & ~3;
+  const double* P = sPlan;   // split layout: rows of 32, zero past m
+  const double* Rm = P + dp * 32;
+  const double* Fm = Rm + dp * 32;
+  const double* Vd = Fm + dp * 32;
+  const double* L = Vd + dp;
+  const bool fallback = L[m * m] != 0.0;
+  double proj = 0.0;
+  if (!fallback) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) proj += fy[i] * fy[i];
+  } else {
+    for (int c = 0; c < m; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < M; ++r)
+        if (r < m) s += L[r * m + c] * fy[r];
+      proj += s * s;
+    }
+  }
+  double rss = tss - proj;
+  if (rss < 0.0) rss = 0.0;
+  const double sigma2 = rss / a.df;
+  double sscov = 0.0;
+  for (int jj = 0; jj < d; ++jj) {
+    double coef = 0.0, b = 0.0, fc = 0.0;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      coef += P[jj * 32 + i] * fy[i];
+      b += Rm[jj * 32 + i] * fy[i];
+      fc += Fm[jj * 32 + i] * fy[i];
+    }
+    sscov += fc * fc;
+    if (valid) {
+      // t_test (src/testing.rs:1036-1048)
+      const double se = sqrt(sigma2 * Vd[jj]);
+      double tv = 0.0;
+      if (se >= 1e-300 && isfinite(se)) {
+        tv = coef / se;
+        if (!isfinite(tv)) tv = 0.0;
+      }
+      a.tstat[j * d + jj] = tv;
+      a.beta[j * d + jj] = b;
+    }
+  }
+  if (valid) {   // src/testing.rs:394-404
+    double r0 = 0.0, r1 = 0.0, r2v = 0.0;
+    if (tss > 1e-300) {
+      double sslat = tss - rss - sscov;
+      if (sslat < 0.0) sslat = 0.0;
+      r0 = sscov / tss;
+      r1 = sslat / tss;
+      r2v = rss / tss;
+    }
+    a.r2[j * 3 + 0] = r0;
+    a.r2[j * 3 + 1] = r1;
+    a.r2[j * 3 + 2] = r2v;
+  }
+}
+
+// Exchange of two doubles between the lane halves (v_permlane32_swap on both
+// words): x's upper 32 lanes take y's lower 32 lanes and y's lower 32 lanes
+// take x's upper 32 lanes.  With x = y every lane gets the lower partner's
+// value in x and the upper partner's in y.
+__device__ __forceinline__ void swap_halves(double& x, double& y) {
+  const uint64_t ux = (uint64_t)__double_as_longlong(x), uy = (uint64_t)__double_as_longlong(y);
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)ux, (uint32_t)uy, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(ux >> 32), (uint32_t)(uy >> 32), false, false);
+  x = __longlong_as_double((long long)(((uint64_t)hi[0] << 32) | lo[0]));
+  y = __longlong_as_double((long long)(((uint64_t)hi[1] << 32) | lo[1]));
+}
+
 // x_lower + x_upper of a lane-swap pair: v_permlane16_swap (row r of 16 lanes
 // with row r ^ 1) or v_permlane32_swap (lanes l with l ^ 32), applied to both
 // words of the double.  Swapping a register with itself leaves the lower
