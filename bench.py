@@ -53,12 +53,14 @@ def main():
                     help="association prefix of the CPU baseline (SNPs)")
     ap.add_argument("--cpu-est-snps", type=int, default=20000,
                     help="estimation SNPs of the CPU baseline's RSVD (scaled to the workload's subset)")
-    ap.add_argument("--workload", choices=["lfmm2", "nmf", "batch", "host"], default="lfmm2",
+    ap.add_argument("--workload", choices=["lfmm2", "nmf", "batch", "host", "mfma"], default="lfmm2",
                     help="lfmm2: the headline fit (default); nmf: NMF imputation estimation and the "
                          "NMF-imputed fit at n=5000, p=2M, nmf-k=8, 10 iterations, K=8, d=2 (FP64); "
                          "batch: 64 independent fits of seeded uneven shapes, batched vs looped; "
                          "host: the headline fit with the genotypes in host memory, streamed over "
-                         "PCIe chunk by chunk (PCIe-inclusive rate)")
+                         "PCIe chunk by chunk (PCIe-inclusive rate); "
+                         "mfma: sustained int8 / FP8 / BF16 MFMA rates and the cost per genotype fragment of "
+                         "each operand format of the precision comparison")
     ap.add_argument("--host-memory", choices=["pageable", "pinned"], default="pageable",
                     help="host buffer of the genotypes (--workload host)")
     ap.add_argument("--chunk-snps", type=int, default=0, help="rows per streamed chunk (--workload host)")
@@ -95,6 +97,9 @@ def main():
         return
     if args.workload == "batch":
         batch_workload(args)
+        return
+    if args.workload == "mfma":
+        mfma_workload(args)
         return
     if args.workload == "host":
         host_workload(args)
@@ -476,6 +481,64 @@ def host_workload(args):
     print(json.dumps(line), flush=True)
     geno.free()
     del hbuf_t
+
+
+# Operand terms per genotype fragment of each format in the precision
+# comparison (the dense terms against the dosage operand plus those against
+# the missing/observed operand that the centring needs), as in its cost
+# model; int8 is the library's layout at K = 20, d = 4.
+PRECISION_FORMATS = [
+    ("int8 2/3 terms (library)", 0, (2 * 20 + 3 * 4) / 24 + 2),
+    ("bf16 hi+lo, fp32 acc", 2, 2 + 2),
+    ("bf16 1 term, fp32 acc", 2, 1 + 1),
+    ("fp8 e4m3 x2, fp32 acc", 1, 2 + 2),
+    ("fp8 e4m3 x3, fp32 acc", 1, 3 + 3),
+]
+
+
+def mfma_workload(args):
+    """Sustained dense MFMA rate of the three formats that hold genotype codes
+    exactly (slfmm_debug_mfma_rate: whole chip, 2 waves per SIMD, 4 chains per
+    wave), and the time per genotype fragment (32 SNPs x 32 samples against
+    one 32-row dense tile) of each format of the precision comparison:
+    terms x 2*32^3 operations / measured rate."""
+    import ctypes as C
+    import torch
+    import schnellfmm_amd as S
+    from schnellfmm_amd._lib import lib
+    torch.cuda.init()
+    ctx = S.Context(0)
+    names = {0: "int8 (v_mfma_i32_32x32x32_i8)", 1: "fp8 e4m3 (v_mfma_scale_f32_32x32x64_f8f6f4)",
+             2: "bf16 (v_mfma_f32_32x32x16_bf16)"}
+    rate = {}
+    for kind in (0, 1, 2):
+        best = 0.0
+        for _ in range(max(args.steps, 1)):
+            t = C.c_double()
+            ctx.check(lib().slfmm_debug_mfma_rate(ctx.h, kind, 200_000, C.byref(t)))
+            best = max(best, t.value)
+        rate[kind] = best
+        log(f"[bench] {names[kind]}: {best:.0f} TOPS sustained")
+    frag_ops = 2.0 * 32 ** 3
+    fmts = []
+    for name, kind, terms in PRECISION_FORMATS:
+        ns = terms * frag_ops / (rate[kind] * 1e12) * 1e9
+        fmts.append({"format": name, "operand_terms": round(terms, 2), "mfma": names[kind],
+                     "ns_per_fragment_chip": ns, "relative_to_int8": None})
+    base = fmts[0]["ns_per_fragment_chip"]
+    for f in fmts:
+        f["relative_to_int8"] = f["ns_per_fragment_chip"] / base
+    peaks = {0: 5000.0, 1: 5000.0, 2: 2500.0}
+    line = {
+        "metric": "sustained dense MFMA rate by operand format (TOPS)", "value": rate[0], "unit": "TOPS",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "higher_is_better": True, "vs_baseline": None,
+        "dtype": "i8 / fp8 e4m3 / bf16", "data": "fixed operand bytes",
+        "config": {"workload": "mfma rates: 2 waves per SIMD, 4 accumulator chains per wave, 200000 MFMAs each"},
+        "rates_tops": {names[k]: rate[k] for k in rate},
+        "fraction_of_dense_peak": {names[k]: rate[k] / peaks[k] for k in rate},
+        "formats": fmts, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
 
 
 def i8_kernel_name(K, d, mode):

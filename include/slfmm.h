@@ -329,6 +329,14 @@ int slfmm_graph_stats(slfmm_ctx* ctx, uint64_t* captured, uint64_t* replayed);
  * all row-major.  A GPU test pins the lane layout with it. */
 int slfmm_debug_mfma_i8(slfmm_ctx* ctx, const int8_t* a, const int8_t* b, int32_t* d);
 
+/* Diagnostic (no reference counterpart): sustained dense MFMA rate on the
+ * whole chip, in tera-operations per second (2 per multiply-add), of kind 0
+ * v_mfma_i32_32x32x32_i8, 1 v_mfma_scale_f32_32x32x64_f8f6f4 with FP8 e4m3
+ * operands, 2 v_mfma_f32_32x32x16_bf16: iters MFMAs on each of 4
+ * accumulators per wave, 2 waves per SIMD.  The benchmark's precision
+ * comparison prices int8, FP8 and BF16 passes with it. */
+int slfmm_debug_mfma_rate(slfmm_ctx* ctx, int32_t kind, uint64_t iters, double* tops);
+
 /* Diagnostic (no reference counterpart, host only): the packed-operand row
  * layout of the int8 SNP-side kernels for m functionals, columns >= k3 with 3
  * dosage-side terms and columns >= km with 2 miss-side terms.

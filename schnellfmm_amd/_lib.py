@@ -89,6 +89,7 @@ SIGNATURES = {
     "slfmm_fit_lfmm2_nmf": (C.c_int, [vp, vp, vp, u64, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp, vp, u32]),
     "slfmm_debug_mfma_i8": (C.c_int, [vp, vp, vp, vp]),
     "slfmm_debug_i8_layout": (C.c_int, [i32, i32, i32, vp, vp, u64]),
+    "slfmm_debug_mfma_rate": (C.c_int, [vp, i32, u64, vp]),
     "slfmm_set_tuning": (C.c_int, [vp, vp]),
     "slfmm_get_tuning": (C.c_int, [vp, vp]),
     "slfmm_profile_enable": (C.c_int, [vp, i32]),

@@ -15,8 +15,8 @@ OUT = os.path.join(HERE, "libslfmm.so")
 OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["geno.hip", "dense.hip", "assoc.hip", "assoc_i8.hip", "rsvd.hip", "nmf.hip", "noalloc.hip", "api.cpp",
-           "output.cpp"]
+SOURCES = ["geno.hip", "dense.hip", "assoc.hip", "assoc_i8.hip", "rsvd.hip", "nmf.hip", "noalloc.hip", "mfma_rate.hip",
+           "api.cpp", "output.cpp"]
 # geno.hip reproduces the reference's decode arithmetic bit for bit: no FMA contraction.
 EXTRA = {"geno.hip": ["-ffp-contract=off"]}
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -1066,6 +1066,13 @@ int slfmm_debug_mfma_i8(slfmm_ctx* ctx, const int8_t* a, const int8_t* b, int32_
   });
 }
 
+int slfmm_debug_mfma_rate(slfmm_ctx* ctx, int32_t kind, uint64_t iters, double* tops) {
+  return guarded(ctx, [&] {
+    SLFMM_CHECK(ctx && tops, SLFMM_E_ARG, "NULL argument");
+    mfma_rate(ctx, kind, iters, tops);
+  });
+}
+
 int slfmm_debug_i8_layout(int32_t m, int32_t k3, int32_t km, int32_t* dims, int32_t* rows, uint64_t cap) {
   return guarded(nullptr, [&] {
     SLFMM_CHECK(dims, SLFMM_E_ARG, "NULL argument");

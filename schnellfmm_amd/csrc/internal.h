@@ -323,6 +323,8 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
 // ---- assoc_i8.hip: single-MFMA lane-layout probe (test-only entry point) ----
 void probe_mfma_i8(slfmm_ctx* ctx, const int8_t* hA, const int8_t* hB, int32_t* hD);
 void i8_layout(int m, int k3, int km, int32_t* dims, int32_t* rows, uint64_t cap);
+// ---- mfma_rate.hip: sustained MFMA rate of int8 / FP8 / BF16 (diagnostic) ----
+void mfma_rate(slfmm_ctx* ctx, int kind, uint64_t iters, double* tops);
 // ---- noalloc.hip ----------------------------------------------------------
 // Resolves the device address of the allocation-free entry point's static
 // scratch array on the current device (once per device).
