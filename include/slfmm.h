@@ -291,9 +291,10 @@ int slfmm_fit_lfmm2_nmf(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* su
 typedef struct {
   /* int8 SNP-side kernels (association pass, RSVD sketch) with at most 3 row
    * tiles: 2 tiles of 32 SNPs per wave (256 SNPs per workgroup) when that
-   * grid has at least this many workgroups, else 1 tile (128 SNPs per
-   * workgroup).  0: always 2. */
-  uint32_t snp_two_tiles_min_wg;
+   * grid holds at least this many workgroup blocks (workgroups x 256-sample
+   * blocks per row), else 1 tile (128 SNPs per workgroup: twice the
+   * workgroups for short or narrow panels).  0: always 2. */
+  uint32_t snp_two_tiles_min_blocks;
   /* int8 RSVD forward pass: target number of workgroups (sample blocks x
    * SNP chunks); fewer chunks mean fewer partial sums to reduce. */
   uint32_t fwd_target_wg;

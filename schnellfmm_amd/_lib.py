@@ -43,7 +43,7 @@ class SynthSpec(C.Structure):
 
 class Tuning(C.Structure):
     """slfmm_tuning: launch shapes (tile and workgroup-count choices)."""
-    _fields_ = [("snp_two_tiles_min_wg", u32), ("fwd_target_wg", u32), ("gram_max_blocks", u32),
+    _fields_ = [("snp_two_tiles_min_blocks", u32), ("fwd_target_wg", u32), ("gram_max_blocks", u32),
                 ("batch_fit_ms", dbl), ("batch_gb_per_ms", dbl)]
 
 

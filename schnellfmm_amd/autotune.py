@@ -6,13 +6,14 @@ Runs on the GPU.  For each launch-shape choice it times the candidates on
 synthetic panels held in HBM and writes the table that every Context of this
 package applies (schnellfmm_amd.tuning), together with the timings behind it:
 
-* snp_two_tiles_min_wg: the int8 association kernel with 1 and with 2
-  32-SNP tiles per wave, over panels from 20 000 to 2M SNPs and two operand
-  widths; 2 tiles are used from the smallest 2-tile grid size at and above
+* snp_two_tiles_min_blocks: the int8 SNP-side kernels (association pass and
+  RSVD sketch) with 1 and with 2 32-SNP tiles per wave, over panels from
+  1 000 to 10 000 samples and 20 000 to 2M SNPs; 2 tiles are used from the
+  smallest 2-tile grid work (workgroups x 256-sample blocks) at and above
   which 2 tiles are never slower (within 1 %).
 * fwd_target_wg: the int8 RSVD forward pass at 256 .. 2048 target
-  workgroups, at two estimation shapes; the target with the least summed
-  relative time.
+  workgroups, at two estimation shapes; the target with the least total
+  time.
 * gram_max_blocks: the whole randomized SVD with 256 .. 2048 Gram blocks.
 * batch_streams, batch_fit_ms: the batched fit of the 64 seeded problems of
   uneven shape with 4 .. 16 streams and three per-fit cost estimates.
@@ -43,10 +44,11 @@ def log(msg):
 # ---- decision rules (pure functions of the measurements; CPU-tested) ------------
 
 def pick_two_tile_threshold(rows, tol=0.01):
-    """rows: [(grid2, ms_one_tile, ms_two_tiles)], one per measured shape, where
-    grid2 is the workgroup count of the 2-tile launch.  Returns the smallest
-    grid size g such that every shape with grid2 >= g is no slower with 2
-    tiles (ms2 <= ms1 (1 + tol)); 0 when that holds for every shape."""
+    """rows: [(work2, ms_one_tile, ms_two_tiles)], one per measured shape, where
+    work2 is the work of the 2-tile launch (workgroups x 256-sample blocks).
+    Returns the smallest work w such that every shape with work2 >= w is no
+    slower with 2 tiles (ms2 <= ms1 (1 + tol)); 0 when that holds for every
+    shape."""
     sizes = sorted({g for g, _, _ in rows})
     for g in sizes:
         if all(m2 <= m1 * (1.0 + tol) for gg, m1, m2 in rows if gg >= g):
@@ -54,14 +56,12 @@ def pick_two_tile_threshold(rows, tol=0.01):
     return int(sizes[-1]) + 1   # one tile everywhere measured
 
 
-def pick_min_relative(times):
-    """times: {candidate: [ms per shape]} -> the candidate with the least sum of
-    times relative to each shape's best (ties: the smaller candidate)."""
+def pick_min_total(times):
+    """times: {candidate: [ms per shape]} -> the candidate with the least total
+    time over the shapes (the larger problems weigh most; ties: the smaller
+    candidate)."""
     cands = sorted(times)
-    nshape = len(times[cands[0]])
-    best = [min(times[c][i] for c in cands) for i in range(nshape)]
-    score = {c: sum(times[c][i] / best[i] for i in range(nshape)) for c in cands}
-    return min(cands, key=lambda c: (score[c], c))
+    return min(cands, key=lambda c: (round(sum(times[c]), 9), c))
 
 
 def pick_batch(times):
@@ -124,11 +124,15 @@ class _Bench:
 
 
 def measure_tiles(B, quick):
+    """Both int8 SNP-side kernels with 1 and 2 tiles per wave: the
+    association pass (kernel timer "assoc_stream") and the RSVD sketch
+    ("rsvd_project", with its operand packing)."""
     S = B.S
     rows = []
-    shapes = [(1000, 5, 2), (5000, 20, 4)]
-    ps = [20_000, 50_000, 100_000, 200_000, 500_000] + ([] if quick else [2_000_000])
-    for n, K, d in shapes:
+    assoc = [(1000, 5, 2, [20_000, 50_000, 200_000, 500_000] + ([] if quick else [2_000_000])),
+             (5000, 20, 4, [20_000, 50_000, 100_000, 200_000] + ([] if quick else [500_000])),
+             (10_000, 20, 4, [20_000, 50_000] + ([] if quick else [100_000, 200_000]))]
+    for n, K, d, ps in assoc:
         rng = np.random.default_rng(n + K)
         u = np.ascontiguousarray(rng.standard_normal((n, K)))
         x = np.ascontiguousarray(rng.standard_normal((n, d)))
@@ -140,12 +144,28 @@ def measure_tiles(B, quick):
             fn = (lambda g=g: S.test_associations_fused(g, xs, u, cfg))
             res = {}
             for label, thr in (("one", 2 ** 32 - 1), ("two", 0)):
-                B.set(snp_two_tiles_min_wg=thr)
+                B.set(snp_two_tiles_min_blocks=thr)
                 res[label] = B.kernel_ms("assoc_stream", fn, 5)
-            grid2 = math.ceil(p / 256)
-            log(f"[autotune] tiles n={n} m={K + d} p={p} grid2={grid2}: one {res['one']:.3f} ms, two {res['two']:.3f} ms")
-            rows.append((grid2, res["one"], res["two"], n, K + d, p))
+            work2 = math.ceil(p / 256) * math.ceil(n / 256)
+            log(f"[autotune] tiles association n={n} m={K + d} p={p} work2={work2}: one {res['one']:.3f} ms, "
+                f"two {res['two']:.3f} ms")
+            rows.append((work2, res["one"], res["two"], "association", n, K + d, p))
             del g
+    for n, pe in ((2000, 20_000), (10_000, 100_000)) if not quick else ((10_000, 100_000),):
+        g = S.BedFile.synthetic(n, pe, n_pops=8, fst=(0.02, 0.15), maf=(0.05, 0.5), miss_rate=0.02, seed=5,
+                                ctx=B.ctx)
+        x = np.ascontiguousarray(np.random.default_rng(6).standard_normal((n, 2)))
+        cfg = S.Lfmm2Config(k=20, oversampling=10, n_power_iter=1, norm=S.EIGENSTRAT, precision=S.PREC_I8)
+        fn = (lambda g=g: S.estimate_factors(g, "all", x, cfg))
+        res = {}
+        for label, thr in (("one", 2 ** 32 - 1), ("two", 0)):
+            B.set(snp_two_tiles_min_blocks=thr)
+            res[label] = B.kernel_ms("rsvd_project", fn, 3)
+        work2 = math.ceil(pe / 256) * math.ceil(n / 256)
+        log(f"[autotune] tiles sketch n={n} l=30 p_est={pe} work2={work2}: one {res['one']:.3f} ms, "
+            f"two {res['two']:.3f} ms")
+        rows.append((work2, res["one"], res["two"], "rsvd sketch", n, 30, pe))
+        del g
     B.set()
     return rows
 
@@ -231,11 +251,11 @@ def main(argv=None):
     B = _Bench()
     t0 = time.perf_counter()
     tile_rows = measure_tiles(B, args.quick)
-    thr = pick_two_tile_threshold([(g, m1, m2) for g, m1, m2, *_ in tile_rows])
+    thr = pick_two_tile_threshold([(w, m1, m2) for w, m1, m2, *_ in tile_rows])
     fwd = measure_fwd(B, args.quick)
-    fwd_pick = pick_min_relative(fwd)
+    fwd_pick = pick_min_total(fwd)
     gram = measure_gram(B, args.quick)
-    gram_pick = pick_min_relative(gram)
+    gram_pick = pick_min_total(gram)
     batch, count = measure_batch(B, args.quick)
     streams, fit_ms = pick_batch(batch)
     props = B.torch.cuda.get_device_properties(0)
@@ -243,11 +263,12 @@ def main(argv=None):
         "device": f"{props.name} ({getattr(props, 'gcnArchName', '')})",
         "command": "python -m schnellfmm_amd.autotune" + (" --quick" if args.quick else ""),
         "seconds": round(time.perf_counter() - t0, 1),
-        "params": {"snp_two_tiles_min_wg": thr, "fwd_target_wg": fwd_pick, "gram_max_blocks": gram_pick,
+        "params": {"snp_two_tiles_min_blocks": thr, "fwd_target_wg": fwd_pick, "gram_max_blocks": gram_pick,
                    "batch_fit_ms": fit_ms, "batch_gb_per_ms": B.base["batch_gb_per_ms"], "batch_streams": streams},
         "measurements": {
-            "assoc_stream_ms": [{"n": n, "m": m, "p": p, "grid_two_tiles": g, "one_tile": round(m1, 4),
-                                 "two_tiles": round(m2, 4)} for g, m1, m2, n, m, p in tile_rows],
+            "snp_kernel_ms": [{"kernel": kname, "n": n, "m": m, "p": p, "blocks_two_tiles": w,
+                               "one_tile": round(m1, 4), "two_tiles": round(m2, 4)}
+                              for w, m1, m2, kname, n, m, p in tile_rows],
             "rsvd_forward_ms": {"shapes": ["n=2000 p_est=20000", "n=10000 p_est=100000"],
                                 **{str(c): [round(v, 4) for v in vs] for c, vs in fwd.items()}},
             "rsvd_wall_ms_n10000_pest100000": {str(c): round(v[0], 3) for c, v in gram.items()},

@@ -744,7 +744,7 @@ static void queue_fit(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* idx,
   key_add(key, n_idx); key_add(key, identity); key_add(key, d); key_add(key, cfg);
   key_add(key, dx); key_add(key, db0); key_add(key, du); key_add(key, *out); key_add(key, flags);
   const slfmm_tuning& tu = ctx->tuning;   // launch shapes are captured with the graph
-  key_add(key, tu.snp_two_tiles_min_wg); key_add(key, tu.fwd_target_wg); key_add(key, tu.gram_max_blocks);
+  key_add(key, tu.snp_two_tiles_min_blocks); key_add(key, tu.fwd_target_wg); key_add(key, tu.gram_max_blocks);
   const size_t slot0 = ctx->checks.size();
   key_add(key, slot0);
   auto it = ctx->graphs.find(key);

@@ -707,15 +707,17 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
 template <int NT>
 constexpr int snp_tiles_max() { return NT <= 3 ? 2 : 1; }
 
-// Layouts of up to 3 row tiles run 2 tiles per wave when that grid has at
-// least tuning.snp_two_tiles_min_wg workgroups, else 1 (twice the
-// workgroups, for short panels); the per-SNP arithmetic is the same.
+// Layouts of up to 3 row tiles run 2 tiles per wave when that grid holds at
+// least tuning.snp_two_tiles_min_blocks workgroup blocks (workgroups x
+// 256-sample blocks), else 1 (twice the workgroups, for short panels or
+// short rows); the per-SNP arithmetic is the same.
 static void launch_snp(slfmm_ctx* ctx, const Operand& op, SnpArgs& a, int mode) {
   if (a.p == 0) return;
   hipStream_t s = ctx->stream;
   const RowMap& r = op.rmap;
   const int NT = r.NT0 + r.NT1;
-  const int st = NT <= 3 && cdiv(a.p, 2 * 32 * kSnpWaves) >= (uint64_t)ctx->tuning.snp_two_tiles_min_wg ? 2 : 1;
+  const uint64_t blocks2 = cdiv(a.p, 2 * 32 * kSnpWaves) * (uint64_t)a.nblk;
+  const int st = NT <= 3 && blocks2 >= (uint64_t)ctx->tuning.snp_two_tiles_min_blocks ? 2 : 1;
   const uint64_t rows = 32 * (uint64_t)st * kSnpWaves;
   // buffer descriptors address a workgroup's rows and the operand with 32-bit byte offsets
   SLFMM_CHECK((rows + 2 * 32) * a.ld < 0xF0000000ull && (uint64_t)a.nblk * 8 * NT * 1024 < 0xFFFFFFFFull,

@@ -2,7 +2,7 @@
 
 The library picks some tile and workgroup-count shapes per problem from a
 small table: how many 32-SNP tiles a wave of the int8 SNP-side kernels
-carries (1 or 2, by grid size), the workgroup target of the int8 RSVD
+carries (1 or 2, by the work of the grid), the workgroup target of the int8 RSVD
 forward pass, the number of partial blocks of the tall-skinny Gram, the cost
 model of the batched-fit schedule and the default stream count of batched
 fits.  ``python -m schnellfmm_amd.autotune`` measures the candidates on the
@@ -23,9 +23,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_PATH = os.path.join(HERE, "tuning.json")
 
 # the library's built-in values (slfmm_ctx defaults)
-DEFAULTS = {"snp_two_tiles_min_wg": 0, "fwd_target_wg": 512, "gram_max_blocks": 1024,
+DEFAULTS = {"snp_two_tiles_min_blocks": 0, "fwd_target_wg": 512, "gram_max_blocks": 1024,
             "batch_fit_ms": 1.4, "batch_gb_per_ms": 2.3, "batch_streams": 8}
-LIBRARY_KEYS = ("snp_two_tiles_min_wg", "fwd_target_wg", "gram_max_blocks", "batch_fit_ms", "batch_gb_per_ms")
+LIBRARY_KEYS = ("snp_two_tiles_min_blocks", "fwd_target_wg", "gram_max_blocks", "batch_fit_ms", "batch_gb_per_ms")
 
 _table: Optional[dict] = None
 
@@ -44,12 +44,12 @@ def validate(params: dict) -> dict:
         raise ValueError(f"unknown tuning keys: {sorted(bad)}")
     out = dict(DEFAULTS)
     out.update(params)
-    for k in ("snp_two_tiles_min_wg", "fwd_target_wg", "gram_max_blocks", "batch_streams"):
+    for k in ("snp_two_tiles_min_blocks", "fwd_target_wg", "gram_max_blocks", "batch_streams"):
         out[k] = int(out[k])
     if not (1 <= out["fwd_target_wg"] <= 65536 and 1 <= out["gram_max_blocks"] <= 65536):
         raise ValueError("fwd_target_wg and gram_max_blocks must be 1..65536")
-    if not (0 <= out["snp_two_tiles_min_wg"] < 2 ** 32):
-        raise ValueError("snp_two_tiles_min_wg must fit 32 bits")
+    if not (0 <= out["snp_two_tiles_min_blocks"] < 2 ** 32):
+        raise ValueError("snp_two_tiles_min_blocks must fit 32 bits")
     if not (1 <= out["batch_streams"] <= 16):
         raise ValueError("batch_streams must be 1..16")
     if not (out["batch_fit_ms"] >= 0.0 and out["batch_gb_per_ms"] > 0.0):

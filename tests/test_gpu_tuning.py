@@ -18,7 +18,7 @@ def S():
 def test_set_get_round_trip(S):
     from schnellfmm_amd import tuning
     ctx = S.Context(0)
-    p = dict(tuning.DEFAULTS, snp_two_tiles_min_wg=123, fwd_target_wg=777, gram_max_blocks=99, batch_fit_ms=0.5,
+    p = dict(tuning.DEFAULTS, snp_two_tiles_min_blocks=123, fwd_target_wg=777, gram_max_blocks=99, batch_fit_ms=0.5,
              batch_gb_per_ms=3.0)
     tuning.apply(ctx, p)
     got = tuning.read(ctx)
@@ -41,7 +41,7 @@ def test_tile_choice_is_bit_identical(S, K, d):
     cfg = S.Lfmm2Config(k=K, norm=S.EIGENSTRAT, precision=S.PREC_I8, n_power_iter=2)
     outs = []
     for thr in (2 ** 32 - 1, 0):   # one tile per wave everywhere, then two
-        tuning.apply(ctx, dict(tuning.DEFAULTS, snp_two_tiles_min_wg=thr))
+        tuning.apply(ctx, dict(tuning.DEFAULTS, snp_two_tiles_min_blocks=thr))
         outs.append(S.fit_lfmm2(g, 0.05, g, x, cfg))
     a, b = outs
     for f in ("u_hat", "beta", "tstat", "r2", "pvalue", "gif_per_trait"):
