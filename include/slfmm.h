@@ -59,8 +59,10 @@ enum {
  * output structs are device pointers).  SLFMM_EAGER: queue every kernel of
  * a fit directly; without it, fits with device-resident panels and device
  * outputs replay a HIP graph captured by an earlier call with the same
- * shapes, configuration and pointers (no counterpart in the reference). */
-enum { SLFMM_OUT_DEVICE = 1u, SLFMM_EAGER = 2u };
+ * shapes, configuration and pointers (no counterpart in the reference).
+ * SLFMM_NMF_INT8 (slfmm_nmf_estimate): the H update of every iteration runs
+ * on the int8 matrix cores for ranks k <= 8 (see slfmm_nmf_estimate). */
+enum { SLFMM_OUT_DEVICE = 1u, SLFMM_EAGER = 2u, SLFMM_NMF_INT8 = 4u };
 
 /* Lfmm2Config (src/lib.rs:47-98) plus the GPU precision switch. */
 typedef struct {
@@ -234,7 +236,12 @@ int slfmm_fit_lfmm2(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset
  * hold-out draw of entry (sample i, column c of chunk q) in iteration t is
  * element c * n + i of the library's uniform stream seeded cfg->seed + t + q
  * (the reference draws from ChaCha8).  w and h follow `flags`; the CV
- * outputs are host memory. */
+ * outputs are host memory.  With SLFMM_NMF_INT8 and k <= 8 the backward
+ * pass of each iteration (the H update, src/nmf.rs:171-221) sums the filled
+ * genotypes on the int8 matrix cores: W in 3 int8 terms against the exact
+ * dosages and the products W_r W_s in 2 terms against the missing-call
+ * indicator (the fill W_k . H_j enters as (sum over missing k of W_k W_k^T)
+ * H_j), recombined in FP64; everything else stays FP64. */
 int slfmm_nmf_estimate(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* subset_idx, uint64_t n_idx,
                        const slfmm_nmf_config* cfg, const double* omega, double* w, double* h,
                        double* cv_mae, double* cv_mae_mean, uint64_t* mask_count, uint32_t flags);

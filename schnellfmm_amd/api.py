@@ -19,6 +19,7 @@ from ._lib import AssocOut, Config, NmfConfig, SlfmmError, check, lib
 
 CENTER_ONLY, EIGENSTRAT = 0, 1          # SnpNorm, src/bed.rs:19-26
 PREC_FP64, PREC_I8 = 0, 1
+NMF_INT8 = 4                              # slfmm_nmf_estimate flag: H update on the int8 matrix cores
 N_BINS = 100_000
 
 
@@ -365,9 +366,10 @@ class NmfEstResult:
     total_mask_count: int
 
 
-def nmf_impute_estimation(bed: BedFile, subset, config: NmfParams, omega=None) -> NmfEstResult:
+def nmf_impute_estimation(bed: BedFile, subset, config: NmfParams, omega=None, int8: bool = False) -> NmfEstResult:
     """src/nmf.rs:80-249 on the GPU (FP64).  omega (n x (k + 10)) may be
-    injected; otherwise it is drawn from config.seed."""
+    injected; otherwise it is drawn from config.seed.  int8: the H update of
+    every iteration runs on the int8 matrix cores (k <= 8; SLFMM_NMF_INT8)."""
     ctx = bed.ctx
     idx = None if subset is None or (isinstance(subset, str) and subset == "all") else subset_indices(subset, bed.n_snps)
     pe = bed.n_snps if idx is None else len(idx)
@@ -379,7 +381,8 @@ def nmf_impute_estimation(bed: BedFile, subset, config: NmfParams, omega=None) -
     cnt = C.c_uint64()
     om = None if omega is None else _f64(omega)
     ctx.check(lib().slfmm_nmf_estimate(ctx.h, bed.h, _ptr(idx), 0 if idx is None else len(idx), C.byref(cfg),
-                                       _ptr(om), _ptr(w), _ptr(h), _ptr(cv), C.byref(mean), C.byref(cnt), 0))
+                                       _ptr(om), _ptr(w), _ptr(h), _ptr(cv), C.byref(mean), C.byref(cnt),
+                                       NMF_INT8 if int8 else 0))
     return NmfEstResult(w, h, cv[:config.n_iter].copy(), float(mean.value), int(cnt.value))
 
 

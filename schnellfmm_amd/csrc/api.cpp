@@ -957,7 +957,7 @@ int slfmm_nmf_estimate(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* subse
     double* dHt = ctx->buf<double>("api_nmf_ht", pe * K);
     const uint64_t it = cfg->n_iter;
     double* dcv = ctx->buf<double>("api_nmf_cv", 3 * std::max<uint64_t>(it, 1));
-    nmf_run(ctx, Y, g->ld, n, pe, *cfg, dom, dW, dHt, dcv);
+    nmf_run(ctx, Y, g->ld, n, pe, *cfg, dom, dW, dHt, dcv, (flags & SLFMM_NMF_INT8) != 0);
     nmf_transpose_h(ctx->stream, dHt, pe, (int)K, dH);
     if (!dev) {
       to_host(ctx, w, dW, n * K);

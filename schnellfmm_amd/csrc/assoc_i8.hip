@@ -117,11 +117,30 @@ struct RowMap {
   }
 };
 
+// Row layout of the operand of the NMF backward pass (MODE 2): the K
+// columns of W on the dosage side (3 terms each: rows 3 r + t of the first
+// NT0 tiles) and the P = K (K + 1) / 2 products W_r W_s (r <= s, pair
+// p = r K - r (r - 1) / 2 + s - r) on the missing side (2 terms each: rows
+// 2 p + t of the NT1 tiles after them).  Operand column K + p holds pair p.
+struct NmfMap {
+  int K, P, NT0, NT1;
+  static NmfMap make(int K) {
+    const int P = K * (K + 1) / 2;
+    return NmfMap{K, P, (3 * K + 31) / 32, (2 * P + 31) / 32};
+  }
+  __host__ __device__ int2 info(int row) const {
+    if (row < 32 * NT0) return row < 3 * K ? make_int2(row / 3, row % 3) : make_int2(-1, 0);
+    const int r = row - 32 * NT0;
+    return r < 2 * P ? make_int2(K + r / 2, r % 2) : make_int2(-1, 0);
+  }
+};
+
 // Apack for the SNP-side kernels: rows = functional terms, K = samples.
 // Fragment (blk, s, rt) is 1 KB (8 k-steps of 32 samples per 256-sample
 // block): lane l holds row 32 rt + l % 32, byte 4v + c = sample
 // 256 blk + 128 (l / 32) + 16 s + 4 c + v.
-__global__ void k_pack_rows(const double* __restrict__ Q, uint64_t ldq, uint64_t n, RowMap rmap,
+template <class Map>
+__global__ void k_pack_rows(const double* __restrict__ Q, uint64_t ldq, uint64_t n, Map rmap,
                             const double* __restrict__ sc, int NT, uint64_t total, int8_t* __restrict__ out) {
   constexpr int KS = 8;
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
@@ -341,6 +360,12 @@ struct SnpArgs {
   EpiArgs ea;
   double* Z;
   uint64_t ldZ;
+  // MODE 2 (NMF backward pass): H^T rows updated in place, the operand's
+  // column scales, W^T W (K x K) and the clamp eps
+  double* Ht;
+  const double* nmf_sc;
+  const double* WtW;
+  double eps;
 };
 
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
@@ -558,6 +583,69 @@ __host__ __device__ constexpr int snp_plan_bytes(int NQ) {
               4 * NQ * NQ + 1);
 }
 
+// NMF backward pass for the SNP of this lane's column (src/nmf.rs:171-221):
+// with W's columns on the dosage side and the products W_r W_s on the
+// missing side (NmfMap), the filled sum is
+//   num_r = sum_k W_kr y_kj + sum_{k missing} W_kr (W_k . H_j)
+//         = A_r + sum_s G_rs H_js,  A_r = sum_{k obs} W_kr g_kj,
+//                                    G_rs = sum_{k missing} W_kr W_ks,
+// so no per-call fill is formed.  One exchange per register gives both lane
+// halves every row of the column; A and G are recombined in FP64 from the
+// exact int32 sums of their terms, and the reference's update follows:
+// den_r = (W^T W H_j)_r, H_jr *= num_r / den_r where den_r > eps, clamped to
+// eps.  Half 0 stores.  ls: column scales at [0, K + P), W^T W at [64, ...).
+template <int K, int NT0, int NT1>
+__device__ __forceinline__ void nmf_bwd_epilogue(const SnpArgs& a, const v16i (&acc)[NT0 + NT1], const double* ls,
+                                                 uint64_t j, int h) {
+  constexpr int P = K * (K + 1) / 2;
+  double val[K + P];
+#pragma unroll
+  for (int c = 0; c < K + P; ++c) val[c] = 0.0;
+#pragma unroll
+  for (int rt = 0; rt < NT0 + NT1; ++rt)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      uint32_t x = (uint32_t)acc[rt][v], y = x;
+      const auto sw = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+      const int r0 = 32 * rt + 8 * (v >> 2) + (v & 3);   // lower half's row; the upper half's is r0 + 4
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        const int row = r0 + 4 * hh;
+        const double iv = (double)(int)(hh ? sw[1] : sw[0]);
+        if (row < 32 * NT0) {
+          if (row < 3 * K) {
+            const int c = row / 3, tm = row % 3;
+            val[c] += (tm == 0 ? 1.0 : (tm == 1 ? 1.0 / 254.0 : 1.0 / 64516.0)) * iv;
+          }
+        } else {
+          const int rr = row - 32 * NT0;
+          if (rr < 2 * P) val[K + rr / 2] += (rr % 2 == 0 ? 1.0 : 1.0 / 254.0) * iv;
+        }
+      }
+    }
+  const bool valid = j < a.p && h == 0;
+  double hj[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) hj[r] = valid ? a.Ht[j * K + r] : 0.0;
+  const double* wtw = ls + 64;
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    double num = ls[r] * val[r];
+#pragma unroll
+    for (int s2 = 0; s2 < K; ++s2) {
+      const int lo = r < s2 ? r : s2, hi = r < s2 ? s2 : r;
+      const int pidx = lo * K - lo * (lo - 1) / 2 + (hi - lo);
+      num += (ls[K + pidx] * val[K + pidx]) * hj[s2];
+    }
+    double den = 0.0;
+#pragma unroll
+    for (int s2 = 0; s2 < K; ++s2) den += wtw[r * K + s2] * hj[s2];
+    double v = hj[r];
+    if (den > a.eps) v *= num / den;
+    if (valid) a.Ht[j * K + r] = v > a.eps ? v : a.eps;
+  }
+}
+
 template <int NQ, int NT0, int NT1, int ST, int MODE>
 __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
   constexpr int NT = NT0 + NT1, KS = 8, W = kSnpWaves, CH = KS * NT;
@@ -630,6 +718,15 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
   }
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
+  if constexpr (MODE == 2) {
+    double* ls = reinterpret_cast<double*>(A1);   // column scales [0, K + P), W^T W at [64, 64 + K^2)
+    const int mq = NQ + NQ * (NQ + 1) / 2;
+    for (int e = t; e < mq; e += 64 * W) ls[e] = a.nmf_sc[e];
+    for (int e = t; e < NQ * NQ; e += 64 * W) ls[64 + e] = a.WtW[e];
+    __syncthreads();
+    nmf_bwd_epilogue<NQ, NT0, NT1>(a, acc[0], ls, base + col, h);
+    return;
+  }
 
   // ---- epilogue: the plan goes to A0, the weight table to A1 ----------------
   const double* plan = nullptr;
@@ -791,6 +888,60 @@ void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_
   a.packed = packed; a.ld = ld; a.p = p; a.n = n; a.nblk = (int)op.nblk; a.norm = norm;
   a.A = op.pack; a.wtab = op.wtab; a.Z = Z; a.ldZ = ldZ;
   launch_snp(ctx, op, a, 1);
+}
+
+// ---- NMF backward pass on the int8 matrix cores --------------------------------
+// [W | W_r W_s] (n x (K + P)), one thread per sample (pairs r <= s in NmfMap order)
+__global__ void k_nmf_wprod(const double* __restrict__ W, uint64_t n, int K, double* __restrict__ Qn, int mq) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double* w = W + k * K;
+  double* q = Qn + k * mq;
+  int p = K;
+  for (int r = 0; r < K; ++r) {
+    q[r] = w[r];
+    for (int s2 = r; s2 < K; ++s2) q[p++] = w[r] * w[s2];
+  }
+}
+
+// H^T (pe x K) <- the reference's H update from W (n x K) and W^T W on the
+// rows Y (pe SNPs, row stride ld), K <= kNmfMaxK8.
+void nmf_backward_i8(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const double* W, int K,
+                     const double* WtW, double eps, double* Ht) {
+  SLFMM_CHECK(K >= 1 && K <= kNmfMaxK8, SLFMM_E_ARG, "the int8 NMF pass supports rank 1..8");
+  SLFMM_CHECK(n <= 8000000, SLFMM_E_ARG, "too many samples for the int8 kernels");
+  if (pe == 0) return;
+  hipStream_t s = ctx->stream;
+  const NmfMap map = NmfMap::make(K);
+  const int mq = K + map.P, NT = map.NT0 + map.NT1;
+  double* Qn = ctx->buf<double>("nmf8_q", n * mq);
+  k_nmf_wprod<<<(unsigned)cdiv(n, 256), 256, 0, s>>>(W, n, K, Qn, mq);
+  double* sc = ctx->buf<double>("nmf8_sc", mq);
+  colabsmax_scale(ctx, Qn, mq, n, mq, nullptr, nullptr, sc);
+  const uint64_t nblk = cdiv(n, 256);
+  const uint64_t total = nblk * 8 * NT * 1024;
+  int8_t* pack = ctx->buf<int8_t>("nmf8_pack", total);
+  k_pack_rows<<<(unsigned)std::min<uint64_t>(cdiv(total, 256), 16384), 256, 0, s>>>(Qn, mq, n, map, sc, NT, total,
+                                                                                  pack);
+  SnpArgs a{};
+  a.packed = Y; a.ld = ld; a.p = pe; a.n = n; a.nblk = (int)nblk;
+  a.rb = (uint32_t)std::min<uint64_t>(rup(cdiv(n, 4), 16), ld);
+  a.m = mq; a.S = 1; a.fill = 0; a.A = pack;
+  a.Ht = Ht; a.nmf_sc = sc; a.WtW = WtW; a.eps = eps;
+  const uint64_t rows = 32 * kSnpWaves;   // one tile per wave
+  SLFMM_CHECK((rows + 2 * 32) * a.ld < 0xF0000000ull && total < 0xFFFFFFFFull, SLFMM_E_ARG,
+              "panel rows too long for the int8 kernels");
+  const unsigned grid = (unsigned)cdiv(pe, rows);
+  switch (K) {
+#define SLFMM_NMF8(KK)                                                                                        \
+  case KK:                                                                                                    \
+    k_snp_i8<KK, (3 * KK + 31) / 32, (KK * (KK + 1) + 31) / 32, 1, 2><<<grid, 64 * kSnpWaves, 0, s>>>(a);     \
+    break;
+    SLFMM_NMF8(1) SLFMM_NMF8(2) SLFMM_NMF8(3) SLFMM_NMF8(4) SLFMM_NMF8(5) SLFMM_NMF8(6) SLFMM_NMF8(7)
+    SLFMM_NMF8(8)
+#undef SLFMM_NMF8
+  }
+  SLFMM_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------

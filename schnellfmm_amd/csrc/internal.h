@@ -49,6 +49,7 @@ constexpr int kRowBlk = 32;          // rows per FP64 functional row-block
 constexpr uint64_t kSlack = 128;
 constexpr int kMaxM = 64;   // functionals per SNP (K + d, or K + oversampling)
 constexpr int kNmfMaxK = 16; // NMF components
+constexpr int kNmfMaxK8 = 8;  // NMF rank of the int8 matrix-core backward pass
 // every NMF rank 1 .. kNmfMaxK gets its own kernel instantiation
 #define SLFMM_NMF_RANKS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
 static_assert(kNmfMaxK == 16, "rank instantiations cover 1 .. kNmfMaxK");
@@ -349,8 +350,14 @@ void precompute_dev(slfmm_ctx* ctx, const double* d_xs, uint64_t n, uint64_t d, 
 // NMF estimation (src/nmf.rs:80-249) on packed rows Y (pe SNPs): device
 // outputs W (n x K), Ht (pe x K) and cv (3 per iteration: NMF MAE, mean MAE,
 // held-out count); omega (n x (K + 10)) is drawn from cfg.seed when null.
+// With i8 (and K <= kNmfMaxK8) the backward pass (the H update) runs on the
+// int8 matrix cores (nmf_backward_i8); everything else is FP64.
 void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const slfmm_nmf_config& cfg,
-             const double* d_omega, double* W, double* Ht, double* cv);
+             const double* d_omega, double* W, double* Ht, double* cv, bool i8 = false);
+// assoc_i8.hip: H^T (pe x K) <- the NMF H update (src/nmf.rs:171-221) from W
+// (n x K) and W^T W on the rows Y, on the int8 matrix cores (K <= kNmfMaxK8).
+void nmf_backward_i8(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const double* W, int K,
+                     const double* WtW, double eps, double* Ht);
 void nmf_transpose_h(hipStream_t s, const double* Ht, uint64_t pe, int K, double* H);
 // Ht[j][r] = max(F[j][r], 0)
 void relu_rows(hipStream_t s, const double* F, uint64_t ldf, uint64_t p, int K, double* Ht);

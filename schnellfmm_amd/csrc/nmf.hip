@@ -322,7 +322,7 @@ static void normalize_w(slfmm_ctx* ctx, double* W, uint64_t n, double* Ht, uint6
 // count per iteration) are device outputs; omega (n x (K + 10)) may be
 // injected, else it is drawn from seed.
 void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const slfmm_nmf_config& cfg,
-             const double* d_omega, double* W, double* Ht, double* cv) {
+             const double* d_omega, double* W, double* Ht, double* cv, bool i8) {
   hipStream_t s = ctx->stream;
   const int K = (int)cfg.k;
   const uint64_t l2 = cfg.k + 10;
@@ -396,10 +396,16 @@ void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t
     // backward: H update in place, fill from the new W and the old H
     ts_gram(ctx, W, K, K, W, K, K, n, WtW);
     const unsigned bgrid = (unsigned)cdiv(pe, 256);
-    switch (K) {
+    if (i8 && K <= kNmfMaxK8) {
+      slfmm_timed tm(ctx, "nmf_backward");
+      nmf_backward_i8(ctx, Y, ld, n, pe, W, K, WtW, eps, Ht);
+    } else {
+      slfmm_timed tm(ctx, "nmf_backward");
+      switch (K) {
 #define SLFMM_BWD(R) case R: k_nmf_bwd<R><<<bgrid, 256, 0, s>>>(Y, ld, n, pe, W, WtW, eps, Ht); break;
-      SLFMM_NMF_RANKS(SLFMM_BWD)
+        SLFMM_NMF_RANKS(SLFMM_BWD)
 #undef SLFMM_BWD
+      }
     }
     SLFMM_HIP(hipGetLastError());
     normalize_w(ctx, W, n, Ht, pe, K, eps);

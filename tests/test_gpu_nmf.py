@@ -106,3 +106,44 @@ def test_nmf_rejects_mixed_precision(S):
     g = S.BedFile.from_packed(O.encode_bed(pan.g), n, p)
     with pytest.raises(S.api.SlfmmError, match="FP64"):
         S.fit_lfmm2_nmf(g, None, g, pan.x, S.Lfmm2Config(k=2, precision=S.PREC_I8), S.NmfParams(k=2, n_iter=1))
+
+
+@pytest.mark.parametrize("n,p,k,n_iter,rate,chunk,miss", [
+    (300, 1500, 3, 5, None, 400, 0.05),
+    (257, 2000, 8, 4, 0.5, 333, 0.10),     # rank 8 (4 row tiles), ragged sample tile, subset
+    (1100, 3000, 6, 6, None, 1000, 0.15),
+])
+def test_nmf_int8_backward_vs_oracle(S, n, p, k, n_iter, rate, chunk, miss):
+    """The H update on the int8 matrix cores (SLFMM_NMF_INT8): W in 3 int8
+    terms, the products W_r W_s in 2, recombined in FP64 from exact int32
+    sums.  Tolerances: W and H within 1e-5 of the largest entry (about the
+    FP32 level the NMF needs) after every multiplicative iteration,
+    per-iteration CV MAE within 1e-5 relative, held-out counts exact (the
+    same mask stream)."""
+    print(f"[nmf int8] n={n} p={p} k={k} iters={n_iter}", flush=True)
+    packed, bps = _panel(n, p, seed=n + p + k, miss=miss)
+    idx = np.arange(p) if rate is None else O.subset_indices(rate, p)
+    seed, cv_rate, eps = 42, 0.01, 1e-16
+    omega = synth.normal_matrix(seed, n, k + 10)
+    ref = O.nmf_impute_estimation(packed, bps, n, idx, k, n_iter, eps, cv_rate, chunk, seed, omega,
+                                  synth.uniform_stream)
+    g = S.BedFile.from_packed(packed, n, p)
+    cfg = S.NmfParams(k=k, n_iter=n_iter, eps=eps, cv_rate=cv_rate, chunk_size=chunk, seed=seed)
+    got = S.nmf_impute_estimation(g, rate, cfg, omega=omega, int8=True)
+    f64 = S.nmf_impute_estimation(g, rate, cfg, omega=omega)
+    assert _close(got.w, ref.w, 1e-5) and _close(got.h_full, ref.h_full, 1e-5)
+    assert not (np.array_equal(got.w, f64.w) and np.array_equal(got.h_full, f64.h_full))   # the int8 pass ran
+    assert got.total_mask_count == ref.total_mask_count > 0
+    assert np.allclose(got.cv_mae_per_iter, ref.cv_mae_per_iter, rtol=1e-5, atol=0)
+    assert np.isclose(got.cv_mae_mean_impute, ref.cv_mae_mean_impute, rtol=1e-12, atol=0)
+
+
+def test_nmf_int8_rank_above_8_runs_fp64(S):
+    """Ranks above 8 take the FP64 backward pass: the int8 flag changes nothing."""
+    packed, bps = _panel(200, 800, seed=9, miss=0.05)
+    g = S.BedFile.from_packed(packed, 200, 800)
+    cfg = S.NmfParams(k=10, n_iter=2, eps=1e-16, cv_rate=0.01, chunk_size=400, seed=3)
+    om = synth.normal_matrix(3, 200, 20)
+    a = S.nmf_impute_estimation(g, None, cfg, omega=om, int8=True)
+    b = S.nmf_impute_estimation(g, None, cfg, omega=om)
+    assert np.array_equal(a.w, b.w) and np.array_equal(a.h_full, b.h_full)
