@@ -276,19 +276,24 @@ def nmf_workload(args):
     dt_est, est = timed(lambda: S.nmf_impute_estimation(geno, None, cfg), steps, warmup)
     log(f"[bench] nmf estimation: {dt_est * 1e3:.1f} ms")
     dt_est8, est8 = timed(lambda: S.nmf_impute_estimation(geno, None, cfg, int8=True), steps, warmup)
-    log(f"[bench] nmf estimation, H update on the int8 matrix cores: {dt_est8 * 1e3:.1f} ms")
+    log(f"[bench] nmf estimation, both passes on the int8 matrix cores: {dt_est8 * 1e3:.1f} ms")
 
-    def backward_ms(int8):   # device time of the backward pass (H update) per iteration, HIP events
+    def pass_ms(int8):   # device time per iteration of each NMF pass, HIP events
         import ctypes as C
         from schnellfmm_amd._lib import lib
         ctx.check(lib().slfmm_profile_enable(ctx.h, 1))
         S.nmf_impute_estimation(geno, None, cfg, int8=int8)
-        tot, cnt = C.c_double(), C.c_uint64()
-        ctx.check(lib().slfmm_profile_read(ctx.h, b"nmf_backward", C.byref(tot), C.byref(cnt), 1))
+        out = {}
+        for name in ("nmf_forward", "nmf_backward", "nmf_cv"):
+            tot, cnt = C.c_double(), C.c_uint64()
+            ctx.check(lib().slfmm_profile_read(ctx.h, name.encode(), C.byref(tot), C.byref(cnt), 1))
+            out[name] = tot.value / max(cnt.value, 1)
         ctx.check(lib().slfmm_profile_enable(ctx.h, 0))
-        return tot.value / max(cnt.value, 1)
-    bw64, bw8 = backward_ms(False), backward_ms(True)
-    log(f"[bench] backward pass per iteration: fp64 {bw64:.2f} ms, int8 {bw8:.2f} ms")
+        return out
+    t64, t8 = pass_ms(False), pass_ms(True)
+    log(f"[bench] per iteration, fp64 / int8: forward {t64['nmf_forward']:.2f} / {t8['nmf_forward']:.2f} ms, "
+        f"backward {t64['nmf_backward']:.2f} / {t8['nmf_backward']:.2f} ms, "
+        f"cv {t64['nmf_cv']:.2f} / {t8['nmf_cv']:.2f} ms")
     dt_fit, fit = timed(lambda: S.fit_lfmm2_nmf(geno, None, geno, x, lc, cfg), max(steps // 2, 1), min(warmup, 1))
     log(f"[bench] nmf fit: {dt_fit * 1e3:.1f} ms")
     bytes_pass = p * math.ceil(n / 4)
@@ -306,13 +311,14 @@ def nmf_workload(args):
                      "traffic": None, "kernel": "nmf iteration (forward + backward + CV)",
                      "genotype_bytes_per_pass": bytes_pass, "passes": passes},
         "cv_mae_last": float(est.cv_mae_per_iter[-1]), "cv_mae_mean_impute": float(est.cv_mae_mean_impute),
-        "int8_backward": {"estimation_ms": dt_est8 * 1e3, "backward_ms_per_iter_fp64": bw64,
-                          "backward_ms_per_iter_int8": bw8,
-                          "backward_int8_gb_per_s": p * math.ceil(n / 4) / (bw8 * 1e-3) / 1e9,
-                          "cv_mae_last": float(est8.cv_mae_per_iter[-1]),
-                          "max_rel_dw_vs_fp64": float(np.max(np.abs(est8.w - est.w)) / np.max(np.abs(est.w))),
-                          "max_rel_dh_vs_fp64": float(np.max(np.abs(est8.h_full - est.h_full))
-                                                      / np.max(np.abs(est.h_full)))},
+        "ms_per_iter_fp64": {k[4:]: v for k, v in t64.items()},
+        "int8": {"estimation_ms": dt_est8 * 1e3, "ms_per_iter": {k[4:]: v for k, v in t8.items()},
+                 "forward_gb_per_s": bytes_pass / (t8["nmf_forward"] * 1e-3) / 1e9,
+                 "backward_gb_per_s": bytes_pass / (t8["nmf_backward"] * 1e-3) / 1e9,
+                 "cv_mae_last": float(est8.cv_mae_per_iter[-1]),
+                 "max_rel_dw_vs_fp64": float(np.max(np.abs(est8.w - est.w)) / np.max(np.abs(est.w))),
+                 "max_rel_dh_vs_fp64": float(np.max(np.abs(est8.h_full - est.h_full))
+                                             / np.max(np.abs(est.h_full)))},
         "fit_ms": dt_fit * 1e3, "fit_snps_per_s": p / dt_fit, "fit_gif": float(fit.gif),
         "gwas_cv": [float(v) for v in fit.gwas_cv],
         "cpu_baseline": None,

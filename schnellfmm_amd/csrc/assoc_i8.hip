@@ -1000,20 +1000,53 @@ struct GStore1 {
 
 // WV waves x 4 tiles of 16 samples per workgroup share each staged B block.
 constexpr int kFwdWaves = 8;
+// Epilogue of the RSVD forward pass: the 2 terms of Qt and of mu Qt of
+// column 16 cb + l16 sit in tiles cb, NCB + cb, 2 NCB + cb, 3 NCB + cb.
 template <int NCB>
+struct FwdRsvdEpi {
+  static constexpr int NTB = 4 * NCB, NG = 2 * NCB;   // tiles; the first NG meet the dosage fragment
+  const double* csc; int l; double* part; uint64_t ldp;
+  template <int ST>
+  __device__ void operator()(const v4i (&acc)[ST][NTB], uint64_t sbase, uint64_t n, int g, int l16,
+                             uint64_t chunk) const {
+    // lane-local combination: output (sample 4g + r of tile st, column 16 cb + l16)
+#pragma unroll
+    for (int st = 0; st < ST; ++st)
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        const int col = 16 * cb + l16;
+        if (col >= l) continue;
+        const double w0 = csc[col], w1 = csc[col] / 254.0, m0 = csc[l + col], m1 = csc[l + col] / 254.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint64_t k = sbase + 16 * st + 4 * g + r;
+          if (k >= n) continue;
+          double v = w0 * (double)acc[st][cb][r] + w1 * (double)acc[st][NCB + cb][r] +
+                     m0 * (double)acc[st][2 * NCB + cb][r] + m1 * (double)acc[st][3 * NCB + cb][r];
+          part[(chunk * n + k) * ldp + col] = v;
+        }
+      }
+  }
+};
+
+// Forward pass on the sample-major copy with a tile plan: tiles [0, NG)
+// meet the dosage fragment, [NG, NTB) the missing-call fragment; Epi
+// combines the int32 sums of a workgroup's SNP chunk into its partial row.
+// Grid: (sample blocks of 64 kFwdWaves, SNP chunks of nblk_chunk blocks).
+template <class Epi>
 __global__ void __launch_bounds__(64 * kFwdWaves) k_fwd_i8(const uint8_t* __restrict__ YT, uint64_t ldt, uint64_t n,
-                                                           int blk0, int nblk_chunk, int nblk_total,
-                                                           const v4i* __restrict__ Bp, const double* __restrict__ csc,
-                                                           int l, double* __restrict__ part, uint64_t ldp) {
-  constexpr int NTB = 4 * NCB;
+                                                           int nblk_chunk, int nblk_total, const v4i* __restrict__ Bp,
+                                                           Epi epi) {
+  constexpr int NTB = Epi::NTB, NG = Epi::NG;
   constexpr int KS = 4;
   constexpr int ST = 4;
   constexpr int NTH = 64 * kFwdWaves;
   constexpr int BLK = KS * NTB * 64;
   __shared__ v4i Bs[2 * BLK];
+  const int bx = (int)blockIdx.x, by = (int)blockIdx.y;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, l16 = lane & 15;
-  const uint64_t sbase = (uint64_t)blockIdx.x * (ST * 16 * kFwdWaves) + wave * (ST * 16);
-  const int b0 = blk0 + blockIdx.y * nblk_chunk;
+  const uint64_t sbase = (uint64_t)bx * (ST * 16 * kFwdWaves) + wave * (ST * 16);
+  const int b0 = by * nblk_chunk;
   const int b1 = min(nblk_total, b0 + nblk_chunk);
   const uint8_t* rowp[ST];
 #pragma unroll
@@ -1071,10 +1104,10 @@ __global__ void __launch_bounds__(64 * kFwdWaves) k_fwd_i8(const uint8_t* __rest
       for (int st = 0; st < ST; ++st) {
         Frag f = decode_word(gw[st][s], dummy0, dummy1);
 #pragma unroll
-        for (int ct = 0; ct < 2 * NCB; ++ct)
+        for (int ct = 0; ct < NG; ++ct)
           acc[st][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f.g0, bf[ct], acc[st][ct], 0, 0, 0);
 #pragma unroll
-        for (int ct = 2 * NCB; ct < NTB; ++ct)
+        for (int ct = NG; ct < NTB; ++ct)
           acc[st][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f.miss, bf[ct], acc[st][ct], 0, 0, 0);
       }
     }
@@ -1092,23 +1125,28 @@ __global__ void __launch_bounds__(64 * kFwdWaves) k_fwd_i8(const uint8_t* __rest
     __syncthreads();
   }
   (void)dummy0; (void)dummy1; (void)dummy2;
-  // lane-local combination: output (sample 4g + r of tile st, column 16 cb + l16)
-#pragma unroll
-  for (int st = 0; st < ST; ++st)
-#pragma unroll
-    for (int cb = 0; cb < NCB; ++cb) {
-      const int col = 16 * cb + l16;
-      if (col >= l) continue;
-      const double w0 = csc[col], w1 = csc[col] / 254.0, m0 = csc[l + col], m1 = csc[l + col] / 254.0;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const uint64_t k = sbase + 16 * st + 4 * g + r;
-        if (k >= n) continue;
-        double v = w0 * (double)acc[st][cb][r] + w1 * (double)acc[st][NCB + cb][r] +
-                   m0 * (double)acc[st][2 * NCB + cb][r] + m1 * (double)acc[st][3 * NCB + cb][r];
-        part[((uint64_t)blockIdx.y * n + k) * ldp + col] = v;
-      }
-    }
+  epi(acc, sbase, n, g, l16, (uint64_t)by);
+}
+
+// SNP blocks per chunk of a k_fwd_i8 launch: about fwd_target_wg
+// workgroups, rounded to whole rounds of the workgroups the device holds at
+// once (CUs x the kernel's occupancy), so no round runs a few stragglers;
+// at least min_chunks chunks.
+template <class Kern>
+static uint64_t fwd_blocks_per_chunk(slfmm_ctx* ctx, Kern kern, uint64_t nblk, uint64_t sblocks,
+                                     uint64_t min_chunks) {
+  uint64_t& resident = ctx->resident_wgs[reinterpret_cast<const void*>(kern)];
+  if (resident == 0) {
+    int cus = 0, occ = 0;
+    SLFMM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    SLFMM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64 * kFwdWaves, 0));
+    resident = (uint64_t)std::max(cus, 1) * (uint64_t)std::max(occ, 1);
+  }
+  const uint64_t target = std::max<uint32_t>(1u, ctx->tuning.fwd_target_wg);
+  const uint64_t rounds = std::max<uint64_t>(1, (target + resident / 2) / resident);
+  uint64_t nchunk = std::max<uint64_t>(1, rounds * resident / sblocks);
+  nchunk = std::min(std::max(nchunk, min_chunks), nblk);
+  return cdiv(nblk, nchunk);
 }
 
 __global__ void k_fwd_reduce(const double* __restrict__ part, uint64_t n, int l, uint64_t ldp, int nchunk,
@@ -1143,22 +1181,241 @@ void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_
   double* vhat = ctx->buf<double>("fw_vhat", l);
   colreduce<1, kColSum>(ctx, FnVhat{Qz, ldq, ssnp, mu, csc}, GStore1{vhat}, pe, (int)l);
   const uint64_t sblocks = cdiv(n, 64 * kFwdWaves);
-  // about tuning.fwd_target_wg workgroups: fewer SNP chunks mean fewer
-  // partial sums to reduce
-  const uint64_t target = std::max<uint32_t>(1u, ctx->tuning.fwd_target_wg);
-  uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(nblk, cdiv(target, sblocks)));
-  const uint64_t per = cdiv(nblk, nchunk);
-  nchunk = cdiv(nblk, per);
+  // a chunk's int32 sums stay exact below 2^31 / 254 SNPs (|q| <= 127, dosage <= 2)
+  const uint64_t per = NCB == 1 ? fwd_blocks_per_chunk(ctx, k_fwd_i8<FwdRsvdEpi<1>>, nblk, sblocks, cdiv(pe, 8u << 20))
+                                : fwd_blocks_per_chunk(ctx, k_fwd_i8<FwdRsvdEpi<2>>, nblk, sblocks, cdiv(pe, 8u << 20));
+  const uint64_t nchunk = cdiv(nblk, per);
   const uint64_t ldp = l;
   double* part = ctx->buf<double>("fw_part", nchunk * n * ldp);
-  dim3 grid((unsigned)sblocks, (unsigned)nchunk);
+  const dim3 grid((unsigned)sblocks, (unsigned)nchunk);
   if (NCB == 1)
-    k_fwd_i8<1><<<grid, 64 * kFwdWaves, 0, s>>>(YT, ldt, n, 0, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp), csc,
-                                     (int)l, part, ldp);
+    k_fwd_i8<<<grid, 64 * kFwdWaves, 0, s>>>(YT, ldt, n, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp),
+                                            FwdRsvdEpi<1>{csc, (int)l, part, ldp});
   else
-    k_fwd_i8<2><<<grid, 64 * kFwdWaves, 0, s>>>(YT, ldt, n, 0, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp), csc,
-                                     (int)l, part, ldp);
+    k_fwd_i8<<<grid, 64 * kFwdWaves, 0, s>>>(YT, ldt, n, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp),
+                                            FwdRsvdEpi<2>{csc, (int)l, part, ldp});
   k_fwd_reduce<<<(unsigned)cdiv(n * l, 256), 256, 0, s>>>(part, n, (int)l, ldp, (int)nchunk, vhat, W, ldw);
+  SLFMM_HIP(hipGetLastError());
+}
+
+// ---- NMF forward pass on the int8 matrix cores ---------------------------------
+// With the missing calls of sample k filled by W_k . H_j (src/nmf.rs:131-147),
+//   num_W[k][r] = sum_j y'_kj H_jr + sum_s W_ks G_k[r][s],
+//   G_k[r][s] = sum_{j missing in k} H_jr H_js,
+// where y' is the dosage with missing calls as 0.  Both sums run on the
+// sample-major copy: the dosage fragment meets H in 3 int8 terms per column,
+// the missing-call fragment meets the pairwise products H_r H_s (r <= s) in
+// 2.  Slots of the 16-column tiles: g tiles hold term t of column r at
+// 3 r + t, miss tiles term t of pair p at 16 NG + 2 p + t.
+struct NmfFwdMap {
+  int K, P, NG, NM;
+  static NmfFwdMap make(int K) {
+    const int P = K * (K + 1) / 2;
+    return NmfFwdMap{K, P, (3 * K + 15) / 16, (2 * P + 15) / 16};
+  }
+  // (column of [H | H_r H_s], term) of a slot; column -1: zero slot
+  __host__ __device__ int2 info(int slot) const {
+    if (slot < 16 * NG) return slot < 3 * K ? make_int2(slot / 3, slot % 3) : make_int2(-1, 0);
+    const int r = slot - 16 * NG;
+    return r < 2 * P ? make_int2(K + r / 2, r % 2) : make_int2(-1, 0);
+  }
+};
+
+// Pair p (NmfMap order: r = 0.., s = r..K-1) -> (r, s)
+__host__ __device__ __forceinline__ int2 nmf_pair(int p, int K) {
+  int r = 0;
+  while (p >= K - r) { p -= K - r; ++r; }
+  return make_int2(r, r + p);
+}
+
+// Column maxima of |[H | H_r H_s]| per workgroup: each thread keeps the K + P
+// running maxima of its rows (one row = one SNP's K loadings), then the
+// workgroup combines them in LDS; k_colfin finishes across workgroups.
+template <int K>
+__global__ void __launch_bounds__(256) k_nmf_hmax(const double* __restrict__ Ht, uint64_t pe, uint64_t rpb,
+                                                  double* __restrict__ part) {
+  constexpr int M = K + K * (K + 1) / 2;
+  __shared__ double red[M][257];
+  const int t = threadIdx.x;
+  double mx[M];
+#pragma unroll
+  for (int c = 0; c < M; ++c) mx[c] = 0.0;
+  const uint64_t r0 = (uint64_t)blockIdx.x * rpb, r1 = min(pe, r0 + rpb);
+  for (uint64_t r = r0 + t; r < r1; r += 256) {
+    double h[K];
+#pragma unroll
+    for (int a = 0; a < K; ++a) h[a] = Ht[r * K + a];
+#pragma unroll
+    for (int a = 0; a < K; ++a) mx[a] = fmax(mx[a], fabs(h[a]));
+    int q = K;
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+      for (int b = a; b < K; ++b, ++q) mx[q] = fmax(mx[q], fabs(h[a] * h[b]));
+  }
+#pragma unroll
+  for (int c = 0; c < M; ++c) red[c][t] = mx[c];
+  __syncthreads();
+  if (t < M) {
+    double v = 0.0;
+    for (int i = 0; i < 256; ++i) v = fmax(v, red[t][i]);
+    part[(uint64_t)blockIdx.x * M + t] = v;
+  }
+}
+
+struct GScaleMax {
+  double* out;
+  __device__ void operator()(int c, const double (&v)[1]) const { out[c] = v[0] > 0.0 ? v[0] / 127.0 : 1.0; }
+};
+
+// B operand of k_fwd_i8 from Ht (pe x K); one workgroup per block of 64 KS
+// SNPs, whose loadings are staged in LDS.  Lane chunk (s, ct, lane) of the
+// block: lane g 16 + l16 holds slot 16 ct + l16 of the 16 SNPs
+// 16 KS g + 16 s + r3 at byte (r3 & 3) 4 + (r3 >> 2).
+template <int KS>
+__global__ void __launch_bounds__(256) k_pack_nmf_cols(const double* __restrict__ Ht, uint64_t pe, NmfFwdMap map,
+                                                       const double* __restrict__ sc, v4i* __restrict__ out) {
+  __shared__ double hs[64 * KS * kNmfMaxK8];
+  const int K = map.K, NTB = map.NG + map.NM;
+  const uint64_t snp0 = (uint64_t)blockIdx.x * 64 * KS;
+  for (int i = threadIdx.x; i < 64 * KS * K; i += 256) hs[i] = snp0 + i / K < pe ? Ht[snp0 * K + i] : 0.0;
+  __syncthreads();
+  const int nch = KS * NTB * 64;
+  for (int c = threadIdx.x; c < nch; c += 256) {
+    const int lane = c & 63, frag = c >> 6, ct = frag % NTB, s = frag / NTB, g = lane >> 4;
+    const int2 ci = map.info(16 * ct + (lane & 15));
+    uint32_t word[4] = {0u, 0u, 0u, 0u};
+    if (ci.x >= 0) {
+      int a = ci.x, b = -1;
+      if (a >= K) {
+        const int2 rs = nmf_pair(a - K, K);
+        a = rs.x; b = rs.y;
+      }
+      const double inv = 1.0 / sc[ci.x];
+      const double* h = hs + (16 * KS * g + 16 * s) * K;
+#pragma unroll 4
+      for (int r3 = 0; r3 < 16; ++r3) {
+        const double v = b >= 0 ? h[r3 * K + a] * h[r3 * K + b] : h[r3 * K + a];
+        const int q = quant_term(v * inv, ci.y);
+        word[r3 & 3] |= (uint32_t)(q & 0xFF) << (8 * (r3 >> 2));
+      }
+    }
+    out[(uint64_t)blockIdx.x * nch + c] = v4i{(int)word[0], (int)word[1], (int)word[2], (int)word[3]};
+  }
+}
+
+// Epilogue: the int32 sums of every slot go out as they are (the chunk sums
+// are exact in 64 bits; the terms are recombined after them).
+template <int NTB_, int NG_>
+struct FwdRawEpi {
+  static constexpr int NTB = NTB_, NG = NG_;
+  int32_t* part;
+  template <int ST>
+  __device__ void operator()(const v4i (&acc)[ST][NTB], uint64_t sbase, uint64_t n, int g, int l16,
+                             uint64_t chunk) const {
+#pragma unroll
+    for (int st = 0; st < ST; ++st)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint64_t k = sbase + 16 * st + 4 * g + r;
+        if (k >= n) continue;
+        int32_t* row = part + (chunk * n + k) * (16 * NTB);
+#pragma unroll
+        for (int ct = 0; ct < NTB; ++ct) row[16 * ct + l16] = acc[st][ct][r];
+      }
+  }
+};
+
+// Chunk sums of every slot (chunk order, exact in 64 bits): one thread per
+// (sample, slot), so a wave reads whole partial rows.
+__global__ void k_sum_slots(const int32_t* __restrict__ part, uint64_t count, int nchunk, double* __restrict__ out) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  int64_t t = 0;
+  for (int c = 0; c < nchunk; ++c) t += part[(uint64_t)c * count + e];
+  out[e] = (double)t;
+}
+
+// num_W[k][r] = A_r + sum_s W_ks G[r][s] from the slot sums of sample k and
+// the W of the current iterate
+__global__ void k_nmf_fwd_combine(const double* __restrict__ ssum, uint64_t n, NmfFwdMap map,
+                                  const double* __restrict__ sc, const double* __restrict__ W,
+                                  double* __restrict__ num) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int K = map.K;
+  if (e >= n * (uint64_t)K) return;
+  const uint64_t k = e / K;
+  const int r = (int)(e % K);
+  const double* S = ssum + k * 16 * (uint64_t)(map.NG + map.NM);
+  double v = sc[r] * (S[3 * r] + S[3 * r + 1] / 254.0 + S[3 * r + 2] / 64516.0);
+  int p = 0;
+  for (int a = 0; a < K; ++a)
+    for (int b = a; b < K; ++b, ++p) {
+      if (a != r && b != r) continue;
+      const int slot = 16 * map.NG + 2 * p;
+      v += W[k * K + (a == r ? b : a)] * (sc[K + p] * (S[slot] + S[slot + 1] / 254.0));
+    }
+  num[e] = v;
+}
+
+// num_W (n x K) of the reference's forward pass from the sample-major copy
+// YT (n x ldt bytes) of the estimation rows, Ht (pe x K) and the current W.
+void nmf_forward_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_t n, uint64_t pe, const double* Ht,
+                    const double* W, int K, double* num) {
+  SLFMM_CHECK(K >= 1 && K <= kNmfMaxK8, SLFMM_E_ARG, "the int8 NMF pass supports rank 1..8");
+  hipStream_t s = ctx->stream;
+  const NmfFwdMap map = NmfFwdMap::make(K);
+  const int KS = 4, NTB = map.NG + map.NM, mq = K + map.P;
+  double* sc = ctx->buf<double>("nmf8f_sc", mq);
+  {
+    uint64_t nb = std::min<uint64_t>(std::max<uint64_t>(cdiv(pe, 2048), 1), 1024);
+    const uint64_t rpb = cdiv(pe, nb);
+    nb = cdiv(pe, rpb);
+    double* hpart = ctx->buf<double>("nmf8f_hpart", nb * mq);
+    switch (K) {
+#define SLFMM_HMAX(KK) case KK: k_nmf_hmax<KK><<<(unsigned)nb, 256, 0, s>>>(Ht, pe, rpb, hpart); break;
+      SLFMM_HMAX(1) SLFMM_HMAX(2) SLFMM_HMAX(3) SLFMM_HMAX(4) SLFMM_HMAX(5) SLFMM_HMAX(6) SLFMM_HMAX(7) SLFMM_HMAX(8)
+#undef SLFMM_HMAX
+    }
+    k_colfin<1, kColMax, GScaleMax><<<(unsigned)cdiv(mq, 4), 256, 0, s>>>(hpart, (int)nb, mq, GScaleMax{sc});
+  }
+  const uint64_t nblk = cdiv(pe, 64 * KS);
+  const uint64_t n16 = nblk * KS * NTB * 64;   // 16-byte lane chunks
+  SLFMM_CHECK(n16 * 16 < 0xFFFFFFFFull, SLFMM_E_ARG, "estimation panel too large for the int8 NMF forward pass");
+  v4i* Bp = ctx->buf<v4i>("nmf8f_pack", n16);
+  k_pack_nmf_cols<4><<<(unsigned)nblk, 256, 0, s>>>(Ht, pe, map, sc, Bp);
+  const uint64_t sblocks = cdiv(n, 64 * kFwdWaves);
+  // a chunk's int32 sums stay exact below 2^31 / 254 SNPs (|q| <= 127, dosage <= 2)
+  uint64_t per = 0;
+  switch (K) {
+#define SLFMM_NMF8F(KK)                                                                                       \
+  case KK:                                                                                                    \
+    per = fwd_blocks_per_chunk(                                                                               \
+        ctx, k_fwd_i8<FwdRawEpi<(3 * KK + 15) / 16 + (KK * (KK + 1) + 15) / 16, (3 * KK + 15) / 16>>, nblk,   \
+        sblocks, cdiv(pe, 8u << 20));                                                                         \
+    break;
+    SLFMM_NMF8F(1) SLFMM_NMF8F(2) SLFMM_NMF8F(3) SLFMM_NMF8F(4) SLFMM_NMF8F(5) SLFMM_NMF8F(6) SLFMM_NMF8F(7)
+    SLFMM_NMF8F(8)
+#undef SLFMM_NMF8F
+  }
+  const uint64_t nchunk = cdiv(nblk, per);
+  int32_t* part = ctx->buf<int32_t>("nmf8f_part", nchunk * n * 16 * NTB);
+  const dim3 grid((unsigned)sblocks, (unsigned)nchunk);
+  switch (K) {
+#define SLFMM_NMF8F(KK)                                                                                       \
+  case KK:                                                                                                    \
+    k_fwd_i8<<<grid, 64 * kFwdWaves, 0, s>>>(                                                                 \
+        YT, ldt, n, (int)per, (int)nblk, Bp,                                                                  \
+        FwdRawEpi<(3 * KK + 15) / 16 + (KK * (KK + 1) + 15) / 16, (3 * KK + 15) / 16>{part});                 \
+    break;
+    SLFMM_NMF8F(1) SLFMM_NMF8F(2) SLFMM_NMF8F(3) SLFMM_NMF8F(4) SLFMM_NMF8F(5) SLFMM_NMF8F(6) SLFMM_NMF8F(7)
+    SLFMM_NMF8F(8)
+#undef SLFMM_NMF8F
+  }
+  const uint64_t nslot = n * 16 * NTB;
+  double* ssum = ctx->buf<double>("nmf8f_ssum", nslot);
+  k_sum_slots<<<(unsigned)cdiv(nslot, 256), 256, 0, s>>>(part, nslot, (int)nchunk, ssum);
+  k_nmf_fwd_combine<<<(unsigned)cdiv(n * K, 256), 256, 0, s>>>(ssum, n, map, sc, W, num);
   SLFMM_HIP(hipGetLastError());
 }
 

@@ -85,6 +85,8 @@ struct slfmm_ctx {
   T* buf(const std::string& name, size_t count) {
     return static_cast<T*>(scratch(name, count * sizeof(T)));
   }
+  // workgroups of a kernel the device holds at once (CUs x occupancy), by kernel
+  std::map<const void*, uint64_t> resident_wgs;
   // HIP-event timing of named kernels (enabled by slfmm_profile_enable)
   bool profiling = false;
   struct Timer { double ms = 0.0; uint64_t launches = 0; std::vector<std::pair<hipEvent_t, hipEvent_t>> pending; };
@@ -350,14 +352,20 @@ void precompute_dev(slfmm_ctx* ctx, const double* d_xs, uint64_t n, uint64_t d, 
 // NMF estimation (src/nmf.rs:80-249) on packed rows Y (pe SNPs): device
 // outputs W (n x K), Ht (pe x K) and cv (3 per iteration: NMF MAE, mean MAE,
 // held-out count); omega (n x (K + 10)) is drawn from cfg.seed when null.
-// With i8 (and K <= kNmfMaxK8) the backward pass (the H update) runs on the
-// int8 matrix cores (nmf_backward_i8); everything else is FP64.
+// With i8 (and K <= kNmfMaxK8) the forward pass (num_W, on a sample-major
+// copy of Y) and the backward pass (the H update) run on the int8 matrix
+// cores (nmf_forward_i8, nmf_backward_i8); everything else is FP64.
 void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const slfmm_nmf_config& cfg,
              const double* d_omega, double* W, double* Ht, double* cv, bool i8 = false);
 // assoc_i8.hip: H^T (pe x K) <- the NMF H update (src/nmf.rs:171-221) from W
 // (n x K) and W^T W on the rows Y, on the int8 matrix cores (K <= kNmfMaxK8).
 void nmf_backward_i8(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const double* W, int K,
                      const double* WtW, double eps, double* Ht);
+// assoc_i8.hip: num_W (n x K) of the NMF forward pass (src/nmf.rs:131-147,
+// missing calls filled by W_k . H_j) from the sample-major copy YT (n x ldt
+// bytes) of the rows, Ht (pe x K) and W (K <= kNmfMaxK8).
+void nmf_forward_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_t n, uint64_t pe, const double* Ht,
+                    const double* W, int K, double* num);
 void nmf_transpose_h(hipStream_t s, const double* Ht, uint64_t pe, int K, double* H);
 // Ht[j][r] = max(F[j][r], 0)
 void relu_rows(hipStream_t s, const double* F, uint64_t ldf, uint64_t p, int K, double* Ht);

@@ -375,20 +375,35 @@ void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t
   uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(cdiv(pe, 256), cdiv(2048, ntile)));
   const uint64_t spc = cdiv(pe, nchunk);
   nchunk = cdiv(pe, spc);
-  double* part = ctx->buf<double>("nmf_part", nchunk * n * K);
+  double* part = i8 && K <= kNmfMaxK8 ? nullptr : ctx->buf<double>("nmf_part", nchunk * n * K);
   double* num = ctx->buf<double>("nmf_num", n * K);
   double* hh = ctx->buf<double>("nmf_hh", K * K);
   const unsigned cvblk = (unsigned)cdiv(pe, 256);
   double* cvpart = ctx->buf<double>("nmf_cvpart", (uint64_t)cvblk * 3);
+  const bool mc = i8 && K <= kNmfMaxK8;
+  uint8_t* YT = nullptr;
+  uint64_t ldt = 0;
+  if (mc) {   // sample-major copy for the forward passes
+    ldt = rup(cdiv(pe, 4), 16);
+    YT = ctx->buf<uint8_t>("nmf_YT", n * ldt + kSlack);
+    launch_fill(s, reinterpret_cast<double*>(YT + n * ldt), 0.0, kSlack / 8);   // the transpose writes every row byte
+    launch_transpose(s, Y, ld, n, pe, YT, ldt);
+  }
   for (uint64_t it = 0; it < cfg.n_iter; ++it) {
     // forward: W update from the current reconstruction
-    const dim3 fgrid((unsigned)ntile, (unsigned)nchunk);
-    switch (K) {
+    if (mc) {
+      slfmm_timed tm(ctx, "nmf_forward");
+      nmf_forward_i8(ctx, YT, ldt, n, pe, Ht, W, K, num);
+    } else {
+      slfmm_timed tm(ctx, "nmf_forward");
+      const dim3 fgrid((unsigned)ntile, (unsigned)nchunk);
+      switch (K) {
 #define SLFMM_FWD(R) case R: k_nmf_fwd<R><<<fgrid, 256, 0, s>>>(Y, ld, n, pe, spc, W, Ht, part); break;
-      SLFMM_NMF_RANKS(SLFMM_FWD)
+        SLFMM_NMF_RANKS(SLFMM_FWD)
 #undef SLFMM_FWD
+      }
+      k_nmf_sum_chunks<<<(unsigned)cdiv(n * K, 256), 256, 0, s>>>(part, n * K, (int)nchunk, num);
     }
-    k_nmf_sum_chunks<<<(unsigned)cdiv(n * K, 256), 256, 0, s>>>(part, n * K, (int)nchunk, num);
     ts_gram(ctx, Ht, K, K, Ht, K, K, pe, hh);
     k_nmf_wupd<<<(unsigned)cdiv(n, 256), 256, 0, s>>>(W, num, hh, n, K, eps);
     SLFMM_HIP(hipGetLastError());
@@ -396,7 +411,7 @@ void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t
     // backward: H update in place, fill from the new W and the old H
     ts_gram(ctx, W, K, K, W, K, K, n, WtW);
     const unsigned bgrid = (unsigned)cdiv(pe, 256);
-    if (i8 && K <= kNmfMaxK8) {
+    if (mc) {
       slfmm_timed tm(ctx, "nmf_backward");
       nmf_backward_i8(ctx, Y, ld, n, pe, W, K, WtW, eps, Ht);
     } else {
@@ -410,6 +425,7 @@ void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t
     SLFMM_HIP(hipGetLastError());
     normalize_w(ctx, W, n, Ht, pe, K, eps);
     // held-out error of this iteration
+    slfmm_timed tm(ctx, "nmf_cv");
     k_nmf_cv<<<cvblk, 256, 0, s>>>(Y, ld, n, pe, W, Ht, K, nmiss, sumg, cfg.seed + it, cfg.chunk_size,
                                    u01_threshold(cfg.cv_rate), cvpart);
     k_nmf_cv_final<<<1, 256, 0, s>>>(cvpart, (int)cvblk, cv + 3 * it);

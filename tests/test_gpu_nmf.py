@@ -112,14 +112,18 @@ def test_nmf_rejects_mixed_precision(S):
     (300, 1500, 3, 5, None, 400, 0.05),
     (257, 2000, 8, 4, 0.5, 333, 0.10),     # rank 8 (4 row tiles), ragged sample tile, subset
     (1100, 3000, 6, 6, None, 1000, 0.15),
+    (530, 40001, 1, 3, None, 5000, 0.08),   # rank 1, several SNP chunks, ragged last block
+    (1030, 20000, 5, 3, None, 2000, 0.03),  # 3 sample workgroups of the forward pass
 ])
-def test_nmf_int8_backward_vs_oracle(S, n, p, k, n_iter, rate, chunk, miss):
-    """The H update on the int8 matrix cores (SLFMM_NMF_INT8): W in 3 int8
-    terms, the products W_r W_s in 2, recombined in FP64 from exact int32
-    sums.  Tolerances: W and H within 1e-5 of the largest entry (about the
-    FP32 level the NMF needs) after every multiplicative iteration,
-    per-iteration CV MAE within 1e-5 relative, held-out counts exact (the
-    same mask stream)."""
+def test_nmf_int8_vs_oracle(S, n, p, k, n_iter, rate, chunk, miss):
+    """Both NMF passes on the int8 matrix cores (SLFMM_NMF_INT8).  Forward:
+    the dosages meet H in 3 int8 terms and the missing-call indicator the
+    products H_r H_s in 2, on the sample-major copy; backward: W in 3 terms,
+    W_r W_s in 2; every sum is recombined in FP64 from exact int32 sums.
+    Tolerances: W and H within 1e-5 of the largest entry (about the FP32
+    level the NMF needs) after every multiplicative iteration, per-iteration
+    CV MAE within 1e-5 relative, held-out counts exact (the same mask
+    stream)."""
     print(f"[nmf int8] n={n} p={p} k={k} iters={n_iter}", flush=True)
     packed, bps = _panel(n, p, seed=n + p + k, miss=miss)
     idx = np.arange(p) if rate is None else O.subset_indices(rate, p)
