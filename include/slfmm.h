@@ -60,8 +60,9 @@ enum {
  * a fit directly; without it, fits with device-resident panels and device
  * outputs replay a HIP graph captured by an earlier call with the same
  * shapes, configuration and pointers (no counterpart in the reference).
- * SLFMM_NMF_INT8 (slfmm_nmf_estimate): the H update of every iteration runs
- * on the int8 matrix cores for ranks k <= 8 (see slfmm_nmf_estimate). */
+ * SLFMM_NMF_INT8 (slfmm_nmf_estimate): the forward pass (the W update's
+ * numerator) and the H update of every iteration run on the int8 matrix
+ * cores for ranks k <= 8 (see slfmm_nmf_estimate). */
 enum { SLFMM_OUT_DEVICE = 1u, SLFMM_EAGER = 2u, SLFMM_NMF_INT8 = 4u };
 
 /* Lfmm2Config (src/lib.rs:47-98) plus the GPU precision switch. */
@@ -236,12 +237,18 @@ int slfmm_fit_lfmm2(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset
  * hold-out draw of entry (sample i, column c of chunk q) in iteration t is
  * element c * n + i of the library's uniform stream seeded cfg->seed + t + q
  * (the reference draws from ChaCha8).  w and h follow `flags`; the CV
- * outputs are host memory.  With SLFMM_NMF_INT8 and k <= 8 the backward
- * pass of each iteration (the H update, src/nmf.rs:171-221) sums the filled
- * genotypes on the int8 matrix cores: W in 3 int8 terms against the exact
- * dosages and the products W_r W_s in 2 terms against the missing-call
- * indicator (the fill W_k . H_j enters as (sum over missing k of W_k W_k^T)
- * H_j), recombined in FP64; everything else stays FP64. */
+ * outputs are host memory.  With SLFMM_NMF_INT8 and k <= 8 both passes of
+ * each iteration sum the filled genotypes on the int8 matrix cores, with
+ * exact int32 sums recombined in FP64:
+ *   forward (num_W, src/nmf.rs:131-147), on a sample-major copy of the rows:
+ *   H in 3 int8 terms against the dosages and the products H_r H_s in 2
+ *   against the missing-call indicator (the fill W_k . H_j enters as
+ *   (sum over missing j of H_j H_j^T) W_k);
+ *   backward (the H update, src/nmf.rs:171-221): W in 3 terms against the
+ *   dosages, W_r W_s in 2 against the indicator (the fill enters as
+ *   (sum over missing k of W_k W_k^T) H_j).
+ * W and H then agree with the FP64 path to about 1e-6 of their largest
+ * entries; everything else (initialisation, updates, CV) stays FP64. */
 int slfmm_nmf_estimate(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* subset_idx, uint64_t n_idx,
                        const slfmm_nmf_config* cfg, const double* omega, double* w, double* h,
                        double* cv_mae, double* cv_mae_mean, uint64_t* mask_count, uint32_t flags);

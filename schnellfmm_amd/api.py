@@ -19,7 +19,7 @@ from ._lib import AssocOut, Config, NmfConfig, SlfmmError, check, lib
 
 CENTER_ONLY, EIGENSTRAT = 0, 1          # SnpNorm, src/bed.rs:19-26
 PREC_FP64, PREC_I8 = 0, 1
-NMF_INT8 = 4                              # slfmm_nmf_estimate flag: H update on the int8 matrix cores
+NMF_INT8 = 4                              # slfmm_nmf_estimate flag: both NMF passes on the int8 matrix cores
 N_BINS = 100_000
 
 
@@ -368,8 +368,9 @@ class NmfEstResult:
 
 def nmf_impute_estimation(bed: BedFile, subset, config: NmfParams, omega=None, int8: bool = False) -> NmfEstResult:
     """src/nmf.rs:80-249 on the GPU (FP64).  omega (n x (k + 10)) may be
-    injected; otherwise it is drawn from config.seed.  int8: the H update of
-    every iteration runs on the int8 matrix cores (k <= 8; SLFMM_NMF_INT8)."""
+    injected; otherwise it is drawn from config.seed.  int8: the forward
+    pass and the H update of every iteration run on the int8 matrix cores
+    (k <= 8; SLFMM_NMF_INT8)."""
     ctx = bed.ctx
     idx = None if subset is None or (isinstance(subset, str) and subset == "all") else subset_indices(subset, bed.n_snps)
     pe = bed.n_snps if idx is None else len(idx)

@@ -361,6 +361,15 @@ void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t
 // (n x K) and W^T W on the rows Y, on the int8 matrix cores (K <= kNmfMaxK8).
 void nmf_backward_i8(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const double* W, int K,
                      const double* WtW, double eps, double* Ht);
+// assoc_i8.hip: RSVD passes on the int8 matrix cores (l <= 32).  Sketch /
+// backward: Z[j][c] = s_j (B^T y_j)_c for every SNP row (y normalised by
+// norm, missing calls at the mean).  Forward: W = Y Q_z on the sample-major
+// copy packedT.
+void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
+                     const double* d_B, uint64_t l, int norm, double* Z, uint64_t ldZ);
+void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* packedT, uint64_t ldt, uint64_t n, uint64_t p,
+                        const double* Qz, uint64_t ldq, uint64_t l, const double* mu, const double* sc,
+                        double* W, uint64_t ldw);
 // assoc_i8.hip: num_W (n x K) of the NMF forward pass (src/nmf.rs:131-147,
 // missing calls filled by W_k . H_j) from the sample-major copy YT (n x ldt
 // bytes) of the rows, Ht (pe x K) and W (K <= kNmfMaxK8).

@@ -370,17 +370,20 @@ void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t
   SLFMM_HIP(hipGetLastError());
   normalize_w(ctx, W, n, Ht, pe, K, eps);
 
-  // -- iterations
+  // -- iterations.  With i8 (K <= kNmfMaxK8) both passes run on the int8
+  // matrix cores; the initialisation above stays FP64: its functionals feed
+  // an SVD, and int8 sums there moved W by 3e-5 of its largest entry after 5
+  // iterations (the passes alone: about 1e-6).
+  const bool mc = i8 && K <= kNmfMaxK8;
   const uint64_t ntile = cdiv(n, 256);
   uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(cdiv(pe, 256), cdiv(2048, ntile)));
   const uint64_t spc = cdiv(pe, nchunk);
   nchunk = cdiv(pe, spc);
-  double* part = i8 && K <= kNmfMaxK8 ? nullptr : ctx->buf<double>("nmf_part", nchunk * n * K);
+  double* part = mc ? nullptr : ctx->buf<double>("nmf_part", nchunk * n * K);
   double* num = ctx->buf<double>("nmf_num", n * K);
   double* hh = ctx->buf<double>("nmf_hh", K * K);
   const unsigned cvblk = (unsigned)cdiv(pe, 256);
   double* cvpart = ctx->buf<double>("nmf_cvpart", (uint64_t)cvblk * 3);
-  const bool mc = i8 && K <= kNmfMaxK8;
   uint8_t* YT = nullptr;
   uint64_t ldt = 0;
   if (mc) {   // sample-major copy for the forward passes

@@ -14,12 +14,6 @@
 
 namespace slfmm {
 
-void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uint64_t p,
-                     const double* d_B, uint64_t l, int norm, double* Z, uint64_t ldZ);
-void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* packedT, uint64_t ldt, uint64_t n, uint64_t p,
-                        const double* Qz, uint64_t ldq, uint64_t l, const double* mu, const double* sc,
-                        double* W, uint64_t ldw);
-
 // ---- forward pass: W[k][c] = sum_j v_jk * sc_j * Qz[j][c] --------------------
 // One workgroup per (256-sample tile, SNP chunk, 32-column block); partial
 // sums per chunk are reduced in a fixed order afterwards.  With FILL the
