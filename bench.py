@@ -296,6 +296,9 @@ def nmf_workload(args):
         f"cv {t64['nmf_cv']:.2f} / {t8['nmf_cv']:.2f} ms")
     dt_fit, fit = timed(lambda: S.fit_lfmm2_nmf(geno, None, geno, x, lc, cfg), max(steps // 2, 1), min(warmup, 1))
     log(f"[bench] nmf fit: {dt_fit * 1e3:.1f} ms")
+    dt_fit8, fit8 = timed(lambda: S.fit_lfmm2_nmf(geno, None, geno, x, lc, cfg, nmf_int8=True), max(steps // 2, 1),
+                          min(warmup, 1))
+    log(f"[bench] nmf fit, NMF passes on the int8 matrix cores: {dt_fit8 * 1e3:.1f} ms")
     bytes_pass = p * math.ceil(n / 4)
     passes = 2 + 3 * cfg.n_iter
     line = {
@@ -320,6 +323,9 @@ def nmf_workload(args):
                  "max_rel_dh_vs_fp64": float(np.max(np.abs(est8.h_full - est.h_full))
                                              / np.max(np.abs(est.h_full)))},
         "fit_ms": dt_fit * 1e3, "fit_snps_per_s": p / dt_fit, "fit_gif": float(fit.gif),
+        "fit_int8_nmf": {"ms": dt_fit8 * 1e3, "gif": float(fit8.gif),
+                         "max_abs_dlog10p_vs_fp64": float(np.max(np.abs(np.log10(fit8.pvalue)
+                                                                         - np.log10(fit.pvalue))))},
         "gwas_cv": [float(v) for v in fit.gwas_cv],
         "cpu_baseline": None,
     }

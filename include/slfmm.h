@@ -288,7 +288,10 @@ int slfmm_fit_lfmm2_batch(slfmm_ctx* ctx, slfmm_problem* problems, uint64_t coun
  * the estimation CV error per iteration, then the mean-imputation error on the
  * final iteration's mask (results.nmf_cv_mean, src/lib.rs:259); gwas_cv (3, host: NMF MAE, mean-fill
  * MAE, held-out count) enables the association-phase CV with rate
- * nmf->cv_rate and seed cfg->seed (NULL disables it). */
+ * nmf->cv_rate and seed cfg->seed (NULL disables it).  With SLFMM_NMF_INT8
+ * in flags the NMF estimation runs its update passes on the int8 matrix
+ * cores (nmf k <= 8, as slfmm_nmf_estimate); the filled RSVD and
+ * association passes stay FP64. */
 int slfmm_fit_lfmm2_nmf(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset_idx, uint64_t n_idx,
                         const slfmm_geno* g, const double* x, uint64_t d, const slfmm_config* cfg,
                         const slfmm_nmf_config* nmf, const double* b0, const double* nmf_omega, double* u_hat,

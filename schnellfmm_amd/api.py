@@ -487,8 +487,12 @@ def fit_lfmm2_batch(problems, n_streams: Optional[int] = None, with_hist: bool =
 
 
 def fit_lfmm2_nmf(y_est: BedFile, est_subset, y_full: BedFile, x, config: Lfmm2Config, nmf: NmfParams,
-                  b0=None, nmf_omega=None, gwas_cv: bool = True, with_hist: bool = False) -> TestResults:
-    """Full pipeline with NMF imputation, src/lib.rs:162-263 (FP64)."""
+                  b0=None, nmf_omega=None, gwas_cv: bool = True, with_hist: bool = False,
+                  nmf_int8: bool = False) -> TestResults:
+    """Full pipeline with NMF imputation, src/lib.rs:162-263 (FP64).
+    nmf_int8: the NMF estimation's update passes run on the int8 matrix
+    cores (nmf k <= 8; SLFMM_NMF_INT8); the filled RSVD and association
+    passes stay FP64."""
     ctx = y_full.ctx
     x = _f64(x)
     idx = None if est_subset is None or (isinstance(est_subset, str) and est_subset == "all") \
@@ -505,7 +509,7 @@ def fit_lfmm2_nmf(y_est: BedFile, est_subset, y_full: BedFile, x, config: Lfmm2C
     gcv = np.zeros(3) if gwas_cv else None
     ctx.check(lib().slfmm_fit_lfmm2_nmf(ctx.h, y_est.h, _ptr(idx), 0 if idx is None else len(idx), y_full.h,
                                         _ptr(x), d, C.byref(cfg), C.byref(ncfg), _ptr(b), _ptr(om), _ptr(u),
-                                        C.byref(out), _ptr(cv_iter), _ptr(gcv), 0))
+                                        C.byref(out), _ptr(cv_iter), _ptr(gcv), NMF_INT8 if nmf_int8 else 0))
     beta, t, r2, p, hist, gpt, gif = arrs
     res = TestResults(u, float(gif[0]), gpt, beta, t, r2, p, hist)
     res.nmf_cv = cv_iter[:nmf.n_iter].copy()

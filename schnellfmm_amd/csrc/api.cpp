@@ -1004,7 +1004,7 @@ int slfmm_fit_lfmm2_nmf(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* su
     double* W = ctx->buf<double>("fit_nmf_w", n * kn);
     double* Ht = ctx->buf<double>("fit_nmf_ht", pe * kn);
     double* dcv = ctx->buf<double>("fit_nmf_cv", 3 * std::max<uint64_t>(nmf->n_iter, 1));
-    nmf_run(ctx, Y, est->ld, n, pe, *nmf, dom, W, Ht, dcv);
+    nmf_run(ctx, Y, est->ld, n, pe, *nmf, dom, W, Ht, dcv, (flags & SLFMM_NMF_INT8) != 0);
     // W^+ = safe_inv(W^T W) W^T, kept as its transpose Wp = W (W^T W)^-1 (src/lib.rs:199-206)
     double* WtW = ctx->buf<double>("fit_nmf_wtw", kn * kn);
     double* Winv = ctx->buf<double>("fit_nmf_winv", kn * kn);
