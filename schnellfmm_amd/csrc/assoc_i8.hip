@@ -998,13 +998,16 @@ struct GStore1 {
   __device__ void operator()(int c, const double (&v)[1]) const { out[c] = v[0]; }
 };
 
-// WV waves x 4 tiles of 16 samples per workgroup share each staged B block.
-constexpr int kFwdWaves = 8;
+// Epi::WV waves x 4 tiles of 16 samples per workgroup share each staged B
+// block: 8 for the RSVD pass, 4 for the NMF pass (two workgroups per CU,
+// whose barriers interleave; measured 1.09 vs 1.19 ms per NMF forward pass
+// at n = 5000, p = 2M, and 0.90 vs 0.87 ms per RSVD fit the other way).
 // Epilogue of the RSVD forward pass: the 2 terms of Qt and of mu Qt of
 // column 16 cb + l16 sit in tiles cb, NCB + cb, 2 NCB + cb, 3 NCB + cb.
 template <int NCB>
 struct FwdRsvdEpi {
   static constexpr int NTB = 4 * NCB, NG = 2 * NCB;   // tiles; the first NG meet the dosage fragment
+  static constexpr int WV = 8;                        // waves per workgroup
   const double* csc; int l; double* part; uint64_t ldp;
   template <int ST>
   __device__ void operator()(const v4i (&acc)[ST][NTB], uint64_t sbase, uint64_t n, int g, int l16,
@@ -1032,20 +1035,21 @@ struct FwdRsvdEpi {
 // Forward pass on the sample-major copy with a tile plan: tiles [0, NG)
 // meet the dosage fragment, [NG, NTB) the missing-call fragment; Epi
 // combines the int32 sums of a workgroup's SNP chunk into its partial row.
-// Grid: (sample blocks of 64 kFwdWaves, SNP chunks of nblk_chunk blocks).
+// Grid: (sample blocks of 64 Epi::WV, SNP chunks of nblk_chunk blocks).
 template <class Epi>
-__global__ void __launch_bounds__(64 * kFwdWaves) k_fwd_i8(const uint8_t* __restrict__ YT, uint64_t ldt, uint64_t n,
+__global__ void __launch_bounds__(64 * Epi::WV) k_fwd_i8(const uint8_t* __restrict__ YT, uint64_t ldt, uint64_t n,
                                                            int nblk_chunk, int nblk_total, const v4i* __restrict__ Bp,
                                                            Epi epi) {
-  constexpr int NTB = Epi::NTB, NG = Epi::NG;
+  constexpr int NTB = Epi::NTB, NG = Epi::NG, WV = Epi::WV;
   constexpr int KS = 4;
   constexpr int ST = 4;
-  constexpr int NTH = 64 * kFwdWaves;
-  constexpr int BLK = KS * NTB * 64;
+  constexpr int CH = KS * NTB;   // 1 KB B chunks (64 lanes x 16 B) per block
+  constexpr int BLK = CH * 64;
   __shared__ v4i Bs[2 * BLK];
   const int bx = (int)blockIdx.x, by = (int)blockIdx.y;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, l16 = lane & 15;
-  const uint64_t sbase = (uint64_t)bx * (ST * 16 * kFwdWaves) + wave * (ST * 16);
+  const int t = threadIdx.x, lane = t & 63, g = lane >> 4, l16 = lane & 15;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);   // wave-uniform: scalar DMA offsets
+  const uint64_t sbase = (uint64_t)bx * (ST * 16 * WV) + wave * (ST * 16);
   const int b0 = by * nblk_chunk;
   const int b1 = min(nblk_total, b0 + nblk_chunk);
   const uint8_t* rowp[ST];
@@ -1061,27 +1065,27 @@ __global__ void __launch_bounds__(64 * kFwdWaves) k_fwd_i8(const uint8_t* __rest
 #pragma unroll
     for (int ct = 0; ct < NTB; ++ct) acc[st][ct] = v4i{0, 0, 0, 0};
   uint32_t dummy0 = 0, dummy1 = 0, dummy2 = 0;
-  constexpr int PER = (BLK + NTH - 1) / NTH;
-  if (b0 < b1) {
-    for (int q = 0; q < PER; ++q) {
-      int idx = t + NTH * q;
-      if (idx < BLK) Bs[idx] = Bp[(uint64_t)b0 * BLK + idx];
-    }
-  }
+  // B blocks go straight from global memory into the LDS ring by DMA: the
+  // waves take the block's chunks in turn (byte offsets < 2^32, checked by
+  // the launchers)
+  const rsrc_t brs = make_rsrc(Bp, (uint32_t)((uint64_t)nblk_total * BLK * 16));
+  auto stage_b = [&](int kb, int slot) {
+    unsigned char* dst = reinterpret_cast<unsigned char*>(Bs + slot * BLK);
+    for (int c = wave; c < CH; c += WV)
+      buf_to_lds(brs, 16 * lane, (uint32_t)(((uint64_t)kb * CH + c) * 1024), dst + c * 1024);
+  };
   uint32_t gw[ST][KS], gn[ST][KS];
-  if (b0 < b1)
+  if (b0 < b1) {
+    stage_b(b0, 0);
 #pragma unroll
     for (int st = 0; st < ST; ++st) load_words<KS>(rowp[st] + (uint64_t)b0 * 16 * KS, gw[st]);
+  }
+  wait_vmcnt<0>();
   __syncthreads();
   for (int kb = b0; kb < b1; ++kb) {
     const bool more = kb + 1 < b1;
-    v4i pre[PER];
     if (more) {
-#pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        int idx = t + NTH * q;
-        if (idx < BLK) pre[q] = Bp[(uint64_t)(kb + 1) * BLK + idx];
-      }
+      stage_b(kb + 1, (kb + 1 - b0) & 1);
 #pragma unroll
       for (int st = 0; st < ST; ++st) load_words<KS>(rowp[st] + (uint64_t)(kb + 1) * 16 * KS, gn[st]);
     }
@@ -1113,15 +1117,11 @@ __global__ void __launch_bounds__(64 * kFwdWaves) k_fwd_i8(const uint8_t* __rest
     }
     if (more) {
 #pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        int idx = t + NTH * q;
-        if (idx < BLK) Bs[((kb + 1 - b0) & 1) * BLK + idx] = pre[q];
-      }
-#pragma unroll
       for (int st = 0; st < ST; ++st)
 #pragma unroll
         for (int s = 0; s < KS; ++s) gw[st][s] = gn[st][s];
     }
+    wait_vmcnt<0>();   // this wave's B DMA has landed
     __syncthreads();
   }
   (void)dummy0; (void)dummy1; (void)dummy2;
@@ -1133,13 +1133,13 @@ __global__ void __launch_bounds__(64 * kFwdWaves) k_fwd_i8(const uint8_t* __rest
 // once (CUs x the kernel's occupancy), so no round runs a few stragglers;
 // at least min_chunks chunks.
 template <class Kern>
-static uint64_t fwd_blocks_per_chunk(slfmm_ctx* ctx, Kern kern, uint64_t nblk, uint64_t sblocks,
+static uint64_t fwd_blocks_per_chunk(slfmm_ctx* ctx, Kern kern, int threads, uint64_t nblk, uint64_t sblocks,
                                      uint64_t min_chunks) {
   uint64_t& resident = ctx->resident_wgs[reinterpret_cast<const void*>(kern)];
   if (resident == 0) {
     int cus = 0, occ = 0;
     SLFMM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    SLFMM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64 * kFwdWaves, 0));
+    SLFMM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, threads, 0));
     resident = (uint64_t)std::max(cus, 1) * (uint64_t)std::max(occ, 1);
   }
   const uint64_t target = std::max<uint32_t>(1u, ctx->tuning.fwd_target_wg);
@@ -1180,20 +1180,22 @@ void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_
     k_pack_cols<2, 4><<<pgrid, 256, 0, s>>>(Qz, ldq, pe, (int)l, ssnp, mu, csc, (uint32_t)nsnp, Bp);
   double* vhat = ctx->buf<double>("fw_vhat", l);
   colreduce<1, kColSum>(ctx, FnVhat{Qz, ldq, ssnp, mu, csc}, GStore1{vhat}, pe, (int)l);
-  const uint64_t sblocks = cdiv(n, 64 * kFwdWaves);
+  constexpr int WV = FwdRsvdEpi<1>::WV;
+  const uint64_t sblocks = cdiv(n, 64 * WV);
   // a chunk's int32 sums stay exact below 2^31 / 254 SNPs (|q| <= 127, dosage <= 2)
-  const uint64_t per = NCB == 1 ? fwd_blocks_per_chunk(ctx, k_fwd_i8<FwdRsvdEpi<1>>, nblk, sblocks, cdiv(pe, 8u << 20))
-                                : fwd_blocks_per_chunk(ctx, k_fwd_i8<FwdRsvdEpi<2>>, nblk, sblocks, cdiv(pe, 8u << 20));
+  const uint64_t per = NCB == 1
+                           ? fwd_blocks_per_chunk(ctx, k_fwd_i8<FwdRsvdEpi<1>>, 64 * WV, nblk, sblocks, cdiv(pe, 8u << 20))
+                           : fwd_blocks_per_chunk(ctx, k_fwd_i8<FwdRsvdEpi<2>>, 64 * WV, nblk, sblocks, cdiv(pe, 8u << 20));
   const uint64_t nchunk = cdiv(nblk, per);
   const uint64_t ldp = l;
   double* part = ctx->buf<double>("fw_part", nchunk * n * ldp);
   const dim3 grid((unsigned)sblocks, (unsigned)nchunk);
   if (NCB == 1)
-    k_fwd_i8<<<grid, 64 * kFwdWaves, 0, s>>>(YT, ldt, n, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp),
-                                            FwdRsvdEpi<1>{csc, (int)l, part, ldp});
+    k_fwd_i8<<<grid, 64 * WV, 0, s>>>(YT, ldt, n, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp),
+                                     FwdRsvdEpi<1>{csc, (int)l, part, ldp});
   else
-    k_fwd_i8<<<grid, 64 * kFwdWaves, 0, s>>>(YT, ldt, n, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp),
-                                            FwdRsvdEpi<2>{csc, (int)l, part, ldp});
+    k_fwd_i8<<<grid, 64 * WV, 0, s>>>(YT, ldt, n, (int)per, (int)nblk, reinterpret_cast<const v4i*>(Bp),
+                                     FwdRsvdEpi<2>{csc, (int)l, part, ldp});
   k_fwd_reduce<<<(unsigned)cdiv(n * l, 256), 256, 0, s>>>(part, n, (int)l, ldp, (int)nchunk, vhat, W, ldw);
   SLFMM_HIP(hipGetLastError());
 }
@@ -1309,6 +1311,7 @@ __global__ void __launch_bounds__(256) k_pack_nmf_cols(const double* __restrict_
 template <int NTB_, int NG_>
 struct FwdRawEpi {
   static constexpr int NTB = NTB_, NG = NG_;
+  static constexpr int WV = 4;   // waves per workgroup
   int32_t* part;
   template <int ST>
   __device__ void operator()(const v4i (&acc)[ST][NTB], uint64_t sbase, uint64_t n, int g, int l16,
@@ -1384,15 +1387,16 @@ void nmf_forward_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_t n,
   SLFMM_CHECK(n16 * 16 < 0xFFFFFFFFull, SLFMM_E_ARG, "estimation panel too large for the int8 NMF forward pass");
   v4i* Bp = ctx->buf<v4i>("nmf8f_pack", n16);
   k_pack_nmf_cols<4><<<(unsigned)nblk, 256, 0, s>>>(Ht, pe, map, sc, Bp);
-  const uint64_t sblocks = cdiv(n, 64 * kFwdWaves);
+  constexpr int WV = FwdRawEpi<1, 1>::WV;
+  const uint64_t sblocks = cdiv(n, 64 * WV);
   // a chunk's int32 sums stay exact below 2^31 / 254 SNPs (|q| <= 127, dosage <= 2)
   uint64_t per = 0;
   switch (K) {
 #define SLFMM_NMF8F(KK)                                                                                       \
   case KK:                                                                                                    \
     per = fwd_blocks_per_chunk(                                                                               \
-        ctx, k_fwd_i8<FwdRawEpi<(3 * KK + 15) / 16 + (KK * (KK + 1) + 15) / 16, (3 * KK + 15) / 16>>, nblk,   \
-        sblocks, cdiv(pe, 8u << 20));                                                                         \
+        ctx, k_fwd_i8<FwdRawEpi<(3 * KK + 15) / 16 + (KK * (KK + 1) + 15) / 16, (3 * KK + 15) / 16>>, 64 * WV, \
+        nblk, sblocks, cdiv(pe, 8u << 20));                                                                   \
     break;
     SLFMM_NMF8F(1) SLFMM_NMF8F(2) SLFMM_NMF8F(3) SLFMM_NMF8F(4) SLFMM_NMF8F(5) SLFMM_NMF8F(6) SLFMM_NMF8F(7)
     SLFMM_NMF8F(8)
@@ -1404,7 +1408,7 @@ void nmf_forward_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_t n,
   switch (K) {
 #define SLFMM_NMF8F(KK)                                                                                       \
   case KK:                                                                                                    \
-    k_fwd_i8<<<grid, 64 * kFwdWaves, 0, s>>>(                                                                 \
+    k_fwd_i8<<<grid, 64 * WV, 0, s>>>(                                                                        \
         YT, ldt, n, (int)per, (int)nblk, Bp,                                                                  \
         FwdRawEpi<(3 * KK + 15) / 16 + (KK * (KK + 1) + 15) / 16, (3 * KK + 15) / 16>{part});                 \
     break;
