@@ -10,7 +10,9 @@
 * NMF (n = 5 000, p = 2M, 8 Dirichlet(0.2) components, structured 10 %
   missingness, nmf-k = 8, 10 iterations, cv-rate 5e-4): full-size invariants,
   and the estimation on a 3 000-SNP subset against the oracle (W, H within
-  1e-8, CV MAE within 1e-9, held-out counts exact).
+  1e-8, CV MAE within 1e-9, held-out counts exact); with both update passes
+  on the int8 matrix cores, W and H within 1e-5 of the FP64 run at full size
+  and of the oracle on the subset, held-out counts exact.
 * inbred lines (n = 1 135, p = 1.8M, K = 12): mixed against FP64 on every SNP
   and the FP64 association against the oracle on a 6 000-SNP slice.
 Each test prints its workload before the GPU work starts."""
@@ -165,26 +167,46 @@ def test_nmf_full_size_invariants(S, nmf_panel):
     assert abs(got.total_mask_count / 10 - expect) < 0.03 * expect
     _log(f"[workload] NMF: cv MAE {cv[-1]:.4f} vs mean-impute {got.cv_mae_mean_impute:.4f}, "
          f"{got.total_mask_count} held out")
+    _log("[workload] NMF: estimation with both passes on the int8 matrix cores")
+    i8 = S.nmf_impute_estimation(g, None, cfg, int8=True)
+    dw = np.max(np.abs(i8.w - got.w)) / np.abs(got.w).max()
+    dh = np.max(np.abs(i8.h_full - got.h_full)) / np.abs(got.h_full).max()
+    _log(f"[workload] NMF int8 vs FP64: max |dW| {dw:.2e}, max |dH| {dh:.2e} of the largest entries")
+    assert dw <= 1e-5 and dh <= 1e-5
+    assert i8.total_mask_count == got.total_mask_count
+    assert np.allclose(i8.cv_mae_per_iter, cv, rtol=1e-5, atol=0)
 
 
-def test_nmf_subset_vs_oracle(S, nmf_panel):
+_NMF_SUB = dict(n=5000, m=3000, k=8, iters=10, seed=42, cv_rate=5e-4, chunk=1000)
+
+
+@pytest.fixture(scope="module")
+def nmf_subset_ref(nmf_panel):
+    c = _NMF_SUB
+    packed = nmf_panel.read_packed(0, c["m"]).tobytes()
+    omega = synth.normal_matrix(c["seed"], c["n"], c["k"] + 10)
+    _log(f"[workload] NMF subset: oracle on {c['m']} SNPs")
+    ref = O.nmf_impute_estimation(packed, (c["n"] + 3) // 4, c["n"], np.arange(c["m"]), c["k"], c["iters"], 1e-16,
+                                  c["cv_rate"], c["chunk"], c["seed"], omega, synth.uniform_stream)
+    return ref, omega
+
+
+@pytest.mark.parametrize("int8", [False, True])
+def test_nmf_subset_vs_oracle(S, nmf_panel, nmf_subset_ref, int8):
     g = nmf_panel
-    n, m, k, iters = 5000, 3000, 8, 10
-    packed = g.read_packed(0, m).tobytes()
-    bps = (n + 3) // 4
-    seed, cv_rate, chunk = 42, 5e-4, 1000
-    omega = synth.normal_matrix(seed, n, k + 10)
-    _log(f"[workload] NMF subset: oracle on {m} SNPs")
-    ref = O.nmf_impute_estimation(packed, bps, n, np.arange(m), k, iters, 1e-16, cv_rate, chunk, seed, omega,
-                                  synth.uniform_stream)
+    ref, omega = nmf_subset_ref
+    tol, tol_cv = (1e-5, 1e-5) if int8 else (1e-8, 1e-9)
+    m, k, iters, seed = _NMF_SUB["m"], _NMF_SUB["k"], _NMF_SUB["iters"], _NMF_SUB["seed"]
+    cv_rate, chunk = _NMF_SUB["cv_rate"], _NMF_SUB["chunk"]
+    _log(f"[workload] NMF subset: GPU estimation ({'int8 passes' if int8 else 'FP64'})")
     cfg = S.NmfParams(k=k, n_iter=iters, cv_rate=cv_rate, seed=seed, chunk_size=chunk)
-    got = S.nmf_impute_estimation(g, np.arange(m), cfg, omega=omega)
+    got = S.nmf_impute_estimation(g, np.arange(m), cfg, omega=omega, int8=int8)
     scale_w, scale_h = np.abs(ref.w).max(), np.abs(ref.h_full).max()
-    assert np.max(np.abs(got.w - ref.w)) <= 1e-8 * scale_w
-    assert np.max(np.abs(got.h_full - ref.h_full)) <= 1e-8 * scale_h
+    assert np.max(np.abs(got.w - ref.w)) <= tol * scale_w
+    assert np.max(np.abs(got.h_full - ref.h_full)) <= tol * scale_h
     assert got.total_mask_count == ref.total_mask_count > 0
-    assert np.allclose(got.cv_mae_per_iter, ref.cv_mae_per_iter, rtol=1e-9, atol=0)
-    assert np.isclose(got.cv_mae_mean_impute, ref.cv_mae_mean_impute, rtol=1e-12, atol=0)
+    assert np.allclose(got.cv_mae_per_iter, ref.cv_mae_per_iter, rtol=tol_cv, atol=0)
+    assert np.isclose(got.cv_mae_mean_impute, ref.cv_mae_mean_impute, rtol=1e-12 if not int8 else 1e-5, atol=0)
 
 
 # ---------------------------------------------------------------------------
