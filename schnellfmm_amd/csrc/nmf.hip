@@ -167,6 +167,11 @@ __global__ void __launch_bounds__(256) k_nmf_bwd(const uint8_t* __restrict__ pac
 // (j % chunk) * n + k of the stream seeded mask_seed + seq (column, then
 // row, every entry drawn); it is held out if the draw is < cv_rate and the
 // call is observed.  Per-workgroup sums, combined in a fixed order.
+// One thread per SNP.  The draws of a 16-code word are unrolled (a counter
+// offset per code, a hit bit at bit 2c of the word's mask, missing calls and
+// padding cleared with the codes' own bits), and the rare held-out calls are
+// then taken in sample order from the mask, so the hot loop has no branch
+// and the sums keep the per-entry order.
 __global__ void __launch_bounds__(256) k_nmf_cv(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t n,
                                                 uint64_t p, const double* __restrict__ W,
                                                 const double* __restrict__ Ht, int K,
@@ -179,27 +184,35 @@ __global__ void __launch_bounds__(256) k_nmf_cv(const uint8_t* __restrict__ pack
   const uint64_t j = (uint64_t)blockIdx.x * 256 + t;
   double e_nmf = 0.0, e_mean = 0.0;
   unsigned long long cnt = 0;
-  if (j < p) {
+  if (j < p && cv_thresh > 0) {
     const int64_t nobs = (int64_t)n - nmiss[j];
     const double mu = nobs > 0 ? (double)sumg[j] / (double)nobs : 0.0;
     const uint64_t seed = mask_seed + j / chunk;
-    const uint64_t base = (j % chunk) * n;
     // draw base + k of the stream: u01 < cv_rate  <=>  (mix >> 11) < cv_thresh
-    uint64_t ctr = stream_counter(seed, base);
+    uint64_t ctr = stream_counter(seed, (j % chunk) * n);
     const uint4* row = reinterpret_cast<const uint4*>(packed + j * ld);
     const uint64_t nkb = cdiv(n, 64);
-    for (uint64_t kb = 0; kb < nkb; ++kb) {
+    for (uint64_t kb = 0; kb < nkb; ++kb, ctr += 64 * kStreamStep) {
       const uint4 q = row[kb];
       const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
       const int lim = (int)std::min<uint64_t>(64, n - kb * 64);
-      for (int kk = 0; kk < lim; ++kk, ctr += kStreamStep) {
-        const uint64_t k = kb * 64 + kk;
-        const bool drawn = (mix_counter(ctr) >> 11) < cv_thresh;
-        const uint32_t code = code_of(wd[kk >> 4], kk & 15);
-        if (drawn && code != 1u) {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        uint32_t hit = 0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+          hit |= (mix_counter(ctr + (uint64_t)(16 * w + c) * kStreamStep) >> 11) < cv_thresh ? 1u << (2 * c) : 0u;
+        const uint32_t x = wd[w];
+        hit &= ~(x & ~(x >> 1) & 0x55555555u);   // code 01 (missing) at bit 2c
+        const int valid = lim - 16 * w;           // codes past the panel's last sample
+        if (valid < 16) hit &= valid <= 0 ? 0u : (1u << (2 * valid)) - 1u;
+        while (hit) {
+          const int b = __builtin_ctz(hit);
+          hit &= hit - 1;
+          const uint64_t k = kb * 64 + 16 * w + (b >> 1);
           double rec = 0.0;
           for (int r = 0; r < K; ++r) rec += W[k * K + r] * Ht[j * K + r];
-          const double g = dosage_of(code);
+          const double g = dosage_of((x >> b) & 3u);
           e_nmf += fabs(g - rec);
           e_mean += fabs(g - mu);
           cnt += 1;
