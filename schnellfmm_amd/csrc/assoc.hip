@@ -26,12 +26,15 @@ namespace slfmm {
 // with v = g - mu_j on observed calls and 0 on missing calls (mean
 // imputation after centring, src/bed.rs:339-343).  Tile: 256 SNPs x 32 rows;
 // wave w -> rows 16*(w&1)..+16 of the row block, SNP group w>>1; lane ->
-// 2 SNPs.  A k-block of 64 samples of A is staged in LDS.
+// 2 SNPs.  The 16 A values of a sample are the same for every lane of the
+// wave, so they come in by scalar loads (two samples' rows in flight) and
+// feed the FMAs from SGPRs: no LDS staging and no barrier (the LDS
+// broadcast of one value per two FMAs bounded the earlier form).
 // ---------------------------------------------------------------------------
 // With a fill rank FK > 0, missing calls take the value W_k . H_j - mu_j (NMF
 // imputation after centring, rank FK = fs.K): the 64 W rows of the block are
-// staged beside A, and each lane keeps the FK loadings of its two SNPs in
-// registers.
+// staged in LDS (read only where a call is missing), and each lane keeps the
+// FK loadings of its two SNPs in registers.
 template <int FK>
 __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t p,
                                                   uint64_t npad, const double* __restrict__ A, uint64_t lda,
@@ -39,9 +42,9 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
                                                   uint64_t ldF, const double* __restrict__ outscale, FillSpec fs) {
   constexpr bool FILL = FK > 0;
   constexpr int KR = FILL ? FK : 1;
-  __shared__ double As[64][33];
   __shared__ double Ws[FILL ? 64 : 1][KR + 1];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int t = threadIdx.x, lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);   // wave-uniform: A rows by scalar loads
   const int half = w & 1, grp = w >> 1;
   const int rb = blockIdx.y;
   const uint64_t j0 = (uint64_t)blockIdx.x * 256 + grp * 128 + lane;
@@ -63,49 +66,53 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
   for (int r = 0; r < 16; ++r) { acc0[r] = 0.0; acc1[r] = 0.0; }
   const uint4* row0 = reinterpret_cast<const uint4*>(packed + (ok0 ? j0 : 0) * ld);
   const uint4* row1 = reinterpret_cast<const uint4*>(packed + (ok1 ? j1 : 0) * ld);
+  const double* Acol = A + rb * 32 + half * 16;
   const uint64_t nkb = npad / 64;
   for (uint64_t kb = 0; kb < nkb; ++kb) {
-    __syncthreads();
-    for (int e = t; e < 64 * 32; e += 256) {
-      int kk = e >> 5, r = e & 31;
-      As[kk][r] = A[(kb * 64 + kk) * lda + rb * 32 + r];
-    }
-    if constexpr (FILL) {
+    const uint4 q0 = ok0 ? row0[kb] : make_uint4(0, 0, 0, 0);
+    const uint4 q1 = ok1 ? row1[kb] : make_uint4(0, 0, 0, 0);
+    if constexpr (FILL) {   // the block's W rows, read only where a call is missing
+      __syncthreads();
       for (int e = t; e < 64 * KR; e += 256) {
         const int kk = e / KR, r = e % KR;
         const uint64_t k = kb * 64 + kk;
         Ws[kk][r] = k < fs.n ? fs.W[k * KR + r] : 0.0;
       }
+      __syncthreads();
     }
-    uint4 q0 = ok0 ? row0[kb] : make_uint4(0, 0, 0, 0);
-    uint4 q1 = ok1 ? row1[kb] : make_uint4(0, 0, 0, 0);
-    __syncthreads();
-    uint32_t w0[4] = {q0.x, q0.y, q0.z, q0.w}, w1[4] = {q1.x, q1.y, q1.z, q1.w};
-#pragma unroll 4
-    for (int kk = 0; kk < 64; ++kk) {
-      uint32_t c0 = (w0[kk >> 4] >> (2 * (kk & 15))) & 3u;
-      uint32_t c1 = (w1[kk >> 4] >> (2 * (kk & 15))) & 3u;
+#pragma unroll 1
+    for (int wq = 0; wq < 4; ++wq) {   // one 16-code word per SNP
+    const uint32_t x0 = wq == 0 ? q0.x : wq == 1 ? q0.y : wq == 2 ? q0.z : q0.w;
+    const uint32_t x1 = wq == 0 ? q1.x : wq == 1 ? q1.y : wq == 2 ? q1.z : q1.w;
+#pragma unroll 2   // two samples' A rows (64 SGPRs) in flight
+    for (int c = 0; c < 16; ++c) {
+      const uint64_t k = kb * 64 + 16 * wq + c;
+      const uint32_t c0 = (x0 >> (2 * c)) & 3u;
+      const uint32_t c1 = (x1 >> (2 * c)) & 3u;
       double miss0 = 0.0, miss1 = 0.0;
       if constexpr (FILL) {
         if (c0 == 1u || c1 == 1u) {
           double f0 = 0.0, f1 = 0.0;
 #pragma unroll
           for (int r = 0; r < KR; ++r) {
-            f0 += Ws[kk][r] * h0[r];
-            f1 += Ws[kk][r] * h1[r];
+            const double wr = Ws[16 * wq + c][r];
+            f0 += wr * h0[r];
+            f1 += wr * h1[r];
           }
           miss0 = f0 - m0;
           miss1 = f1 - m1;
         }
       }
-      double v0 = (c0 & 2u) ? ((c0 & 1u) ? l03 : l02) : ((c0 & 1u) ? miss0 : l00);
-      double v1 = (c1 & 2u) ? ((c1 & 1u) ? l13 : l12) : ((c1 & 1u) ? miss1 : l10);
+      const double v0 = (c0 & 2u) ? ((c0 & 1u) ? l03 : l02) : ((c0 & 1u) ? miss0 : l00);
+      const double v1 = (c1 & 2u) ? ((c1 & 1u) ? l13 : l12) : ((c1 & 1u) ? miss1 : l10);
+      const double* ak = Acol + k * lda;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        double a = As[kk][half * 16 + r];
+        const double a = ak[r];
         acc0[r] += a * v0;
         acc1[r] += a * v1;
       }
+    }
     }
   }
   const double s0 = (outscale && ok0) ? outscale[j0] : 1.0, s1 = (outscale && ok1) ? outscale[j1] : 1.0;
