@@ -150,47 +150,48 @@ void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t
 // thread per row; S1 = sum of the fills, S2 = sum of their squares, and with
 // Wp the sum of the Wp rows over the missing calls.
 // ---------------------------------------------------------------------------
+// One thread per SNP row; the W (and Wp) rows of a sample are the same for
+// every thread, so they come in by scalar loads (the rank K is a template
+// parameter), and the fill's sums are taken on every call with the
+// observed ones adding zero: no per-call branch, and the sums keep the
+// per-entry order.
+template <int K>
 __global__ void __launch_bounds__(256) k_fill_stats(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t rows,
                                                     FillSpec fs, const double* __restrict__ Wp,
                                                     double* __restrict__ S1, double* __restrict__ S2,
                                                     double* __restrict__ SWp) {
-  __shared__ double sW[64][kNmfMaxK + 1];
-  __shared__ double sP[64][kNmfMaxK + 1];
   const int t = threadIdx.x;
   const uint64_t j = (uint64_t)blockIdx.x * 256 + t;
   const bool ok = j < rows;
-  const int K = fs.K;
-  double h[kNmfMaxK], sp[kNmfMaxK];
+  double h[K], sp[K];
 #pragma unroll
-  for (int r = 0; r < kNmfMaxK; ++r) {
-    h[r] = (ok && r < K) ? fs.Ht[j * K + r] : 0.0;
+  for (int r = 0; r < K; ++r) {
+    h[r] = ok ? fs.Ht[j * K + r] : 0.0;
     sp[r] = 0.0;
   }
   double s1 = 0.0, s2 = 0.0;
   const uint4* row = reinterpret_cast<const uint4*>(packed + (ok ? j : 0) * ld);
-  const uint64_t nkb = cdiv(fs.n, 64);
+  const uint64_t n = fs.n, nkb = cdiv(n, 64);
   for (uint64_t kb = 0; kb < nkb; ++kb) {
-    __syncthreads();
-    for (int e = t; e < 64 * kNmfMaxK; e += 256) {
-      const int kk = e / kNmfMaxK, r = e % kNmfMaxK;
-      const uint64_t k = kb * 64 + kk;
-      const bool in = k < fs.n && r < K;
-      sW[kk][r] = in ? fs.W[k * K + r] : 0.0;
-      if (Wp) sP[kk][r] = in ? Wp[k * K + r] : 0.0;
-    }
     const uint4 q = ok ? row[kb] : make_uint4(0, 0, 0, 0);
-    __syncthreads();
-    const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
-    for (int kk = 0; kk < 64; ++kk) {
-      if (((wd[kk >> 4] >> (2 * (kk & 15))) & 3u) != 1u) continue;
-      double f = 0.0;
+#pragma unroll 1
+    for (int wq = 0; wq < 4; ++wq) {
+      const uint32_t x = wq == 0 ? q.x : wq == 1 ? q.y : wq == 2 ? q.z : q.w;
+      const uint64_t kw = kb * 64 + 16 * wq;
+      const int cn = (int)std::min<uint64_t>(16, kw < n ? n - kw : 0);   // samples of this word
+#pragma unroll 2
+      for (int c = 0; c < cn; ++c) {
+        const uint64_t k = kw + c;
+        const bool miss = ((x >> (2 * c)) & 3u) == 1u;
+        double f = 0.0;
 #pragma unroll
-      for (int r = 0; r < kNmfMaxK; ++r) f += sW[kk][r] * h[r];
-      s1 += f;
-      s2 += f * f;
-      if (Wp)
+        for (int r = 0; r < K; ++r) f += fs.W[k * K + r] * h[r];
+        s1 += miss ? f : 0.0;
+        s2 += miss ? f * f : 0.0;
+        if (Wp)
 #pragma unroll
-        for (int r = 0; r < kNmfMaxK; ++r) sp[r] += sP[kk][r];
+          for (int r = 0; r < K; ++r) sp[r] += miss ? Wp[k * K + r] : 0.0;
+      }
     }
   }
   if (!ok) return;
@@ -228,7 +229,12 @@ void fill_moments(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t r
   double* S2 = ctx->buf<double>("fl_s2", rows);
   launch_stats(s, packed, ld, fs.n, rows, nmiss, sumg, nhom);
   if (rows) {
-    k_fill_stats<<<(unsigned)cdiv(rows, 256), 256, 0, s>>>(packed, ld, rows, fs, Wp, S1, S2, SWp);
+    switch (fs.K) {
+#define SLFMM_FST(R) \
+  case R: k_fill_stats<R><<<(unsigned)cdiv(rows, 256), 256, 0, s>>>(packed, ld, rows, fs, Wp, S1, S2, SWp); break;
+      SLFMM_NMF_RANKS(SLFMM_FST)
+#undef SLFMM_FST
+    }
     k_fill_moments<<<(unsigned)cdiv(rows, 256), 256, 0, s>>>(nmiss, sumg, nhom, S1, S2, fs.n, rows, norm, mu, sc, tss);
   }
   SLFMM_HIP(hipGetLastError());

@@ -476,39 +476,65 @@ __global__ void k_chunk_obs_base(const int32_t* __restrict__ nmiss, uint64_t n, 
   }
 }
 
+// One thread per SNP.  The first pass draws every observed call and records
+// the sample indices of the first kGwasList held-out calls in LDS; when a
+// SNP holds out no more than that, the other two passes walk the list
+// instead of drawing again (same calls, same order).
+constexpr int kGwasList = 16;
 __global__ void __launch_bounds__(256) k_gwas_cv(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t n,
                                                  uint64_t p, const double* __restrict__ W,
                                                  const double* __restrict__ Wp, int K,
                                                  const double* __restrict__ Hraw, uint64_t ldh,
                                                  const double* __restrict__ SWp, const int32_t* __restrict__ nmiss,
                                                  const int32_t* __restrict__ sumg, const uint64_t* __restrict__ base,
-                                                 uint64_t seed, uint64_t chunk, double cv_rate,
+                                                 uint64_t seed, uint64_t chunk, uint64_t cv_thresh,
                                                  double* __restrict__ part) {
   __shared__ double r0[256], r1[256];
   __shared__ unsigned long long r2[256];
+  __shared__ uint32_t held[kGwasList][256];
   const int t = threadIdx.x;
   const uint64_t j = (uint64_t)blockIdx.x * 256 + t;
   double e1 = 0.0, e2 = 0.0;
   unsigned long long cnt = 0;
   if (j < p) {
-    const uint64_t sd = seed + j / chunk;
-    const uint64_t b0 = base[j];
-    const uint4* row = reinterpret_cast<const uint4*>(packed + j * ld);
+    // draw rank of the stream seeded seed + j / chunk: base[j] + (observed calls before)
+    const uint64_t ctr0 = stream_counter(seed + j / chunk, base[j]);
+    const uint8_t* rowb = packed + j * ld;
+    const uint4* row = reinterpret_cast<const uint4*>(rowb);
     const uint64_t nkb = cdiv(n, 64);
-    // pass A: held-out count and dosage sum
-    uint64_t rank = 0, m = 0;
-    double sgm = 0.0;
-    for (uint64_t kb = 0; kb < nkb; ++kb) {
-      const uint4 q = row[kb];
-      const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
-      const int lim = (int)std::min<uint64_t>(64, n - kb * 64);
-      for (int kk = 0; kk < lim; ++kk) {
-        const uint32_t code = code_of(wd[kk >> 4], kk & 15);
-        if (code == 1u) continue;
-        if (u01(sd, b0 + rank) < cv_rate) { ++m; sgm += dosage_of(code); }
-        ++rank;
+    // fn(k, code) for every held-out call in sample order, by drawing again
+    auto redraw = [&](auto&& fn) {
+      uint64_t ctr = ctr0;
+      for (uint64_t kb = 0; kb < nkb; ++kb) {
+        const uint4 q = row[kb];
+        const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
+        const int lim = (int)std::min<uint64_t>(64, n - kb * 64);
+        for (int kk = 0; kk < lim; ++kk) {
+          const uint32_t code = code_of(wd[kk >> 4], kk & 15);
+          if (code == 1u) continue;
+          if ((mix_counter(ctr) >> 11) < cv_thresh) fn(kb * 64 + kk, code);
+          ctr += kStreamStep;
+        }
       }
-    }
+    };
+    // pass A: held-out count and dosage sum, the first kGwasList indices kept
+    uint64_t m = 0;
+    double sgm = 0.0;
+    redraw([&](uint64_t k, uint32_t code) {
+      if (m < kGwasList) held[m][t] = (uint32_t)k;
+      ++m;
+      sgm += dosage_of(code);
+    });
+    auto each_held = [&](auto&& fn) {
+      if (m <= kGwasList) {
+        for (uint64_t i = 0; i < m; ++i) {
+          const uint32_t k = held[i][t];
+          fn((uint64_t)k, (uint32_t)(rowb[k >> 2] >> (2 * (k & 3))) & 3u);
+        }
+      } else {
+        redraw(fn);
+      }
+    };
     if (m > 0) {
       const int64_t nobs = (int64_t)n - nmiss[j];
       const double mu = nobs > 0 ? (double)sumg[j] / (double)nobs : 0.0;
@@ -517,44 +543,20 @@ __global__ void __launch_bounds__(256) k_gwas_cv(const uint8_t* __restrict__ pac
       double h[kNmfMaxK];
       for (int r = 0; r < K; ++r) h[r] = Hraw[j * ldh + r] + (mum - mu) * SWp[j * K + r];
       // pass B: corrections of the held-out calls
-      rank = 0;
-      for (uint64_t kb = 0; kb < nkb; ++kb) {
-        const uint4 q = row[kb];
-        const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
-        const int lim = (int)std::min<uint64_t>(64, n - kb * 64);
-        for (int kk = 0; kk < lim; ++kk) {
-          const uint32_t code = code_of(wd[kk >> 4], kk & 15);
-          if (code == 1u) continue;
-          if (u01(sd, b0 + rank) < cv_rate) {
-            const uint64_t k = kb * 64 + kk;
-            const double dlt = mum - dosage_of(code);
-            for (int r = 0; r < K; ++r) h[r] += Wp[k * K + r] * dlt;
-          }
-          ++rank;
-        }
-      }
+      each_held([&](uint64_t k, uint32_t code) {
+        const double dlt = mum - dosage_of(code);
+        for (int r = 0; r < K; ++r) h[r] += Wp[k * K + r] * dlt;
+      });
       for (int r = 0; r < K; ++r) h[r] = h[r] > 0.0 ? h[r] : 0.0;
       // pass C: errors of the held-out calls
-      rank = 0;
-      for (uint64_t kb = 0; kb < nkb; ++kb) {
-        const uint4 q = row[kb];
-        const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
-        const int lim = (int)std::min<uint64_t>(64, n - kb * 64);
-        for (int kk = 0; kk < lim; ++kk) {
-          const uint32_t code = code_of(wd[kk >> 4], kk & 15);
-          if (code == 1u) continue;
-          if (u01(sd, b0 + rank) < cv_rate) {
-            const uint64_t k = kb * 64 + kk;
-            double pred = 0.0;
-            for (int r = 0; r < K; ++r) pred += W[k * K + r] * h[r];
-            const double g = dosage_of(code);
-            e1 += fabs(g - pred);
-            e2 += fabs(g - mum);
-            ++cnt;
-          }
-          ++rank;
-        }
-      }
+      each_held([&](uint64_t k, uint32_t code) {
+        double pred = 0.0;
+        for (int r = 0; r < K; ++r) pred += W[k * K + r] * h[r];
+        const double g = dosage_of(code);
+        e1 += fabs(g - pred);
+        e2 += fabs(g - mum);
+        ++cnt;
+      });
     }
   }
   r0[t] = e1; r1[t] = e2; r2[t] = cnt;
@@ -584,8 +586,8 @@ void gwas_cv(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uin
   k_chunk_obs_base<<<(unsigned)cdiv(nq, 64), 64, 0, s>>>(nmiss, n, p, chunk, base);
   const unsigned nblk = (unsigned)cdiv(p, 256);
   double* part = ctx->buf<double>("gcv_part", (uint64_t)nblk * 3);
-  k_gwas_cv<<<nblk, 256, 0, s>>>(packed, ld, n, p, W, Wp, K, Hraw, ldh, SWp, nmiss, sumg, base, seed, chunk, cv_rate,
-                                 part);
+  k_gwas_cv<<<nblk, 256, 0, s>>>(packed, ld, n, p, W, Wp, K, Hraw, ldh, SWp, nmiss, sumg, base, seed, chunk,
+                                 u01_threshold(cv_rate), part);
   k_nmf_cv_final<<<1, 256, 0, s>>>(part, (int)nblk, out3);
   SLFMM_HIP(hipGetLastError());
 }
