@@ -22,11 +22,12 @@ namespace slfmm {
 
 
 // ---------------------------------------------------------------------------
-// FP64 functional kernel:  F[j][32*rb + r] = sum_k A[k][32*rb + r] * v_jk
+// FP64 functional kernel:  F[j][2RW*rb + r] = sum_k A[k][2RW*rb + r] * v_jk
 // with v = g - mu_j on observed calls and 0 on missing calls (mean
-// imputation after centring, src/bed.rs:339-343).  Tile: 256 SNPs x 32 rows;
-// wave w -> rows 16*(w&1)..+16 of the row block, SNP group w>>1; lane ->
-// 2 SNPs.  The 16 A values of a sample are the same for every lane of the
+// imputation after centring, src/bed.rs:339-343).  Tile: 256 SNPs x 2 RW
+// rows; wave w -> rows RW*(w&1)..+RW of the row block, SNP group w>>1;
+// lane -> 2 SNPs.  RW (4, 8, 12 or 16) follows the operand's width, so
+// narrow operands do not pay for padding rows.  The 16 A values of a sample are the same for every lane of the
 // wave, so they come in by scalar loads (two samples' rows in flight) and
 // feed the FMAs from SGPRs: no LDS staging and no barrier (the LDS
 // broadcast of one value per two FMAs bounded the earlier form).
@@ -35,7 +36,7 @@ namespace slfmm {
 // imputation after centring, rank FK = fs.K): the 64 W rows of the block are
 // staged in LDS (read only where a call is missing), and each lane keeps the
 // FK loadings of its two SNPs in registers.
-template <int FK>
+template <int FK, int RW>
 __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t p,
                                                   uint64_t npad, const double* __restrict__ A, uint64_t lda,
                                                   const double* __restrict__ mu, double* __restrict__ F,
@@ -61,12 +62,12 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
       h1[r] = ok1 ? fs.Ht[j1 * KR + r] : 0.0;
     }
   }
-  double acc0[16], acc1[16];
+  double acc0[RW], acc1[RW];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) { acc0[r] = 0.0; acc1[r] = 0.0; }
+  for (int r = 0; r < RW; ++r) { acc0[r] = 0.0; acc1[r] = 0.0; }
   const uint4* row0 = reinterpret_cast<const uint4*>(packed + (ok0 ? j0 : 0) * ld);
   const uint4* row1 = reinterpret_cast<const uint4*>(packed + (ok1 ? j1 : 0) * ld);
-  const double* Acol = A + rb * 32 + half * 16;
+  const double* Acol = A + rb * 2 * RW + half * RW;
   const uint64_t nkb = npad / 64;
   for (uint64_t kb = 0; kb < nkb; ++kb) {
     const uint4 q0 = ok0 ? row0[kb] : make_uint4(0, 0, 0, 0);
@@ -107,7 +108,7 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
       const double v1 = (c1 & 2u) ? ((c1 & 1u) ? l13 : l12) : ((c1 & 1u) ? miss1 : l10);
       const double* ak = Acol + k * lda;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
+      for (int r = 0; r < RW; ++r) {
         const double a = ak[r];
         acc0[r] += a * v0;
         acc1[r] += a * v1;
@@ -118,30 +119,46 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
   const double s0 = (outscale && ok0) ? outscale[j0] : 1.0, s1 = (outscale && ok1) ? outscale[j1] : 1.0;
   if (ok0)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) F[j0 * ldF + rb * 32 + half * 16 + r] = acc0[r] * s0;
+    for (int r = 0; r < RW; ++r) F[j0 * ldF + rb * 2 * RW + half * RW + r] = acc0[r] * s0;
   if (ok1)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) F[j1 * ldF + rb * 32 + half * 16 + r] = acc1[r] * s1;
+    for (int r = 0; r < RW; ++r) F[j1 * ldF + rb * 2 * RW + half * RW + r] = acc1[r] * s1;
 }
 
 // Host launcher shared with the RSVD projection (Z = Y^T B) and the NMF
 // functionals; `fill` (optional) selects NMF imputation of missing calls.
-void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t p, uint64_t npad,
-                     const double* A, uint64_t lda, int nrb, const double* mu, double* F, uint64_t ldF,
-                     const double* outscale, const FillSpec* fill) {
-  if (p == 0) return;
-  dim3 grid((unsigned)cdiv(p, 256), (unsigned)nrb);
+// Columns [0, m) of F are written (A and F hold at least rup(m, 32)).
+template <int RW>
+static void launch_func_rw(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t p, uint64_t npad,
+                           const double* A, uint64_t lda, int m, const double* mu, double* F, uint64_t ldF,
+                           const double* outscale, const FillSpec* fill) {
+  const dim3 grid((unsigned)cdiv(p, 256), (unsigned)cdiv(m, 2 * RW));
   if (fill && fill->W) {
     SLFMM_CHECK(fill->K >= 1 && fill->K <= kNmfMaxK, SLFMM_E_ARG, "NMF rank must be 1..16");
     switch (fill->K) {
-#define SLFMM_FUNC(R) \
-  case R: k_func_f64<R><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, *fill); break;
+#define SLFMM_FUNC(R)                                                                                         \
+  case R:                                                                                                     \
+    k_func_f64<R, RW><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, *fill);         \
+    break;
       SLFMM_NMF_RANKS(SLFMM_FUNC)
 #undef SLFMM_FUNC
     }
   } else {
-    k_func_f64<0><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, FillSpec{});
+    k_func_f64<0, RW><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, FillSpec{});
   }
+}
+
+void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t p, uint64_t npad,
+                     const double* A, uint64_t lda, int m, const double* mu, double* F, uint64_t ldF,
+                     const double* outscale, const FillSpec* fill) {
+  if (p == 0) return;
+  SLFMM_CHECK(m >= 1 && lda >= rup((uint64_t)m, (uint64_t)kRowBlk) && ldF >= (uint64_t)m, SLFMM_E_ARG,
+              "FP64 functionals: operand or output stride below the operand width");
+  // row blocks of 2 RW: one block up to 24 rows, blocks of 32 beyond (within rup(m, 32))
+  if (m <= 8) launch_func_rw<4>(s, packed, ld, p, npad, A, lda, m, mu, F, ldF, outscale, fill);
+  else if (m <= 16) launch_func_rw<8>(s, packed, ld, p, npad, A, lda, m, mu, F, ldF, outscale, fill);
+  else if (m <= 24) launch_func_rw<12>(s, packed, ld, p, npad, A, lda, m, mu, F, ldF, outscale, fill);
+  else launch_func_rw<16>(s, packed, ld, p, npad, A, lda, m, mu, F, ldF, outscale, fill);
   SLFMM_HIP(hipGetLastError());
 }
 
@@ -735,7 +752,7 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
     launch_mean_scale(s, nmiss + j0, sumg + j0, n, v.p, cfg.norm, mu + j0, sc + j0);
     {
       slfmm_timed tm(ctx, "assoc_stream");
-      launch_func_f64(s, v.d, v.ld, v.p, npad, A, lda, nrb, mu + j0, F, lda, nullptr, nullptr);
+      launch_func_f64(s, v.d, v.ld, v.p, npad, A, lda, (int)m, mu + j0, F, lda, nullptr, nullptr);
     }
     epilogue(e, F);
   };
@@ -807,7 +824,7 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
     double* F = ctx->buf<double>("as_F", p * lda);
     {
       slfmm_timed tm(ctx, "assoc_stream");
-      launch_func_f64(s, g->d, g->ld, p, npad, A, lda, nrb, fmu, F, lda, nullptr, &fs);
+      launch_func_f64(s, g->d, g->ld, p, npad, A, lda, (int)m, fmu, F, lda, nullptr, &fs);
     }
     epilogue(e, F);
   }

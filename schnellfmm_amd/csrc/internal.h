@@ -294,10 +294,11 @@ struct FillSpec {
 // ---- assoc.hip --------------------------------------------------------------
 struct AssocPlan;   // defined in assoc.hip
 // FP64 functionals F[j][c] = sum_k A[k][c] v_kj of the centred rows (v = g - mu,
-// missing calls 0, or W_k . H_j - mu with a fill); A is npad x lda (lda a
-// multiple of 32, nrb = lda / 32 row blocks); F rows scaled by outscale.
+// missing calls 0, or W_k . H_j - mu with a fill) for the columns c < m; A is
+// npad x lda (lda >= rup(m, 32), zero past column m); F rows scaled by
+// outscale.
 void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t p, uint64_t npad,
-                     const double* A, uint64_t lda, int nrb, const double* mu, double* F, uint64_t ldF,
+                     const double* A, uint64_t lda, int m, const double* mu, double* F, uint64_t ldF,
                      const double* outscale, const FillSpec* fill = nullptr);
 void pad_operand(hipStream_t s, const double* Q, uint64_t ldq, uint64_t n, int m, double* A, uint64_t lda,
                  uint64_t npad);

@@ -315,7 +315,7 @@ void raw_functionals(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, 
   SLFMM_CHECK(ldout >= lda, SLFMM_E_ARG, "raw_functionals: output stride");
   double* A = ctx->buf<double>("nmf_A", npad * lda);
   pad_operand(s, Op, ldo, n, (int)m, A, lda, npad);
-  launch_func_f64(s, Y, ld, pe, npad, A, lda, nrb, mu, out, ldout, nullptr);
+  launch_func_f64(s, Y, ld, pe, npad, A, lda, (int)m, mu, out, ldout, nullptr);
   double* cs = ctx->buf<double>("nmf_cs", m);
   colreduce<1, kColSum>(ctx, FnColSum{Op, ldo}, GStore{cs}, n, (int)m);
   k_add_mean_term<<<(unsigned)cdiv(pe * m, 256), 256, 0, s>>>(out, ldout, pe, (int)m, mu, cs);

@@ -319,7 +319,7 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
       rsvd_project_i8(ctx, Y, g->ld, n, pe, B, l, cfg.norm, Z, lda);
     } else {
       k_pad_rows<<<(unsigned)cdiv(npad * lda, 256), 256, 0, s>>>(B, l, n, (int)l, A, lda, npad);
-      launch_func_f64(s, Y, g->ld, pe, npad, A, lda, nrb, mu, Z, lda, sc, fill);
+      launch_func_f64(s, Y, g->ld, pe, npad, A, lda, (int)l, mu, Z, lda, sc, fill);
     }
   };
   // The bases between power iterations carry only their column span forward,
