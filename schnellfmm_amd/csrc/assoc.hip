@@ -22,12 +22,14 @@ namespace slfmm {
 
 
 // ---------------------------------------------------------------------------
-// FP64 functional kernel:  F[j][2RW*rb + r] = sum_k A[k][2RW*rb + r] * v_jk
+// FP64 functional kernel:  F[j][NH RW rb + r] = sum_k A[k][NH RW rb + r] v_jk
 // with v = g - mu_j on observed calls and 0 on missing calls (mean
-// imputation after centring, src/bed.rs:339-343).  Tile: 256 SNPs x 2 RW
-// rows; wave w -> rows RW*(w&1)..+RW of the row block, SNP group w>>1;
-// lane -> 2 SNPs.  RW (4, 8, 12 or 16) follows the operand's width, so
-// narrow operands do not pay for padding rows.  The 16 A values of a sample are the same for every lane of the
+// imputation after centring, src/bed.rs:339-343).  Tile: (4 / NH) x 128
+// SNPs x NH RW rows; with NH = 2, wave w -> rows RW (w & 1)..+RW of the row
+// block and SNP group w >> 1; with NH = 1 every wave takes all RW rows and
+// its own SNP group; lane -> 2 SNPs.  (RW, NH) follows the operand's width
+// ((8, 1), (8, 2), (12, 2), then blocks of (16, 2)), so narrow operands pay
+// neither for padding rows nor for decoding codes for a few FMAs.  The 16 A values of a sample are the same for every lane of the
 // wave, so they come in by scalar loads (two samples' rows in flight) and
 // feed the FMAs from SGPRs: no LDS staging and no barrier (the LDS
 // broadcast of one value per two FMAs bounded the earlier form).
@@ -36,7 +38,7 @@ namespace slfmm {
 // imputation after centring, rank FK = fs.K): the 64 W rows of the block are
 // staged in LDS (read only where a call is missing), and each lane keeps the
 // FK loadings of its two SNPs in registers.
-template <int FK, int RW>
+template <int FK, int RW, int NH>
 __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t p,
                                                   uint64_t npad, const double* __restrict__ A, uint64_t lda,
                                                   const double* __restrict__ mu, double* __restrict__ F,
@@ -46,9 +48,9 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
   __shared__ double Ws[FILL ? 64 : 1][KR + 1];
   const int t = threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);   // wave-uniform: A rows by scalar loads
-  const int half = w & 1, grp = w >> 1;
+  const int half = NH == 2 ? (w & 1) : 0, grp = NH == 2 ? (w >> 1) : w;
   const int rb = blockIdx.y;
-  const uint64_t j0 = (uint64_t)blockIdx.x * 256 + grp * 128 + lane;
+  const uint64_t j0 = (uint64_t)blockIdx.x * (512 / NH) + grp * 128 + lane;
   const uint64_t j1 = j0 + 64;
   const bool ok0 = j0 < p, ok1 = j1 < p;
   const double m0 = ok0 ? mu[j0] : 0.0, m1 = ok1 ? mu[j1] : 0.0;
@@ -67,7 +69,7 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
   for (int r = 0; r < RW; ++r) { acc0[r] = 0.0; acc1[r] = 0.0; }
   const uint4* row0 = reinterpret_cast<const uint4*>(packed + (ok0 ? j0 : 0) * ld);
   const uint4* row1 = reinterpret_cast<const uint4*>(packed + (ok1 ? j1 : 0) * ld);
-  const double* Acol = A + rb * 2 * RW + half * RW;
+  const double* Acol = A + rb * NH * RW + half * RW;
   const uint64_t nkb = npad / 64;
   for (uint64_t kb = 0; kb < nkb; ++kb) {
     const uint4 q0 = ok0 ? row0[kb] : make_uint4(0, 0, 0, 0);
@@ -119,32 +121,32 @@ __global__ void __launch_bounds__(256) k_func_f64(const uint8_t* __restrict__ pa
   const double s0 = (outscale && ok0) ? outscale[j0] : 1.0, s1 = (outscale && ok1) ? outscale[j1] : 1.0;
   if (ok0)
 #pragma unroll
-    for (int r = 0; r < RW; ++r) F[j0 * ldF + rb * 2 * RW + half * RW + r] = acc0[r] * s0;
+    for (int r = 0; r < RW; ++r) F[j0 * ldF + rb * NH * RW + half * RW + r] = acc0[r] * s0;
   if (ok1)
 #pragma unroll
-    for (int r = 0; r < RW; ++r) F[j1 * ldF + rb * 2 * RW + half * RW + r] = acc1[r] * s1;
+    for (int r = 0; r < RW; ++r) F[j1 * ldF + rb * NH * RW + half * RW + r] = acc1[r] * s1;
 }
 
 // Host launcher shared with the RSVD projection (Z = Y^T B) and the NMF
 // functionals; `fill` (optional) selects NMF imputation of missing calls.
 // Columns [0, m) of F are written (A and F hold at least rup(m, 32)).
-template <int RW>
+template <int RW, int NH>
 static void launch_func_rw(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t p, uint64_t npad,
                            const double* A, uint64_t lda, int m, const double* mu, double* F, uint64_t ldF,
                            const double* outscale, const FillSpec* fill) {
-  const dim3 grid((unsigned)cdiv(p, 256), (unsigned)cdiv(m, 2 * RW));
+  const dim3 grid((unsigned)cdiv(p, 512 / NH), (unsigned)cdiv(m, NH * RW));
   if (fill && fill->W) {
     SLFMM_CHECK(fill->K >= 1 && fill->K <= kNmfMaxK, SLFMM_E_ARG, "NMF rank must be 1..16");
     switch (fill->K) {
 #define SLFMM_FUNC(R)                                                                                         \
   case R:                                                                                                     \
-    k_func_f64<R, RW><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, *fill);         \
+    k_func_f64<R, RW, NH><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, *fill);     \
     break;
       SLFMM_NMF_RANKS(SLFMM_FUNC)
 #undef SLFMM_FUNC
     }
   } else {
-    k_func_f64<0, RW><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, FillSpec{});
+    k_func_f64<0, RW, NH><<<grid, 256, 0, s>>>(packed, ld, p, npad, A, lda, mu, F, ldF, outscale, FillSpec{});
   }
 }
 
@@ -154,11 +156,13 @@ void launch_func_f64(hipStream_t s, const uint8_t* packed, uint64_t ld, uint64_t
   if (p == 0) return;
   SLFMM_CHECK(m >= 1 && lda >= rup((uint64_t)m, (uint64_t)kRowBlk) && ldF >= (uint64_t)m, SLFMM_E_ARG,
               "FP64 functionals: operand or output stride below the operand width");
-  // row blocks of 2 RW: one block up to 24 rows, blocks of 32 beyond (within rup(m, 32))
-  if (m <= 8) launch_func_rw<4>(s, packed, ld, p, npad, A, lda, m, mu, F, ldF, outscale, fill);
-  else if (m <= 16) launch_func_rw<8>(s, packed, ld, p, npad, A, lda, m, mu, F, ldF, outscale, fill);
-  else if (m <= 24) launch_func_rw<12>(s, packed, ld, p, npad, A, lda, m, mu, F, ldF, outscale, fill);
-  else launch_func_rw<16>(s, packed, ld, p, npad, A, lda, m, mu, F, ldF, outscale, fill);
+  // row blocks of NH RW: one block up to 24 rows, blocks of 32 beyond (within rup(m, 32))
+  // measured at n = 5000, p = 2M: m = 8 without a fill 8.4 -> 6.4 ms from
+  // (4, 2) to (8, 1); m = 10 with a fill 19.3 ms at (8, 2), 21.2 at (16, 1)
+  if (m <= 8) launch_func_rw<8, 1>(s, packed, ld, p, npad, A, lda, m, mu, F, ldF, outscale, fill);
+  else if (m <= 16) launch_func_rw<8, 2>(s, packed, ld, p, npad, A, lda, m, mu, F, ldF, outscale, fill);
+  else if (m <= 24) launch_func_rw<12, 2>(s, packed, ld, p, npad, A, lda, m, mu, F, ldF, outscale, fill);
+  else launch_func_rw<16, 2>(s, packed, ld, p, npad, A, lda, m, mu, F, ldF, outscale, fill);
   SLFMM_HIP(hipGetLastError());
 }
 
