@@ -194,8 +194,12 @@ def test_association_singular_design_falls_back(S, capfd):
     assert np.allclose(got.beta, ref.beta, rtol=1e-6, atol=1e-9)
 
 
-def test_rsvd_fp64_vs_oracle_injected_sketch(S):
-    n, p, K, os_ = 250, 1500, 4, 10
+@pytest.mark.parametrize("n,K,os_", [
+    (250, 4, 10),
+    (700, 8, 10),   # l = 18: two 16-column blocks of the forward pass, a partial 512-sample tile
+])
+def test_rsvd_fp64_vs_oracle_injected_sketch(S, n, K, os_):
+    p = 1500
     pan, packed, bps = _panel(n=n, p=p, seed=11, d=2)
     xs = O.center_covariates(pan.x)
     pre = O.precompute(xs, 1e-5)
