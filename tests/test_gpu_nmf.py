@@ -149,6 +149,26 @@ def test_nmf_int8_vs_oracle(S, n, p, k, n_iter, rate, chunk, miss):
     assert np.isclose(got.cv_mae_mean_impute, ref.cv_mae_mean_impute, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("k", [2, 4, 7])
+def test_nmf_int8_remaining_ranks_vs_oracle(S, k):
+    """The int8 passes at the ranks the shapes above leave out (their tile
+    plans differ: forward 1+1, 1+2 and 2+4 tiles of 16, backward 1+1, 1+1
+    and 1+2 tiles of 32), same tolerances."""
+    n, p, iters, chunk = 333, 1200, 3, 500
+    print(f"[nmf int8] n={n} p={p} k={k} iters={iters}", flush=True)
+    packed, bps = _panel(n, p, seed=100 + k, miss=0.08)
+    seed, cv_rate, eps = 11, 0.02, 1e-16
+    omega = synth.normal_matrix(seed, n, k + 10)
+    ref = O.nmf_impute_estimation(packed, bps, n, np.arange(p), k, iters, eps, cv_rate, chunk, seed, omega,
+                                  synth.uniform_stream)
+    g = S.BedFile.from_packed(packed, n, p)
+    cfg = S.NmfParams(k=k, n_iter=iters, eps=eps, cv_rate=cv_rate, chunk_size=chunk, seed=seed)
+    got = S.nmf_impute_estimation(g, None, cfg, omega=omega, int8=True)
+    assert _close(got.w, ref.w, 1e-5) and _close(got.h_full, ref.h_full, 1e-5)
+    assert got.total_mask_count == ref.total_mask_count > 0
+    assert np.allclose(got.cv_mae_per_iter, ref.cv_mae_per_iter, rtol=1e-5, atol=0)
+
+
 def test_nmf_int8_rank_above_8_runs_fp64(S):
     """Ranks above 8 take the FP64 backward pass: the int8 flag changes nothing."""
     packed, bps = _panel(200, 800, seed=9, miss=0.05)
