@@ -299,6 +299,10 @@ def nmf_workload(args):
     dt_fit8, fit8 = timed(lambda: S.fit_lfmm2_nmf(geno, None, geno, x, lc, cfg, nmf_int8=True), max(steps // 2, 1),
                           min(warmup, 1))
     log(f"[bench] nmf fit, NMF passes on the int8 matrix cores: {dt_fit8 * 1e3:.1f} ms")
+    lc8 = S.Lfmm2Config(k=8, norm=S.EIGENSTRAT, chunk_size=10_000, n_power_iter=2, seed=42, precision=S.PREC_I8)
+    dt_fitm, fitm = timed(lambda: S.fit_lfmm2_nmf(geno, None, geno, x, lc8, cfg, nmf_int8=True), max(steps // 2, 1),
+                          min(warmup, 1))
+    log(f"[bench] nmf fit in mixed precision (int8 NMF passes and filled functionals): {dt_fitm * 1e3:.1f} ms")
     bytes_pass = p * math.ceil(n / 4)
     passes = 2 + 3 * cfg.n_iter
     line = {
@@ -326,6 +330,9 @@ def nmf_workload(args):
         "fit_int8_nmf": {"ms": dt_fit8 * 1e3, "gif": float(fit8.gif),
                          "max_abs_dlog10p_vs_fp64": float(np.max(np.abs(np.log10(fit8.pvalue)
                                                                          - np.log10(fit.pvalue))))},
+        "fit_mixed": {"ms": dt_fitm * 1e3, "gif": float(fitm.gif),
+                      "max_abs_dlog10p_vs_fp64": float(np.max(np.abs(np.log10(fitm.pvalue)
+                                                                      - np.log10(fit.pvalue))))},
         "gwas_cv": [float(v) for v in fit.gwas_cv],
         "cpu_baseline": None,
     }

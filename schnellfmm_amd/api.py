@@ -489,10 +489,12 @@ def fit_lfmm2_batch(problems, n_streams: Optional[int] = None, with_hist: bool =
 def fit_lfmm2_nmf(y_est: BedFile, est_subset, y_full: BedFile, x, config: Lfmm2Config, nmf: NmfParams,
                   b0=None, nmf_omega=None, gwas_cv: bool = True, with_hist: bool = False,
                   nmf_int8: bool = False) -> TestResults:
-    """Full pipeline with NMF imputation, src/lib.rs:162-263 (FP64).
-    nmf_int8: the NMF estimation's update passes run on the int8 matrix
-    cores (nmf k <= 8; SLFMM_NMF_INT8); the filled RSVD and association
-    passes stay FP64."""
+    """Full pipeline with NMF imputation, src/lib.rs:162-263.  With
+    config.precision PREC_I8 the filled sketch/backward passes and the filled
+    association functionals run on the int8 matrix cores (nmf k <= 8); the
+    forward passes, moments and epilogue stay FP64.  nmf_int8: the NMF
+    estimation's update passes run on the int8 matrix cores too (nmf k <= 8;
+    SLFMM_NMF_INT8)."""
     ctx = y_full.ctx
     x = _f64(x)
     idx = None if est_subset is None or (isinstance(est_subset, str) and est_subset == "all") \

@@ -987,7 +987,6 @@ int slfmm_fit_lfmm2_nmf(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* su
     slfmm_config c = *cfg;
     c.impute = SLFMM_IMPUTE_MEAN;   // the fills are passed explicitly below
     check_cfg(&c);
-    SLFMM_CHECK(c.precision == SLFMM_PREC_FP64, SLFMM_E_ARG, "NMF imputation runs on the FP64 path (use SLFMM_PREC_FP64)");
     SLFMM_CHECK(nmf->k >= 1 && nmf->k <= 16, SLFMM_E_ARG, "NMF rank k must be 1..16");
     hipStream_t s = ctx->stream;
     const uint64_t n = g->n, K = c.k, l = K + c.oversampling, kn = nmf->k;

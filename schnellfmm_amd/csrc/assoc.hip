@@ -708,8 +708,11 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
   AssocPlan pl = make_plan(ctx, n, d_xs, d, d_u, k, ridge);
   const uint64_t m = pl.m;
   const bool with_nmf = nmf && nmf->W;
-  // the int8 kernels carry at most 32 functionals (K + d); beyond that the FP64 kernels run
-  const bool i8 = cfg.precision == SLFMM_PREC_I8 && m <= 32;
+  // the int8 kernels carry at most 32 functionals (K + d); beyond that the FP64 kernels run.
+  // With an NMF fill the functionals come from the filled int8 pass where its
+  // plans reach (nmf_i8), and the counts, moments and epilogue stay FP64.
+  const bool i8 = cfg.precision == SLFMM_PREC_I8 && m <= 32 && !with_nmf;
+  const bool nmf_i8 = cfg.precision == SLFMM_PREC_I8 && with_nmf && fill_functionals_i8_supported((int)m, nmf->K);
   // per-SNP integer counts of the FP64 kernels (the int8 kernels count in registers)
   int32_t* nmiss = i8 ? nullptr : ctx->buf<int32_t>("as_nmiss", p);
   int32_t* sumg = i8 ? nullptr : ctx->buf<int32_t>("as_sumg", p);
@@ -719,8 +722,6 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
   ea.norm = cfg.norm; ea.df = (double)(n - 1 - d - k); ea.plan = pl.plan;
   ea.n_miss = nmiss; ea.sum_g = sumg; ea.n_hom = nhom;
   ea.beta = d_beta; ea.tstat = d_t; ea.r2 = d_r2;
-  SLFMM_CHECK(!(cfg.precision == SLFMM_PREC_I8 && with_nmf), SLFMM_E_ARG,
-              "NMF imputation runs on the FP64 path (use SLFMM_PREC_FP64)");
   SLFMM_CHECK(!(with_nmf && g->host), SLFMM_E_ARG, "NMF imputation needs a device-resident genotype panel");
   // FP64 operand: the basis padded to whole row blocks (shared by every row range)
   const uint64_t npad = rup(n, 64);
@@ -828,7 +829,8 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
     double* F = ctx->buf<double>("as_F", p * lda);
     {
       slfmm_timed tm(ctx, "assoc_stream");
-      launch_func_f64(s, g->d, g->ld, p, npad, A, lda, (int)m, fmu, F, lda, nullptr, &fs);
+      if (nmf_i8) fill_functionals_i8(ctx, g->d, g->ld, n, p, pl.Q, m, (int)m, nmf->W, Ht, K, fmu, nullptr, F, lda);
+      else launch_func_f64(s, g->d, g->ld, p, npad, A, lda, (int)m, fmu, F, lda, nullptr, &fs);
     }
     epilogue(e, F);
   }

@@ -366,6 +366,9 @@ struct SnpArgs {
   const double* nmf_sc;
   const double* WtW;
   double eps;
+  // MODE 3 (raw sums): int32 sum of packed row r for SNP j at raw[r * ldraw + j]
+  int32_t* raw;
+  uint64_t ldraw;
 };
 
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
@@ -718,6 +721,17 @@ __global__ void __launch_bounds__(64 * kSnpWaves, 2) k_snp_i8(SnpArgs a) {
   }
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
+  if constexpr (MODE == 3) {
+    // register e of tile rt holds row 32 rt + 8 (e >> 2) + 4 h + (e & 3) of SNP base + col
+    const uint64_t j = base + col;
+    if (j < a.p)
+#pragma unroll
+      for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+          a.raw[(uint64_t)(32 * rt + 8 * (e >> 2) + 4 * h + (e & 3)) * a.ldraw + j] = acc[0][rt][e];
+    return;
+  }
   if constexpr (MODE == 2) {
     double* ls = reinterpret_cast<double*>(A1);   // column scales [0, K + P), W^T W at [64, 64 + K^2)
     const int mq = NQ + NQ * (NQ + 1) / 2;
@@ -1198,6 +1212,120 @@ void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_
                                      FwdRsvdEpi<2>{csc, (int)l, part, ldp});
   k_fwd_reduce<<<(unsigned)cdiv(n * l, 256), 256, 0, s>>>(part, n, (int)l, ldp, (int)nchunk, vhat, W, ldw);
   SLFMM_HIP(hipGetLastError());
+}
+
+// ---- NMF-filled functionals on the int8 matrix cores -------------------------------
+// With the missing calls of SNP j filled by W_k . H_j (NMF imputation,
+// src/parallel.rs:252-321), the centred functional of operand column c is
+//   F_jc = sum_{k obs} A_kc g_kj + sum_r H_jr M_jcr - mu_j sum_k A_kc,
+//   M_jcr = sum_{k missing at j} A_kc W_kr,
+// so the dosage fragment meets A (3 int8 terms) and the missing-call
+// fragment the products A_c W_r (2 terms): one SNP-side pass (mode 3) per
+// group of columns writes the exact int32 sums of every packed row, and
+// k_fill_combine forms F in FP64 with the SNP's own H_j.  Operand of a group
+// of g columns (NmfMap with K -> g, P -> g K): [A_c | A_c W_r (c, r)].
+__global__ void k_fill_prod(const double* __restrict__ A, uint64_t lda, uint64_t n, int c0, int g,
+                            const double* __restrict__ W, int K, double* __restrict__ Qn) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int mq = g + g * K;
+  double* q = Qn + k * mq;
+  for (int c = 0; c < g; ++c) {
+    const double a = A[k * lda + c0 + c];
+    q[c] = a;
+    for (int r = 0; r < K; ++r) q[g + c * K + r] = a * W[k * K + r];
+  }
+}
+
+// F[j][c0 + c] (times outscale[j] if given) from the raw sums of one group;
+// one thread per (column, SNP), consecutive threads on consecutive SNPs.
+__global__ void k_fill_combine(const int32_t* __restrict__ raw, uint64_t ldraw, uint64_t p, NmfMap map, int c0,
+                               const double* __restrict__ sc, const double* __restrict__ Ht,
+                               const double* __restrict__ mu, const double* __restrict__ colsum,
+                               const double* __restrict__ outscale, double* __restrict__ F, uint64_t ldF) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = map.K, K = map.P / map.K;
+  if (e >= (uint64_t)g * p) return;
+  const int c = (int)(e / p);
+  const uint64_t j = e % p;
+  auto R = [&](int row) { return (double)raw[(uint64_t)row * ldraw + j]; };
+  double f = sc[c] * (R(3 * c) + R(3 * c + 1) / 254.0 + R(3 * c + 2) / 64516.0);
+  const int m0 = 32 * map.NT0;
+  for (int r = 0; r < K; ++r) {
+    const int q = c * K + r;
+    f += Ht[j * K + r] * (sc[g + q] * (R(m0 + 2 * q) + R(m0 + 2 * q + 1) / 254.0));
+  }
+  f -= mu[j] * colsum[c0 + c];
+  F[j * ldF + c0 + c] = outscale ? f * outscale[j] : f;
+}
+
+// Tile plans of the mode-3 pass: (NT0, NT1) with NT0 + NT1 <= 6.
+#define SLFMM_SNP_FILL(X) X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(2, 1) X(2, 2) X(2, 3) X(2, 4)
+
+// Largest column group of a mode-3 pass at fill rank K.
+static int fill_group(int m, int K) {
+  for (int g = m; g >= 1; --g) {
+    const int nt0 = (3 * g + 31) / 32, nt1 = (2 * g * K + 31) / 32;
+    if (nt0 <= 2 && nt0 + nt1 <= 6) return g;
+  }
+  return 0;
+}
+
+struct FnColA {
+  const double* A; uint64_t lda;
+  __device__ void operator()(uint64_t r, int c, double (&v)[1]) const { v[0] = A[r * lda + c]; }
+};
+
+bool fill_functionals_i8_supported(int m, int K) { return m >= 1 && K >= 1 && K <= kNmfMaxK8 && fill_group(m, K) >= 1; }
+
+void fill_functionals_i8(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t p, const double* A,
+                         uint64_t lda, int m, const double* W, const double* Ht, int K, const double* mu,
+                         const double* outscale, double* F, uint64_t ldF) {
+  SLFMM_CHECK(fill_functionals_i8_supported(m, K), SLFMM_E_ARG, "no int8 plan for these filled functionals");
+  SLFMM_CHECK(n <= 8000000, SLFMM_E_ARG, "too many samples for the int8 kernels");
+  if (p == 0) return;
+  hipStream_t s = ctx->stream;
+  const int gmax = fill_group(m, K);
+  double* colsum = ctx->buf<double>("fi8_colsum", m);
+  colreduce<1, kColSum>(ctx, FnColA{A, lda}, GStore1{colsum}, n, m);
+  const uint64_t nblk = cdiv(n, 256);
+  for (int c0 = 0; c0 < m; c0 += gmax) {
+    const int g = std::min(gmax, m - c0);
+    const NmfMap map{g, g * K, (3 * g + 31) / 32, (2 * g * K + 31) / 32};
+    const int mq = g + g * K, NT = map.NT0 + map.NT1;
+    double* Qn = ctx->buf<double>("fi8_q", n * mq);
+    k_fill_prod<<<(unsigned)cdiv(n, 256), 256, 0, s>>>(A, lda, n, c0, g, W, K, Qn);
+    double* sc = ctx->buf<double>("fi8_sc", mq);
+    for (int q0 = 0; q0 < mq; q0 += 64)   // column reductions take up to 64 columns
+      colabsmax_scale(ctx, Qn + q0, mq, n, std::min(64, mq - q0), nullptr, nullptr, sc + q0);
+    const uint64_t total = nblk * 8 * NT * 1024;
+    int8_t* pack = ctx->buf<int8_t>("fi8_pack", total);
+    k_pack_rows<<<(unsigned)std::min<uint64_t>(cdiv(total, 256), 16384), 256, 0, s>>>(Qn, mq, n, map, sc, NT, total,
+                                                                                    pack);
+    const uint64_t ldraw = rup(p, 64);
+    int32_t* raw = ctx->buf<int32_t>("fi8_raw", (uint64_t)32 * NT * ldraw);
+    SnpArgs a{};
+    a.packed = Y; a.ld = ld; a.p = p; a.n = n; a.nblk = (int)nblk;
+    a.rb = (uint32_t)std::min<uint64_t>(rup(cdiv(n, 4), 16), ld);
+    a.m = mq; a.S = 1; a.fill = 0; a.A = pack;
+    a.raw = raw; a.ldraw = ldraw;
+    const uint64_t rows = 32 * kSnpWaves;   // one tile per wave
+    SLFMM_CHECK((rows + 2 * 32) * a.ld < 0xF0000000ull && total < 0xFFFFFFFFull, SLFMM_E_ARG,
+                "panel rows too long for the int8 kernels");
+    const unsigned grid = (unsigned)cdiv(p, rows);
+    bool done = false;
+#define X(A0_, A1_)                                                                               \
+    if (!done && map.NT0 == A0_ && map.NT1 == A1_) {                                              \
+      k_snp_i8<1, A0_, A1_, 1, 3><<<grid, 64 * kSnpWaves, 0, s>>>(a);                             \
+      done = true;                                                                                \
+    }
+    SLFMM_SNP_FILL(X)
+#undef X
+    SLFMM_CHECK(done, SLFMM_E_ARG, "no int8 kernel for this filled-functional layout");
+    k_fill_combine<<<(unsigned)cdiv((uint64_t)g * p, 256), 256, 0, s>>>(raw, ldraw, p, map, c0, sc, Ht, mu, colsum,
+                                                                       outscale, F, ldF);
+    SLFMM_HIP(hipGetLastError());
+  }
 }
 
 // ---- NMF forward pass on the int8 matrix cores ---------------------------------

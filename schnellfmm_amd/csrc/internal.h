@@ -371,6 +371,15 @@ void rsvd_project_i8(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_
 void rsvd_accumulate_i8(slfmm_ctx* ctx, const uint8_t* packedT, uint64_t ldt, uint64_t n, uint64_t p,
                         const double* Qz, uint64_t ldq, uint64_t l, const double* mu, const double* sc,
                         double* W, uint64_t ldw);
+// assoc_i8.hip: centred functionals of NMF-filled rows on the int8 matrix
+// cores, F[j][c] = sum_k A[k][c] (y_kj - mu_j) for c < m with the missing
+// calls of SNP j filled by W_k . H_j (W n x K, Ht p x K), times outscale[j]
+// if given; A is n x m (row stride lda).  Supported when fill rank K <= 8 and
+// the column groups fit the mode-3 tile plans (fill_functionals_i8_supported).
+bool fill_functionals_i8_supported(int m, int K);
+void fill_functionals_i8(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t p, const double* A,
+                         uint64_t lda, int m, const double* W, const double* Ht, int K, const double* mu,
+                         const double* outscale, double* F, uint64_t ldF);
 // assoc_i8.hip: num_W (n x K) of the NMF forward pass (src/nmf.rs:131-147,
 // missing calls filled by W_k . H_j) from the sample-major copy YT (n x ldt
 // bytes) of the rows, Ht (pe x K) and W (K <= kNmfMaxK8).

@@ -280,8 +280,6 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   double* mu = ctx->buf<double>("rs_mu", pe);
   double* sc = ctx->buf<double>("rs_sc", pe);
   const bool filled = fill && fill->W;
-  SLFMM_CHECK(!(filled && cfg.precision == SLFMM_PREC_I8), SLFMM_E_ARG,
-              "NMF imputation runs on the FP64 path (use SLFMM_PREC_FP64)");
   if (filled) {   // NMF in-RAM fill of the estimation rows: moments of the filled rows
     fill_moments(ctx, Y, g->ld, pe, *fill, cfg.norm, nullptr, mu, sc, nullptr, nullptr);
   } else {
@@ -289,8 +287,12 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
     launch_mean_scale(s, nmiss, sumg, n, pe, cfg.norm, mu, sc);
   }
 
-  // the int8 sketch carries at most 32 columns; wider sketches run the FP64 kernels
-  const bool mixed = cfg.precision == SLFMM_PREC_I8 && l <= 32;
+  // the int8 sketch carries at most 32 columns; wider sketches run the FP64
+  // kernels.  With an NMF fill the sketch and backward passes take the
+  // filled int8 pass where its plans reach (fill_i8); the forward passes
+  // stay FP64.
+  const bool mixed = cfg.precision == SLFMM_PREC_I8 && l <= 32 && !filled;
+  const bool fill_i8 = cfg.precision == SLFMM_PREC_I8 && filled && fill_functionals_i8_supported((int)l, fill->K);
   uint8_t* YT = nullptr;
   uint64_t ldt = 0;
   if (mixed) {   // sample-major copy for the forward passes
@@ -317,6 +319,8 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
     slfmm_timed tm(ctx, "rsvd_project");
     if (mixed) {
       rsvd_project_i8(ctx, Y, g->ld, n, pe, B, l, cfg.norm, Z, lda);
+    } else if (fill_i8) {
+      fill_functionals_i8(ctx, Y, g->ld, n, pe, B, l, (int)l, fill->W, fill->Ht, fill->K, mu, sc, Z, lda);
     } else {
       k_pad_rows<<<(unsigned)cdiv(npad * lda, 256), 256, 0, s>>>(B, l, n, (int)l, A, lda, npad);
       launch_func_f64(s, Y, g->ld, pe, npad, A, lda, (int)l, mu, Z, lda, sc, fill);

@@ -107,12 +107,42 @@ def test_fit_lfmm2_nmf_vs_oracle(S, int8, cv_rate):
     assert np.max(np.abs(ref.t)) > 4   # the panel has real signal
 
 
-def test_nmf_rejects_mixed_precision(S):
-    n, p = 64, 40
-    pan = synth.balding_nichols(n, p, n_pops=2, miss_rate=0.05, d=1, n_causal=0, seed=3)
-    g = S.BedFile.from_packed(O.encode_bed(pan.g), n, p)
-    with pytest.raises(S.api.SlfmmError, match="FP64"):
-        S.fit_lfmm2_nmf(g, None, g, pan.x, S.Lfmm2Config(k=2, precision=S.PREC_I8), S.NmfParams(k=2, n_iter=1))
+@pytest.mark.parametrize("nmf_int8", [False, True])
+def test_fit_lfmm2_nmf_mixed_vs_oracle(S, nmf_int8):
+    """The NMF-imputed fit in mixed precision: the filled sketch/backward
+    passes and the filled association functionals on the int8 matrix cores
+    (A, or B, in 3 int8 terms against the dosages, the products A_c W_r in 2
+    against the missing-call indicator), counts, moments, forward passes and
+    epilogue in FP64; optionally the NMF update passes in int8 too.  Against
+    the FP64 oracle: U_hat angles < 1e-4 rad, |delta log10 p| <= 1e-3, GWAS-
+    phase CV counts exact and errors within 1e-5 relative (they use the FP64
+    H_j)."""
+    n, p, K, d = 300, 2000, 3, 2
+    pan = synth.balding_nichols(n, p, n_pops=3, miss_rate=0.10, d=d, n_causal=20, effect=1.0, seed=77)
+    packed = O.encode_bed(pan.g)
+    bps = O.bytes_per_snp(n)
+    xs = O.center_covariates(pan.x)
+    pre = O.precompute(xs, 1e-5)
+    b0 = pre.m.T @ np.random.default_rng(4).standard_normal((n, K + 10))
+    kn, seed_nmf, cv_rate = 3, 7, 0.02
+    omega = synth.normal_matrix(seed_nmf, n, kn + 10)
+    cfg_o = O.Config(k=K, norm=O.EIGENSTRAT, chunk_size=500, n_power_iter=2, seed=42)
+    est = O.subset_indices(0.5, p)
+    nmf_o = dict(k=kn, n_iter=3, eps=1e-16, cv_rate=cv_rate, seed=seed_nmf, omega=omega,
+                 uniform=synth.uniform_stream)
+    u_ref, ref, _, nres = O.fit_lfmm2(packed, bps, n, p, pan.x, cfg_o, est, b0=b0, nmf=nmf_o)
+    g = S.BedFile.from_packed(packed, n, p)
+    cfg = S.Lfmm2Config(k=K, norm=S.EIGENSTRAT, chunk_size=500, n_power_iter=2, seed=42, precision=S.PREC_I8)
+    nmf = S.NmfParams(k=kn, n_iter=3, cv_rate=cv_rate, seed=seed_nmf, chunk_size=500)
+    got = S.fit_lfmm2_nmf(g, 0.5, g, pan.x, cfg, nmf, b0=b0, nmf_omega=omega, nmf_int8=nmf_int8)
+    ang = np.max(O.principal_angles(got.u_hat, u_ref))
+    lp = np.abs(O.log10_pvalue(got.tstat, got.gif_per_trait) - O.log10_pvalue(ref.t, ref.gif_per_trait))
+    print(f"[nmf mixed] angles {ang:.2e}, max |dlog10p| {lp.max():.2e}", flush=True)
+    assert ang < 1e-4
+    assert lp.max() <= 1e-3
+    assert got.gwas_cv[2] == ref.cv[2] > 0
+    assert np.allclose(got.gwas_cv[:2], ref.cv[:2], rtol=1e-5, atol=0)
+    assert np.max(np.abs(ref.t)) > 4
 
 
 @pytest.mark.parametrize("n,p,k,n_iter,rate,chunk,miss", [
