@@ -1551,6 +1551,132 @@ void nmf_forward_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_t n,
   SLFMM_HIP(hipGetLastError());
 }
 
+// ---- NMF-filled RSVD forward pass on the int8 matrix cores ------------------------
+// With the missing calls of sample k filled by W_k . H_j (NmfInRam,
+// src/parallel.rs:284-321), the centred forward product is
+//   W_kc = sum_j g'_kj Qt_jc + sum_r W_kr sum_{j missing in k} H_jr Qt_jc
+//          - sum_j mu_j Qt_jc,     Qt = diag(sc) Q_z,
+// with g' the dosage (missing calls 0).  Per group of g columns, one pass of
+// k_fwd_i8 over the sample-major copy: the dosage fragment meets Qt_c in 3
+// int8 terms, the missing-call fragment the products H_r Qt_c in 2
+// (NmfFwdMap with K -> g, P -> g K); the exact int32 slot sums are combined
+// in FP64 with the sample's own W_k.  The last term (vhat) is an FP64 column
+// sum.
+struct FnFillB {
+  const double* Qz; uint64_t ldq; const double* sc; const double* Ht; int K, c0, g;
+  __device__ double val(uint64_t j, int x) const {
+    if (x < g) return sc[j] * Qz[j * ldq + c0 + x];
+    const int q = x - g;
+    return Ht[j * K + q % K] * (sc[j] * Qz[j * ldq + c0 + q / K]);
+  }
+  __device__ void operator()(uint64_t j, int x, double (&v)[1]) const { v[0] = fabs(val(j, x)); }
+};
+struct FnVhatF {
+  const double* Qz; uint64_t ldq; const double* sc; const double* mu;
+  __device__ void operator()(uint64_t j, int c, double (&v)[1]) const { v[0] = mu[j] * (sc[j] * Qz[j * ldq + c]); }
+};
+
+// B operand of one group (layout of k_pack_nmf_cols): one workgroup per
+// block of 64 KS SNPs.
+template <int KS>
+__global__ void __launch_bounds__(256) k_pack_fill_cols(FnFillB fb, uint64_t pe, NmfFwdMap map,
+                                                        const double* __restrict__ bsc, v4i* __restrict__ out) {
+  const int NTB = map.NG + map.NM;
+  const uint64_t snp0 = (uint64_t)blockIdx.x * 64 * KS;
+  const int nch = KS * NTB * 64;
+  for (int c = threadIdx.x; c < nch; c += 256) {
+    const int lane = c & 63, frag = c >> 6, ct = frag % NTB, s = frag / NTB, g = lane >> 4;
+    const int2 ci = map.info(16 * ct + (lane & 15));
+    uint32_t word[4] = {0u, 0u, 0u, 0u};
+    if (ci.x >= 0) {
+      const double inv = 1.0 / bsc[ci.x];
+      const uint64_t j0 = snp0 + 16 * KS * g + 16 * s;
+#pragma unroll 4
+      for (int r3 = 0; r3 < 16; ++r3) {
+        const uint64_t j = j0 + r3;
+        if (j >= pe) break;
+        const int q = quant_term(fb.val(j, ci.x) * inv, ci.y);
+        word[r3 & 3] |= (uint32_t)(q & 0xFF) << (8 * (r3 >> 2));
+      }
+    }
+    out[(uint64_t)blockIdx.x * nch + c] = v4i{(int)word[0], (int)word[1], (int)word[2], (int)word[3]};
+  }
+}
+
+// W[k][c0 + c] from the slot sums of sample k
+__global__ void k_fill_fwd_combine(const double* __restrict__ ssum, uint64_t n, NmfFwdMap map, int c0,
+                                   const double* __restrict__ bsc, const double* __restrict__ Wf,
+                                   const double* __restrict__ vhat, double* __restrict__ W, uint64_t ldw) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = map.K, K = map.P / map.K;
+  if (e >= n * (uint64_t)g) return;
+  const uint64_t k = e / g;
+  const int c = (int)(e % g);
+  const double* S = ssum + k * 16 * (uint64_t)(map.NG + map.NM);
+  double v = bsc[c] * (S[3 * c] + S[3 * c + 1] / 254.0 + S[3 * c + 2] / 64516.0);
+  for (int r = 0; r < K; ++r) {
+    const int q = c * K + r, slot = 16 * map.NG + 2 * q;
+    v += Wf[k * K + r] * (bsc[g + q] * (S[slot] + S[slot + 1] / 254.0));
+  }
+  W[k * ldw + c0 + c] = v - vhat[c0 + c];
+}
+
+// Tile plans (NG, NM) of the filled forward pass (NG + NM <= 7, fill rank <= 8).
+#define SLFMM_FWD_FILL(X) \
+  X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5) X(3, 2) X(3, 3) X(3, 4) X(4, 3)
+
+static int fill_fwd_group(int l, int K) {
+  for (int g = std::min(l, 64); g >= 1; --g)
+    if ((3 * g + 15) / 16 + (2 * g * K + 15) / 16 <= 7) return g;
+  return 0;
+}
+
+bool fill_forward_i8_supported(int l, int K) { return l >= 1 && K >= 1 && K <= kNmfMaxK8 && fill_fwd_group(l, K) >= 1; }
+
+void fill_forward_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_t n, uint64_t pe, const double* Qz,
+                     uint64_t ldq, int l, const double* mu, const double* sc, const double* Wf, const double* Ht, int K,
+                     double* W, uint64_t ldw) {
+  SLFMM_CHECK(fill_forward_i8_supported(l, K), SLFMM_E_ARG, "no int8 plan for this filled forward pass");
+  hipStream_t s = ctx->stream;
+  constexpr int KS = 4, WV = FwdRawEpi<1, 1>::WV;
+  double* vhat = ctx->buf<double>("ff8_vhat", l);
+  colreduce<1, kColSum>(ctx, FnVhatF{Qz, ldq, sc, mu}, GStore1{vhat}, pe, l);
+  const uint64_t nblk = cdiv(pe, 64 * KS), sblocks = cdiv(n, 64 * WV);
+  const int gmax = fill_fwd_group(l, K);
+  for (int c0 = 0; c0 < l; c0 += gmax) {
+    const int g = std::min(gmax, l - c0);
+    const NmfFwdMap map{g, g * K, (3 * g + 15) / 16, (2 * g * K + 15) / 16};
+    const int NTB = map.NG + map.NM, mq = g + g * K;
+    const FnFillB fb{Qz, ldq, sc, Ht, K, c0, g};
+    double* bsc = ctx->buf<double>("ff8_sc", mq);
+    colreduce<1, kColMax>(ctx, fb, GScaleMax{bsc}, pe, mq);
+    const uint64_t n16 = nblk * KS * NTB * 64;
+    SLFMM_CHECK(n16 * 16 < 0xFFFFFFFFull, SLFMM_E_ARG, "estimation panel too large for the int8 filled forward pass");
+    v4i* Bp = ctx->buf<v4i>("ff8_pack", n16);
+    k_pack_fill_cols<KS><<<(unsigned)nblk, 256, 0, s>>>(fb, pe, map, bsc, Bp);
+    uint64_t per = 0;
+    bool done = false;
+#define X(NG_, NM_)                                                                                         \
+    if (!done && map.NG == NG_ && map.NM == NM_) {                                                          \
+      per = fwd_blocks_per_chunk(ctx, k_fwd_i8<FwdRawEpi<NG_ + NM_, NG_>>, 64 * WV, nblk, sblocks,         \
+                                 cdiv(pe, 8u << 20));                                                       \
+      const uint64_t nchunk = cdiv(nblk, per);                                                              \
+      int32_t* part = ctx->buf<int32_t>("ff8_part", nchunk * n * 16 * NTB);                                 \
+      k_fwd_i8<<<dim3((unsigned)sblocks, (unsigned)nchunk), 64 * WV, 0, s>>>(                               \
+          YT, ldt, n, (int)per, (int)nblk, Bp, FwdRawEpi<NG_ + NM_, NG_>{part});                            \
+      const uint64_t nslot = n * 16 * NTB;                                                                  \
+      double* ssum = ctx->buf<double>("ff8_ssum", nslot);                                                   \
+      k_sum_slots<<<(unsigned)cdiv(nslot, 256), 256, 0, s>>>(part, nslot, (int)nchunk, ssum);              \
+      k_fill_fwd_combine<<<(unsigned)cdiv(n * g, 256), 256, 0, s>>>(ssum, n, map, c0, bsc, Wf, vhat, W, ldw); \
+      done = true;                                                                                          \
+    }
+    SLFMM_FWD_FILL(X)
+#undef X
+    SLFMM_CHECK(done, SLFMM_E_ARG, "no int8 kernel for this filled forward layout");
+    SLFMM_HIP(hipGetLastError());
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Layout probe (test-only): one v_mfma_i32_16x16x64_i8 with the fragment
 // mapping the kernels above assume.  A is 16 x 64, B is 64 x 16, row-major.

@@ -380,6 +380,14 @@ bool fill_functionals_i8_supported(int m, int K);
 void fill_functionals_i8(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t p, const double* A,
                          uint64_t lda, int m, const double* W, const double* Ht, int K, const double* mu,
                          const double* outscale, double* F, uint64_t ldF);
+// assoc_i8.hip: the RSVD forward pass over NMF-filled rows (before N is
+// applied) on the int8 matrix cores: W[k][c] = sum_j (y_kj - mu_j) sc_j Qz[j][c] for c < l
+// with the missing calls of sample k filled by W_k . H_j (Wf n x K, Ht pe x K),
+// from the sample-major copy YT (n x ldt bytes).  Fill rank K <= 8.
+bool fill_forward_i8_supported(int l, int K);
+void fill_forward_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_t n, uint64_t pe, const double* Qz,
+                     uint64_t ldq, int l, const double* mu, const double* sc, const double* Wf, const double* Ht, int K,
+                     double* W, uint64_t ldw);
 // assoc_i8.hip: num_W (n x K) of the NMF forward pass (src/nmf.rs:131-147,
 // missing calls filled by W_k . H_j) from the sample-major copy YT (n x ldt
 // bytes) of the rows, Ht (pe x K) and W (K <= kNmfMaxK8).

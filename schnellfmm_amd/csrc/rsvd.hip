@@ -288,14 +288,14 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   }
 
   // the int8 sketch carries at most 32 columns; wider sketches run the FP64
-  // kernels.  With an NMF fill the sketch and backward passes take the
-  // filled int8 pass where its plans reach (fill_i8); the forward passes
-  // stay FP64.
+  // kernels.  With an NMF fill the sketch/backward and forward passes take
+  // the filled int8 passes where their plans reach (fill_i8, fill_fwd_i8).
   const bool mixed = cfg.precision == SLFMM_PREC_I8 && l <= 32 && !filled;
   const bool fill_i8 = cfg.precision == SLFMM_PREC_I8 && filled && fill_functionals_i8_supported((int)l, fill->K);
+  const bool fill_fwd_i8 = cfg.precision == SLFMM_PREC_I8 && filled && fill_forward_i8_supported((int)l, fill->K);
   uint8_t* YT = nullptr;
   uint64_t ldt = 0;
-  if (mixed) {   // sample-major copy for the forward passes
+  if (mixed || fill_fwd_i8) {   // sample-major copy for the forward passes
     ldt = rup(cdiv(pe, 4), 16);
     YT = ctx->buf<uint8_t>("rs_YT", n * ldt + kSlack);
     launch_fill(s, reinterpret_cast<double*>(YT + n * ldt), 0.0, kSlack / 8);   // the transpose writes every row byte
@@ -338,6 +338,8 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
     {
       slfmm_timed tm(ctx, "rsvd_forward");
       if (mixed) rsvd_accumulate_i8(ctx, YT, ldt, n, pe, Zq, lda, l, mu, sc, W, l);
+      else if (fill_fwd_i8)
+        fill_forward_i8(ctx, YT, ldt, n, pe, Zq, lda, (int)l, mu, sc, fill->W, fill->Ht, fill->K, W, l);
       else accumulate_f64(ctx, Y, g->ld, n, pe, Zq, lda, l, mu, sc, W, l, fill);
     }
     apply_N(ctx, Qd, d, dlam, W, l, l, n, false);
