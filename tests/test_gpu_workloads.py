@@ -209,6 +209,27 @@ def test_nmf_subset_vs_oracle(S, nmf_panel, nmf_subset_ref, int8):
     assert np.isclose(got.cv_mae_mean_impute, ref.cv_mae_mean_impute, rtol=1e-12 if not int8 else 1e-5, atol=0)
 
 
+def test_nmf_fit_mixed_vs_fp64_full_size(S, nmf_panel):
+    """The NMF-imputed fit (K = 8, d = 2, nmf-k = 8) in mixed precision (int8
+    NMF passes, filled sketch/backward, forward and association passes)
+    against the same fit in FP64 on every SNP: |delta log10 p| <= 1e-3, the
+    same GIFs to 1e-6, r2 partitions."""
+    g = nmf_panel
+    n = g.n_samples
+    x = np.random.default_rng(5).standard_normal((n, 2))
+    cfg = S.NmfParams(k=8, n_iter=10, cv_rate=5e-4, seed=42, chunk_size=10_000)
+    base = dict(k=8, norm=S.EIGENSTRAT, chunk_size=10_000, n_power_iter=2, seed=42)
+    _log("[workload] NMF fit: FP64")
+    f64 = S.fit_lfmm2_nmf(g, None, g, x, S.Lfmm2Config(**base), cfg)
+    _log("[workload] NMF fit: mixed precision")
+    mix = S.fit_lfmm2_nmf(g, None, g, x, S.Lfmm2Config(precision=S.PREC_I8, **base), cfg, nmf_int8=True)
+    lp = np.abs(O.log10_pvalue(mix.tstat, mix.gif_per_trait) - O.log10_pvalue(f64.tstat, f64.gif_per_trait))
+    _log(f"[workload] NMF fit mixed vs FP64: max |dlog10p| {lp.max():.2e}")
+    assert lp.max() <= 1e-3
+    assert np.allclose(mix.gif_per_trait, f64.gif_per_trait, rtol=1e-6)
+    assert _r2_partition_ok(mix.r2)
+
+
 # ---------------------------------------------------------------------------
 # inbred lines at full size
 # ---------------------------------------------------------------------------
