@@ -290,8 +290,14 @@ int slfmm_fit_lfmm2_batch(slfmm_ctx* ctx, slfmm_problem* problems, uint64_t coun
  * MAE, held-out count) enables the association-phase CV with rate
  * nmf->cv_rate and seed cfg->seed (NULL disables it).  With SLFMM_NMF_INT8
  * in flags the NMF estimation runs its update passes on the int8 matrix
- * cores (nmf k <= 8, as slfmm_nmf_estimate); the filled RSVD and
- * association passes stay FP64. */
+ * cores (nmf k <= 8, as slfmm_nmf_estimate).  With cfg->precision
+ * SLFMM_PREC_I8 (nmf k <= 8) the passes over the NMF-filled rows run there
+ * too: the RSVD sketch/backward and forward passes and the association
+ * functionals split each filled sum into the dosages against the operand (3
+ * int8 terms) and the missing-call indicator against the operand's products
+ * with W (or H) (2 terms), recombined in FP64 with the SNP's own H_j (the
+ * sample's own W_k); counts, moments, H_j, the CV and the epilogue stay
+ * FP64.  Layouts past the int8 tile plans run those passes in FP64. */
 int slfmm_fit_lfmm2_nmf(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset_idx, uint64_t n_idx,
                         const slfmm_geno* g, const double* x, uint64_t d, const slfmm_config* cfg,
                         const slfmm_nmf_config* nmf, const double* b0, const double* nmf_omega, double* u_hat,
