@@ -184,8 +184,6 @@ def run(a: argparse.Namespace):
 
     nmf = bool(a.nmf_impute)
     prec = A.PREC_I8 if a.precision == "mixed" else A.PREC_FP64
-    if nmf and prec != A.PREC_FP64:
-        raise ValueError("NMF imputation runs on the FP64 path (use --precision fp64)")
     cfg = A.Lfmm2Config(k=a.k, lam=a.lam, chunk_size=a.chunk_size, oversampling=a.oversampling,
                         n_power_iter=a.power_iter, seed=a.seed, n_workers=a.threads,
                         progress=sys.stderr.isatty(),
@@ -212,7 +210,8 @@ def run(a: argparse.Namespace):
     if nmf:
         ncfg = A.NmfParams(k=a.nmf_k if a.nmf_k is not None else a.k, n_iter=a.nmf_iter, eps=1e-16,
                            cv_rate=a.nmf_cv_rate, chunk_size=a.chunk_size, seed=a.seed)
-        res = A.fit_lfmm2_nmf(y_est, subset, y_full, x, cfg, ncfg)
+        # mixed precision also runs the NMF update passes on the int8 matrix cores
+        res = A.fit_lfmm2_nmf(y_est, subset, y_full, x, cfg, ncfg, nmf_int8=prec == A.PREC_I8)
     else:
         res = A.fit_lfmm2(y_est, subset, y_full, x, cfg)
 
