@@ -168,16 +168,54 @@ __global__ void __launch_bounds__(256) k_nmf_bwd(const uint8_t* __restrict__ pac
 // row, every entry drawn); it is held out if the draw is < cv_rate and the
 // call is observed.  Per-workgroup sums, combined in a fixed order.
 // One thread per SNP.  The draws of a 16-code word are unrolled (a counter
-// offset per code, a hit bit at bit 2c of the word's mask, missing calls and
-// padding cleared with the codes' own bits), and the rare held-out calls are
-// then taken in sample order from the mask, so the hot loop has no branch
-// and the sums keep the per-entry order.
+// offset per code, a hit mask), and the rare drawn calls are then taken in
+// sample order from the mask, so the hot loop has no branch and the sums
+// keep the per-entry order.
+// Iteration it draws chunk seq from the stream seeded seed + it + seq, so an
+// iteration reuses the streams of the one before, shifted by a chunk: the
+// nchunk + n_iter - 1 distinct streams are drawn once (k_cv_lists: the
+// sample indices of the first kCvList draws below the rate of each (stream,
+// row-in-chunk), in sample order, and their count), and each iteration
+// walks its SNPs' lists; a SNP whose list overflowed draws again.
+constexpr int kCvList = 16;
+
+// Draw bits of 16 consecutive elements of a stream: bit c set if element
+// base + c (counter ctr + c kStreamStep) falls below the rate.
+__device__ __forceinline__ uint32_t cv_draw16(uint64_t ctr, uint64_t cv_thresh) {
+  uint32_t hit = 0;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) hit |= (mix_counter(ctr + (uint64_t)c * kStreamStep) >> 11) < cv_thresh ? 1u << c : 0u;
+  return hit;
+}
+
+__global__ void __launch_bounds__(256) k_cv_lists(uint64_t n, uint64_t seed0, uint64_t ns, uint64_t jl_count,
+                                                  uint64_t cv_thresh, uint32_t* __restrict__ lists,
+                                                  uint8_t* __restrict__ counts) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ns * jl_count) return;
+  const uint64_t srel = e / jl_count, jl = e % jl_count;
+  const uint64_t ctr0 = stream_counter(seed0 + srel, jl * n);
+  uint32_t cnt = 0;
+  for (uint64_t k0 = 0; k0 < n; k0 += 16) {
+    uint32_t hit = cv_draw16(ctr0 + k0 * kStreamStep, cv_thresh);
+    if (n - k0 < 16) hit &= (1u << (n - k0)) - 1u;
+    while (hit) {
+      const int b = __builtin_ctz(hit);
+      hit &= hit - 1;
+      if (cnt < (uint32_t)kCvList) lists[e * kCvList + cnt] = (uint32_t)(k0 + b);
+      cnt += cnt < 255u ? 1u : 0u;
+    }
+  }
+  counts[e] = (uint8_t)cnt;
+}
+
 __global__ void __launch_bounds__(256) k_nmf_cv(const uint8_t* __restrict__ packed, uint64_t ld, uint64_t n,
                                                 uint64_t p, const double* __restrict__ W,
                                                 const double* __restrict__ Ht, int K,
                                                 const int32_t* __restrict__ nmiss, const int32_t* __restrict__ sumg,
                                                 uint64_t mask_seed, uint64_t chunk, uint64_t cv_thresh,
-                                                double* __restrict__ part) {
+                                                const uint32_t* __restrict__ lists, const uint8_t* __restrict__ counts,
+                                                uint64_t jl_count, uint64_t it, double* __restrict__ part) {
   __shared__ double r0[256], r1[256];
   __shared__ unsigned long long r2[256];
   const int t = threadIdx.x;
@@ -187,35 +225,42 @@ __global__ void __launch_bounds__(256) k_nmf_cv(const uint8_t* __restrict__ pack
   if (j < p && cv_thresh > 0) {
     const int64_t nobs = (int64_t)n - nmiss[j];
     const double mu = nobs > 0 ? (double)sumg[j] / (double)nobs : 0.0;
-    const uint64_t seed = mask_seed + j / chunk;
-    // draw base + k of the stream: u01 < cv_rate  <=>  (mix >> 11) < cv_thresh
-    uint64_t ctr = stream_counter(seed, (j % chunk) * n);
-    const uint4* row = reinterpret_cast<const uint4*>(packed + j * ld);
-    const uint64_t nkb = cdiv(n, 64);
-    for (uint64_t kb = 0; kb < nkb; ++kb, ctr += 64 * kStreamStep) {
-      const uint4 q = row[kb];
-      const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
-      const int lim = (int)std::min<uint64_t>(64, n - kb * 64);
+    auto held = [&](uint64_t k, uint32_t code) {   // a drawn call; held out if observed
+      if (code == 1u) return;
+      double rec = 0.0;
+      for (int r = 0; r < K; ++r) rec += W[k * K + r] * Ht[j * K + r];
+      const double g = dosage_of(code);
+      e_nmf += fabs(g - rec);
+      e_mean += fabs(g - mu);
+      cnt += 1;
+    };
+    const uint64_t le = lists ? ((it + j / chunk) * jl_count + j % chunk) : 0;
+    const uint32_t nl = lists ? counts[le] : 0u;
+    if (lists && nl <= (uint32_t)kCvList) {
+      const uint8_t* rowb = packed + j * ld;
+      for (uint32_t i = 0; i < nl; ++i) {
+        const uint32_t k = lists[le * kCvList + i];
+        held(k, (uint32_t)(rowb[k >> 2] >> (2 * (k & 3))) & 3u);
+      }
+    } else {
+      // draw base + k of the stream: u01 < cv_rate  <=>  (mix >> 11) < cv_thresh
+      uint64_t ctr = stream_counter(mask_seed + j / chunk, (j % chunk) * n);
+      const uint4* row = reinterpret_cast<const uint4*>(packed + j * ld);
+      const uint64_t nkb = cdiv(n, 64);
+      for (uint64_t kb = 0; kb < nkb; ++kb, ctr += 64 * kStreamStep) {
+        const uint4 q = row[kb];
+        const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
+        const int lim = (int)std::min<uint64_t>(64, n - kb * 64);
 #pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        uint32_t hit = 0;
-#pragma unroll
-        for (int c = 0; c < 16; ++c)
-          hit |= (mix_counter(ctr + (uint64_t)(16 * w + c) * kStreamStep) >> 11) < cv_thresh ? 1u << (2 * c) : 0u;
-        const uint32_t x = wd[w];
-        hit &= ~(x & ~(x >> 1) & 0x55555555u);   // code 01 (missing) at bit 2c
-        const int valid = lim - 16 * w;           // codes past the panel's last sample
-        if (valid < 16) hit &= valid <= 0 ? 0u : (1u << (2 * valid)) - 1u;
-        while (hit) {
-          const int b = __builtin_ctz(hit);
-          hit &= hit - 1;
-          const uint64_t k = kb * 64 + 16 * w + (b >> 1);
-          double rec = 0.0;
-          for (int r = 0; r < K; ++r) rec += W[k * K + r] * Ht[j * K + r];
-          const double g = dosage_of((x >> b) & 3u);
-          e_nmf += fabs(g - rec);
-          e_mean += fabs(g - mu);
-          cnt += 1;
+        for (int w = 0; w < 4; ++w) {
+          uint32_t hit = cv_draw16(ctr + (uint64_t)(16 * w) * kStreamStep, cv_thresh);
+          const int valid = lim - 16 * w;   // codes past the panel's last sample
+          if (valid < 16) hit &= valid <= 0 ? 0u : (1u << valid) - 1u;
+          while (hit) {
+            const int b = __builtin_ctz(hit);
+            hit &= hit - 1;
+            held(kb * 64 + 16 * w + b, (wd[w] >> (2 * b)) & 3u);
+          }
         }
       }
     }
@@ -397,6 +442,21 @@ void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t
   double* hh = ctx->buf<double>("nmf_hh", K * K);
   const unsigned cvblk = (unsigned)cdiv(pe, 256);
   double* cvpart = ctx->buf<double>("nmf_cvpart", (uint64_t)cvblk * 3);
+  // CV draws of the nchunk + n_iter - 1 distinct streams, once (when that is
+  // fewer draws than n_iter passes and the lists stay within 2 GiB)
+  const uint64_t cv_thresh = u01_threshold(cfg.cv_rate);
+  const uint64_t cv_nchunk = cdiv(pe, cfg.chunk_size), cv_jl = std::min<uint64_t>(cfg.chunk_size, pe);
+  const uint64_t cv_ns = cv_nchunk + cfg.n_iter - 1;
+  uint32_t* cv_lists = nullptr;
+  uint8_t* cv_counts = nullptr;
+  if (cv_thresh > 0 && cfg.n_iter > 1 && cv_ns * cv_jl < cfg.n_iter * pe && cv_ns * cv_jl * (4 * kCvList + 1) < (2ull << 30)) {
+    cv_lists = ctx->buf<uint32_t>("nmf_cvlists", cv_ns * cv_jl * kCvList);
+    cv_counts = ctx->buf<uint8_t>("nmf_cvcounts", cv_ns * cv_jl);
+    slfmm_timed tm(ctx, "nmf_cv_lists");
+    k_cv_lists<<<(unsigned)cdiv(cv_ns * cv_jl, 256), 256, 0, s>>>(n, cfg.seed, cv_ns, cv_jl, cv_thresh, cv_lists,
+                                                                 cv_counts);
+    SLFMM_HIP(hipGetLastError());
+  }
   uint8_t* YT = nullptr;
   uint64_t ldt = 0;
   if (mc) {   // sample-major copy for the forward passes
@@ -442,8 +502,8 @@ void nmf_run(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t
     normalize_w(ctx, W, n, Ht, pe, K, eps);
     // held-out error of this iteration
     slfmm_timed tm(ctx, "nmf_cv");
-    k_nmf_cv<<<cvblk, 256, 0, s>>>(Y, ld, n, pe, W, Ht, K, nmiss, sumg, cfg.seed + it, cfg.chunk_size,
-                                   u01_threshold(cfg.cv_rate), cvpart);
+    k_nmf_cv<<<cvblk, 256, 0, s>>>(Y, ld, n, pe, W, Ht, K, nmiss, sumg, cfg.seed + it, cfg.chunk_size, cv_thresh,
+                                   cv_lists, cv_counts, cv_jl, it, cvpart);
     k_nmf_cv_final<<<1, 256, 0, s>>>(cvpart, (int)cvblk, cv + 3 * it);
     SLFMM_HIP(hipGetLastError());
   }
