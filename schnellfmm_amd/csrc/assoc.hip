@@ -240,7 +240,7 @@ __global__ void k_fill_moments(const int32_t* __restrict__ nmiss, const int32_t*
 }
 
 void fill_moments(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t rows, const FillSpec& fs, int norm,
-                  const double* Wp, double* mu, double* sc, double* tss, double* SWp) {
+                  const double* Wp, double* mu, double* sc, double* tss, double* SWp, bool i8) {
   hipStream_t s = ctx->stream;
   SLFMM_CHECK(fs.K >= 1 && fs.K <= kNmfMaxK, SLFMM_E_ARG, "NMF rank must be 1..16");
   int32_t* nmiss = ctx->buf<int32_t>("fl_nmiss", rows);
@@ -250,11 +250,15 @@ void fill_moments(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t r
   double* S2 = ctx->buf<double>("fl_s2", rows);
   launch_stats(s, packed, ld, fs.n, rows, nmiss, sumg, nhom);
   if (rows) {
-    switch (fs.K) {
+    if (i8 && fs.K <= kNmfMaxK8) {
+      fill_stats_i8(ctx, packed, ld, fs.n, rows, fs.W, fs.Ht, fs.K, Wp, S1, S2, SWp);
+    } else {
+      switch (fs.K) {
 #define SLFMM_FST(R) \
   case R: k_fill_stats<R><<<(unsigned)cdiv(rows, 256), 256, 0, s>>>(packed, ld, rows, fs, Wp, S1, S2, SWp); break;
-      SLFMM_NMF_RANKS(SLFMM_FST)
+        SLFMM_NMF_RANKS(SLFMM_FST)
 #undef SLFMM_FST
+      }
     }
     k_fill_moments<<<(unsigned)cdiv(rows, 256), 256, 0, s>>>(nmiss, sumg, nhom, S1, S2, fs.n, rows, norm, mu, sc, tss);
   }
@@ -819,7 +823,8 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
     double* fmu = ctx->buf<double>("as_nmf_mu", p);
     const bool cv = nmf->cv_rate > 0.0 && nmf->cv_out;
     double* SWp = cv ? ctx->buf<double>("as_nmf_swp", p * K) : nullptr;
-    fill_moments(ctx, g->d, g->ld, p, fs, cfg.norm, cv ? nmf->Wp : nullptr, fmu, fsc, ftss, SWp);
+    fill_moments(ctx, g->d, g->ld, p, fs, cfg.norm, cv ? nmf->Wp : nullptr, fmu, fsc, ftss, SWp,
+                 cfg.precision == SLFMM_PREC_I8);
     if (cv)
       gwas_cv(ctx, g->d, g->ld, n, p, nmf->W, nmf->Wp, K, Hraw, ldh, SWp, nmiss, sumg, nmf->cv_seed, nmf->chunk,
               nmf->cv_rate, nmf->cv_out);

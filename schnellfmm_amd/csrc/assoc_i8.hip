@@ -1238,25 +1238,29 @@ __global__ void k_fill_prod(const double* __restrict__ A, uint64_t lda, uint64_t
 }
 
 // F[j][c0 + c] (times outscale[j] if given) from the raw sums of one group;
-// one thread per (column, SNP), consecutive threads on consecutive SNPs.
+// one thread per SNP (its H_j read once), consecutive threads on consecutive
+// SNPs for every raw row.
 __global__ void k_fill_combine(const int32_t* __restrict__ raw, uint64_t ldraw, uint64_t p, NmfMap map, int c0,
                                const double* __restrict__ sc, const double* __restrict__ Ht,
                                const double* __restrict__ mu, const double* __restrict__ colsum,
                                const double* __restrict__ outscale, double* __restrict__ F, uint64_t ldF) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= p) return;
   const int g = map.K, K = map.P / map.K;
-  if (e >= (uint64_t)g * p) return;
-  const int c = (int)(e / p);
-  const uint64_t j = e % p;
+  double h[kNmfMaxK8];
+  for (int r = 0; r < K; ++r) h[r] = Ht[j * K + r];
+  const double mj = mu[j], os = outscale ? outscale[j] : 1.0;
   auto R = [&](int row) { return (double)raw[(uint64_t)row * ldraw + j]; };
-  double f = sc[c] * (R(3 * c) + R(3 * c + 1) / 254.0 + R(3 * c + 2) / 64516.0);
   const int m0 = 32 * map.NT0;
-  for (int r = 0; r < K; ++r) {
-    const int q = c * K + r;
-    f += Ht[j * K + r] * (sc[g + q] * (R(m0 + 2 * q) + R(m0 + 2 * q + 1) / 254.0));
+  for (int c = 0; c < g; ++c) {
+    double f = sc[c] * (R(3 * c) + R(3 * c + 1) / 254.0 + R(3 * c + 2) / 64516.0);
+    for (int r = 0; r < K; ++r) {
+      const int q = c * K + r;
+      f += h[r] * (sc[g + q] * (R(m0 + 2 * q) + R(m0 + 2 * q + 1) / 254.0));
+    }
+    f -= mj * colsum[c0 + c];
+    F[j * ldF + c0 + c] = f * os;
   }
-  f -= mu[j] * colsum[c0 + c];
-  F[j * ldF + c0 + c] = outscale ? f * outscale[j] : f;
 }
 
 // Tile plans of the mode-3 pass: (NT0, NT1) with NT0 + NT1 <= 6.
@@ -1322,10 +1326,98 @@ void fill_functionals_i8(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t
     SLFMM_SNP_FILL(X)
 #undef X
     SLFMM_CHECK(done, SLFMM_E_ARG, "no int8 kernel for this filled-functional layout");
-    k_fill_combine<<<(unsigned)cdiv((uint64_t)g * p, 256), 256, 0, s>>>(raw, ldraw, p, map, c0, sc, Ht, mu, colsum,
-                                                                       outscale, F, ldF);
+    k_fill_combine<<<(unsigned)cdiv(p, 256), 256, 0, s>>>(raw, ldraw, p, map, c0, sc, Ht, mu, colsum, outscale, F,
+                                                         ldF);
     SLFMM_HIP(hipGetLastError());
   }
+}
+
+// ---- moments of NMF-filled rows on the int8 matrix cores ---------------------------
+// S1_j = sum_{k missing} W_k . H_j = H_j . a_j,  S2_j = H_j^T G_j H_j and
+// SWp_j = sum_{k missing} Wp_k, with a_j = sum_{k missing} W_k and G_j =
+// sum_{k missing} W_k W_k^T: one mode-3 pass whose missing-call side meets
+// [W_r | W_r W_s (r <= s) | Wp_r] in 2 int8 terms (a zero column on the
+// dosage side), then an FP64 combine with the SNP's own H_j.
+__global__ void k_fstat_cols(const double* __restrict__ W, const double* __restrict__ Wp, uint64_t n, int K,
+                             double* __restrict__ Qn, int mq) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double* w = W + k * K;
+  double* q = Qn + k * mq;
+  q[0] = 0.0;
+  int c = 1;
+  for (int r = 0; r < K; ++r) q[c++] = w[r];
+  for (int r = 0; r < K; ++r)
+    for (int s2 = r; s2 < K; ++s2) q[c++] = w[r] * w[s2];
+  if (Wp)
+    for (int r = 0; r < K; ++r) q[c++] = Wp[k * K + r];
+}
+
+__global__ void k_fstat_combine(const int32_t* __restrict__ raw, uint64_t ldraw, uint64_t p, NmfMap map, int K,
+                                const double* __restrict__ sc, const double* __restrict__ Ht,
+                                double* __restrict__ S1, double* __restrict__ S2, double* __restrict__ SWp) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= p) return;
+  const int m0 = 32 * map.NT0;
+  // miss-side column q (operand column 1 + q): scale sc[1 + q], rows m0 + 2 q, + 1
+  auto M = [&](int q) {
+    return sc[1 + q] * ((double)raw[(uint64_t)(m0 + 2 * q) * ldraw + j] +
+                        (double)raw[(uint64_t)(m0 + 2 * q + 1) * ldraw + j] / 254.0);
+  };
+  double h[kNmfMaxK8];
+  for (int r = 0; r < K; ++r) h[r] = Ht[j * K + r];
+  double s1 = 0.0, s2 = 0.0;
+  for (int r = 0; r < K; ++r) s1 += h[r] * M(r);
+  int q = K;
+  for (int r = 0; r < K; ++r)
+    for (int s3 = r; s3 < K; ++s3, ++q) s2 += (r == s3 ? 1.0 : 2.0) * h[r] * h[s3] * M(q);
+  S1[j] = s1;
+  S2[j] = s2;
+  if (SWp)
+    for (int r = 0; r < K; ++r) SWp[j * K + r] = M(q + r);
+}
+
+void fill_stats_i8(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t p, const double* W,
+                   const double* Ht, int K, const double* Wp, double* S1, double* S2, double* SWp) {
+  SLFMM_CHECK(K >= 1 && K <= kNmfMaxK8, SLFMM_E_ARG, "the int8 fill statistics support rank 1..8");
+  SLFMM_CHECK(n <= 8000000, SLFMM_E_ARG, "too many samples for the int8 kernels");
+  if (p == 0) return;
+  hipStream_t s = ctx->stream;
+  const int nm = K + K * (K + 1) / 2 + (Wp ? K : 0), mq = 1 + nm;
+  const NmfMap map{1, nm, 1, (2 * nm + 31) / 32};
+  const int NT = map.NT0 + map.NT1;
+  double* Qn = ctx->buf<double>("fs8_q", n * mq);
+  k_fstat_cols<<<(unsigned)cdiv(n, 256), 256, 0, s>>>(W, Wp, n, K, Qn, mq);
+  double* sc = ctx->buf<double>("fs8_sc", mq);
+  for (int q0 = 0; q0 < mq; q0 += 64)   // column reductions take up to 64 columns
+    colabsmax_scale(ctx, Qn + q0, mq, n, std::min(64, mq - q0), nullptr, nullptr, sc + q0);
+  const uint64_t nblk = cdiv(n, 256);
+  const uint64_t total = nblk * 8 * NT * 1024;
+  int8_t* pack = ctx->buf<int8_t>("fs8_pack", total);
+  k_pack_rows<<<(unsigned)std::min<uint64_t>(cdiv(total, 256), 16384), 256, 0, s>>>(Qn, mq, n, map, sc, NT, total,
+                                                                                  pack);
+  const uint64_t ldraw = rup(p, 64);
+  int32_t* raw = ctx->buf<int32_t>("fs8_raw", (uint64_t)32 * NT * ldraw);
+  SnpArgs a{};
+  a.packed = Y; a.ld = ld; a.p = p; a.n = n; a.nblk = (int)nblk;
+  a.rb = (uint32_t)std::min<uint64_t>(rup(cdiv(n, 4), 16), ld);
+  a.m = mq; a.S = 1; a.fill = 0; a.A = pack;
+  a.raw = raw; a.ldraw = ldraw;
+  const uint64_t rows = 32 * kSnpWaves;
+  SLFMM_CHECK((rows + 2 * 32) * a.ld < 0xF0000000ull && total < 0xFFFFFFFFull, SLFMM_E_ARG,
+              "panel rows too long for the int8 kernels");
+  const unsigned grid = (unsigned)cdiv(p, rows);
+  bool done = false;
+#define X(A0_, A1_)                                                                               \
+  if (!done && map.NT0 == A0_ && map.NT1 == A1_) {                                                \
+    k_snp_i8<1, A0_, A1_, 1, 3><<<grid, 64 * kSnpWaves, 0, s>>>(a);                               \
+    done = true;                                                                                  \
+  }
+  SLFMM_SNP_FILL(X)
+#undef X
+  SLFMM_CHECK(done, SLFMM_E_ARG, "no int8 kernel for the fill statistics layout");
+  k_fstat_combine<<<(unsigned)cdiv(p, 256), 256, 0, s>>>(raw, ldraw, p, map, K, sc, Ht, S1, S2, SWp);
+  SLFMM_HIP(hipGetLastError());
 }
 
 // ---- NMF forward pass on the int8 matrix cores ---------------------------------

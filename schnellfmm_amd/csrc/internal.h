@@ -305,8 +305,9 @@ void pad_operand(hipStream_t s, const double* Q, uint64_t ldq, uint64_t n, int m
 // Per-row moments of a filled panel: mu = (sum_g + S1) / n over all calls,
 // sc = compute_scale(mu, norm), tss = sc^2 sum_k (y_k - mu)^2 (y filled).
 // With Wp (n x K) also SWp[j] = sum over missing k of Wp[k] (rows x K).
+// With i8 (rank <= 8) the fill sums come from fill_stats_i8.
 void fill_moments(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t rows, const FillSpec& fs,
-                  int norm, const double* Wp, double* mu, double* sc, double* tss, double* SWp);
+                  int norm, const double* Wp, double* mu, double* sc, double* tss, double* SWp, bool i8 = false);
 // Association with NMF on-the-fly imputation (src/parallel.rs:252-282): the
 // caller passes W, the pseudo-inverse rows Wp = W (W^T W)^-1 and the GWAS
 // CV settings; cv_out (3) receives (NMF MAE, mean MAE, count) if non-null.
@@ -380,6 +381,11 @@ bool fill_functionals_i8_supported(int m, int K);
 void fill_functionals_i8(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t p, const double* A,
                          uint64_t lda, int m, const double* W, const double* Ht, int K, const double* mu,
                          const double* outscale, double* F, uint64_t ldF);
+// assoc_i8.hip: S1 = sum of the fills, S2 = sum of their squares and (with
+// Wp) SWp = sum of the Wp rows over each row's missing calls, on the int8
+// matrix cores (rank K <= 8); the FP64 counterpart is fill_moments' kernel.
+void fill_stats_i8(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t p, const double* W,
+                   const double* Ht, int K, const double* Wp, double* S1, double* S2, double* SWp);
 // assoc_i8.hip: the RSVD forward pass over NMF-filled rows (before N is
 // applied) on the int8 matrix cores: W[k][c] = sum_j (y_kj - mu_j) sc_j Qz[j][c] for c < l
 // with the missing calls of sample k filled by W_k . H_j (Wf n x K, Ht pe x K),

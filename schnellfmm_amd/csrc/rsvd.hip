@@ -281,7 +281,8 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   double* sc = ctx->buf<double>("rs_sc", pe);
   const bool filled = fill && fill->W;
   if (filled) {   // NMF in-RAM fill of the estimation rows: moments of the filled rows
-    fill_moments(ctx, Y, g->ld, pe, *fill, cfg.norm, nullptr, mu, sc, nullptr, nullptr);
+    fill_moments(ctx, Y, g->ld, pe, *fill, cfg.norm, nullptr, mu, sc, nullptr, nullptr,
+                 cfg.precision == SLFMM_PREC_I8);
   } else {
     launch_stats(s, Y, g->ld, n, pe, nmiss, sumg, nhom);
     launch_mean_scale(s, nmiss, sumg, n, pe, cfg.norm, mu, sc);
