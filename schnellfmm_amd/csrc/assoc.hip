@@ -815,7 +815,8 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
     const uint64_t ldh = rup(K, kRowBlk);
     double* Hraw = ctx->buf<double>("as_nmf_hraw", p * ldh);
     double* Ht = ctx->buf<double>("as_nmf_ht", p * K);
-    raw_functionals(ctx, g->d, g->ld, n, p, mu, nmf->Wp, K, K, Hraw, ldh);
+    if (cfg.precision == SLFMM_PREC_I8) raw_functionals_i8(ctx, g->d, g->ld, n, p, mu, nmf->Wp, K, Hraw, ldh);
+    else raw_functionals(ctx, g->d, g->ld, n, p, mu, nmf->Wp, K, K, Hraw, ldh);
     relu_rows(s, Hraw, ldh, p, K, Ht);
     fs.W = nmf->W; fs.Ht = Ht; fs.K = K; fs.n = n;
     double* fsc = ctx->buf<double>("as_nmf_sc", p);

@@ -1668,26 +1668,79 @@ struct FnVhatF {
   __device__ void operator()(uint64_t j, int c, double (&v)[1]) const { v[0] = mu[j] * (sc[j] * Qz[j * ldq + c]); }
 };
 
+// Largest column group of the filled forward pass at rank K (at most 7
+// tiles of 16 per pass, at most 16 columns).
+constexpr int fill_fwd_gmax(int K) { return 112 / (3 + 2 * K) < 16 ? 112 / (3 + 2 * K) : 16; }
+
+// Column maxima of |B| of one group per workgroup (rows of SNPs, Qt and H
+// values in registers; K and the group bound are template parameters).
+template <int K>
+__global__ void __launch_bounds__(256) k_fill_bmax(FnFillB fb, uint64_t pe, uint64_t rpb, double* __restrict__ part) {
+  constexpr int G = fill_fwd_gmax(K), M = G + G * K;
+  __shared__ double red[M][257];
+  const int t = threadIdx.x, g = fb.g, mq = g + g * K;
+  double mx[M];
+#pragma unroll
+  for (int x = 0; x < M; ++x) mx[x] = 0.0;
+  const uint64_t r0 = (uint64_t)blockIdx.x * rpb, r1 = min(pe, r0 + rpb);
+  for (uint64_t j = r0 + t; j < r1; j += 256) {
+    double qt[G], h[K];
+#pragma unroll
+    for (int c = 0; c < G; ++c) qt[c] = c < g ? fb.sc[j] * fb.Qz[j * fb.ldq + fb.c0 + c] : 0.0;
+#pragma unroll
+    for (int r = 0; r < K; ++r) h[r] = fb.Ht[j * K + r];
+#pragma unroll
+    for (int c = 0; c < G; ++c) {
+      mx[c] = fmax(mx[c], fabs(qt[c]));
+#pragma unroll
+      for (int r = 0; r < K; ++r) mx[G + c * K + r] = fmax(mx[G + c * K + r], fabs(h[r] * qt[c]));
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < M; ++x) red[x][t] = mx[x];
+  __syncthreads();
+  for (int x = t; x < mq; x += 256) {   // operand column x of the group
+    const int i = x < g ? x : G + (x - g);   // (x - g) = c K + r
+    double v = 0.0;
+    for (int q = 0; q < 256; ++q) v = fmax(v, red[i][q]);
+    part[(uint64_t)blockIdx.x * mq + x] = v;
+  }
+}
+
 // B operand of one group (layout of k_pack_nmf_cols): one workgroup per
-// block of 64 KS SNPs.
+// block of 64 KS SNPs, whose Qt and H values are staged in LDS.
 template <int KS>
 __global__ void __launch_bounds__(256) k_pack_fill_cols(FnFillB fb, uint64_t pe, NmfFwdMap map,
                                                         const double* __restrict__ bsc, v4i* __restrict__ out) {
-  const int NTB = map.NG + map.NM;
+  __shared__ double qs[64 * KS][16];
+  __shared__ double hs[64 * KS][kNmfMaxK8];
+  const int NTB = map.NG + map.NM, g = fb.g, K = fb.K;
   const uint64_t snp0 = (uint64_t)blockIdx.x * 64 * KS;
+  for (int e = threadIdx.x; e < 64 * KS * 16; e += 256) {
+    const int jj = e >> 4, c = e & 15;
+    const uint64_t j = snp0 + jj;
+    qs[jj][c] = j < pe && c < g ? fb.sc[j] * fb.Qz[j * fb.ldq + fb.c0 + c] : 0.0;
+  }
+  for (int e = threadIdx.x; e < 64 * KS * K; e += 256) {
+    const int jj = e / K, r = e % K;
+    const uint64_t j = snp0 + jj;
+    hs[jj][r] = j < pe ? fb.Ht[j * K + r] : 0.0;
+  }
+  __syncthreads();
   const int nch = KS * NTB * 64;
   for (int c = threadIdx.x; c < nch; c += 256) {
-    const int lane = c & 63, frag = c >> 6, ct = frag % NTB, s = frag / NTB, g = lane >> 4;
+    const int lane = c & 63, frag = c >> 6, ct = frag % NTB, s = frag / NTB, gg = lane >> 4;
     const int2 ci = map.info(16 * ct + (lane & 15));
     uint32_t word[4] = {0u, 0u, 0u, 0u};
     if (ci.x >= 0) {
       const double inv = 1.0 / bsc[ci.x];
-      const uint64_t j0 = snp0 + 16 * KS * g + 16 * s;
+      const int col = ci.x < g ? ci.x : (ci.x - g) / K, r = ci.x < g ? -1 : (ci.x - g) % K;
+      const int jb = 16 * KS * gg + 16 * s;
 #pragma unroll 4
       for (int r3 = 0; r3 < 16; ++r3) {
-        const uint64_t j = j0 + r3;
-        if (j >= pe) break;
-        const int q = quant_term(fb.val(j, ci.x) * inv, ci.y);
+        const double qt = qs[jb + r3][col];
+        const double v = r < 0 ? qt : hs[jb + r3][r] * qt;
+        const int q = quant_term(v * inv, ci.y);
         word[r3 & 3] |= (uint32_t)(q & 0xFF) << (8 * (r3 >> 2));
       }
     }
@@ -1715,10 +1768,10 @@ __global__ void k_fill_fwd_combine(const double* __restrict__ ssum, uint64_t n, 
 
 // Tile plans (NG, NM) of the filled forward pass (NG + NM <= 7, fill rank <= 8).
 #define SLFMM_FWD_FILL(X) \
-  X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5) X(3, 2) X(3, 3) X(3, 4) X(4, 3)
+  X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5) X(3, 2) X(3, 3) X(3, 4)
 
 static int fill_fwd_group(int l, int K) {
-  for (int g = std::min(l, 64); g >= 1; --g)
+  for (int g = std::min(l, fill_fwd_gmax(K)); g >= 1; --g)
     if ((3 * g + 15) / 16 + (2 * g * K + 15) / 16 <= 7) return g;
   return 0;
 }
@@ -1741,7 +1794,19 @@ void fill_forward_i8(slfmm_ctx* ctx, const uint8_t* YT, uint64_t ldt, uint64_t n
     const int NTB = map.NG + map.NM, mq = g + g * K;
     const FnFillB fb{Qz, ldq, sc, Ht, K, c0, g};
     double* bsc = ctx->buf<double>("ff8_sc", mq);
-    colreduce<1, kColMax>(ctx, fb, GScaleMax{bsc}, pe, mq);
+    {
+      uint64_t nb = std::min<uint64_t>(std::max<uint64_t>(cdiv(pe, 2048), 1), 1024);
+      const uint64_t rpb = cdiv(pe, nb);
+      nb = cdiv(pe, rpb);
+      double* bpart = ctx->buf<double>("ff8_bpart", nb * mq);
+      switch (K) {
+#define SLFMM_BMAX(KK) case KK: k_fill_bmax<KK><<<(unsigned)nb, 256, 0, s>>>(fb, pe, rpb, bpart); break;
+        SLFMM_BMAX(1) SLFMM_BMAX(2) SLFMM_BMAX(3) SLFMM_BMAX(4) SLFMM_BMAX(5) SLFMM_BMAX(6) SLFMM_BMAX(7)
+        SLFMM_BMAX(8)
+#undef SLFMM_BMAX
+      }
+      k_colfin<1, kColMax, GScaleMax><<<(unsigned)cdiv(mq, 4), 256, 0, s>>>(bpart, (int)nb, mq, GScaleMax{bsc});
+    }
     const uint64_t n16 = nblk * KS * NTB * 64;
     SLFMM_CHECK(n16 * 16 < 0xFFFFFFFFull, SLFMM_E_ARG, "estimation panel too large for the int8 filled forward pass");
     v4i* Bp = ctx->buf<v4i>("ff8_pack", n16);

@@ -410,5 +410,8 @@ void gwas_cv(slfmm_ctx* ctx, const uint8_t* packed, uint64_t ld, uint64_t n, uin
 // the row mean on missing calls); ldout >= rup(m, 32).
 void raw_functionals(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const double* mu,
                      const double* Op, uint64_t ldo, uint64_t m, double* out, uint64_t ldout);
+// raw_functionals on the int8 matrix cores (m <= 32; no padding of Op).
+void raw_functionals_i8(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const double* mu,
+                        const double* Op, uint64_t m, double* out, uint64_t ldout);
 
 }  // namespace slfmm

@@ -367,6 +367,17 @@ void raw_functionals(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, 
   SLFMM_HIP(hipGetLastError());
 }
 
+// The same functionals on the int8 matrix cores (m <= 32): the centred
+// sketch pass (rsvd_project_i8, centring only) plus mu_j * colsum.
+void raw_functionals_i8(slfmm_ctx* ctx, const uint8_t* Y, uint64_t ld, uint64_t n, uint64_t pe, const double* mu,
+                        const double* Op, uint64_t m, double* out, uint64_t ldout) {
+  rsvd_project_i8(ctx, Y, ld, n, pe, Op, m, SLFMM_NORM_CENTER_ONLY, out, ldout);
+  double* cs = ctx->buf<double>("nmf_cs", m);
+  colreduce<1, kColSum>(ctx, FnColSum{Op, m}, GStore{cs}, n, (int)m);
+  k_add_mean_term<<<(unsigned)cdiv(pe * m, 256), 256, 0, ctx->stream>>>(out, ldout, pe, (int)m, mu, cs);
+  SLFMM_HIP(hipGetLastError());
+}
+
 static void normalize_w(slfmm_ctx* ctx, double* W, uint64_t n, double* Ht, uint64_t pe, int K, double eps) {
   double* cs = ctx->buf<double>("nmf_wsum", K);
   colreduce<1, kColSum>(ctx, FnColSum{W, (uint64_t)K}, GStore{cs}, n, K);
