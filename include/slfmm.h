@@ -296,7 +296,8 @@ int slfmm_fit_lfmm2_batch(slfmm_ctx* ctx, slfmm_problem* problems, uint64_t coun
  * functionals split each filled sum into the dosages against the operand (3
  * int8 terms) and the missing-call indicator against the operand's products
  * with W (or H) (2 terms), recombined in FP64 with the SNP's own H_j (the
- * sample's own W_k); counts, moments, H_j, the CV and the epilogue stay
+ * sample's own W_k); the fill moments and the association's H_j run there
+ * as well; counts, the NMF initialisation, the CV and the epilogue stay
  * FP64.  Layouts past the int8 tile plans run those passes in FP64. */
 int slfmm_fit_lfmm2_nmf(slfmm_ctx* ctx, const slfmm_geno* est, const int64_t* subset_idx, uint64_t n_idx,
                         const slfmm_geno* g, const double* x, uint64_t d, const slfmm_config* cfg,
