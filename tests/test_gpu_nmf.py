@@ -107,16 +107,17 @@ def test_fit_lfmm2_nmf_vs_oracle(S, int8, cv_rate):
     assert np.max(np.abs(ref.t)) > 4   # the panel has real signal
 
 
-@pytest.mark.parametrize("nmf_int8", [False, True])
-def test_fit_lfmm2_nmf_mixed_vs_oracle(S, nmf_int8):
+@pytest.mark.parametrize("nmf_int8,kn", [(False, 3), (True, 3), (True, 10)])
+def test_fit_lfmm2_nmf_mixed_vs_oracle(S, nmf_int8, kn):
     """The NMF-imputed fit in mixed precision: the filled sketch/backward
     passes and the filled association functionals on the int8 matrix cores
     (A, or B, in 3 int8 terms against the dosages, the products A_c W_r in 2
     against the missing-call indicator), counts, moments, forward passes and
     epilogue in FP64; optionally the NMF update passes in int8 too.  Against
     the FP64 oracle: U_hat angles < 1e-4 rad, |delta log10 p| <= 1e-3, GWAS-
-    phase CV counts exact and errors within 1e-5 relative (they use the FP64
-    H_j)."""
+    phase CV counts exact and errors within 1e-5 relative.  nmf-k = 10 is
+    past the int8 fill plans: the filled passes run in FP64 there, with the
+    same tolerances."""
     n, p, K, d = 300, 2000, 3, 2
     pan = synth.balding_nichols(n, p, n_pops=3, miss_rate=0.10, d=d, n_causal=20, effect=1.0, seed=77)
     packed = O.encode_bed(pan.g)
@@ -124,7 +125,7 @@ def test_fit_lfmm2_nmf_mixed_vs_oracle(S, nmf_int8):
     xs = O.center_covariates(pan.x)
     pre = O.precompute(xs, 1e-5)
     b0 = pre.m.T @ np.random.default_rng(4).standard_normal((n, K + 10))
-    kn, seed_nmf, cv_rate = 3, 7, 0.02
+    seed_nmf, cv_rate = 7, 0.02
     omega = synth.normal_matrix(seed_nmf, n, kn + 10)
     cfg_o = O.Config(k=K, norm=O.EIGENSTRAT, chunk_size=500, n_power_iter=2, seed=42)
     est = O.subset_indices(0.5, p)
