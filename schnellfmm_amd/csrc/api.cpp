@@ -77,7 +77,9 @@ void slfmm_ctx::upload(void* dst, const void* src, size_t bytes) {
     if (up_buf) (void)hipHostFree(up_buf);
     up_buf = nullptr;
     up_cap = 0;
-    const size_t cap = std::max<size_t>(std::max<size_t>(bytes, 2 * off), 1 << 20);
+    // room for this call's uploads so far and this one, twice over: a call
+    // that repeats the same uploads then fits without growing again
+    const size_t cap = std::max<size_t>(2 * (off + bytes), 1 << 20);
     if (hipHostMalloc(reinterpret_cast<void**>(&up_buf), cap, hipHostMallocDefault) != hipSuccess) {
       (void)hipGetLastError();
       up_buf = nullptr;
@@ -96,6 +98,13 @@ void slfmm_ctx::upload(void* dst, const void* src, size_t bytes) {
 void slfmm_ctx::upload_reset() {
   if (up_top && ev_up) SLFMM_HIP(hipEventSynchronize(ev_up));
   up_top = 0;
+}
+
+void slfmm_ctx::ensure_side_stream() {
+  if (side_stream) return;
+  SLFMM_HIP(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking));
+  SLFMM_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  SLFMM_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
 }
 
 void slfmm_ctx::ensure_copy_stream() {
@@ -160,6 +169,9 @@ std::vector<int> slfmm_ctx::finish_checks() {
 slfmm_ctx::~slfmm_ctx() {
   for (slfmm_ctx* w : workers) delete w;
   if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+  if (side_stream) (void)hipStreamSynchronize(side_stream);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
   for (int b = 0; b < 2; ++b) {
     if (ev_copied[b]) (void)hipEventDestroy(ev_copied[b]);
     if (ev_consumed[b]) (void)hipEventDestroy(ev_consumed[b]);
@@ -175,6 +187,7 @@ slfmm_ctx::~slfmm_ctx() {
   if (up_buf) (void)hipHostFree(up_buf);
   if (ev_up) (void)hipEventDestroy(ev_up);
   if (copy_stream) (void)hipStreamDestroy(copy_stream);
+  if (side_stream) (void)hipStreamDestroy(side_stream);
   if (stream) (void)hipStreamDestroy(stream);
 }
 

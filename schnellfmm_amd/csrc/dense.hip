@@ -98,7 +98,10 @@ __global__ void __launch_bounds__(256) k_gram_partial(const double* __restrict__
 // Same partial Gram for a, b <= 32 with register blocking: each wave takes
 // every fourth row of a 32-row tile, each lane a 4 x 4 block of the 32 x 32
 // output (8 LDS reads per 16 FMAs), and the four waves' sums are added in a
-// fixed order at the end.
+// fixed order at the end.  The next tile's loads are issued into registers
+// before the current tile's products, so their latency overlaps the
+// arithmetic; with B the same matrix as A (same pointer, stride and width)
+// only A is loaded.
 __global__ void __launch_bounds__(256) k_gram_partial32(const double* __restrict__ A, uint64_t lda, int a,
                                                         const double* __restrict__ B, uint64_t ldb, int b,
                                                         uint64_t rows, uint64_t rpb, double* __restrict__ part,
@@ -108,6 +111,7 @@ __global__ void __launch_bounds__(256) k_gram_partial32(const double* __restrict
   __shared__ __align__(16) double sb[32][32];
   __shared__ double red[4][32][33];
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63, ti = lane >> 3, tj = lane & 7;
+  const bool same = A == B && lda == ldb && a == b;
   const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
   const uint64_t r1 = std::min(rows, r0 + rpb);
   double acc[4][4];
@@ -115,17 +119,30 @@ __global__ void __launch_bounds__(256) k_gram_partial32(const double* __restrict
   for (int x = 0; x < 4; ++x)
 #pragma unroll
     for (int y = 0; y < 4; ++y) acc[x][y] = 0.0;
+  // thread t stages column t & 31 of rows (t >> 5) + 8k, k < 4; columns
+  // past a (b) and rows past the range are zero
+  const int c = t & 31, rq = t >> 5;
+  double va[4], vb[4];
+  auto fetch = [&](uint64_t rb) {
+    const int nr = (int)std::min<uint64_t>(32, r1 - rb);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int rr = rq + 8 * k;
+      va[k] = (rr < nr && c < a) ? A[(rb + rr) * lda + c] : 0.0;
+      vb[k] = (!same && rr < nr && c < b) ? B[(rb + rr) * ldb + c] : 0.0;
+    }
+  };
+  if (r0 < r1) fetch(r0);
   for (uint64_t rb = r0; rb < r1; rb += 32) {
     const int nr = (int)std::min<uint64_t>(32, r1 - rb);
     __syncthreads();
-    {   // 8 rows x 32 columns per pass; columns past a (b) and rows past the range are zero
-      const int c = t & 31;
-      for (int rr = t >> 5; rr < 32; rr += 8) {
-        sa[rr][c] = (rr < nr && c < a) ? A[(rb + rr) * lda + c] : 0.0;
-        sb[rr][c] = (rr < nr && c < b) ? B[(rb + rr) * ldb + c] : 0.0;
-      }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      sa[rq + 8 * k][c] = va[k];
+      sb[rq + 8 * k][c] = same ? va[k] : vb[k];
     }
     __syncthreads();
+    if (rb + 32 < r1) fetch(rb + 32);
     for (int rr = wave; rr < nr; rr += 4) {
       double av[4], bv[4];
 #pragma unroll
@@ -220,11 +237,81 @@ __global__ void __launch_bounds__(256) k_ts_gemm(const double* __restrict__ A, u
   }
 }
 
+// Same product for m, q <= 32: column c of M is held in registers by the
+// threads of column c, and a pass of a workgroup stages 64 rows of A in LDS
+// (eight coalesced loads per thread) and forms eight rows of the output per
+// thread.  Each output element is the same dot product, summed in the same
+// order (t = 0, 1, ...), as k_ts_gemm's; the terms past m are zero products
+// of the zero-padded operands, so no branch on m splits the LDS reads.
+constexpr int kTsRows = 64;
+__global__ void __launch_bounds__(256) k_ts_gemm32(const double* __restrict__ A, uint64_t lda, int m,
+                                                   const double* __restrict__ M, uint64_t ldm, int q,
+                                                   uint64_t rows, double alpha, double beta, double* out,
+                                                   uint64_t ldo, const int* gate, const double* dlam, int inverse) {
+  if (gate && *gate != 1) return;
+  __shared__ double sa[kTsRows][33];
+  __shared__ double sm[32][33];
+  const int c = threadIdx.x & 31, rq = threadIdx.x >> 5;
+  // M staged through LDS (coalesced, all loads in flight together), then
+  // column c into registers
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int t = rq + 8 * k;
+    double v = 0.0;
+    if (t < m && c < q) v = M[(uint64_t)t * ldm + c];
+    sm[t][c] = v;
+  }
+  __syncthreads();
+  if (dlam && threadIdx.x < 32) {   // row t of M scaled by -(1 - d_t), or 1 / d_t - 1 for the inverse
+    const int t = threadIdx.x;
+    if (t < m) {
+      const double dl = dlam[t];
+      const double f = inverse ? (1.0 / dl - 1.0) : -(1.0 - dl);
+      for (int cc = 0; cc < q; ++cc) sm[t][cc] *= f;
+    }
+  }
+  __syncthreads();
+  double mc[32];
+#pragma unroll
+  for (int t = 0; t < 32; ++t) mc[t] = sm[t][c];
+  for (uint64_t r0 = (uint64_t)blockIdx.x * kTsRows; r0 < rows; r0 += (uint64_t)gridDim.x * kTsRows) {
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kTsRows / 8; ++k) {
+      const int i = rq + 8 * k;
+      sa[i][c] = (c < m && r0 + i < rows) ? A[(r0 + i) * lda + c] : 0.0;
+    }
+    __syncthreads();
+    if (c < q) {
+#pragma unroll 2
+      for (int k = 0; k < kTsRows / 8; ++k) {
+        const int i = rq + 8 * k;
+        const uint64_t r = r0 + i;
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < 32; ++t) s += sa[i][t] * mc[t];   // zero past m on both sides
+        if (r < rows) {
+          double v = alpha * s;
+          if (beta != 0.0) v += beta * out[r * ldo + c];
+          out[r * ldo + c] = v;
+        }
+      }
+    }
+  }
+}
+
 void ts_gemm(hipStream_t s, const double* A, uint64_t lda, uint64_t m, const double* M,
              uint64_t ldm, uint64_t q, uint64_t rows, double alpha, double beta, double* out,
              uint64_t ldo, const int* gate, const double* dlam, bool inverse) {
   if (rows == 0 || q == 0) return;
   SLFMM_CHECK(m <= (uint64_t)kSm && q <= (uint64_t)kSm, SLFMM_E_ARG, "ts_gemm: too many columns");
+  if (m <= 32 && q <= 32 && rows > 32768) {   // measured: below this the one-row-per-thread kernel is faster
+    const uint64_t blocks = std::min<uint64_t>(cdiv(rows, kTsRows), 4096);
+    k_ts_gemm32<<<(unsigned)blocks, 256, 0, s>>>(A, lda, (int)m, M, ldm, (int)q, rows, alpha, beta, out, ldo, gate,
+                                                 dlam, inverse ? 1 : 0);
+    SLFMM_HIP(hipGetLastError());
+    return;
+  }
   const uint64_t rpb = q <= 32 ? 8 : 4;
   uint64_t blocks = std::min<uint64_t>(cdiv(rows, rpb), 1024);   // M is staged once per workgroup
   k_ts_gemm<<<(unsigned)blocks, 256, 0, s>>>(A, lda, (int)m, M, ldm, (int)q, rows, alpha, beta, out, ldo, gate, dlam,

@@ -114,6 +114,12 @@ struct slfmm_ctx {
   hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_consumed[2] = {nullptr, nullptr};
   void* pinned_buf(int i, size_t bytes);   // grow-only, hipHostMalloc
   void ensure_copy_stream();
+  // A second compute stream (default priority) for independent work that
+  // overlaps the main stream's chain of small kernels: forked from and
+  // joined back into `stream` with the two events (capturable).
+  hipStream_t side_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  void ensure_side_stream();
   // Uploads of host arrays: copied into a grow-only pinned buffer and queued
   // with hipMemcpyAsync from there, so the call returns without waiting for
   // the copy (a pageable source would be staged synchronously).  Slices are

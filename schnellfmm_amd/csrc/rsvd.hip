@@ -296,11 +296,36 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   const bool fill_fwd_i8 = cfg.precision == SLFMM_PREC_I8 && filled && fill_forward_i8_supported((int)l, fill->K);
   uint8_t* YT = nullptr;
   uint64_t ldt = 0;
-  if (mixed || fill_fwd_i8) {   // sample-major copy for the forward passes
+  // The sample-major copy for the forward passes is first read by the first
+  // forward pass, after the first sketch and its QR: outside allocation-free
+  // calls it is made on the side stream, overlapping that chain, and joined
+  // back before the first forward pass (or when this scope is left early, so
+  // that a graph capture always sees the join).
+  struct SideJoin {
+    slfmm_ctx* ctx = nullptr;
+    void join() {
+      if (ctx) SLFMM_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+      ctx = nullptr;
+    }
+    ~SideJoin() {
+      if (ctx) (void)hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+    }
+  } yt_join;
+  if (mixed || fill_fwd_i8) {
     ldt = rup(cdiv(pe, 4), 16);
     YT = ctx->buf<uint8_t>("rs_YT", n * ldt + kSlack);
-    launch_fill(s, reinterpret_cast<double*>(YT + n * ldt), 0.0, kSlack / 8);   // the transpose writes every row byte
-    launch_transpose(s, Y, g->ld, n, pe, YT, ldt);
+    SLFMM_CHECK(ldt % 16 == 0 && g->ld % 16 == 0, SLFMM_E_ARG, "transpose strides must be multiples of 16 bytes");
+    hipStream_t ts = s;
+    if (!ctx->arena) {
+      ctx->ensure_side_stream();
+      ts = ctx->side_stream;
+      SLFMM_HIP(hipEventRecord(ctx->ev_fork, s));
+      SLFMM_HIP(hipStreamWaitEvent(ts, ctx->ev_fork, 0));
+      yt_join.ctx = ctx;
+    }
+    launch_fill(ts, reinterpret_cast<double*>(YT + n * ldt), 0.0, kSlack / 8);   // the transpose writes every row byte
+    launch_transpose(ts, Y, g->ld, n, pe, YT, ldt);
+    if (ts != s) SLFMM_HIP(hipEventRecord(ctx->ev_join, ts));
   }
 
   const uint64_t npad = rup(n, 64);
@@ -336,6 +361,7 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
             ctx->check_slot("", SLFMM_E_NUMERIC, "QR of the RSVD sketch failed (rank deficient)"));
   };
   auto forward = [&]() {   // W = N Y Q_z
+    yt_join.join();
     {
       slfmm_timed tm(ctx, "rsvd_forward");
       if (mixed) rsvd_accumulate_i8(ctx, YT, ldt, n, pe, Zq, lda, l, mu, sc, W, l);
