@@ -468,13 +468,14 @@ __device__ __forceinline__ void wave_lds_sync() {
 // for m <= 32, on one wave: lane l keeps column l of the working matrix in
 // registers, padded to 32 x 32 with an identity block past m (pivots 1,
 // zero couplings: the steps past m change nothing, and no step needs a
-// branch on m).  Step j broadcasts row j of the working matrix and then
-// row j of R through LDS (one write per lane, every lane reads the same
-// values), so the trailing update is independent FMAs per lane.  The
-// kernel runs 32 threads (one wave, lanes 32..63 inactive).  The
-// arithmetic of every element is k_chol's: R[j][l] = A[j][l] / r_jj, then
-// A[i][l] -= R[j][i] R[j][l] for j < i <= l.  lds: 5 x 32 doubles
-// (two row pairs used by alternate steps, then the pivot references).
+// branch on m).  Step j reads its pivot from lane j (readlane into scalar
+// registers) and broadcasts row j of R through LDS (one write per lane,
+// every lane reads the same values), so the trailing update is independent
+// FMAs per lane.  The kernel runs 32 threads (one wave, lanes 32..63
+// inactive).  The arithmetic of every element is k_chol's: R[j][l] =
+// A[j][l] / r_jj, then A[i][l] -= R[j][i] R[j][l] for j < i <= l.  lds:
+// 5 x 32 doubles (rows of R in two slots used by alternate steps, then the
+// pivot references).
 // Returns 0, or j + 1 for a failed pivot j; column l of R is left in col.
 __device__ __forceinline__ int chol32_factor(const double* __restrict__ G, int m, double shift, double tol,
                                              double (&col)[32], double* lds) {
@@ -489,13 +490,11 @@ __device__ __forceinline__ int chol32_factor(const double* __restrict__ G, int m
     col[i] = (lv && i < m) ? G[i * m + l] + (i == l ? shift : 0.0) : (i == l ? 1.0 : 0.0);
   double* d0s = lds + 4 * 32;   // pivot references (the padded diagonal reads 1)
   d0s[l] = lv ? G[l * m + l] + shift : 1.0;
+  wave_lds_sync();
 #pragma unroll
   for (int j = 0; j < 32; ++j) {
-    double* arow = lds + 64 * (j & 1);
-    double* rrow = arow + 32;
-    arow[l] = col[j];   // row j of the working matrix
-    wave_lds_sync();
-    const double dj = arow[j], d0 = d0s[j];
+    double* rrow = lds + 64 * (j & 1) + 32;
+    const double dj = lane_value(col[j], j), d0 = d0s[j];   // pivot: lane j's diagonal element
     if (!(dj > tol * d0) || !(dj < 1e308)) return j + 1;   // wave-uniform
     const double rjj = sqrt(dj);
     const double r = l > j ? col[j] / rjj : (l == j ? rjj : 0.0);   // R[j][l]
