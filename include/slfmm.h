@@ -52,7 +52,9 @@ enum {
                              operands, exact int32 accumulation, FP64 epilogue.
                              The int8 kernels carry at most 32 columns: a sketch
                              with K + oversampling > 32, or an association with
-                             K + d > 32, runs that stage's FP64 kernels. */
+                             K + d > 32, runs that stage's FP64 kernels, and
+                             each such stage is reported once per context on
+                             stderr. */
 };
 
 /* SLFMM_OUT_DEVICE: outputs stay in device memory (the pointers in the

@@ -100,6 +100,12 @@ void slfmm_ctx::upload_reset() {
   up_top = 0;
 }
 
+void slfmm::fp64_stage(slfmm_ctx* ctx, const std::string& stage, const std::string& why) {
+  if (!ctx->fp64_noted.insert(stage).second) return;
+  fprintf(stderr, "schnellfmm_amd: SLFMM_PREC_I8 requested, but the %s runs in FP64 (%s)\n", stage.c_str(),
+          why.c_str());
+}
+
 void slfmm_ctx::ensure_side_stream() {
   if (side_stream) return;
   SLFMM_HIP(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking));

@@ -732,6 +732,12 @@ void assoc_run(slfmm_ctx* ctx, const slfmm_geno* g, const double* d_xs, uint64_t
   // plans reach (nmf_i8), and the counts, moments and epilogue stay FP64.
   const bool i8 = cfg.precision == SLFMM_PREC_I8 && m <= 32 && !with_nmf;
   const bool nmf_i8 = cfg.precision == SLFMM_PREC_I8 && with_nmf && fill_functionals_i8_supported((int)m, nmf->K);
+  if (cfg.precision == SLFMM_PREC_I8 && !with_nmf && !i8)
+    fp64_stage(ctx, "association", "K + d = " + std::to_string(m) + " > 32 columns");
+  if (cfg.precision == SLFMM_PREC_I8 && with_nmf && !nmf_i8)
+    fp64_stage(ctx, "association over the NMF-filled rows",
+               "K + d = " + std::to_string(m) + " with nmf-k = " + std::to_string(nmf->K) +
+                   " is past the int8 fill plans");
   // per-SNP integer counts of the FP64 kernels (the int8 kernels count in registers)
   int32_t* nmiss = i8 ? nullptr : ctx->buf<int32_t>("as_nmiss", p);
   int32_t* sumg = i8 ? nullptr : ctx->buf<int32_t>("as_sumg", p);

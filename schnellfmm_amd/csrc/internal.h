@@ -8,6 +8,7 @@
 #include <map>
 #include <new>
 #include <stdexcept>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -117,6 +118,9 @@ struct slfmm_ctx {
   // A second compute stream (default priority) for independent work that
   // overlaps the main stream's chain of small kernels: forked from and
   // joined back into `stream` with the two events (capturable).
+  // Stages that SLFMM_PREC_I8 asked for but whose shape runs the FP64
+  // kernels, each reported once per context on stderr (fp64_stage).
+  std::set<std::string> fp64_noted;
   hipStream_t side_stream = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void ensure_side_stream();
@@ -262,6 +266,9 @@ void ts_gemm(hipStream_t s, const double* A, uint64_t lda, uint64_t m, const dou
              uint64_t ldm, uint64_t q, uint64_t rows, double alpha, double beta, double* out,
              uint64_t ldo, const int* gate = nullptr, const double* dlam = nullptr, bool inverse = false);
 void launch_fill(hipStream_t s, double* p, double v, uint64_t count);
+// Reports once per context that a stage requested in int8 runs its FP64
+// kernels because of its shape (stage: what runs in FP64; why: the limit).
+void fp64_stage(slfmm_ctx* ctx, const std::string& stage, const std::string& why);
 // dst (rows x cols, ldd) <- src (rows x cols, lds)
 void copy2d(hipStream_t s, const double* src, uint64_t lds, double* dst, uint64_t ldd, uint64_t rows,
             uint64_t cols, const int* gate = nullptr);

@@ -294,6 +294,12 @@ void rsvd_run(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* h_idx, uint64_
   const bool mixed = cfg.precision == SLFMM_PREC_I8 && l <= 32 && !filled;
   const bool fill_i8 = cfg.precision == SLFMM_PREC_I8 && filled && fill_functionals_i8_supported((int)l, fill->K);
   const bool fill_fwd_i8 = cfg.precision == SLFMM_PREC_I8 && filled && fill_forward_i8_supported((int)l, fill->K);
+  if (cfg.precision == SLFMM_PREC_I8 && !filled && !mixed)
+    fp64_stage(ctx, "randomized SVD", "K + oversampling = " + std::to_string(l) + " > 32 columns");
+  if (cfg.precision == SLFMM_PREC_I8 && filled && !(fill_i8 && fill_fwd_i8))
+    fp64_stage(ctx, "randomized SVD over the NMF-filled rows",
+               "K + oversampling = " + std::to_string(l) + " with nmf-k = " + std::to_string(fill->K) +
+                   " is past the int8 fill plans");
   uint8_t* YT = nullptr;
   uint64_t ldt = 0;
   // The sample-major copy for the forward passes is first read by the first

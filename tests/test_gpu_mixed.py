@@ -166,13 +166,17 @@ def test_i8_fit_end_to_end_log10p_bound(S):
     assert np.max(np.abs(ref.t)) > 5       # the panel has real signal
 
 
-def test_i8_wide_shapes_fall_back_to_fp64_kernels(S):
+def test_i8_wide_shapes_fall_back_to_fp64_kernels(S, capfd):
     """Past 32 int8 columns the mixed-precision fit runs the FP64 kernels for
     that stage: K + oversampling > 32 gives the FP64 sketch (so U_hat equals
-    the FP64 fit's bit for bit), and K + d > 32 the FP64 association too."""
+    the FP64 fit's bit for bit), and K + d > 32 the FP64 association too.
+    Each such stage is reported once per context on stderr."""
     n, p = 500, 2500
     pan, packed, bps = _panel(n, p, seed=77, d=1, miss=0.03)
-    g = S.BedFile.from_packed(packed, n, p)
+    g = S.BedFile.from_packed(packed, n, p, ctx=S.Context())   # a fresh context: its notes start empty
+    capfd.readouterr()
+    S.fit_lfmm2(g, 0.5, g, pan.x, S.Lfmm2Config(k=20, norm=S.EIGENSTRAT, precision=S.PREC_I8))
+    assert "SLFMM_PREC_I8 requested" not in capfd.readouterr().err   # every stage in int8
     for K in (30, 40):
         f64 = S.fit_lfmm2(g, 0.5, g, pan.x, S.Lfmm2Config(k=K, norm=S.EIGENSTRAT, precision=S.PREC_FP64))
         mix = S.fit_lfmm2(g, 0.5, g, pan.x, S.Lfmm2Config(k=K, norm=S.EIGENSTRAT, precision=S.PREC_I8))
@@ -181,3 +185,10 @@ def test_i8_wide_shapes_fall_back_to_fp64_kernels(S):
             assert np.array_equal(mix.tstat, f64.tstat)
         dlp = np.abs(O.log10_pvalue(mix.tstat, mix.gif_per_trait) - O.log10_pvalue(f64.tstat, f64.gif_per_trait))
         assert np.max(dlp) <= 1e-3
+        err = capfd.readouterr().err
+        if K == 30:   # first wide fit: the sketch note (40 columns); the association has 31
+            assert "randomized SVD runs in FP64 (K + oversampling = 40 > 32 columns)" in err
+            assert "association runs in FP64" not in err
+        else:         # the sketch note is not repeated; the association (41 columns) is new
+            assert "randomized SVD runs in FP64" not in err
+            assert "association runs in FP64 (K + d = 41 > 32 columns)" in err
