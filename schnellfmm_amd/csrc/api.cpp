@@ -873,6 +873,7 @@ int slfmm_fit_lfmm2_batch(slfmm_ctx* ctx, slfmm_problem* probs, uint64_t count, 
       slfmm_ctx* w = new slfmm_ctx();
       w->device = ctx->device;
       w->tuning = ctx->tuning;
+      w->batch_worker = true;
       if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
         delete w;
         throw Error(SLFMM_E_HIP, "stream creation failed");

@@ -122,6 +122,7 @@ struct slfmm_ctx {
   // kernels, each reported once per context on stderr (fp64_stage).
   std::set<std::string> fp64_noted;
   hipStream_t side_stream = nullptr;
+  bool batch_worker = false;   // a stream of slfmm_fit_lfmm2_batch (problems already overlap)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void ensure_side_stream();
   // Uploads of host arrays: copied into a grow-only pinned buffer and queued
