@@ -667,13 +667,17 @@ __global__ void __launch_bounds__(256) k_safe_inv(const double* __restrict__ In,
         double* x = Xs + t;   // x[i] is Xs[i * m + t]
         for (int i = 0; i < m; ++i) x[i * m] = (i == t) ? 1.0 : 0.0;
         for (int j = 0; j < m; ++j) { int p = piv[j]; double tmp = x[j * m]; x[j * m] = x[p * m]; x[p * m] = tmp; }
+        // (the inner loops are unrolled so that their LDS reads are issued
+        // ahead of the dependent chain of subtractions; same order of terms)
         for (int i = 0; i < m; ++i) {
           double s = x[i * m];
+#pragma unroll 8
           for (int k = 0; k < i; ++k) s -= A[i * m + k] * x[k * m];
           x[i * m] = s;
         }
         for (int i = m - 1; i >= 0; --i) {
           double s = x[i * m];
+#pragma unroll 8
           for (int k = i + 1; k < m; ++k) s -= A[i * m + k] * x[k * m];
           x[i * m] = s / A[i * m + i];
         }
