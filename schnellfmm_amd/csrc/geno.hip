@@ -112,19 +112,25 @@ void geno_upload(slfmm_ctx* ctx, const uint8_t* packed, uint64_t n_phys, uint64_
   SLFMM_HIP(hipGetLastError());
 }
 
-__global__ void k_gather(const uint8_t* __restrict__ src, uint64_t ld, const int64_t* __restrict__ idx,
-                         uint64_t m, uint8_t* __restrict__ dst) {
-  uint64_t r = blockIdx.y + (uint64_t)blockIdx.z * 65535u;
-  if (r >= m) return;
-  const uint4* s = reinterpret_cast<const uint4*>(src + (uint64_t)idx[r] * ld);
-  uint4* d = reinterpret_cast<uint4*>(dst + r * ld);
-  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < ld / 16; w += (uint64_t)gridDim.x * blockDim.x)
-    d[w] = s[w];
+// One wave per gathered row (16 bytes per lane per pass), rows taken in a
+// grid-stride loop by the waves of a fixed grid: a row is a few KB, so one
+// workgroup per row left most of each short workgroup's time to its launch.
+__global__ void __launch_bounds__(256) k_gather(const uint8_t* __restrict__ src, uint64_t ld,
+                                                const int64_t* __restrict__ idx, uint64_t m,
+                                                uint8_t* __restrict__ dst) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = ld / 16;
+  for (uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); r < m; r += (uint64_t)gridDim.x * 4) {
+    const uint4* s = reinterpret_cast<const uint4*>(src + (uint64_t)idx[r] * ld);
+    uint4* d = reinterpret_cast<uint4*>(dst + r * ld);
+    for (uint64_t w = lane; w < nw; w += 64) d[w] = s[w];
+  }
 }
 
 void geno_gather(slfmm_ctx* ctx, const slfmm_geno* g, const int64_t* d_idx, uint64_t m, uint8_t* dst) {
   if (m == 0) return;
-  k_gather<<<grid_rows(cdiv(g->ld / 16, 256), m), 256, 0, ctx->stream>>>(g->d, g->ld, d_idx, m, dst);
+  const unsigned blocks = (unsigned)std::min<uint64_t>(cdiv(m, 4), 8192);
+  k_gather<<<blocks, 256, 0, ctx->stream>>>(g->d, g->ld, d_idx, m, dst);
   SLFMM_HIP(hipGetLastError());
 }
 
